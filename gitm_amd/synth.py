@@ -1,0 +1,272 @@
+"""Seeded synthetic atmospheres for the vertical step.
+
+Builds the column working set that GITM's ``advance_vertical`` reads
+(src/advance_vertical.f90:84-151) for a grid of independent columns, in the
+column-contiguous layout used throughout this package: every per-level field
+is stored ``[column][component][level]`` with the level index fastest and
+levels -1..nAlts+2 at offsets 0..nAlts+3.
+
+The column prep mirrors the reference's own formulas:
+
+* uniform altitudes ``AltMin + (iAlt-0.5)*dz``  (src/initialize.f90:138-141);
+* ``dAlt``, ``1/r`` and ``g = -g0*(R/r)**2``     (src/initialize.f90:148-170);
+* ``Gamma`` and ``cp``                           (src/calc_pressure.f90:56-86);
+* ``MeanMajorMass = sum(m n)/(N+1)``, ``TempUnit = MMM/k``
+                                                 (src/calc_rates.Earth.f90:33-50);
+* ``cMax = max|w_s| + sqrt(Gamma*T)``            (src/calc_timestep.f90:114-118);
+* eddy coefficient pressure ramp                 (src/Earth.f90:376-411).
+
+Random numbers come from a counter-based generator (splitmix64 + Box-Muller),
+so a (seed, stream, index) triple always yields the same value.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+NSPECIES = 6
+NIONS = 10
+KB = 1.3807e-23          # ModConstants.f90:15
+AMU = 1.6726e-27         # ModConstants.f90:11
+R_EARTH = 6372.0 * 1000.0  # ModEarth.f90:73
+VIBRATION = (5.0, 7.0, 7.0, 5.0, 7.0, 5.0)  # ModEarth.f90:307-312
+BASE_DENSITY = (1e17, 1e18, 4e18, 1e13, 1e14, 1e14)  # O O2 N2 N NO He at 100 km
+
+STATE_FIELDS = ("nds", "vvel", "vel", "temp", "rho")
+PREP_FIELDS = ("tempunit", "mmm", "gamma", "cmax", "eddy", "alt", "grav",
+               "invr", "dalt")
+COMPONENTS = {"nds": NSPECIES, "vvel": NSPECIES, "vel": 3, "logns": NSPECIES,
+              "ions": NIONS, "ivel": 3}
+
+
+def species_masses() -> np.ndarray:
+    """Species masses in kg, formed as init_planet does (ModEarth.f90:271-281)."""
+    mo = 16.0 * AMU
+    mn = 14.0 * AMU
+    return np.array([mo, 2.0 * mo, 2.0 * mn, mn, mn + mo, 4.0 * AMU])
+
+
+# ---------------------------------------------------------------------------
+# counter-based random numbers
+# ---------------------------------------------------------------------------
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def _splitmix64(x: np.ndarray) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        z = x + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def _uniform(seed: int, stream: int, n: int) -> np.ndarray:
+    """n doubles in (0, 1) from counter (seed, stream, 0..n-1)."""
+    base = _splitmix64(np.array([(seed * 1000003 + stream) & 0xFFFFFFFFFFFFFFFF],
+                                dtype=np.uint64))[0]
+    with np.errstate(over="ignore"):
+        ctr = np.arange(n, dtype=np.uint64) + base
+    bits = _splitmix64(ctr) >> np.uint64(11)
+    return (bits.astype(np.float64) + 0.5) * (1.0 / 9007199254740992.0)
+
+
+def normal(seed: int, stream: int, shape) -> np.ndarray:
+    """Standard normals (Box-Muller) from counter (seed, stream)."""
+    n = int(np.prod(shape))
+    m = (n + 1) // 2
+    u1 = _uniform(seed, 2 * stream, m)
+    u2 = _uniform(seed, 2 * stream + 1, m)
+    r = np.sqrt(-2.0 * np.log(u1))
+    z = np.concatenate([r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)])
+    return z[:n].reshape(shape)
+
+
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class GridSpec:
+    """A lat/lon grid of GITM blocks of ``nlons x nlats`` cells."""
+
+    nblocks_lon: int
+    nblocks_lat: int
+    nalts: int
+    nlons: int = 9
+    nlats: int = 9
+    stretched: bool = False   # geometric dz 1.5 -> 15 km from 100 km
+    alt_min: float = 100.0e3
+    alt_max: float = 600.0e3
+
+    @property
+    def ncols(self) -> int:
+        return self.nblocks_lon * self.nblocks_lat * self.nlons * self.nlats
+
+
+CONFIGS = {
+    # 2x2 blocks of 9x9, nAlts 50, uniform 100-600 km
+    "cpu_324x50": GridSpec(2, 2, 50),
+    # 5 deg x 5 deg: 8 x 4 blocks of 9x9 = 2592 columns, nAlts 50
+    "5deg_2592x50": GridSpec(8, 4, 50),
+    # 1 deg x 1 deg: 360 x 180 = 40 x 20 blocks of 9x9, nAlts 100 stretched
+    "1deg_64800x100": GridSpec(40, 20, 100, stretched=True),
+}
+
+
+def altitudes(spec: GridSpec) -> np.ndarray:
+    """Altitude of levels -1..nAlts+2 (metres)."""
+    na = spec.nalts
+    k = np.arange(-1, na + 3, dtype=np.float64)
+    if not spec.stretched:
+        dz = (spec.alt_max - spec.alt_min) / na
+        return spec.alt_min + (k - 0.5) * dz
+    # geometric spacing: dz(1) = 1.5 km growing to dz(nAlts-1) = 15 km
+    ratio = 10.0 ** (1.0 / (na - 2))
+    dz = 1.5e3 * ratio ** np.arange(na + 3, dtype=np.float64)
+    alt = np.empty(na + 4)
+    alt[2] = spec.alt_min + 0.5 * dz[0]          # level 1
+    for j in range(3, na + 4):
+        alt[j] = alt[j - 1] + dz[j - 2]
+    alt[1] = alt[2] - dz[0]
+    alt[0] = alt[1] - dz[0]
+    return alt
+
+
+def make_columns(spec: GridSpec, seed: int = 1, *, t0: float = 200.0,
+                 tinf=1000.0, shape_s: float = 0.02e-3, sigma_t: float = math.log(1.05),
+                 w_sigma: float = 5.0, uv_sigma: float = 50.0,
+                 tidal: bool | None = None, g0: float = 9.5,
+                 eddy_coef: float = 50.0, eddy_p0: float = 0.01,
+                 eddy_p1: float = 0.005, ions: bool = False) -> dict:
+    """Seeded synthetic columns in the column-contiguous layout.
+
+    ``tinf`` may be a scalar or an array with one exospheric temperature per
+    column.  With ``tidal`` (default: on for stretched grids) the horizontal
+    winds are three seeded zonal harmonics of 40 m/s amplitude instead of
+    white noise.
+    """
+    if tidal is None:
+        tidal = spec.stretched
+    na = spec.nalts
+    L = na + 4
+    nc = spec.ncols
+    mass = species_masses()
+
+    nlon_tot = spec.nblocks_lon * spec.nlons
+    nlat_tot = spec.nblocks_lat * spec.nlats
+    # column c -> (iLon, iLat, block) with lon fastest inside a block
+    c = np.arange(nc)
+    ilon = c % spec.nlons
+    ilat = (c // spec.nlons) % spec.nlats
+    blk = c // (spec.nlons * spec.nlats)
+    blon = blk % spec.nblocks_lon
+    blat = blk // spec.nblocks_lon
+    glon = blon * spec.nlons + ilon
+    glat = blat * spec.nlats + ilat
+    lat = -0.5 * np.pi + (glat + 0.5) * np.pi / nlat_tot
+    lon = (glon + 0.5) * 2 * np.pi / nlon_tot
+
+    alt1 = altitudes(spec)
+    alt = np.broadcast_to(alt1, (nc, L)).copy()
+    dalt = np.empty_like(alt)
+    dalt[:, 1:L - 1] = 0.5 * (alt[:, 2:L] - alt[:, 0:L - 2])
+    dalt[:, 0] = dalt[:, 1]
+    dalt[:, L - 1] = dalt[:, L - 2]
+    radial = R_EARTH + alt
+    invr = 1 / radial
+    grav = -g0 * (R_EARTH * invr) ** 2
+
+    tinf_c = np.broadcast_to(np.asarray(tinf, dtype=np.float64), (nc,))[:, None]
+    # Bates profile above 100 km, isothermal below it
+    tbase = tinf_c - (tinf_c - t0) * np.exp(-shape_s * np.maximum(alt - spec.alt_min, 0.0))
+    tk = tbase * np.exp(sigma_t * normal(seed, 1, (nc, L)))
+
+    # hydrostatic species densities from the level nearest 100 km
+    nds = np.empty((nc, NSPECIES, L))
+    k100 = 1  # level 0 sits just below 100 km; integrate up and down from it
+    for s in range(NSPECIES):
+        n = np.empty((nc, L))
+        n[:, k100] = BASE_DENSITY[s]
+        for j in range(k100 + 1, L):
+            gm = -0.5 * (grav[:, j] + grav[:, j - 1])
+            tm = 0.5 * (tk[:, j] + tk[:, j - 1])
+            n[:, j] = n[:, j - 1] * (tk[:, j - 1] / tk[:, j]) * \
+                np.exp(-mass[s] * gm * (alt[:, j] - alt[:, j - 1]) / (KB * tm))
+        for j in range(k100 - 1, -1, -1):
+            gm = -0.5 * (grav[:, j] + grav[:, j + 1])
+            tm = 0.5 * (tk[:, j] + tk[:, j + 1])
+            n[:, j] = n[:, j + 1] * (tk[:, j + 1] / tk[:, j]) * \
+                np.exp(mass[s] * gm * (alt[:, j + 1] - alt[:, j]) / (KB * tm))
+        nds[:, s, :] = n
+
+    vvel = w_sigma * normal(seed, 2, (nc, NSPECIES, L))
+    vel = np.empty((nc, 3, L))
+    if tidal:
+        amp = 40.0
+        ph = 2 * np.pi * _uniform(seed, 7, 6)
+        ue = np.zeros((nc, L))
+        vn = np.zeros((nc, L))
+        vert = np.sin(np.pi * (alt - alt1[0]) / (alt1[-1] - alt1[0]))
+        for m in range(3):
+            ue += amp / 3 * np.cos((m + 1) * lon + ph[m])[:, None] * vert
+            vn += amp / 3 * np.sin((m + 1) * lon + ph[m + 3])[:, None] * vert
+        vel[:, 0, :] = ue
+        vel[:, 1, :] = vn
+    else:
+        vel[:, 0:2, :] = uv_sigma * normal(seed, 3, (nc, 2, L))
+    # bulk vertical wind: mass-weighted species winds
+    rho = np.einsum("s,csl->cl", mass, nds)
+    vel[:, 2, :] = np.einsum("s,csl->cl", mass, nds * vvel) / rho
+
+    ntot = nds.sum(axis=1)
+    mmm = np.zeros((nc, L))
+    for s in range(NSPECIES):                       # calc_rates.Earth.f90:36-40
+        mmm = mmm + mass[s] * nds[:, s, :] / (ntot + 1.0)
+    tempunit = mmm / KB                             # :48
+    temp = tk / tempunit                            # normalised temperature
+    gam = np.zeros((nc, L))
+    for s in range(NSPECIES):                       # calc_pressure.f90:56-86
+        gam = gam + nds[:, s, :] / (VIBRATION[s] - 2)
+    gam = gam * 2.0 / ntot + 1
+    cmax = np.abs(vvel).max(axis=1) + np.sqrt(gam * temp)   # calc_timestep.f90:114-118
+    press = ntot * KB * tk
+    eddy = np.where(press > eddy_p0, eddy_coef,
+                    np.where(press > eddy_p1,
+                             eddy_coef * (press - eddy_p1) / (eddy_p0 - eddy_p1), 0.0))
+
+    out = dict(nds=nds, vvel=vvel, vel=vel, temp=temp, rho=rho,
+               tempunit=tempunit, mmm=mmm, gamma=gam, cmax=cmax, eddy=eddy,
+               alt=alt, grav=grav, invr=invr, dalt=dalt,
+               coslat=np.cos(lat), lat=lat, lon=lon)
+    if ions:
+        # O+ and NO+ Chapman-like layers, other ions small, electrons = sum
+        ion = np.empty((nc, NIONS, L))
+        zkm = alt / 1e3
+        op = 1e12 * np.exp(1 - (zkm - 300) / 60 - np.exp(-(zkm - 300) / 60))
+        nop = 1e11 * np.exp(1 - (zkm - 120) / 20 - np.exp(-(zkm - 120) / 20))
+        ion[:, 0] = op * np.exp(0.05 * normal(seed, 11, (nc, L))) + 1e3
+        ion[:, 1] = nop * np.exp(0.05 * normal(seed, 12, (nc, L))) + 1e3
+        for i in range(2, NIONS - 1):
+            ion[:, i] = 1e6 + 1e3 * i
+        ion[:, NIONS - 1] = ion[:, :NIONS - 1].sum(axis=1)
+        out["ions"] = ion
+        out["ivel"] = 20.0 * normal(seed, 13, (nc, 3, L))
+        out["sza"] = np.arccos(np.clip(np.cos(lat) * np.cos(lon - np.pi), -1, 1))
+        out["mlat"] = np.degrees(lat)
+    return out
+
+
+@dataclasses.dataclass
+class StepParams:
+    """Run parameters of the vertical step (ModInputs defaults unless noted)."""
+
+    dt: float = 2.0
+    omega_body: float = 2.0 * 3.141592653589793 / (24.0 * 3600.0)  # ModInputs.f90:285
+    max_vert_vel: float = 1000.0       # ModInputs.f90:224
+    beta_limiter: float = 2.0          # ModInputs.f90:104
+    use_neutral_friction: bool = True  # ModInputs.f90:240
+    use_coriolis: bool = True          # ModInputs.f90:235
+    use_ion_advection: bool = False    # reference default T (ModInputs.f90:311)
+    use_improved_ion_advection: bool = True  # ModInputs.f90:259
+    use_nighttime_ion_bcs: bool = True       # ModInputs.f90:260
+    min_tec: float = 2.0               # ModInputs.f90:261
+    vert_tau: float = 1.0e9            # ModEarth.f90:325

@@ -1,0 +1,116 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of the C restatement (liboracle.so)."""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+AUSM = 1
+RUSANOV = 0
+
+ERRORS = {0: "ok", 1: "non-positive density", 2: "singular friction matrix",
+          3: "negative ion density at top", 4: "bad arguments"}
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("n_alts", ctypes.c_int), ("dt", ctypes.c_double),
+                ("omega_body", ctypes.c_double), ("max_vert_vel", ctypes.c_double),
+                ("beta_limiter", ctypes.c_double),
+                ("use_neutral_friction", ctypes.c_int), ("use_coriolis", ctypes.c_int),
+                ("use_ion_advection", ctypes.c_int),
+                ("use_improved_ion_advection", ctypes.c_int),
+                ("use_nighttime_ion_bcs", ctypes.c_int), ("min_tec", ctypes.c_double),
+                ("vert_tau", ctypes.c_double)]
+
+
+_P = ctypes.POINTER(ctypes.c_double)
+
+
+class _Cols(ctypes.Structure):
+    _fields_ = [("n_cols", ctypes.c_int), ("n_alts", ctypes.c_int)] + [
+        (n, _P) for n in ("nds", "vvel", "vel", "temp", "rho", "logns", "nden",
+                          "tempunit", "mmm", "gamma", "cmax", "eddy", "alt", "grav",
+                          "invr", "dalt", "coslat", "ions", "ivel", "sza", "mlat")]
+
+
+class _Status(ctypes.Structure):
+    _fields_ = [("code", ctypes.c_int), ("col", ctypes.c_int), ("level", ctypes.c_int)]
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = ctypes.CDLL(LIB)
+        _lib.ov_advance.restype = ctypes.c_int
+        _lib.ov_advance_threaded.restype = ctypes.c_int
+        _lib.ov_limiter_mc.restype = ctypes.c_double
+        _lib.ov_limiter_mc.argtypes = [ctypes.c_double] * 3
+    return _lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_P)
+
+
+def advance(cols: dict, params, solver: int = AUSM, n_steps: int = 1,
+            threads: int = 1) -> dict:
+    """Run the oracle on a copy of ``cols``; returns the updated arrays.
+
+    ``cols`` uses the column-contiguous layout of gitm_amd.synth.
+    Raises RuntimeError with the oracle's status on failure.
+    """
+    nc, L = cols["temp"].shape
+    na = L - 4
+    out = {k: np.ascontiguousarray(np.array(cols[k], dtype=np.float64))
+           for k in ("nds", "vvel", "vel", "temp", "rho")}
+    out["logns"] = np.zeros_like(out["nds"])
+    out["nden"] = np.zeros_like(out["temp"])
+    ions = params.use_ion_advection
+    if ions:
+        out["ions"] = np.ascontiguousarray(np.array(cols["ions"], dtype=np.float64))
+    keep = {k: np.ascontiguousarray(cols[k], dtype=np.float64)
+            for k in ("tempunit", "mmm", "gamma", "cmax", "eddy", "alt", "grav",
+                      "invr", "dalt", "coslat")}
+    if ions:
+        for k in ("ivel", "sza", "mlat"):
+            keep[k] = np.ascontiguousarray(cols[k], dtype=np.float64)
+    p = _Params(na, params.dt, params.omega_body, params.max_vert_vel,
+                params.beta_limiter, int(params.use_neutral_friction),
+                int(params.use_coriolis), int(ions),
+                int(params.use_improved_ion_advection),
+                int(params.use_nighttime_ion_bcs), params.min_tec, params.vert_tau)
+    c = _Cols()
+    c.n_cols = nc
+    c.n_alts = na
+    for k, v in out.items():
+        setattr(c, k, _ptr(v))
+    for k, v in keep.items():
+        setattr(c, k, _ptr(v))
+    st = _Status()
+    if threads > 1:
+        rc = lib().ov_advance_threaded(ctypes.byref(p), ctypes.byref(c), solver,
+                                       n_steps, threads, ctypes.byref(st))
+    else:
+        rc = lib().ov_advance(ctypes.byref(p), ctypes.byref(c), solver, n_steps,
+                              ctypes.byref(st))
+    if rc:
+        raise RuntimeError(f"oracle failed: {ERRORS.get(rc, rc)} (column {st.col},"
+                           f" level {st.level})")
+    return out

@@ -1,0 +1,104 @@
+/*
+ * gitm_vert_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement of GITM's per-column vertical dynamics step, used as the
+ * parity checker for the HIP implementation.  Nothing in the product library
+ * (gitm_amd/) links, loads or calls this code; only tests/, the smoke check and
+ * the CPU-baseline leg of bench.py do.
+ *
+ * Data layout ("column-contiguous", shared with the fixtures): every per-level
+ * field of one column is stored with the level index fastest, levels
+ * -1..nAlts+2 at offsets 0..L-1 (L = nAlts+4); multi-component fields are
+ * [col][component][level].
+ */
+#ifndef GITM_VERT_ORACLE_H
+#define GITM_VERT_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OV_NSPECIES 6  /* O, O2, N2, N(4S), NO, He   (ModEarth.f90:12-18)   */
+#define OV_NIONS 10    /* O+ NO+ O2+ N2+ N+ O(2D)+ O(2P)+ H+ He+ e-
+                          (ModEarth.f90:28-38)                           */
+#define OV_NIONS_ADVECT 2 /* ModEarth.f90:39                              */
+#define OV_LMAX 260    /* nAlts <= 256                                    */
+
+enum { OV_SOLVER_RUSANOV = 0, OV_SOLVER_AUSM = 1 };
+
+enum {
+  OV_OK = 0,
+  OV_ERR_NONPOSITIVE_DENSITY = 1, /* advance_vertical.f90:96-100          */
+  OV_ERR_SINGULAR_MATRIX = 2,     /* ludcmp.f:14-17                       */
+  OV_ERR_NEGATIVE_ION_TOP = 3,    /* set_vertical_bcs.Earth.f90:617-620   */
+  OV_ERR_BAD_ARGS = 4
+};
+
+typedef struct {
+  int n_alts;
+  double dt;                  /* Dt                                        */
+  double omega_body;          /* OmegaBodyInput       ModInputs.f90:285    */
+  double max_vert_vel;        /* MaximumVerticalVelocity ModInputs.f90:224 */
+  double beta_limiter;        /* BetaLimiter          ModInputs.f90:104    */
+  int use_neutral_friction;   /* ModInputs.f90:240                         */
+  int use_coriolis;           /* ModInputs.f90:235                         */
+  int use_ion_advection;      /* ModInputs.f90:311                         */
+  int use_improved_ion_advection; /* ModInputs.f90:259                     */
+  int use_nighttime_ion_bcs;  /* ModInputs.f90:260                         */
+  double min_tec;             /* ModInputs.f90:261                         */
+  double vert_tau;            /* VertTau(:) (init_planet, ModEarth.f90:325) */
+} ov_params;
+
+typedef struct {
+  int n_cols;
+  int n_alts;
+  /* state, in/out */
+  double *nds;   /* NDensityS        [col][6][L]                       */
+  double *vvel;  /* VerticalVelocity [col][6][L]                       */
+  double *vel;   /* Velocity (E,N,U) [col][3][L]                       */
+  double *temp;  /* Temperature (normalised by TempUnit) [col][L]      */
+  double *rho;   /* Rho              [col][L]                          */
+  /* outputs */
+  double *logns; /* LogNS            [col][6][L]                       */
+  double *nden;  /* NDensity         [col][L]                          */
+  /* column inputs (prep) */
+  const double *tempunit, *mmm, *gamma, *cmax, *eddy; /* [col][L]      */
+  const double *alt, *grav, *invr, *dalt;             /* [col][L]      */
+  const double *coslat;                               /* [col]         */
+  /* ions: may be NULL when use_ion_advection == 0 */
+  double *ions;        /* IDensityS incl. electrons [col][10][L]       */
+  const double *ivel;  /* IVelocity                 [col][3][L]        */
+  const double *sza;   /* SZA                       [col]              */
+  const double *mlat;  /* MLatitude(:,:,nAlts,:)    [col]              */
+} ov_columns;
+
+typedef struct {
+  int code;
+  int col;
+  int level;
+} ov_status;
+
+/* Advance every column by n_steps vertical steps with the chosen solver.
+ * Returns OV_OK or the first error (status filled). */
+int ov_advance(const ov_params *p, const ov_columns *c, int solver,
+               int n_steps, ov_status *status);
+
+/* Column-parallel variant (POSIX threads), for the CPU baseline. */
+int ov_advance_threaded(const ov_params *p, const ov_columns *c, int solver,
+                        int n_steps, int n_threads, ov_status *status);
+
+/* Building blocks exposed for unit tests. */
+double ov_limiter_mc(double beta, double d_up, double d_down);
+int ov_lu_solve6(double a[6][6], double b[6]); /* ludcmp + lubksb */
+void ov_facevalues_4th(int n_alts, const double *dalt_f, const double *inv_dalt_f,
+                       double beta, const double *var, double *left,
+                       double *right);
+void ov_kt_facevalues(int n_alts, const double *dalt_f, const double *inv_dalt_f,
+                      double beta, const double *var, double *left_m12,
+                      double *right_m12, double *left_p12, double *right_p12);
+void ov_species_masses(double mass[OV_NSPECIES]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,72 @@
+"""TEST INFRASTRUCTURE ONLY: runs the reference Fortran build (oracle/_ref).
+
+The executable is compiled from the reference sources by ``make -C oracle
+ref``; it exists only where those sources were present at build time.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EXE = os.path.join(HERE, "_ref", "gitm_vert_ref")
+MAGIC = 1196840530
+REF_NALTS = 50  # src/ModSize.f90.orig
+
+
+def available() -> bool:
+    return os.path.exists(EXE)
+
+
+def _fortran_order(a: np.ndarray) -> np.ndarray:
+    # [col][comp][level] C-order == Fortran (level, comp, col)
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def advance(cols: dict, params, solver: int = 1, n_steps: int = 1) -> dict:
+    nc, L = cols["temp"].shape
+    na = L - 4
+    if na != REF_NALTS:
+        raise ValueError(f"reference build is compiled for nAlts={REF_NALTS}")
+    ions = bool(params.use_ion_advection)
+    hdr = np.zeros(16, dtype=np.int32)
+    hdr[:10] = [MAGIC, nc, na, solver, n_steps, int(params.use_neutral_friction),
+                int(params.use_coriolis), int(ions),
+                int(params.use_improved_ion_advection),
+                int(params.use_nighttime_ion_bcs)]
+    par = np.zeros(8)
+    par[:5] = [params.dt, params.omega_body, params.max_vert_vel,
+               params.beta_limiter, params.min_tec]
+    order = ["nds", "vvel", "vel", "temp", "rho", "tempunit", "mmm", "gamma",
+             "cmax", "eddy", "alt", "grav", "invr", "dalt", "coslat"]
+    if ions:
+        order += ["ions", "ivel", "sza", "mlat"]
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        fout = os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(hdr.tobytes())
+            f.write(par.tobytes())
+            for k in order:
+                f.write(_fortran_order(cols[k]).tobytes())
+        subprocess.run([EXE, fin, fout], check=True, capture_output=True)
+        raw = np.fromfile(fout, dtype=np.uint8)
+    status = raw[:16].view(np.int32)
+    if status[0]:
+        raise RuntimeError(f"reference status {status}")
+    vals = raw[16:].view(np.float64)
+    out = {}
+    pos = 0
+    shapes = [("nds", (nc, 6, L)), ("vvel", (nc, 6, L)), ("vel", (nc, 3, L)),
+              ("temp", (nc, L)), ("rho", (nc, L)), ("logns", (nc, 6, L)),
+              ("nden", (nc, L))]
+    if ions:
+        shapes.append(("ions", (nc, 10, L)))
+    for k, shp in shapes:
+        n = int(np.prod(shp))
+        out[k] = vals[pos:pos + n].reshape(shp).copy()
+        pos += n
+    return out

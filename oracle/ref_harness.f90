@@ -1,0 +1,199 @@
+! ref_harness.f90 -- TEST INFRASTRUCTURE ONLY.
+!
+! Driver that runs the reference GITM vertical step (compiled from the
+! reference sources in place, see oracle/Makefile) on columns read from a
+! binary file and writes the updated columns back.  Each column is loaded
+! into grid cell (iLon,iLat,iBlock) = (1,1,1) of the reference's module
+! globals, the reference subroutine advance_vertical(1,1,1)
+! (src/advance_vertical.f90:45-219) is called n_steps times, and the result
+! is read back from the same globals.  Columns are independent in the
+! reference, so this is the same computation advance_vertical_all performs
+! over a block (src/advance_vertical.f90:20-29).
+!
+! File formats (stream, native little-endian):
+!   input : int32 hdr(16), real64 par(8), then the canonical column arrays
+!           (level fastest, [col][component][level]) in the order
+!           nds(6) vvel(6) vel(3) temp rho tempunit mmm gamma cmax eddy alt
+!           grav invr dalt, coslat(col), and if ions: ions(10) ivel(3),
+!           sza(col), mlat(col)
+!   output: int32 status(4), then nds(6) vvel(6) vel(3) temp rho logns(6)
+!           nden, and if ions: ions(10)
+program gitm_vert_ref
+
+  use ModSizeGitm
+  use ModPlanet
+  use ModGITM
+  use ModInputs
+  use ModSources, only: KappaEddyDiffusion, EuvHeating, init_mod_sources
+  use ModEUV, only: SZA
+
+  implicit none
+
+  integer :: hdr(16), status(4)
+  real :: par(8)
+  integer :: nCols, nA, solver, nStepsH, iCol, iStep, L, k, s, d, nIonsIO
+  logical :: doIons
+  real, allocatable :: nds(:,:,:), vvel(:,:,:), vel(:,:,:), temp(:,:), rhoH(:,:)
+  real, allocatable :: tempunit_(:,:), mmm(:,:), gam(:,:), cmax(:,:), eddy(:,:)
+  real, allocatable :: alt(:,:), grav(:,:), invr(:,:), dalt(:,:), coslat(:)
+  real, allocatable :: ions(:,:,:), ivel(:,:,:), sza_(:), mlat(:)
+  real, allocatable :: logns_o(:,:,:), nden_o(:,:)
+  character(len=512) :: fin, fout
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+
+  open(10, file=trim(fin), access='stream', form='unformatted', status='old')
+  read(10) hdr
+  read(10) par
+  nCols  = hdr(2)
+  nA     = hdr(3)
+  solver = hdr(4)
+  nStepsH = hdr(5)
+  if (hdr(1) /= 1196840530 .or. nA /= nAlts) then
+     write(*,*) 'ref_harness: bad header or nAlts mismatch', hdr(1), nA, nAlts
+     stop 2
+  endif
+  L = nA + 4
+  doIons = hdr(8) /= 0
+
+  allocate(nds(L,nSpecies,nCols), vvel(L,nSpecies,nCols), vel(L,3,nCols))
+  allocate(temp(L,nCols), rhoH(L,nCols), tempunit_(L,nCols), mmm(L,nCols))
+  allocate(gam(L,nCols), cmax(L,nCols), eddy(L,nCols), alt(L,nCols))
+  allocate(grav(L,nCols), invr(L,nCols), dalt(L,nCols), coslat(nCols))
+  allocate(logns_o(L,nSpecies,nCols), nden_o(L,nCols))
+  read(10) nds
+  read(10) vvel
+  read(10) vel
+  read(10) temp
+  read(10) rhoH
+  read(10) tempunit_
+  read(10) mmm
+  read(10) gam
+  read(10) cmax
+  read(10) eddy
+  read(10) alt
+  read(10) grav
+  read(10) invr
+  read(10) dalt
+  read(10) coslat
+  nIonsIO = nIons
+  if (doIons) then
+     allocate(ions(L,nIons,nCols), ivel(L,3,nCols), sza_(nCols), mlat(nCols))
+     read(10) ions
+     read(10) ivel
+     read(10) sza_
+     read(10) mlat
+  endif
+  close(10)
+
+  ! Model set-up the reference performs before any step
+  nBlocks = 1
+  call init_planet
+  call init_mod_gitm
+  call init_mod_sources
+
+  ! Run flags (ModInputs) for this path
+  UseAUSMSolver              = (solver == 1)
+  UseNeutralFriction         = (hdr(6) /= 0)
+  UseCoriolis                = (hdr(7) /= 0)
+  UseIonAdvection            = doIons
+  UseImprovedIonAdvection    = (hdr(9) /= 0)
+  UseNighttimeIonBCs         = (hdr(10) /= 0)
+  UseMSIS                    = .false.
+  UseGSWMTides               = .false.
+  UseWACCMTides              = .false.
+  UseBcPerturbation          = .false.
+  UseDamping                 = .false.
+  Dt                         = par(1)
+  OmegaBodyInput             = par(2)
+  MaximumVerticalVelocity    = par(3)
+  BetaLimiter                = par(4)
+  MinTEC                     = par(5)
+  ! calc_rates.Earth.f90:83-84
+  ThermalDiffCoefS(1:nSpecies) = 0.0
+  ThermalDiffCoefS(iHe_) = -0.38
+
+  ! Fields advance_vertical reads but the vertical step never uses
+  KappaTemp = 0.0
+  dAltDLon_CB = 0.0
+  dAltDLat_CB = 0.0
+  ViscCoef = 0.0
+  ViscCoefS = 0.0
+  cp = 0.0
+  EuvHeating = 0.0
+  Latitude = 0.0
+  Longitude = 0.0
+  MLatitude = 0.0
+  SZA = 0.0
+  IVelocity = 0.0
+  IDensityS = 1.0e6
+
+  status = 0
+
+  do iCol = 1, nCols
+     do s = 1, nSpecies
+        NDensityS(1,1,-1:nA+2,s,1) = nds(:,s,iCol)
+        VerticalVelocity(1,1,-1:nA+2,s,1) = vvel(:,s,iCol)
+     enddo
+     do d = 1, 3
+        Velocity(1,1,-1:nA+2,d,1) = vel(:,d,iCol)
+     enddo
+     Temperature(1,1,-1:nA+2,1) = temp(:,iCol)
+     Rho(1,1,-1:nA+2,1) = rhoH(:,iCol)
+     TempUnit(1,1,-1:nA+2) = tempunit_(:,iCol)
+     MeanMajorMass(1,1,-1:nA+2) = mmm(:,iCol)
+     Gamma(1,1,-1:nA+2,1) = gam(:,iCol)
+     cMax_GDB(1,1,0:nA+1,iUp_,1) = cmax(2:L-1,iCol)
+     KappaEddyDiffusion(1,1,-1:nA+2,1) = eddy(:,iCol)
+     Altitude_GB(1,1,-1:nA+2,1) = alt(:,iCol)
+     Gravity_GB(1,1,-1:nA+2,1) = grav(:,iCol)
+     InvRadialDistance_GB(1,1,-1:nA+2,1) = invr(:,iCol)
+     dAlt_GB(1,1,-1:nA+2,1) = dalt(:,iCol)
+     CosLatitude(1,1) = coslat(iCol)
+     if (doIons) then
+        do s = 1, nIons
+           IDensityS(1,1,-1:nA+2,s,1) = ions(:,s,iCol)
+        enddo
+        do d = 1, 3
+           IVelocity(1,1,-1:nA+2,d,1) = ivel(:,d,iCol)
+        enddo
+        SZA(1,1,1) = sza_(iCol)
+        MLatitude(1,1,nA,1) = mlat(iCol)
+     endif
+
+     do iStep = 1, nStepsH
+        call advance_vertical(1,1,1)
+     enddo
+
+     do s = 1, nSpecies
+        nds(:,s,iCol) = NDensityS(1,1,-1:nA+2,s,1)
+        vvel(:,s,iCol) = VerticalVelocity(1,1,-1:nA+2,s,1)
+        logns_o(:,s,iCol) = LogNS(1,1,-1:nA+2,s,1)
+     enddo
+     do d = 1, 3
+        vel(:,d,iCol) = Velocity(1,1,-1:nA+2,d,1)
+     enddo
+     temp(:,iCol) = Temperature(1,1,-1:nA+2,1)
+     rhoH(:,iCol) = Rho(1,1,-1:nA+2,1)
+     nden_o(:,iCol) = NDensity(1,1,-1:nA+2,1)
+     if (doIons) then
+        do s = 1, nIons
+           ions(:,s,iCol) = IDensityS(1,1,-1:nA+2,s,1)
+        enddo
+     endif
+  enddo
+
+  open(11, file=trim(fout), access='stream', form='unformatted', status='replace')
+  write(11) status
+  write(11) nds
+  write(11) vvel
+  write(11) vel
+  write(11) temp
+  write(11) rhoH
+  write(11) logns_o
+  write(11) nden_o
+  if (doIons) write(11) ions
+  close(11)
+
+end program gitm_vert_ref
