@@ -1,0 +1,256 @@
+// gitm_vert.hip -- HIP kernels and C ABI of the MI355X vertical step.
+//
+// Entry points are declared in include/gitm_vert.h.  One workgroup per
+// column (gitm_vert_columns column index = blockIdx.x), NT = 64*W threads,
+// one level per thread; see vert_column.h for the column algorithm.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "vert_params.h"
+
+namespace {
+
+struct GpuCtx {
+  unsigned long long *err;
+  __device__ int lane() const { return (int)threadIdx.x; }
+  __device__ void sync() const { __syncthreads(); }
+  __device__ bool sync_or(bool p) const { return __syncthreads_or(p ? 1 : 0) != 0; }
+  __device__ void report(unsigned long long key) const { atomicMin(err, key); }
+};
+
+template <int NT, int SOLVER>
+__global__ void __launch_bounds__(NT) vert_column_kernel(gv::KParams kp, gitm_vert_columns io) {
+  extern __shared__ double lds[];
+  GpuCtx cx{kp.err};
+  gv::run_column<NT, SOLVER>(cx, kp, io, (int)blockIdx.x, lds);
+}
+
+template <int NT>
+void launch(int solver, dim3 grid, hipStream_t stream, const gv::KParams &kp,
+            const gitm_vert_columns &c) {
+  if (solver == GITM_VERT_AUSM)
+    hipLaunchKernelGGL((vert_column_kernel<NT, GITM_VERT_AUSM>), grid, dim3(NT),
+                       gv::LdsSize<NT>::bytes, stream, kp, c);
+  else
+    hipLaunchKernelGGL((vert_column_kernel<NT, GITM_VERT_RUSANOV>), grid, dim3(NT),
+                       gv::LdsSize<NT>::bytes, stream, kp, c);
+}
+
+__global__ void reset_error_word(unsigned long long *e) { *e = ~0ULL; }
+
+}  // namespace
+
+struct gitm_vert_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  unsigned long long *d_err = nullptr;
+  int waves = 0;
+  // device staging of the host-array entry point
+  std::vector<void *> stage;
+  std::vector<size_t> stage_bytes;
+};
+
+#define GV_HIP(call)                         \
+  do {                                       \
+    hipError_t e_ = (call);                  \
+    if (e_ != hipSuccess) return GITM_VERT_EHIP; \
+  } while (0)
+
+extern "C" {
+
+void gitm_vert_default_params(gitm_vert_params *p) {
+  if (p) gv::default_params(p);
+}
+
+const char *gitm_vert_strerror(int code) {
+  switch (code) {
+    case GITM_VERT_OK: return "ok";
+    case GITM_VERT_ENONPOSITIVE: return "non-positive neutral density entering a column";
+    case GITM_VERT_ESINGULAR: return "singular neutral-friction matrix";
+    case GITM_VERT_ENEGION: return "negative ion density too close to the upper boundary";
+    case GITM_VERT_EINVAL: return "invalid arguments";
+    case GITM_VERT_EUNSUPPORTED:
+      return "unsupported option (UseMSIS, tides, BC perturbation, UseDamping, ions)";
+    case GITM_VERT_EHIP: return "HIP runtime error";
+    case GITM_VERT_ENOMEM: return "out of device memory";
+    default: return "unknown error";
+  }
+}
+
+int gitm_vert_create(gitm_vert_handle **out, int device) {
+  if (!out) return GITM_VERT_EINVAL;
+  *out = nullptr;
+  int dev = device;
+  if (dev < 0) GV_HIP(hipGetDevice(&dev));
+  GV_HIP(hipSetDevice(dev));
+  gitm_vert_handle *h = new gitm_vert_handle();
+  h->device = dev;
+  if (hipMalloc(&h->d_err, sizeof(unsigned long long)) != hipSuccess) {
+    delete h;
+    return GITM_VERT_ENOMEM;
+  }
+  unsigned long long init = ~0ULL;
+  if (hipMemcpy(h->d_err, &init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(h->d_err);
+    delete h;
+    return GITM_VERT_EHIP;
+  }
+  *out = h;
+  return GITM_VERT_OK;
+}
+
+int gitm_vert_destroy(gitm_vert_handle *h) {
+  if (!h) return GITM_VERT_OK;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  else (void)hipDeviceSynchronize();
+  for (void *p : h->stage) (void)hipFree(p);
+  (void)hipFree(h->d_err);
+  delete h;
+  return GITM_VERT_OK;
+}
+
+int gitm_vert_set_stream(gitm_vert_handle *h, void *hip_stream) {
+  if (!h) return GITM_VERT_EINVAL;
+  h->stream = (hipStream_t)hip_stream;
+  return GITM_VERT_OK;
+}
+
+int gitm_vert_set_tile(gitm_vert_handle *h, int waves_per_column) {
+  if (!h || waves_per_column < 0 || waves_per_column > 4) return GITM_VERT_EINVAL;
+  h->waves = waves_per_column;
+  return GITM_VERT_OK;
+}
+
+int gitm_vert_advance(gitm_vert_handle *h, const gitm_vert_params *p,
+                      const gitm_vert_columns *c, int solver, int n_steps) {
+  if (!h) return GITM_VERT_EINVAL;
+  int rc = gv::check_args(p, c, solver, n_steps);
+  if (rc) return rc;
+  const int ncols = c->n_lons * c->n_lats * c->n_blocks;
+  if (n_steps == 0) return GITM_VERT_OK;
+  int waves = h->waves ? h->waves : gv::auto_waves(c->n_alts);
+  if (64 * waves < c->n_alts + 4) return GITM_VERT_EINVAL;
+  GV_HIP(hipSetDevice(h->device));
+  gv::KParams kp = gv::make_kparams(p, c, solver, n_steps, h->d_err);
+  const dim3 grid(ncols);
+  switch (waves) {
+    case 1: launch<64>(solver, grid, h->stream, kp, *c); break;
+    case 2: launch<128>(solver, grid, h->stream, kp, *c); break;
+    case 3: launch<192>(solver, grid, h->stream, kp, *c); break;
+    case 4: launch<256>(solver, grid, h->stream, kp, *c); break;
+    default: return GITM_VERT_EINVAL;
+  }
+  GV_HIP(hipGetLastError());
+  return GITM_VERT_OK;
+}
+
+int gitm_vert_get_error(gitm_vert_handle *h, gitm_vert_error *err) {
+  if (!h) return GITM_VERT_EINVAL;
+  GV_HIP(hipSetDevice(h->device));
+  unsigned long long key = ~0ULL;
+  GV_HIP(hipMemcpyAsync(&key, h->d_err, sizeof(key), hipMemcpyDeviceToHost, h->stream));
+  GV_HIP(hipStreamSynchronize(h->stream));
+  if (key != ~0ULL) {
+    hipLaunchKernelGGL(reset_error_word, dim3(1), dim3(1), 0, h->stream, h->d_err);
+    GV_HIP(hipGetLastError());
+    GV_HIP(hipStreamSynchronize(h->stream));
+  }
+  gitm_vert_error tmp;
+  if (!err) err = &tmp;
+  gv::decode_error(key, err);
+  return err->code;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Host-array entry point (the Fortran drop-in): stage every field on the
+// device, run, copy the written fields back.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct FieldSpec {
+  gitm_vert_field gitm_vert_columns::*member;
+  int ncomp;
+  int last_level_offset;  // last level read = nAlts + this
+  bool written;
+  bool per_column;
+};
+
+const FieldSpec kFields[] = {
+    {&gitm_vert_columns::ndensity_s, 6, 2, true, false},
+    {&gitm_vert_columns::vertical_velocity, 6, 2, true, false},
+    {&gitm_vert_columns::velocity, 3, 2, true, false},
+    {&gitm_vert_columns::temperature, 1, 2, true, false},
+    {&gitm_vert_columns::rho, 1, 2, true, false},
+    {&gitm_vert_columns::log_ns, 6, 2, true, false},
+    {&gitm_vert_columns::ndensity, 1, 2, true, false},
+    {&gitm_vert_columns::temp_unit, 1, 2, false, false},
+    {&gitm_vert_columns::mean_major_mass, 1, 2, false, false},
+    {&gitm_vert_columns::gamma, 1, 2, false, false},
+    {&gitm_vert_columns::cmax, 1, 1, false, false},
+    {&gitm_vert_columns::eddy, 1, 2, false, false},
+    {&gitm_vert_columns::altitude, 1, 2, false, false},
+    {&gitm_vert_columns::gravity, 1, 2, false, false},
+    {&gitm_vert_columns::inv_radial_distance, 1, 2, false, false},
+    {&gitm_vert_columns::dalt, 1, 2, false, false},
+    {&gitm_vert_columns::cos_latitude, 1, 0, false, true},
+};
+
+size_t field_span(const gitm_vert_columns &c, const gitm_vert_field &f, const FieldSpec &fs) {
+  if (f.s_lon < 0 || f.s_lat < 0 || f.s_blk < 0 || f.s_lev < 0 || f.s_comp < 0) return 0;
+  long long last = (long long)(c.n_lons - 1) * f.s_lon + (long long)(c.n_lats - 1) * f.s_lat +
+                   (long long)(c.n_blocks - 1) * f.s_blk + (long long)(fs.ncomp - 1) * f.s_comp;
+  if (!fs.per_column)
+    last += (long long)(c.n_alts + fs.last_level_offset - f.first_level) * f.s_lev;
+  return (size_t)(last + 1);
+}
+
+}  // namespace
+
+extern "C" int gitm_vert_advance_host(gitm_vert_handle *h, const gitm_vert_params *p,
+                                      const gitm_vert_columns *c, int solver, int n_steps) {
+  if (!h) return GITM_VERT_EINVAL;
+  int rc = gv::check_args(p, c, solver, n_steps);
+  if (rc) return rc;
+  GV_HIP(hipSetDevice(h->device));
+  const size_t nf = sizeof(kFields) / sizeof(kFields[0]);
+  if (h->stage.size() < nf) {
+    h->stage.resize(nf, nullptr);
+    h->stage_bytes.resize(nf, 0);
+  }
+  gitm_vert_columns d = *c;
+  for (size_t i = 0; i < nf; i++) {
+    const FieldSpec &fs = kFields[i];
+    const gitm_vert_field &f = c->*fs.member;
+    if (!f.ptr) continue;
+    size_t n = field_span(*c, f, fs);
+    if (!n) return GITM_VERT_EINVAL;
+    size_t bytes = n * sizeof(double);
+    if (h->stage_bytes[i] < bytes) {
+      if (h->stage[i]) (void)hipFree(h->stage[i]);
+      h->stage[i] = nullptr;
+      h->stage_bytes[i] = 0;
+      if (hipMalloc(&h->stage[i], bytes) != hipSuccess) return GITM_VERT_ENOMEM;
+      h->stage_bytes[i] = bytes;
+    }
+    GV_HIP(hipMemcpyAsync(h->stage[i], f.ptr, bytes, hipMemcpyHostToDevice, h->stream));
+    (d.*fs.member).ptr = (double *)h->stage[i];
+  }
+  rc = gitm_vert_advance(h, p, &d, solver, n_steps);
+  if (rc) return rc;
+  for (size_t i = 0; i < nf; i++) {
+    const FieldSpec &fs = kFields[i];
+    const gitm_vert_field &f = c->*fs.member;
+    if (!f.ptr || !fs.written) continue;
+    size_t bytes = field_span(*c, f, fs) * sizeof(double);
+    GV_HIP(hipMemcpyAsync(f.ptr, h->stage[i], bytes, hipMemcpyDeviceToHost, h->stream));
+  }
+  return gitm_vert_get_error(h, nullptr);
+}

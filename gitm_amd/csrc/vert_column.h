@@ -1,0 +1,1233 @@
+// vert_column.h -- the column-step kernels of GITM's vertical dynamics,
+// written for CDNA4 (gfx950).
+//
+// Mapping: one workgroup of NT = 64*W threads advances one (lon, lat) column.
+// Thread j owns level k = j-1 (levels -1..nAlts+2, L = nAlts+4 <= NT) and
+// keeps that level's RK4 state in registers.  Everything a level needs from
+// its neighbours (the i+-2 reconstruction stencils, face states, the
+// hydrostatic recurrence, ghost cells) is staged through LDS.  All RK4
+// stages, the AUSM+-up / Rusanov fluxes, the boundary conditions and the
+// implicit neutral-friction solve run inside one launch; the column goes to
+// HBM once on entry and once on exit of n_steps vertical steps.
+//
+// Floating-point work is FP64 throughout and keeps the reference's operation
+// order (Fortran evaluates left to right; the build disables contraction), so
+// results differ from the reference only through the last-bit behaviour of
+// exp/log/pow/tanh.  Face quantities that the reference computes twice (once
+// as the M12 face of cell i, once as the P12 face of cell i-1,
+// src/vertical_solver_ausm.f90:2096-2104) are computed once per face here.
+//
+// The same source compiles for the host (GV_HOST_EMULATION) where a test
+// harness runs the NT threads of a workgroup as CPU threads to check the
+// kernel logic against the CPU oracle bit for bit.
+#pragma once
+
+#include <stdint.h>
+
+#include "gitm_vert.h"
+
+#ifdef GV_HOST_EMULATION
+#include <cmath>
+#define GV_FN static inline
+#define GV_MEM inline
+#else
+#include <hip/hip_runtime.h>
+#define GV_FN static __device__ __forceinline__
+#define GV_MEM __device__ __forceinline__
+#endif
+
+namespace gv {
+
+constexpr int NSP = 6;
+constexpr int NIONS = 10;
+constexpr double KB = 1.3807e-23;         // ModConstants.f90:15
+constexpr double PI = 3.141592653589793;  // ModConstants.f90:23
+
+// species (ModEarth.f90:12-18, 0-based)
+constexpr int S_O = 0, S_N4S = 3, S_NO = 4;
+GV_FN constexpr bool photochemical(int s) { return s == 3 || s == 4; }
+
+// per-level state kept in registers
+constexpr int F_LR = 0;   // LogRho
+constexpr int F_T = 1;    // Temp (= Temperature*TempUnit)
+constexpr int F_LN = 2;   // LogNS(1:6)
+constexpr int F_W = 8;    // VertVel(1:6)
+constexpr int F_VE = 14;  // Vel_GD(iEast_)
+constexpr int F_VN = 15;  // Vel_GD(iNorth_)
+constexpr int F_VU = 16;  // Vel_GD(iUp_)
+constexpr int NF = 17;
+
+// Diff0/1e17 and DiffExp (ModEarth.f90:237-261); both symmetric
+constexpr double D0R[NSP][NSP] = {{0.000, 0.969, 0.969, 0.969, 0.715, 3.440},
+                                  {0.969, 0.000, 0.715, 0.969, 0.715, 3.208},
+                                  {0.969, 0.715, 0.000, 0.969, 0.527, 2.939},
+                                  {0.969, 0.969, 0.969, 0.000, 0.969, 2.939},
+                                  {0.715, 0.715, 0.527, 0.969, 0.000, 2.939},
+                                  {3.440, 3.208, 2.939, 2.939, 2.939, 0.000}};
+constexpr double DEX[NSP][NSP] = {{0.000, 0.774, 0.774, 0.774, 0.750, 0.749},
+                                  {0.774, 0.000, 0.750, 0.774, 0.750, 0.710},
+                                  {0.774, 0.750, 0.000, 0.774, 0.810, 0.718},
+                                  {0.774, 0.774, 0.774, 0.000, 0.774, 0.718},
+                                  {0.750, 0.750, 0.810, 0.774, 0.000, 0.718},
+                                  {0.749, 0.710, 0.718, 0.718, 0.718, 0.000}};
+
+// 4-point reconstruction of calc_facevalues_alts_* (vertical_solver_ausm.f90
+// :1052-1053); Factor2 is the literal 0.0416667, not 1/24.
+constexpr double FACT1 = 0.6250000, FACT2 = 0.0416667;
+
+// Kernel-side run constants (built on the host from gitm_vert_params).
+struct KParams {
+  double dt, beta, maxvv, min_tec, vert_tau, omega;
+  double mass[NSP], tdc[NSP];
+  double log_min_kp, log_max_kp, log_min_ku, log_max_ku;  // log(1e-19), ...
+  int n_alts, n_steps, solver;
+  int friction, coriolis, ions, improved_ions, nighttime_ion_bcs;
+  unsigned long long *err;  // first-error word, see error_key()
+};
+
+// Error word: the lowest (column, code, level) wins an atomic min.
+GV_FN unsigned long long error_key(int col, int code, int level) {
+  return ((unsigned long long)(unsigned)col << 32) |
+         ((unsigned long long)(unsigned)code << 16) |
+         (unsigned long long)(unsigned)(level + 1024);
+}
+
+// LDS layout of one column: NSTAT static tables and NBUF exchange buffers of
+// pitch NT+4 (two guard cells on each side, so j-2..j+2 stays inside), a
+// scalar area and an int buffer.
+enum StatArray { A_DF = 0, A_IF, A_DC, A_CMF, A_KP, A_KU, NSTAT };
+constexpr int NBUF = 16;
+enum Scalar {
+  SC_ALT0 = 0, SC_MMM_M1, SC_MMM_0, SC_MG0, SC_MGT1, SC_MGT2,
+  SC_CA = 8,    // bottom extrapolation coefficients for level 0 (5)
+  SC_CB = 13,   // ... for level -1 (5)
+  SC_BOT = 18,  // bottom BC results: 6 species x 6 values
+  SC_TOP = 54,  // top BC results: 6 species x 4 values
+  NSCAL = 80
+};
+
+template <int NT>
+struct LdsSize {
+  static constexpr int P = NT + 4;
+  static constexpr unsigned long bytes =
+      ((unsigned long)(NSTAT + NBUF) * P + NSCAL) * sizeof(double) +
+      (unsigned long)P * sizeof(int);
+};
+
+// ---------------------------------------------------------------------------
+// small numerics
+// ---------------------------------------------------------------------------
+GV_FN double dmax(double a, double b) { return a > b ? a : b; }
+GV_FN double dmin(double a, double b) { return a < b ? a : b; }
+
+// Limiter_mc, ModLimiter.f90:16-36
+GV_FN double limiter_mc(double beta, double up, double dn) {
+  if (up > 0.0) {
+    if (dn > 0.0) return dmin(dmin(beta * up, beta * dn), (up + dn) * 0.5);
+    return 0.0;
+  }
+  if (dn < 0.0) return dmax(dmax(beta * up, beta * dn), (up + dn) * 0.5);
+  return 0.0;
+}
+
+// Crout LU with implicit partial pivoting + substitution for the 6x6
+// friction system.  Same pivot rule as src/ludcmp.f (rows scaled by
+// 1/max|a_ij|, the last maximum wins, a zero pivot becomes 1e-20) and the
+// leading-zero skip of src/lubksb.f.  Fully unrolled: every array index is a
+// compile-time constant and row interchanges are selects, so the system stays
+// in registers.  Returns false for a zero row (singular matrix).
+GV_FN bool lu_solve6(double (&a)[6][6], double (&b)[6]) {
+  double sc[6];
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    double big = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+      double v = fabs(a[i][j]);
+      if (v > big) big = v;
+    }
+    if (big == 0.0) ok = false;
+    sc[i] = 1. / big;
+  }
+  int perm[6];
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+#pragma unroll
+    for (int i = 0; i < j; i++) {
+      double s = a[i][j];
+#pragma unroll
+      for (int q = 0; q < i; q++) s = s - a[i][q] * a[q][j];
+      a[i][j] = s;
+    }
+    double best = 0.0;
+    int piv = j;
+#pragma unroll
+    for (int i = j; i < 6; i++) {
+      double s = a[i][j];
+#pragma unroll
+      for (int q = 0; q < j; q++) s = s - a[i][q] * a[q][j];
+      a[i][j] = s;
+      double t = sc[i] * fabs(s);
+      if (t >= best) {
+        piv = i;
+        best = t;
+      }
+    }
+#pragma unroll
+    for (int r = j + 1; r < 6; r++) {
+      if (piv == r) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+          double t = a[r][q];
+          a[r][q] = a[j][q];
+          a[j][q] = t;
+        }
+        sc[r] = sc[j];
+      }
+    }
+    perm[j] = piv;
+    if (a[j][j] == 0.) a[j][j] = 1.0e-20;
+    if (j != 5) {
+      double r = 1. / a[j][j];
+#pragma unroll
+      for (int i = j + 1; i < 6; i++) a[i][j] = a[i][j] * r;
+    }
+  }
+  int first = -1;
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    double s = b[i];
+#pragma unroll
+    for (int r = i + 1; r < 6; r++)
+      if (perm[i] == r) {
+        s = b[r];
+        b[r] = b[i];
+      }
+    if (first >= 0) {
+#pragma unroll
+      for (int q = 0; q < i; q++)
+        if (q >= first) s = s - a[i][q] * b[q];
+    } else if (s != 0.) {
+      first = i;
+    }
+    b[i] = s;
+  }
+#pragma unroll
+  for (int i = 5; i >= 0; i--) {
+    double s = b[i];
+#pragma unroll
+    for (int q = i + 1; q < 6; q++) s = s - a[i][q] * b[q];
+    b[i] = s / a[i][i];
+  }
+  return ok;
+}
+
+// calc_neutral_friction, calc_neutral_friction.f90:47-106, one level.
+GV_FN bool neutral_friction(const KParams &P, double dtin, double (&vel)[NSP],
+                            double nt, const double (&ns)[NSP],
+                            const double (&glc)[NSP], double eddy, double temp) {
+  double tdij[NSP][NSP];
+#pragma unroll
+  for (int i = 0; i < NSP; i++)
+#pragma unroll
+    for (int q = i + 1; q < NSP; q++) {
+      // Diff0 and DiffExp are symmetric, so TempDij(i,j) == TempDij(j,i)
+      const double d0 = 1.0e17 * D0R[i][q];
+      double v = (1.0e-04) * (d0 * pow(temp, DEX[i][q])) / (nt * (1.0e-06));
+      tdij[i][q] = v;
+      tdij[q][i] = v;
+    }
+  double m[NSP][NSP], ec[NSP];
+  const double denscale = 1.0 / nt;
+#pragma unroll
+  for (int i = 0; i < NSP; i++) {
+    const double kTOverM = KB * temp / P.mass[i];
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < NSP; q++) {
+      if (q == i) continue;
+      double c = kTOverM * denscale * ns[q] / tdij[i][q];
+      sum = sum + c;
+      m[i][q] = -dtin * c;
+    }
+    m[i][i] = 1.0 + dtin * sum;
+    ec[i] = -1.0 * eddy * glc[i];
+  }
+  bool ok = lu_solve6(m, vel);
+#pragma unroll
+  for (int i = 0; i < NSP; i++) vel[i] = vel[i] + ec[i];
+  return ok;
+}
+
+// one-sided 5-point extrapolation coefficients, set_vertical_bcs.Earth.f90
+// :228-248 (h1..h4 are the spacings above the point being filled)
+GV_FN void bc_fwd_coefs(double h1, double h2, double h3, double h4, double *C) {
+  double H1 = h1, H2 = h1 + h2, H3 = h1 + h2 + h3, H4 = h1 + h2 + h3 + h4;
+  C[0] = -1.0 * (H2 * H3 * H4 + H1 * H3 * H4 + H1 * H2 * H4 + H1 * H2 * H3) /
+         (H1 * H2 * H3 * H4);
+  C[1] = 1.0 * (H2 * H3 * H4) / (h1 * h2 * (h2 + h3) * (h2 + h3 + h4));
+  C[2] = -1.0 * (H1 * H3 * H4) / (H2 * h2 * h3 * (h3 + h4));
+  C[3] = 1.0 * (H1 * H2 * H4) / (H3 * (h3 + h2) * h3 * h4);
+  C[4] = -1.0 * (H1 * H2 * H3) / (H4 * (h2 + h3 + h4) * (h3 + h4) * h4);
+}
+
+// Per-thread constants of the owned level.
+struct LevelStatic {
+  double eg;    // EffectiveGravity = g + Centrifugal/InvR (ausm :582-589)
+  double mgh;   // 0.5*(eg(k) + eg(k-1))                    (ausm :697)
+  double gm1;   // Gamma - 1
+  double gl1, gl2, gl3, gr1, gr2, gr3;  // gamma factors of face k|k+1
+  double hf;    // 0.35*exp(-(z - z0)/45 km)               (ausm :950)
+  double ap, am, vol;  // face areas r^2 and volume (r^3)/3  (ausm :2029-2040)
+  double invr, eddy, tu, grav;
+  double glc[5];  // 5-point d ln(rho_s/rho)/dz weights     (ausm :659-676)
+};
+
+// ---------------------------------------------------------------------------
+// One column.  Ctx supplies lane(), sync(), sync_or(bool) and report(key).
+// ---------------------------------------------------------------------------
+template <int NT, class Ctx>
+struct Column {
+  static constexpr int P = NT + 4;
+  Ctx &cx;
+  const KParams &kp;
+  double *lds;
+  const int j, k, nA, L;
+  const bool lv;  // owns a level
+  int col;
+  LevelStatic S;
+  double U[NF];
+  double centrifugal, coriolis;
+
+  GV_MEM Column(Ctx &c, const KParams &p, double *l, int column)
+      : cx(c), kp(p), lds(l), j(c.lane()), k(c.lane() - 1), nA(p.n_alts),
+        L(p.n_alts + 4), lv(c.lane() < p.n_alts + 4), col(column) {}
+
+  GV_MEM double &st(int a, int jj) const { return lds[a * P + jj + 2]; }
+  GV_MEM double &xb(int b, int jj) const { return lds[(NSTAT + b) * P + jj + 2]; }
+  GV_MEM double &sc(int i) const { return lds[(NSTAT + NBUF) * P + i]; }
+  GV_MEM int &ib(int jj) const {
+    return reinterpret_cast<int *>(lds + (NSTAT + NBUF) * P + NSCAL)[jj + 2];
+  }
+  GV_MEM bool face() const { return k >= 0 && k <= nA; }
+  GV_MEM bool cell() const { return k >= 1 && k <= nA; }
+  GV_MEM void fail(int code, int level) { cx.report(error_key(col, code, level)); }
+
+  // -------------------------------------------------------------------------
+  // Column prep: geometry and per-level constants (advance_vertical.f90
+  // :84-151 plus the state-independent parts of the stage functions).
+  // -------------------------------------------------------------------------
+  GV_MEM void setup(double alt, double grav, double invr, double dalt, double mmm,
+                    double gam, double eddy, double cmax, double tu, double coslat) {
+    centrifugal = (coslat * kp.omega) * (coslat * kp.omega);  // :103
+    coriolis = 2 * coslat * kp.omega;                         // :104
+    S.eg = grav + centrifugal / invr;
+    S.invr = invr;
+    S.eddy = eddy;
+    S.tu = tu;
+    S.grav = grav;
+    S.gm1 = gam - 1.0;
+    const double rd = 1.0 / invr;  // RadialDistance_C (ausm :595-597)
+    xb(0, j) = alt;
+    xb(1, j) = S.eg;
+    xb(2, j) = gam;
+    xb(3, j) = cmax;
+    xb(4, j) = rd;
+    if (j == 0) sc(SC_MMM_M1) = mmm;
+    if (j == 1) {
+      sc(SC_MMM_0) = mmm;
+      sc(SC_ALT0) = alt;
+    }
+    cx.sync();
+    // dAlt_F(k) = Alt(k) - Alt(k-1), dAlt_F(-1) = dAlt_F(0)  (:149-151)
+    const double dF = (j >= 1) ? alt - xb(0, j - 1) : xb(0, 1) - alt;
+    st(A_DF, j) = dF;
+    st(A_IF, j) = 1.0 / dF;
+    st(A_DC, j) = dalt;
+    // Rusanov dissipation speed of face k-1|k: 0.5*max(cMax(k-1), cMax(k))
+    st(A_CMF, j) = 0.5 * dmax(xb(3, j - 1), cmax);
+    S.mgh = (j >= 1) ? 0.5 * (S.eg + xb(1, j - 1)) : 0.0;
+    if (j == 1) sc(SC_MG0) = S.mgh;
+    if (j == nA + 2) sc(SC_MGT1) = S.mgh;
+    if (j == nA + 3) sc(SC_MGT2) = S.mgh;
+    // Kp/Ku blends, vertical_solver_ausm.f90:1320-1352 (same for all species)
+    if (k >= 1 && k <= nA) {
+      const double t = (1.0 + tanh((alt - 160.0e+03) / 15.0e+03));
+      st(A_KP, j) = exp(kp.log_min_kp + 0.5 * (kp.log_max_kp - kp.log_min_kp) * t);
+      st(A_KU, j) = exp(kp.log_min_ku + 0.5 * (kp.log_max_ku - kp.log_min_ku) * t);
+    } else {
+      st(A_KP, j) = 0.0;
+      st(A_KU, j) = 0.0;
+    }
+    // spherical areas and volume, vertical_solver_ausm.f90:2029-2040
+    {
+      const double LR = 0.5 * (rd + xb(4, j - 1));
+      const double RR = 0.5 * (rd + xb(4, j + 1));
+      S.ap = RR * RR;
+      S.am = LR * LR;
+      S.vol = (1.0 / 3.0) * (RR * RR * RR - LR * LR * LR);
+    }
+    S.hf = 0.35 * exp(-1.0 * (alt - sc(SC_ALT0)) / (45.0e+03));
+    cx.sync();
+    // gamma face states (calc_kt_facevalues of Gamma_1d, ausm :1535)
+    double dlg = 0.0;
+    if (k >= 0 && k <= nA + 1)
+      dlg = limiter_mc(kp.beta, (xb(2, j + 1) - gam) * st(A_IF, j + 1),
+                       (gam - xb(2, j - 1)) * st(A_IF, j));
+    xb(5, j) = dlg;
+    // 5-point gradient weights (ausm :649-676)
+    if (k >= 1 && k <= nA) {
+      const double h1 = st(A_DF, j - 1), h2 = dF, h3 = st(A_DF, j + 1),
+                   h4 = st(A_DF, j + 2);
+      const double MeshH2 = h2 + h1;
+      const double MeshH3 = h3 + h2 + h1;
+      const double MeshH4 = h4 + h3 + h2 + h1;
+      S.glc[0] = (h2 * h3 * (h3 + h4)) / (h1 * MeshH2 * MeshH3 * MeshH4);
+      S.glc[1] = -1.0 * (MeshH2 * h3 * (h3 + h4)) / (h1 * h2 * (h2 + h3) * (h2 + h3 + h4));
+      S.glc[3] = MeshH2 * h2 * (h4 + h3) / (MeshH3 * (h2 + h3) * h3 * h4);
+      S.glc[4] = -1.0 * MeshH2 * h2 * h3 / (MeshH4 * (h2 + h3 + h4) * (h3 + h4) * h4);
+      S.glc[2] = (h2 * h3 * (h3 + h4) + MeshH2 * h3 * (h3 + h4) -
+                  MeshH2 * h2 * (h3 + h4) - MeshH2 * h2 * h3) /
+                 (MeshH2 * h2 * h3 * (h3 + h4));
+    } else {
+      for (int q = 0; q < 5; q++) S.glc[q] = 0.0;
+    }
+    // bottom extrapolation weights for levels 0 and -1 (bc :228-248)
+    if (j == 0) {
+      bc_fwd_coefs(st(A_DF, 3), st(A_DF, 4), st(A_DF, 5), st(A_DF, 6), &sc(SC_CA));
+      bc_fwd_coefs(st(A_DF, 2), st(A_DF, 3), st(A_DF, 4), st(A_DF, 5), &sc(SC_CB));
+    }
+    cx.sync();
+    if (face()) {
+      const double gL = gam + 0.5 * dlg * dF;
+      const double gR = xb(2, j + 1) - 0.5 * xb(5, j + 1) * st(A_DF, j + 1);
+      S.gl1 = 1.0 / (gL - 1.0);
+      S.gl2 = gL / (gL - 1.0);
+      S.gl3 = 2.0 * ((gL - 1.0) / (gL + 1.0));
+      S.gr1 = 1.0 / (gR - 1.0);
+      S.gr2 = gR / (gR - 1.0);
+      S.gr3 = 2.0 * ((gR - 1.0) / (gR + 1.0));
+    } else {
+      S.gl1 = S.gl2 = S.gl3 = S.gr1 = S.gr2 = S.gr3 = 0.0;
+    }
+    cx.sync();
+  }
+
+  // -------------------------------------------------------------------------
+  // set_vertical_bcs, set_vertical_bcs.Earth.f90:8-652 (neutrals, UseMSIS=F,
+  // no tides, no BC perturbation).  Ghost values depend only on the interior
+  // and on Temp(0) and LogNS(0) of the non-photochemical species, which the
+  // BCs never change.
+  // -------------------------------------------------------------------------
+  GV_MEM void bcs() {
+    // stage ln(6), w(6), T, vE, vN, vU
+#pragma unroll
+    for (int f = 0; f < 12; f++) xb(f, j) = U[F_LN + f];
+    xb(12, j) = U[F_T];
+    xb(13, j) = U[F_VE];
+    xb(14, j) = U[F_VN];
+    xb(15, j) = U[F_VU];
+    cx.sync();
+    if (j < NSP) {
+      // bottom, one species per thread (:198-272, :277-344, :382-395)
+      const int sp = j;
+      double *CA = &sc(SC_CA), *CB = &sc(SC_CB);
+      const double dF1 = st(A_DF, 2), dF0 = st(A_DF, 1), dFm1 = st(A_DF, 0);
+      double ln0 = xb(sp, 1), ln1 = xb(sp, 2), ln2 = xb(sp, 3), ln3 = xb(sp, 4),
+             ln4 = xb(sp, 5), ln5 = xb(sp, 6);
+      double n0, nm1;
+      if (photochemical(sp)) {
+        double d = ln1 * CA[0] + ln2 * CA[1] + ln3 * CA[2] + ln4 * CA[3] + ln5 * CA[4];
+        n0 = ln1 - dF1 * d;
+        if (d <= 0.0) n0 = ln1;
+        if (sp == S_NO && n0 < ln1 - 0.333) n0 = ln1 - 0.333;
+        d = n0 * CB[0] + ln1 * CB[1] + ln2 * CB[2] + ln3 * CB[3] + ln4 * CB[4];
+        nm1 = n0 - dF0 * d;
+        if (d <= 0.0) nm1 = n0;
+        if (sp == S_NO && nm1 < n0 - 0.333) nm1 = n0 - 0.333;
+      } else {
+        n0 = ln0;
+        if (sp == S_O) {
+          nm1 = ln0;  // zero gradient below the boundary (:219-222)
+        } else {
+          const double t0 = xb(12, 1), tm1 = t0;  // Temp(-1) = Temp(0)
+          const double MeanGravity = -sc(SC_MG0);
+          const double MeanTemp = 0.5 * (t0 + tm1);
+          const double MeanMass = 0.5 * (sc(SC_MMM_0) + sc(SC_MMM_M1));
+          const double ish = MeanGravity * MeanMass / (MeanTemp * KB);
+          const double ns = exp(ln0) * (t0 / tm1) * exp(1.0 * ish * dFm1);
+          nm1 = log(ns);
+        }
+      }
+      const double w1 = xb(F_W - F_LN + sp, 2), w2 = xb(F_W - F_LN + sp, 3),
+                   w3 = xb(F_W - F_LN + sp, 4), w4 = xb(F_W - F_LN + sp, 5),
+                   w5 = xb(F_W - F_LN + sp, 6);
+      const double w0 = w1 - dF1 * (w1 * CA[0] + w2 * CA[1] + w3 * CA[2] + w4 * CA[3] + w5 * CA[4]);
+      const double wm1 = w0 - dF0 * (w0 * CB[0] + w1 * CB[1] + w2 * CB[2] + w3 * CB[3] + w4 * CB[4]);
+      double w0r = w0, wm1r = wm1;
+      if (photochemical(sp) && w1 > 0.0) {
+        w0r = -1.0 * w1;
+        wm1r = -1.0 * w1;
+      }
+      double *R = &sc(SC_BOT + 6 * sp);
+      R[0] = n0;
+      R[1] = nm1;
+      R[2] = w0;
+      R[3] = wm1;
+      R[4] = w0r;
+      R[5] = wm1r;
+    } else if (j >= 8 && j < 8 + NSP) {
+      // top, one species per thread (:429-441, :471-486)
+      const int sp = j - 8;
+      const int jn = nA + 1;  // offset of level nAlts
+      const double tn = xb(12, jn);
+      const double wn = xb(F_W - F_LN + sp, jn), wnm1 = xb(F_W - F_LN + sp, jn - 1);
+      double w1 = wn, w2 = wn;
+      if (photochemical(sp) && wn < 0.0) {
+        w1 = -wn;
+        w2 = -wnm1;
+      }
+      const double m = kp.mass[sp];
+      double ns = exp(xb(sp, jn));
+      double MeanGravity = -sc(SC_MGT1);
+      double MeanTemp = 0.5 * (tn + tn);
+      double ish = MeanGravity * m / (MeanTemp * KB);
+      ns = ns * (tn / tn) * exp(-ish * st(A_DF, jn + 1));
+      const double l1 = log(ns);
+      MeanGravity = -sc(SC_MGT2);
+      ish = MeanGravity * m / (MeanTemp * KB);
+      ns = ns * (tn / tn) * exp(-ish * st(A_DF, jn + 2));
+      const double l2 = log(ns);
+      double *R = &sc(SC_TOP + 4 * sp);
+      R[0] = l1;
+      R[1] = l2;
+      R[2] = w1;
+      R[3] = w2;
+    }
+    cx.sync();
+    if (j <= 1) {
+      // levels -1 and 0: ghost species, LogRho and bulk wind (:346-362)
+      const int o = (j == 1) ? 0 : 1;
+      double sr = 0.0, ns[NSP];
+#pragma unroll
+      for (int sp = 0; sp < NSP; sp++) {
+        const double *R = &sc(SC_BOT + 6 * sp);
+        U[F_LN + sp] = R[o];
+        U[F_W + sp] = R[4 + o];
+        ns[sp] = exp(R[o]);
+        sr = sr + kp.mass[sp] * ns[sp];
+      }
+      U[F_LR] = log(sr);
+      const double er = exp(U[F_LR]);
+      double vu = 0.0;
+#pragma unroll
+      for (int sp = 0; sp < NSP; sp++)
+        vu = vu + ns[sp] * kp.mass[sp] * sc(SC_BOT + 6 * sp + 2 + o) / er;
+      U[F_VU] = vu;
+      U[F_VE] = 0.0;
+      U[F_VN] = 0.0;
+      if (j == 0) U[F_T] = xb(12, 1);
+    } else if (j == nA + 2 || j == nA + 3) {
+      // levels nAlts+1, nAlts+2 (:420-463, :598-604)
+      const int o = (j == nA + 2) ? 0 : 1;
+      const int jn = nA + 1;
+      double sr = 0.0;
+#pragma unroll
+      for (int sp = 0; sp < NSP; sp++) {
+        const double *R = &sc(SC_TOP + 4 * sp);
+        U[F_LN + sp] = R[o];
+        U[F_W + sp] = R[2 + o];
+        sr = sr + kp.mass[sp] * exp(R[o]);
+      }
+      U[F_LR] = log(sr);
+      U[F_VE] = xb(13, jn);
+      U[F_VN] = xb(14, jn);
+      const double vu = xb(15, jn);
+      U[F_VU] = (vu < 0.0) ? -vu : vu;
+      U[F_T] = xb(12, jn);
+    }
+    cx.sync();
+  }
+
+  // -------------------------------------------------------------------------
+  // 4th-order limited face values and Rusanov gradient / dissipation of up to
+  // 8 fields (calc_rusanov_alts_*, vertical_solver_ausm.f90:1007-1090).
+  // Uses exchange buffers 0..15.  grad/diff are valid on cell threads.
+  // -------------------------------------------------------------------------
+  template <int N>
+  GV_MEM void rusanov_grad_diff(const double (&v)[N], double (&grad)[N],
+                                double (&diff)[N]) {
+    static_assert(N <= 8, "at most 8 fields per pass");
+#pragma unroll
+    for (int f = 0; f < N; f++) xb(f, j) = v[f];
+    cx.sync();
+    const double iF0 = st(A_IF, j), iF1 = st(A_IF, j + 1);
+    const double dF0 = st(A_DF, j), dF1 = st(A_DF, j + 1);
+    double rf[N], ln[N];
+#pragma unroll
+    for (int f = 0; f < N; f++) {
+      const double vm2 = xb(f, j - 2), vm1 = xb(f, j - 1), v0 = v[f],
+                   v1 = xb(f, j + 1), v2 = xb(f, j + 2);
+      double dl = 0.0;
+      if (k >= 1 && k <= nA) {
+        double h = iF1 * 2.0;
+        const double up = h * (FACT1 * (v1 - v0) - FACT2 * (v2 - vm1));
+        h = iF0 * 2.0;
+        const double dn = h * (FACT1 * (v0 - vm1) - FACT2 * (v1 - vm2));
+        dl = limiter_mc(kp.beta, up, dn);
+      } else if (k == 0 || k == nA + 1) {
+        dl = limiter_mc(kp.beta, (v1 - v0) * iF1, (v0 - vm1) * iF0);
+      }
+      rf[f] = v0 - 0.5 * dl * dF0;  // right state of face k-1|k
+      ln[f] = v0 + 0.5 * dl * dF1;  // left state of face k|k+1
+    }
+    cx.sync();
+#pragma unroll
+    for (int f = 0; f < N; f++) {
+      xb(f, j) = rf[f];
+      xb(8 + f, j) = ln[f];
+    }
+    cx.sync();
+    const double dC = st(A_DC, j);
+    const double cm0 = st(A_CMF, j), cm1 = st(A_CMF, j + 1);
+#pragma unroll
+    for (int f = 0; f < N; f++) {
+      const double L0 = xb(8 + f, j - 1), R0 = rf[f], L1 = ln[f], R1 = xb(f, j + 1);
+      grad[f] = 0.5 * (L1 + R1 - L0 - R0) / dC;
+      const double df0 = cm0 * (R0 - L0), df1 = cm1 * (R1 - L1);
+      diff[f] = (df1 - df0) / dC;
+    }
+    cx.sync();
+  }
+
+  // 5-point gradient of ln(rho_s/rho) (vertical_solver_ausm.f90:649-688)
+  GV_MEM void grad_log_cons(const double (&lcs)[NSP], double (&glc)[NSP]) {
+#pragma unroll
+    for (int f = 0; f < NSP; f++) xb(f, j) = lcs[f];
+    cx.sync();
+#pragma unroll
+    for (int f = 0; f < NSP; f++)
+      glc[f] = S.glc[0] * xb(f, j - 2) + S.glc[1] * xb(f, j - 1) + S.glc[2] * lcs[f] +
+               S.glc[3] * xb(f, j + 1) + S.glc[4] * xb(f, j + 2);
+    cx.sync();
+  }
+
+  // -------------------------------------------------------------------------
+  // advance_vertical_1stage_ausm, vertical_solver_ausm.f90:436-999, with
+  // calc_all_fluxes_hydro (:1093-2070).  Writes New (valid on cell threads).
+  // -------------------------------------------------------------------------
+  GV_MEM void ausm_stage(double dtin, double (&New)[NF]) {
+    const double *m = kp.mass;
+    const double T = U[F_T];
+    // --- pointwise (:591-609, :649-652, :752-773)
+    double NS[NSP], rhos[NSP], lrs[NSP], lps[NSP], lcs[NSP];
+    const double rho = exp(U[F_LR]);
+    double nt = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSP; s++) {
+      NS[s] = exp(U[F_LN + s]);
+      nt = nt + NS[s];
+    }
+    const double press = nt * KB * T;
+#pragma unroll
+    for (int s = 0; s < NSP; s++) {
+      rhos[s] = m[s] * NS[s];
+      lrs[s] = log(rhos[s]);
+      lps[s] = log(NS[s] * T * KB);
+      lcs[s] = log(m[s] * NS[s] / rho);
+    }
+    const double te = press / S.gm1 + 0.5 * rho *
+                                          (U[F_VU] * U[F_VU] + U[F_VN] * U[F_VN] +
+                                           U[F_VE] * U[F_VE]);
+    const double momE = rho * U[F_VE], momN = rho * U[F_VN];
+
+    // --- hydrostatic background (:694-751): per-level factors in parallel,
+    // the recurrence in altitude serially (one thread per species)
+    xb(0, j) = T;
+    cx.sync();
+    {
+      const double tm1 = xb(0, j - 1), tp1 = xb(0, j + 1);
+      const double dF = st(A_DF, j);
+      if (lv && j >= 2) {
+        xb(1, j) = tm1 / T;
+        const double MeanTemp = 0.5 * (T + tm1);
+#pragma unroll
+        for (int s = 0; s < NSP; s++) {
+          const double ish = -1.0 * m[s] * S.mgh / (KB * MeanTemp);
+          xb(2 + s, j) = exp(-1.0 * dF * ish);
+        }
+      } else if (j == 0) {
+        xb(1, j) = tp1 / T;
+        const double MeanMass = 0.5 * (sc(SC_MMM_M1) + sc(SC_MMM_0));
+        const double MeanTemp = 0.5 * (tp1 + T);
+        const double ish = -1.0 * MeanMass * sc(SC_MG0) / (KB * MeanTemp);
+        const double b = exp(dF * ish);
+#pragma unroll
+        for (int s = 0; s < NSP; s++) xb(2 + s, j) = b;
+      }
+#pragma unroll
+      for (int s = 0; s < NSP; s++) xb(8 + s, j) = NS[s];
+    }
+    cx.sync();
+    if (j < NSP) {
+      const int s = j;
+      double *a = &xb(1, 0), *b = &xb(2 + s, 0), *h = &xb(8 + s, 0);
+      const double ns0 = h[1];
+      double x = ns0;
+      for (int jj = 2; jj < L; jj++) {
+        x = x * a[jj] * b[jj];
+        h[jj] = x;
+      }
+      h[0] = ns0 * a[0] * b[0];
+    }
+    cx.sync();
+    double lhps[NSP], dev[NSP];
+    int sub = 0;  // bit s: SubtractHydrostatic(k, s)
+#pragma unroll
+    for (int s = 0; s < NSP; s++) {
+      const double hns = xb(8 + s, j);
+      lhps[s] = log(hns * KB * T);
+      const double hrhos = m[s] * hns;
+      const double ratio = fabs(rhos[s] - hrhos) / rhos[s];
+      dev[s] = rhos[s] - hrhos;
+      if (!(fabs(ratio) > 1.0)) sub |= 1 << s;
+    }
+    cx.sync();
+
+    // --- face states (calc_kt_facevalues, :2073-2117), one face k|k+1 per
+    // thread.  Batch 1: ln rho_s, ln p_s, u, v.  Batch 2: ln p_hydro,s, w_s.
+    const double iF0 = st(A_IF, j), iF1 = st(A_IF, j + 1), iF2 = st(A_IF, j + 2);
+    const double dF0 = st(A_DF, j), dF1 = st(A_DF, j + 1);
+    double RSL[NSP], RSR[NSP], PSL[NSP], PSR[NSP], HL[NSP], HR[NSP], VL[NSP], VR[NSP];
+    double VEL, VER, VNL, VNR;
+#define GV_KT(b, f0, OUTL, OUTR)                                                  \
+  do {                                                                            \
+    const double fm1 = xb(b, j - 1), f1 = xb(b, j + 1), f2 = xb(b, j + 2);         \
+    const double d0 = limiter_mc(kp.beta, (f1 - (f0)) * iF1, ((f0) - fm1) * iF0); \
+    const double d1 = limiter_mc(kp.beta, (f2 - f1) * iF2, (f1 - (f0)) * iF1);   \
+    OUTL = (f0) + 0.5 * d0 * dF0;                                                 \
+    OUTR = f1 - 0.5 * d1 * dF1;                                                   \
+  } while (0)
+#pragma unroll
+    for (int s = 0; s < NSP; s++) {
+      xb(s, j) = lrs[s];
+      xb(6 + s, j) = lps[s];
+    }
+    xb(12, j) = U[F_VE];
+    xb(13, j) = U[F_VN];
+    ib(j) = sub;
+    cx.sync();
+    const int subUp = ib(j + 1);  // flags of cell k+1
+#pragma unroll
+    for (int s = 0; s < NSP; s++) {
+      double l, r;
+      GV_KT(s, lrs[s], l, r);
+      RSL[s] = exp(l);
+      RSR[s] = exp(r);
+      GV_KT(6 + s, lps[s], l, r);
+      PSL[s] = exp(l);
+      PSR[s] = exp(r);
+    }
+    GV_KT(12, U[F_VE], VEL, VER);
+    GV_KT(13, U[F_VN], VNL, VNR);
+    cx.sync();
+#pragma unroll
+    for (int s = 0; s < NSP; s++) {
+      xb(s, j) = lhps[s];
+      xb(6 + s, j) = U[F_W + s];
+    }
+    cx.sync();
+#pragma unroll
+    for (int s = 0; s < NSP; s++) {
+      double l, r;
+      GV_KT(s, lhps[s], l, r);
+      HL[s] = exp(l);
+      HR[s] = exp(r);
+      GV_KT(6 + s, U[F_W + s], VL[s], VR[s]);
+    }
+#undef GV_KT
+    cx.sync();
+
+    // --- AUSM+-up fluxes of face k|k+1 (:1459-1970).  The P12 side belongs
+    // to cell k, the M12 side to cell k+1 (with that cell's Kp, Ku, dAlt_C
+    // and SubtractHydrostatic flags).
+    double FRp[NSP], FMp[NSP], NPp[NSP], FRm[NSP], FMm[NSP], NPm[NSP];
+    double EFp = 0.0, MOEp = 0.0, MONp = 0.0, EFm = 0.0, MOEm = 0.0, MONm = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSP; s++) FRp[s] = FMp[s] = NPp[s] = FRm[s] = FMm[s] = NPm[s] = 0.0;
+    if (face()) {
+      double RL = 0.0, RR = 0.0, PL = 0.0, PR = 0.0;
+#pragma unroll
+      for (int s = 0; s < NSP; s++) {
+        RL = RL + RSL[s];
+        RR = RR + RSR[s];
+      }
+#pragma unroll
+      for (int s = 0; s < NSP; s++) {
+        PL = PL + PSL[s];
+        PR = PR + PSR[s];
+      }
+      double VUL = 0.0, VUR = 0.0;  // mass-weighted face winds (:1505-1532)
+#pragma unroll
+      for (int s = 0; s < NSP; s++) {
+        VUL = VUL + RSL[s] * VL[s] / RL;
+        VUR = VUR + RSR[s] * VR[s] / RR;
+      }
+      // bulk energy (:1538-1562), enthalpy (:1569-1615), Liou sound speed
+      // (:1619-1655)
+      const double EL = S.gl1 * PL + 0.5 * RL * (VUL * VUL + VEL * VEL + VNL * VNL);
+      const double ER = S.gr1 * PR + 0.5 * RR * (VUR * VUR + VER * VER + VNR * VNR);
+      const double HtL = 0.5 * (VUL * VUL + VEL * VEL + VNL * VNL) + S.gl2 * PL / RL;
+      const double HtR = 0.5 * (VUR * VUR + VER * VER + VNR * VNR) + S.gr2 * PR / RR;
+      double sub_cs = sqrt(S.gl3 * HtL);
+      const double CSL = (sub_cs * sub_cs) / dmax(sub_cs, VUL);
+      sub_cs = sqrt(S.gr3 * HtR);
+      const double CSR = (sub_cs * sub_cs) / dmax(sub_cs, -1.0 * VUR);
+      const double CS = dmin(CSL, CSR);
+
+      const double kp0 = st(A_KP, j), kp1 = st(A_KP, j + 1);
+      const double ku0 = st(A_KU, j), ku1 = st(A_KU, j + 1);
+      const double rdt0 = st(A_DC, j) / dtin, rdt1 = st(A_DC, j + 1) / dtin;
+      double BNVp = 0.0, BNVm = 0.0;
+      const double RLR = RL + RR;
+#pragma unroll
+      for (int s = 0; s < NSP; s++) {
+        const double MPS = 0.5 * (PSL[s] + PSR[s]);
+        const double MHPS = 0.5 * (HL[s] + HR[s]);
+        const double MRS = 0.5 * (RSL[s] + RSR[s]);
+        const double ML = VL[s] / CS, MR = VR[s] / CS;  // :1689-1706
+        const double M2 = 0.5 * (ML * ML + MR * MR);
+        const double MZ = sqrt(dmin(1.0, dmax(M2, 1.0e-19)));
+        const double FA = MZ * (2.0 - MZ);
+        double MF4P, MF4N;  // :1789-1819 (16*LiouBeta = 2)
+        if (fabs(ML) >= 1.0)
+          MF4P = 0.5 * (ML + fabs(ML));
+        else
+          MF4P = 0.25 * ((ML + 1.0) * (ML + 1.0)) *
+                 (1.0 - 2.0 * (-0.25 * ((ML - 1.0) * (ML - 1.0))));
+        if (fabs(MR) >= 1.0)
+          MF4N = 0.5 * (MR - fabs(MR));
+        else
+          MF4N = -0.25 * ((MR - 1.0) * (MR - 1.0)) *
+                 (1.0 + 2.0 * (0.25 * ((MR + 1.0) * (MR + 1.0))));
+        const double f1 = dmax((1.0 - M2), 0.0);
+        const double pdh = (PSR[s] - HR[s]) - (PSL[s] - HL[s]);
+        const double pd = PSR[s] - PSL[s];
+        const double rvd = (RSR[s]) * VR[s] - (RSL[s]) * VL[s];
+        const double mcs = MRS * CS, facs = FA * CS;
+        {  // P12 side of cell k (:1822-1970)
+          const bool sh = (sub >> s) & 1;
+          const double MP = kp0 * f1 * (sh ? pdh : pd) / (mcs * (facs + rdt0));
+          const double NV = CS * (MF4P + MF4N - MP);
+          NPp[s] = (sh ? (MPS - MHPS) : MPS) - 0.5 * ku0 * CS * rvd;
+          if (NV >= 0.0) {
+            FRp[s] = RSL[s] * NV;
+            FMp[s] = RSL[s] * VL[s] * NV;
+          } else {
+            FRp[s] = RSR[s] * NV;
+            FMp[s] = RSR[s] * VR[s] * NV;
+          }
+          BNVp = BNVp + (RSL[s] + RSR[s]) * NV / RLR;
+        }
+        {  // M12 side of cell k+1
+          const bool sh = (subUp >> s) & 1;
+          const double MP = kp1 * f1 * (sh ? pdh : pd) / (mcs * (facs + rdt1));
+          const double NV = CS * (MF4P + MF4N - MP);
+          NPm[s] = (sh ? (MPS - MHPS) : MPS) - 0.5 * ku1 * CS * rvd;
+          if (NV >= 0.0) {
+            FRm[s] = RSL[s] * NV;
+            FMm[s] = RSL[s] * VL[s] * NV;
+          } else {
+            FRm[s] = RSR[s] * NV;
+            FMm[s] = RSR[s] * VR[s] * NV;
+          }
+          BNVm = BNVm + (RSL[s] + RSR[s]) * NV / RLR;
+        }
+      }
+      // energy and momentum fluxes (:1973-2026)
+      if (BNVp >= 0.0) {
+        EFp = (EL + PL) * BNVp;
+        MOEp = RL * VEL * BNVp;
+        MONp = RL * VNL * BNVp;
+      } else {
+        EFp = (ER + PR) * BNVp;
+        MOEp = RR * VER * BNVp;
+        MONp = RR * VNR * BNVp;
+      }
+      if (BNVm >= 0.0) {
+        EFm = (EL + PL) * BNVm;
+        MOEm = RL * VEL * BNVm;
+        MONm = RL * VNL * BNVm;
+      } else {
+        EFm = (ER + PR) * BNVm;
+        MOEm = RR * VER * BNVm;
+        MONm = RR * VNR * BNVm;
+      }
+    }
+    // divergence of the face fluxes (:2042-2068); cell k takes its M12 side
+    // from the face below
+#pragma unroll
+    for (int s = 0; s < NSP; s++) {
+      xb(s, j) = FRm[s];
+      xb(6 + s, j) = FMm[s];
+    }
+    xb(12, j) = EFm;
+    xb(13, j) = MOEm;
+    xb(14, j) = MONm;
+    cx.sync();
+    double rsf[NSP], msf[NSP], ef = 0.0, mfE = 0.0, mfN = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSP; s++) {
+      rsf[s] = (S.ap * FRp[s] - S.am * xb(s, j - 1)) / S.vol;
+      msf[s] = (S.ap * FMp[s] - S.am * xb(6 + s, j - 1)) / S.vol;
+    }
+    ef = (S.ap * EFp - S.am * xb(12, j - 1)) / S.vol;
+    mfE = (S.ap * MOEp - S.am * xb(13, j - 1)) / S.vol;
+    mfN = (S.ap * MONp - S.am * xb(14, j - 1)) / S.vol;
+    cx.sync();
+#pragma unroll
+    for (int s = 0; s < NSP; s++) xb(s, j) = NPm[s];
+    cx.sync();
+    {
+      const double dC = st(A_DC, j);
+#pragma unroll
+      for (int s = 0; s < NSP; s++) msf[s] = msf[s] + (NPp[s] - xb(s, j - 1)) / dC;
+    }
+    cx.sync();
+
+    // --- gradients the sources need (:611-646): GradTemp, DiffVel(E,N)
+    const double gv3[3] = {T, U[F_VE], U[F_VN]};
+    double gr3[3], df3[3];
+    rusanov_grad_diff<3>(gv3, gr3, df3);
+    double glc[NSP];
+    grad_log_cons(lcs, glc);
+
+    // --- update of cell k (:802-997)
+    if (cell()) {
+      double nrs[NSP], nns[NSP], w[NSP];
+      double nrho = 0.0, nnt = 0.0;
+#pragma unroll
+      for (int s = 0; s < NSP; s++) {
+        nrs[s] = rhos[s] - dtin * (rsf[s]);
+        nns[s] = nrs[s] / m[s];
+        New[F_LN + s] = log(nns[s]);
+        nrho = nrho + nrs[s];
+        nnt = nnt + nns[s];
+      }
+      const double gradT = gr3[0];
+#pragma unroll
+      for (int s = 0; s < NSP; s++) {
+        const double moms = m[s] * NS[s] * U[F_W + s];
+        double nm;
+        if (!((sub >> s) & 1))
+          nm = moms - dtin * (msf[s]) + dtin * rhos[s] * S.eg;
+        else
+          nm = moms - dtin * (msf[s]) + dtin * dev[s] * S.eg;
+        double ww = nm / nrs[s];
+        ww = ww + dtin * (U[F_VN] * U[F_VN] + U[F_VE] * U[F_VE]) * S.invr;
+        if (kp.coriolis) ww = ww + dtin * (coriolis * U[F_VE]);
+        ww = ww - dtin * (kp.tdc[s] * KB * gradT) / m[s];
+        w[s] = ww;
+      }
+      if (kp.friction) {
+        if (!neutral_friction(kp, dtin, w, nnt, nns, glc, S.eddy, T))
+          fail(GITM_VERT_ESINGULAR, k);
+      }
+      double nvu = 0.0;
+#pragma unroll
+      for (int s = 0; s < NSP; s++) {
+        double ww = dmax(-kp.maxvv, w[s]);
+        ww = dmin(kp.maxvv, ww);
+        New[F_W + s] = ww;
+        nvu = nvu + ww * nrs[s] / nrho;
+      }
+      New[F_VU] = nvu;
+      double nv = (momE - dtin * (mfE)) / nrho;
+      New[F_VE] = nv + S.hf * dtin * df3[1];
+      nv = (momN - dtin * (mfN)) / nrho;
+      New[F_VN] = nv + S.hf * dtin * df3[2];
+      const double nte = te - dtin * ef + dtin * rho * U[F_VU] * S.eg;
+      const double np = (nte - 0.5 * nrho *
+                                   (New[F_VU] * New[F_VU] + New[F_VN] * New[F_VN] +
+                                    New[F_VE] * New[F_VE])) *
+                        S.gm1;
+      New[F_T] = np / (KB * nnt);
+      double sr = 0.0;
+#pragma unroll
+      for (int s = 0; s < NSP; s++) sr = sr + m[s] * exp(New[F_LN + s]);
+      New[F_LR] = log(sr);
+    }
+  }
+
+  // -------------------------------------------------------------------------
+  // advance_vertical_1stage (Rusanov), vertical_solver_rusanov.f90:387-686.
+  // Writes New (valid on cell threads).
+  // -------------------------------------------------------------------------
+  GV_MEM void rusanov_stage(double (&New)[NF]) {
+    const double *m = kp.mass;
+    const double dt = kp.dt;
+    const double T = U[F_T];
+    double NS[NSP], lcs[NSP];
+    const double rho = exp(U[F_LR]);
+    double sum = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSP; s++) {
+      NS[s] = exp(U[F_LN + s]);
+      sum = sum + NS[s];
+    }
+    const double nt = exp(log(sum));  // NT = exp(LogNum) (:458-460)
+#pragma unroll
+    for (int s = 0; s < NSP; s++) lcs[s] = log(m[s] * NS[s] / rho);
+    // gradients and dissipation (:462-498)
+    const double va[4] = {T, U[F_VE], U[F_VN], U[F_VU]};
+    double ga[4], da[4];
+    rusanov_grad_diff<4>(va, ga, da);
+    double gln[NSP], dln[NSP], gw[NSP], dw[NSP];
+    {
+      double v6[NSP];
+#pragma unroll
+      for (int s = 0; s < NSP; s++) v6[s] = U[F_LN + s];
+      rusanov_grad_diff<NSP>(v6, gln, dln);
+#pragma unroll
+      for (int s = 0; s < NSP; s++) v6[s] = U[F_W + s];
+      rusanov_grad_diff<NSP>(v6, gw, dw);
+    }
+    double glc[NSP];
+    grad_log_cons(lcs, glc);
+    if (!cell()) return;
+    const double divvel = ga[3] + 2 * U[F_VU] * S.invr;  // :476
+    double w[NSP];
+#pragma unroll
+    for (int s = 0; s < NSP; s++) {  // :550-614
+      const double vv = U[F_W + s];
+      const double divvv = gw[s] + 2 * vv * S.invr;
+      New[F_LN + s] = U[F_LN + s] - dt * (divvv + vv * gln[s]) + dt * dln[s];
+      double ww = vv -
+                  dt * (vv * gw[s] -
+                        (U[F_VN] * U[F_VN] + U[F_VE] * U[F_VE]) * S.invr +
+                        T * gln[s] * KB / m[s] + ga[0] * KB / m[s] - S.grav) +
+                  dt * dw[s] - vv / kp.vert_tau;
+      if (kp.coriolis)
+        ww = ww + dt * (centrifugal / S.invr + coriolis * U[F_VE]);
+      ww = ww - dt * (kp.tdc[s] * KB * ga[0]) / m[s];
+      w[s] = ww;
+    }
+    if (kp.friction) {  // :632-641, old densities
+      if (!neutral_friction(kp, dt, w, nt, NS, glc, S.eddy, T))
+        fail(GITM_VERT_ESINGULAR, k);
+    }
+    double nvu = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSP; s++) {  // :643-658
+      double ww = dmax(-kp.maxvv, w[s]);
+      ww = dmin(kp.maxvv, ww);
+      New[F_W + s] = ww;
+      nvu = nvu + ww * (m[s] * NS[s] / rho);
+    }
+    New[F_VU] = nvu;
+    const double vu = U[F_VU];  // :660-679
+    New[F_VE] = U[F_VE] - dt * vu * ga[1] + dt * da[1];
+    New[F_VN] = U[F_VN] - dt * vu * ga[2] + dt * da[2];
+    New[F_T] = T - dt * (vu * ga[0] + (S.gm1) * (T * divvel)) + dt * da[0];
+    double sr = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSP; s++) sr = sr + m[s] * exp(New[F_LN + s]);
+    New[F_LR] = log(sr);
+  }
+
+  // -------------------------------------------------------------------------
+  // RK4 drivers
+  // -------------------------------------------------------------------------
+  // advance_vertical_1d_ausm, vertical_solver_ausm.f90:26-433.  Stage k+1
+  // input = Y + (New_k - U_k); the final state is
+  // (1/3)*(-Y + S2 + 2*S3 + S4 + (New_4 - U_4)) on 1..nAlts, accumulated in
+  // the reference's left-to-right order; the boundary conditions follow every
+  // stage (ghost levels never enter the combination).
+  GV_MEM void ausm_step() {
+    double Y[NF], acc[NF], New[NF];
+    bcs();
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      Y[f] = U[f];
+      acc[f] = 0.0;
+      New[f] = U[f];
+    }
+    const double dtin[4] = {0.5 * kp.dt, 0.5 * kp.dt, kp.dt, 0.5 * kp.dt};
+    for (int stg = 0; stg < 4; stg++) {
+      ausm_stage(dtin[stg], New);
+      if (cell()) {
+        if (stg < 3) {
+#pragma unroll
+          for (int f = 0; f < NF; f++) {
+            const double nxt = Y[f] + (New[f] - U[f]);
+            if (stg == 0)
+              acc[f] = -1.0 * Y[f] + nxt;
+            else if (stg == 1)
+              acc[f] = acc[f] + 2.0 * nxt;
+            else
+              acc[f] = acc[f] + nxt;
+            U[f] = nxt;
+          }
+        } else {
+#pragma unroll
+          for (int f = 0; f < NF; f++)
+            U[f] = (1.0 / 3.0) * (acc[f] + (New[f] - U[f]));
+        }
+      }
+      bcs();
+    }
+  }
+
+  // advance_vertical_1d_rusanov, vertical_solver_rusanov.f90:10-384:
+  // classic RK4, K_i = New_i - U_i, Y + K/2, Y + K/2, Y + K, then
+  // Y + (K1 + 2K2 + 2K3 + K4)/6.
+  GV_MEM void rusanov_step() {
+    double Y[NF], acc[NF], New[NF];
+    bcs();
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      Y[f] = U[f];
+      acc[f] = 0.0;
+      New[f] = U[f];
+    }
+    for (int stg = 0; stg < 4; stg++) {
+      rusanov_stage(New);
+      if (cell()) {
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+          const double K = New[f] - U[f];
+          if (stg == 0) {
+            acc[f] = K;
+            U[f] = Y[f] + 0.5 * K;
+          } else if (stg == 1) {
+            acc[f] = acc[f] + 2.0 * K;
+            U[f] = Y[f] + 0.5 * K;
+          } else if (stg == 2) {
+            acc[f] = acc[f] + 2.0 * K;
+            U[f] = Y[f] + K;
+          } else {
+            U[f] = Y[f] + (1.0 / 6.0) * (acc[f] + K);
+          }
+        }
+      }
+      bcs();
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Gather / step / scatter of one column (advance_vertical.f90:84-185)
+// ---------------------------------------------------------------------------
+GV_FN long long col_base(const gitm_vert_field &f, int ci, int cj, int cb) {
+  return (long long)ci * f.s_lon + (long long)cj * f.s_lat + (long long)cb * f.s_blk;
+}
+GV_FN double *fld(const gitm_vert_field &f, long long base, int k, int comp) {
+  return f.ptr + base + (long long)(k - f.first_level) * f.s_lev + (long long)comp * f.s_comp;
+}
+
+template <int NT, int SOLVER, class Ctx>
+GV_FN void run_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io,
+                      int col, double *lds) {
+  Column<NT, Ctx> c(cx, kp, lds, col);
+  const int nlb = io.n_lons * io.n_lats;
+  const int ci = col % io.n_lons, cj = (col / io.n_lons) % io.n_lats, cb = col / nlb;
+  const int k = c.k, nA = c.nA;
+  const bool lv = c.lv;
+  double alt = 0.0, grav = 0.0, invr = 1.0, dalt = 1.0, mmm = 0.0, gam = 1.5,
+         eddy = 0.0, cmax = 0.0, tu = 1.0;
+  double nds[NSP];
+#pragma unroll
+  for (int f = 0; f < NF; f++) c.U[f] = 0.0;
+  c.U[F_T] = 1.0;
+#pragma unroll
+  for (int s = 0; s < NSP; s++) nds[s] = 1.0;
+  double coslat;
+  {
+    const gitm_vert_field &f = io.cos_latitude;
+    coslat = f.ptr[col_base(f, ci, cj, cb)];
+  }
+  if (lv) {
+#define GV_LD(F) (*fld(io.F, col_base(io.F, ci, cj, cb), k, 0))
+    alt = GV_LD(altitude);
+    grav = GV_LD(gravity);
+    invr = GV_LD(inv_radial_distance);
+    dalt = GV_LD(dalt);
+    mmm = GV_LD(mean_major_mass);
+    gam = GV_LD(gamma);
+    tu = GV_LD(temp_unit);
+    if (k >= 0 && k <= nA + 1) cmax = GV_LD(cmax);
+    if (k >= 1 && k <= nA) eddy = GV_LD(eddy);
+    const long long bn = col_base(io.ndensity_s, ci, cj, cb);
+    const long long bw = col_base(io.vertical_velocity, ci, cj, cb);
+    const long long bv = col_base(io.velocity, ci, cj, cb);
+#pragma unroll
+    for (int s = 0; s < NSP; s++) {
+      nds[s] = *fld(io.ndensity_s, bn, k, s);
+      c.U[F_W + s] = *fld(io.vertical_velocity, bw, k, s);
+    }
+    c.U[F_VE] = *fld(io.velocity, bv, k, 0);
+    c.U[F_VN] = *fld(io.velocity, bv, k, 1);
+    c.U[F_VU] = *fld(io.velocity, bv, k, 2);
+    c.U[F_T] = GV_LD(temperature);
+    c.U[F_LR] = GV_LD(rho);
+#undef GV_LD
+  }
+  c.setup(alt, grav, invr, dalt, mmm, gam, eddy, cmax, tu, coslat);
+
+  for (int step = 0; step < kp.n_steps; step++) {
+    // gather (:96-116): the densities must be positive on every level
+    bool bad = false;
+#pragma unroll
+    for (int s = 0; s < NSP; s++) bad = bad || (lv && !(nds[s] > 0.0));
+    if (cx.sync_or(bad)) {
+      c.ib(c.j) = bad ? k : 1 << 20;
+      cx.sync();
+      if (c.j == 0) {
+        int lo = 1 << 20;
+        for (int q = 0; q < c.L; q++)
+          if (c.ib(q) < lo) lo = c.ib(q);
+        c.fail(GITM_VERT_ENONPOSITIVE, lo);
+      }
+      return;  // the column is left as it was
+    }
+    c.U[F_LR] = log(c.U[F_LR]);
+    c.U[F_T] = c.U[F_T] * tu;
+#pragma unroll
+    for (int s = 0; s < NSP; s++) c.U[F_LN + s] = log(nds[s]);
+    if (SOLVER == GITM_VERT_AUSM)
+      c.ausm_step();
+    else
+      c.rusanov_step();
+    // scatter (:163-185)
+    c.U[F_LR] = exp(c.U[F_LR]);
+    c.U[F_T] = c.U[F_T] / tu;
+#pragma unroll
+    for (int s = 0; s < NSP; s++) nds[s] = exp(c.U[F_LN + s]);
+  }
+  if (lv) {
+    const long long bn = col_base(io.ndensity_s, ci, cj, cb);
+    const long long bw = col_base(io.vertical_velocity, ci, cj, cb);
+    const long long bv = col_base(io.velocity, ci, cj, cb);
+    double nsum = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSP; s++) {
+      *fld(io.ndensity_s, bn, k, s) = nds[s];
+      *fld(io.vertical_velocity, bw, k, s) = c.U[F_W + s];
+      nsum = nsum + nds[s];
+    }
+    if (io.log_ns.ptr) {
+      const long long bl = col_base(io.log_ns, ci, cj, cb);
+#pragma unroll
+      for (int s = 0; s < NSP; s++) *fld(io.log_ns, bl, k, s) = c.U[F_LN + s];
+    }
+    if (io.ndensity.ptr) *fld(io.ndensity, col_base(io.ndensity, ci, cj, cb), k, 0) = nsum;
+    *fld(io.velocity, bv, k, 0) = c.U[F_VE];
+    *fld(io.velocity, bv, k, 1) = c.U[F_VN];
+    *fld(io.velocity, bv, k, 2) = c.U[F_VU];
+    *fld(io.temperature, col_base(io.temperature, ci, cj, cb), k, 0) = c.U[F_T];
+    *fld(io.rho, col_base(io.rho, ci, cj, cb), k, 0) = c.U[F_LR];
+  }
+}
+
+}  // namespace gv

@@ -1,0 +1,114 @@
+// vert_params.h -- host-side checks and conversion of gitm_vert_params /
+// gitm_vert_columns into the kernel constants (shared by the HIP library and
+// the host build of the column kernels).
+#pragma once
+
+#include <math.h>
+#include <string.h>
+
+#include "gitm_vert.h"
+#include "vert_column.h"
+
+namespace gv {
+
+inline void default_params(gitm_vert_params *p) {
+  memset(p, 0, sizeof(*p));
+  p->dt = 2.0;
+  p->omega_body = 2.0 * 3.141592653589793 / (24.0 * 3600.0);  // ModInputs.f90:285
+  p->max_vertical_velocity = 1000.0;                           // :224
+  p->beta_limiter = 2.0;                                       // :104
+  p->min_tec = 2.0;                                            // :261
+  p->vert_tau = 1.0e9;                                         // ModEarth.f90:325
+  // init_planet, ModEarth.f90:271-281
+  const double amu = 1.6726e-27, mo = 16.0 * amu, mn = 14.0 * amu;
+  p->mass[0] = mo;
+  p->mass[1] = 2.0 * mo;
+  p->mass[2] = 2.0 * mn;
+  p->mass[3] = mn;
+  p->mass[4] = mn + mo;
+  p->mass[5] = 4.0 * amu;
+  p->thermal_diff_coef[5] = -0.38;  // calc_rates.Earth.f90:83-84
+  p->use_neutral_friction = 1;
+  p->use_coriolis = 1;
+  p->use_ion_advection = 0;
+  p->use_improved_ion_advection = 1;
+  p->use_nighttime_ion_bcs = 1;
+}
+
+inline bool field_ok(const gitm_vert_field &f) { return f.ptr != nullptr; }
+
+// Returns GITM_VERT_OK or the error code for unusable arguments.
+inline int check_args(const gitm_vert_params *p, const gitm_vert_columns *c,
+                      int solver, int n_steps) {
+  if (!p || !c) return GITM_VERT_EINVAL;
+  if (solver != GITM_VERT_AUSM && solver != GITM_VERT_RUSANOV) return GITM_VERT_EINVAL;
+  if (n_steps < 0) return GITM_VERT_EINVAL;
+  if (c->n_alts < 6 || c->n_alts > GITM_VERT_MAX_NALTS) return GITM_VERT_EINVAL;
+  if (c->n_lons < 1 || c->n_lats < 1 || c->n_blocks < 1) return GITM_VERT_EINVAL;
+  if ((long long)c->n_lons * c->n_lats * c->n_blocks > 0x7fffffffLL) return GITM_VERT_EINVAL;
+  if (p->use_msis || p->use_gswm_tides || p->use_waccm_tides || p->use_bc_perturbation ||
+      p->during_perturb || p->use_damping)
+    return GITM_VERT_EUNSUPPORTED;
+  if (p->use_ion_advection) return GITM_VERT_EUNSUPPORTED;
+  if (!(p->dt > 0.0) || !(p->vert_tau != 0.0)) return GITM_VERT_EINVAL;
+  for (int s = 0; s < NSP; s++)
+    if (!(p->mass[s] > 0.0)) return GITM_VERT_EINVAL;
+  const gitm_vert_field *req[] = {&c->ndensity_s, &c->vertical_velocity, &c->velocity,
+                                  &c->temperature, &c->rho, &c->temp_unit,
+                                  &c->mean_major_mass, &c->gamma, &c->cmax, &c->eddy,
+                                  &c->altitude, &c->gravity, &c->inv_radial_distance,
+                                  &c->dalt, &c->cos_latitude};
+  for (const gitm_vert_field *f : req)
+    if (!field_ok(*f)) return GITM_VERT_EINVAL;
+  if (c->cmax.first_level < -1 || c->cmax.first_level > 0) return GITM_VERT_EINVAL;
+  return GITM_VERT_OK;
+}
+
+inline KParams make_kparams(const gitm_vert_params *p, const gitm_vert_columns *c,
+                            int solver, int n_steps, unsigned long long *err) {
+  KParams k;
+  memset(&k, 0, sizeof(k));
+  k.dt = p->dt;
+  k.beta = p->beta_limiter;
+  k.maxvv = p->max_vertical_velocity;
+  k.min_tec = p->min_tec;
+  k.vert_tau = p->vert_tau;
+  k.omega = p->omega_body;
+  for (int s = 0; s < NSP; s++) {
+    k.mass[s] = p->mass[s];
+    k.tdc[s] = p->thermal_diff_coef[s];
+  }
+  // alog(MinKpS) etc., vertical_solver_ausm.f90:1317-1332
+  k.log_min_kp = log(1.0e-19);
+  k.log_max_kp = log(1.0e-09);
+  k.log_min_ku = log(0.25);
+  k.log_max_ku = log(0.50);
+  k.n_alts = c->n_alts;
+  k.n_steps = n_steps;
+  k.solver = solver;
+  k.friction = p->use_neutral_friction != 0;
+  k.coriolis = p->use_coriolis != 0;
+  k.ions = p->use_ion_advection != 0;
+  k.improved_ions = p->use_improved_ion_advection != 0;
+  k.nighttime_ion_bcs = p->use_nighttime_ion_bcs != 0;
+  k.err = err;
+  return k;
+}
+
+// Smallest workgroup (multiple of 64 threads) holding the nAlts+4 levels.
+inline int auto_waves(int n_alts) { return (n_alts + 4 + 63) / 64; }
+
+inline void decode_error(unsigned long long key, gitm_vert_error *e) {
+  if (key == ~0ULL) {
+    e->code = GITM_VERT_OK;
+    e->column = -1;
+    e->level = 0;
+  } else {
+    e->column = (int)(key >> 32);
+    e->code = (int)((key >> 16) & 0xffff);
+    e->level = (int)(key & 0xffff) - 1024;
+  }
+  e->reserved = 0;
+}
+
+}  // namespace gv

@@ -1,0 +1,207 @@
+"""Host-side interface of the MI355X vertical step.
+
+Mirrors GITM's ``advance_vertical_all`` (src/advance_vertical.f90:8-39): one
+call advances every column it is given by one (or ``n_steps``) vertical
+steps with the solver ``UseAUSMSolver`` selects (src/advance_vertical.f90
+:157-161).  Stepping errors raise ``VerticalStepError``, the counterpart of the
+reference's ``stop_gitm`` calls.
+
+Arrays use the column-contiguous layout of :mod:`gitm_amd.synth`
+(``[column][component][level]``, levels -1..nAlts+2).  ``advance`` works on
+device-resident torch tensors (FP64, ``device="cuda"``, which is the HIP
+device under ROCm) and queues the kernel on torch's current stream;
+``advance_host`` takes numpy arrays and copies through the library.
+
+The HIP library (``libgitm_vert.so``) must be built (``python -m
+gitm_amd.build``); there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import os
+
+import numpy as np
+
+from . import _abi
+from .synth import StepParams
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgitm_vert.so")
+
+STATE = ("nds", "vvel", "vel", "temp", "rho")
+OUTPUTS = ("logns", "nden")
+INPUTS = ("tempunit", "mmm", "gamma", "cmax", "eddy", "alt", "grav", "invr", "dalt",
+          "coslat")
+
+_lib = None
+
+
+class VerticalStepError(RuntimeError):
+    def __init__(self, code: int, column: int = -1, level: int = 0):
+        self.code, self.column, self.level = code, column, level
+        where = f" (column {column}, level {level})" if column >= 0 else ""
+        super().__init__(f"{_abi.ERRORS.get(code, code)}{where}")
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                               "`python -m gitm_amd.build`")
+        # torch ships its own HIP runtime with the same soname
+        # (libamdhip64.so.7).  Load it first so that the library binds to the
+        # runtime torch's tensors and streams live in; one process must not
+        # carry two HIP runtimes.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        _lib = ctypes.CDLL(LIB_PATH)
+        for name in ("gitm_vert_create", "gitm_vert_destroy", "gitm_vert_set_stream",
+                     "gitm_vert_advance", "gitm_vert_advance_host", "gitm_vert_get_error",
+                     "gitm_vert_set_tile"):
+            getattr(_lib, name).restype = ctypes.c_int
+        _lib.gitm_vert_strerror.restype = ctypes.c_char_p
+        _lib.gitm_vert_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]
+        _lib.gitm_vert_destroy.argtypes = [ctypes.c_void_p]
+        _lib.gitm_vert_set_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _lib.gitm_vert_set_tile.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        _lib.gitm_vert_get_error.argtypes = [ctypes.c_void_p, ctypes.POINTER(_abi.Error)]
+        for name in ("gitm_vert_advance", "gitm_vert_advance_host"):
+            getattr(_lib, name).argtypes = [ctypes.c_void_p, ctypes.POINTER(_abi.Params),
+                                            ctypes.POINTER(_abi.Columns), ctypes.c_int,
+                                            ctypes.c_int]
+    return _lib
+
+
+def make_params(sp: StepParams | None = None) -> _abi.Params:
+    sp = sp or StepParams()
+    p = _abi.Params()
+    lib().gitm_vert_default_params(ctypes.byref(p))
+    p.dt = sp.dt
+    p.omega_body = sp.omega_body
+    p.max_vertical_velocity = sp.max_vert_vel
+    p.beta_limiter = sp.beta_limiter
+    p.min_tec = sp.min_tec
+    p.vert_tau = sp.vert_tau
+    p.use_neutral_friction = int(sp.use_neutral_friction)
+    p.use_coriolis = int(sp.use_coriolis)
+    p.use_ion_advection = int(sp.use_ion_advection)
+    p.use_improved_ion_advection = int(sp.use_improved_ion_advection)
+    p.use_nighttime_ion_bcs = int(sp.use_nighttime_ion_bcs)
+    return p
+
+
+def solver_id(solver) -> int:
+    if isinstance(solver, str):
+        return {"ausm": _abi.AUSM, "rusanov": _abi.RUSANOV}[solver.lower()]
+    return int(solver)
+
+
+class VerticalSolver:
+    """A handle on one HIP device (``gitm_vert_create``)."""
+
+    def __init__(self, device: int = -1, waves_per_column: int = 0):
+        self._h = ctypes.c_void_p()
+        rc = lib().gitm_vert_create(ctypes.byref(self._h), device)
+        if rc:
+            raise VerticalStepError(rc)
+        if waves_per_column:
+            self.set_tile(waves_per_column)
+
+    def close(self) -> None:
+        if self._h:
+            lib().gitm_vert_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tile(self, waves_per_column: int) -> None:
+        rc = lib().gitm_vert_set_tile(self._h, waves_per_column)
+        if rc:
+            raise VerticalStepError(rc)
+
+    def set_stream(self, stream_handle: int) -> None:
+        lib().gitm_vert_set_stream(self._h, ctypes.c_void_p(stream_handle))
+
+    def check(self) -> None:
+        """Synchronise and raise the first stepping error, if any."""
+        e = _abi.Error()
+        rc = lib().gitm_vert_get_error(self._h, ctypes.byref(e))
+        if rc:
+            raise VerticalStepError(rc, e.column, e.level)
+
+    # -- device-resident torch tensors -------------------------------------
+    def advance(self, cols: dict, params: StepParams | None = None, solver="ausm",
+                n_steps: int = 1, check: bool = True) -> None:
+        """Advance device-resident columns in place (torch FP64 tensors).
+
+        ``cols`` holds the STATE and INPUTS tensors (optionally "logns" and
+        "nden" outputs) in the column-contiguous layout.
+        """
+        import torch
+
+        ptrs = {}
+        for k, t in cols.items():
+            if k not in _abi.CANON and k not in _abi.PER_COLUMN:
+                continue
+            if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+                raise ValueError(f"{k}: need a contiguous float64 device tensor")
+            ptrs[k] = t.data_ptr()
+        nc, L = cols["temp"].shape
+        c = _abi.columns_struct(ptrs, nc, L - 4)
+        p = make_params(params)
+        self.set_stream(torch.cuda.current_stream().cuda_stream)
+        rc = lib().gitm_vert_advance(self._h, ctypes.byref(p), ctypes.byref(c),
+                                     solver_id(solver), n_steps)
+        if rc:
+            raise VerticalStepError(rc)
+        if check:
+            self.check()
+
+    # -- host numpy arrays --------------------------------------------------
+    def advance_host(self, cols: dict, params: StepParams | None = None,
+                     solver="ausm", n_steps: int = 1) -> None:
+        """Advance host columns in place (numpy FP64, C-contiguous)."""
+        ptrs = {}
+        for k, a in cols.items():
+            if k not in _abi.CANON and k not in _abi.PER_COLUMN:
+                continue
+            if a.dtype != np.float64 or not a.flags.c_contiguous:
+                raise ValueError(f"{k}: need a C-contiguous float64 array")
+            ptrs[k] = a.ctypes.data
+        nc, L = cols["temp"].shape
+        c = _abi.columns_struct(ptrs, nc, L - 4)
+        p = make_params(params)
+        rc = lib().gitm_vert_advance_host(self._h, ctypes.byref(p), ctypes.byref(c),
+                                          solver_id(solver), n_steps)
+        if rc:
+            e = _abi.Error()
+            raise VerticalStepError(rc)
+
+
+def to_device(cols: dict, device="cuda") -> dict:
+    """Copy the state and input arrays of ``cols`` to the device, adding zeroed
+    LogNS / NDensity outputs."""
+    import torch
+
+    out = {}
+    for k in STATE + INPUTS:
+        out[k] = torch.from_numpy(np.ascontiguousarray(cols[k], dtype=np.float64)).to(device)
+    out["logns"] = torch.zeros_like(out["nds"])
+    out["nden"] = torch.zeros_like(out["temp"])
+    return out
+
+
+def to_host(cols: dict) -> dict:
+    return {k: v.detach().cpu().numpy() for k, v in cols.items()}
+
+
+def replace_params(sp: StepParams, **kw) -> StepParams:
+    return dataclasses.replace(sp, **kw)

@@ -1,0 +1,172 @@
+/*
+ * gitm_vert.h -- C ABI of the MI355X implementation of GITM's vertical
+ * dynamics step (HIP, gfx950).
+ *
+ * The reference path is a set of argument-less Fortran subroutines that talk
+ * through module globals:
+ *
+ *   advance_vertical_all          src/advance_vertical.f90:8-39
+ *     advance_vertical(iLon,iLat,iBlock)                 :45-219
+ *       advance_vertical_1d_ausm  src/vertical_solver_ausm.f90:26-433
+ *       advance_vertical_1d_rusanov src/vertical_solver_rusanov.f90:10-384
+ *
+ * One call of gitm_vert_advance replaces one call of advance_vertical_all for
+ * every column it is given: it gathers each column from the caller's arrays
+ * (described by strides, so GITM's own (-1:nLons+2,-1:nLats+2,-1:nAlts+2,..)
+ * module arrays can be passed as they are), runs the RK4 vertical step with
+ * the solver UseAUSMSolver selects (src/advance_vertical.f90:157-161), and
+ * scatters the result back exactly as advance_vertical does (Rho = exp(LogRho),
+ * Temperature = Temp/TempUnit, NDensityS = exp(LogNS), NDensity = sum, ...).
+ * Plain pointers and sizes only; no C++ or torch types cross this boundary.
+ */
+#ifndef GITM_VERT_H
+#define GITM_VERT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GITM_VERT_NSPECIES 6       /* ModEarth.f90:18 nSpecies            */
+#define GITM_VERT_NIONS 10         /* ModEarth.f90:38 nIons (incl. e-)     */
+#define GITM_VERT_MAX_NALTS 252    /* levels -1..nAlts+2 fit 256 lanes     */
+
+/* Solver selector: UseAUSMSolver (ModInputs.f90:242, #AUSMSOLVER). */
+enum { GITM_VERT_RUSANOV = 0, GITM_VERT_AUSM = 1 };
+
+/* Status codes.  The reference stops the run (stop_gitm) where these are
+ * returned; the caller is expected to do the same. */
+enum {
+  GITM_VERT_OK = 0,
+  GITM_VERT_ENONPOSITIVE = 1, /* NDensityS <= 0 entering a column
+                                 (advance_vertical.f90:96-100)            */
+  GITM_VERT_ESINGULAR = 2,    /* zero row in the friction matrix
+                                 (ludcmp.f:14-17)                          */
+  GITM_VERT_ENEGION = 3,      /* negative ion density too close to the top
+                                 (set_vertical_bcs.Earth.f90:617-620)      */
+  GITM_VERT_EINVAL = 4,       /* bad arguments                             */
+  GITM_VERT_EUNSUPPORTED = 5, /* a flag this library does not implement
+                                 (UseMSIS, tides, BC perturbation,
+                                 UseDamping): run the Fortran path         */
+  GITM_VERT_EHIP = 6,         /* HIP runtime error                         */
+  GITM_VERT_ENOMEM = 7
+};
+
+/* One array of the column working set.  Element (column c, level k,
+ * component m) lives at
+ *   ptr[ ilon*s_lon + ilat*s_lat + iblk*s_blk + (k - first_level)*s_lev
+ *        + m*s_comp ]
+ * where c = ilon + n_lons*(ilat + n_lats*iblk) (0-based) and k runs over
+ * GITM's level index (-1..nAlts+2).  Strides are in elements (doubles). */
+typedef struct gitm_vert_field {
+  double *ptr;
+  int64_t s_lon, s_lat, s_blk;
+  int64_t s_lev;
+  int64_t s_comp;
+  int32_t first_level; /* -1 for (-1:nAlts+2) arrays, 0 for cMax_GDB      */
+  int32_t reserved;
+} gitm_vert_field;
+
+typedef struct gitm_vert_columns {
+  int32_t n_lons, n_lats, n_blocks; /* columns = n_lons*n_lats*n_blocks     */
+  int32_t n_alts;                   /* nAlts (ModSize); levels -1..nAlts+2  */
+  /* state, read and written (advance_vertical.f90:105-118, 163-185) */
+  gitm_vert_field ndensity_s;        /* NDensityS(.., 1:6, ..)     [m^-3]   */
+  gitm_vert_field vertical_velocity; /* VerticalVelocity(.., 1:6, ..)       */
+  gitm_vert_field velocity;          /* Velocity(.., 1:3, ..)  E, N, Up     */
+  gitm_vert_field temperature;       /* Temperature (divided by TempUnit)   */
+  gitm_vert_field rho;               /* Rho                                 */
+  /* outputs only; ptr may be NULL */
+  gitm_vert_field log_ns;            /* LogNS(.., 1:6, ..)                  */
+  gitm_vert_field ndensity;          /* NDensity                            */
+  /* column inputs, read only */
+  gitm_vert_field temp_unit;         /* TempUnit (calc_rates)               */
+  gitm_vert_field mean_major_mass;   /* MeanMajorMass (calc_rates)          */
+  gitm_vert_field gamma;             /* Gamma (calc_pressure)               */
+  gitm_vert_field cmax;              /* cMax_GDB(.., iUp_, ..), levels 0..nAlts+1 */
+  gitm_vert_field eddy;              /* KappaEddyDiffusion                  */
+  gitm_vert_field altitude;          /* Altitude_GB                         */
+  gitm_vert_field gravity;           /* Gravity_GB                          */
+  gitm_vert_field inv_radial_distance; /* InvRadialDistance_GB             */
+  gitm_vert_field dalt;              /* dAlt_GB                             */
+  gitm_vert_field cos_latitude;      /* CosLatitude(iLat, iBlock); s_lev unused */
+  /* ions; used only when use_ion_advection != 0 */
+  gitm_vert_field idensity_s;        /* IDensityS(.., 1:10, ..)  read/written */
+  gitm_vert_field ivelocity;         /* IVelocity(.., 1:3, ..)  read        */
+  gitm_vert_field sza;               /* SZA(iLon,iLat,iBlock)               */
+  gitm_vert_field mlat;              /* MLatitude(iLon,iLat,nAlts,iBlock)   */
+} gitm_vert_columns;
+
+/* Run parameters (ModInputs / ModPlanet values the path reads). */
+typedef struct gitm_vert_params {
+  double dt;                    /* Dt                                        */
+  double omega_body;            /* OmegaBodyInput        ModInputs.f90:285   */
+  double max_vertical_velocity; /* MaximumVerticalVelocity       :224        */
+  double beta_limiter;          /* BetaLimiter                   :104        */
+  double min_tec;               /* MinTEC                        :261        */
+  double vert_tau;              /* VertTau(:) (ModEarth.f90:325), Rusanov    */
+  double mass[GITM_VERT_NSPECIES];              /* Mass(1:6) [kg]           */
+  double thermal_diff_coef[GITM_VERT_NSPECIES]; /* ThermalDiffCoefS         */
+  int32_t use_neutral_friction;       /* ModInputs.f90:240                  */
+  int32_t use_coriolis;               /* :235                               */
+  int32_t use_ion_advection;          /* :311                               */
+  int32_t use_improved_ion_advection; /* :259                               */
+  int32_t use_nighttime_ion_bcs;      /* :260                               */
+  /* Must be 0 (otherwise GITM_VERT_EUNSUPPORTED): */
+  int32_t use_msis;                   /* :190                               */
+  int32_t use_gswm_tides, use_waccm_tides; /* :195-196                      */
+  int32_t use_bc_perturbation;        /* :402                               */
+  int32_t during_perturb;             /* :121                               */
+  int32_t use_damping;                /* :221                               */
+  int32_t reserved;
+} gitm_vert_params;
+
+typedef struct gitm_vert_error {
+  int32_t code;   /* GITM_VERT_* of the first failure                      */
+  int32_t column; /* first failing column (0-based), -1 if none            */
+  int32_t level;  /* level index where known, else -99                     */
+  int32_t reserved;
+} gitm_vert_error;
+
+typedef struct gitm_vert_handle gitm_vert_handle;
+
+/* Fill `p` with the reference defaults (ModInputs.f90, ModEarth.f90
+ * init_planet masses, calc_rates.Earth.f90:83-84 ThermalDiffCoefS). */
+void gitm_vert_default_params(gitm_vert_params *p);
+
+/* Create a handle bound to HIP device `device` (the current device when
+ * device < 0).  Work is queued on the handle's stream (the null stream
+ * until gitm_vert_set_stream). */
+int gitm_vert_create(gitm_vert_handle **out, int device);
+int gitm_vert_destroy(gitm_vert_handle *h);
+int gitm_vert_set_stream(gitm_vert_handle *h, void *hip_stream);
+
+/* Advance every column by n_steps vertical steps (each one a full
+ * advance_vertical gather/solve/scatter) with `solver`.  All pointers in
+ * `c` are device pointers.  Asynchronous: work is queued on the handle's
+ * stream; argument errors are returned at once, errors found while
+ * stepping are reported by gitm_vert_get_error. */
+int gitm_vert_advance(gitm_vert_handle *h, const gitm_vert_params *p,
+                      const gitm_vert_columns *c, int solver, int n_steps);
+
+/* Same, for host arrays (the Fortran drop-in): copies the fields in,
+ * advances, copies the written fields back and synchronises.  Returns the
+ * stepping status (as gitm_vert_get_error would). */
+int gitm_vert_advance_host(gitm_vert_handle *h, const gitm_vert_params *p,
+                           const gitm_vert_columns *c, int solver, int n_steps);
+
+/* Synchronise the handle's stream and return (and clear) the first error
+ * recorded by the kernels since the last call. */
+int gitm_vert_get_error(gitm_vert_handle *h, gitm_vert_error *err);
+
+/* Tile choice: waves per column (1..4) for the next launches; 0 restores
+ * the automatic choice (smallest that holds nAlts+4 levels). */
+int gitm_vert_set_tile(gitm_vert_handle *h, int waves_per_column);
+
+const char *gitm_vert_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -292,6 +292,22 @@ static int neutral_friction(const ov_col *c, double dtin, double vel[NSP][LM],
   return OV_OK;
 }
 
+/* Weights of the 5-point non-uniform derivative at the centre point, with
+ * h1..h4 the spacings between the five points (vertical_solver_ausm.f90
+ * :657-676). */
+void ov_glc_weights(double h1, double h2, double h3, double h4, double w[5]) {
+  double MeshH2 = h2 + h1;
+  double MeshH3 = h3 + h2 + h1;
+  double MeshH4 = h4 + h3 + h2 + h1;
+  w[0] = (h2 * h3 * (h3 + h4)) / (h1 * MeshH2 * MeshH3 * MeshH4);
+  w[1] = -1.0 * (MeshH2 * h3 * (h3 + h4)) / (h1 * h2 * (h2 + h3) * (h2 + h3 + h4));
+  w[3] = MeshH2 * h2 * (h4 + h3) / (MeshH3 * (h2 + h3) * h3 * h4);
+  w[4] = -1.0 * MeshH2 * h2 * h3 / (MeshH4 * (h2 + h3 + h4) * (h3 + h4) * h4);
+  w[2] = (h2 * h3 * (h3 + h4) + MeshH2 * h3 * (h3 + h4) - MeshH2 * h2 * (h3 + h4) -
+          MeshH2 * h2 * h3) /
+         (MeshH2 * h2 * h3 * (h3 + h4));
+}
+
 /* 5-point non-uniform gradient of ln(rho_s/rho),
  * vertical_solver_ausm.f90:649-688 (vertical_solver_rusanov.f90:510-545). */
 static void grad_log_cons(const ov_col *c, const ov_state *u, double ns[NSP][LM],
@@ -303,23 +319,13 @@ static void grad_log_cons(const ov_col *c, const ov_state *u, double ns[NSP][LM]
       lc[sp][X(k)] = log(c->mass[sp] * ns[sp][X(k)] / rho[X(k)]);
   (void)u;
   for (i = 1; i <= nA; i++) {
-    double h1 = c->dAltF[X(i - 1)], h2 = c->dAltF[X(i)];
-    double h3 = c->dAltF[X(i + 1)], h4 = c->dAltF[X(i + 2)];
-    double MeshH2 = h2 + h1;
-    double MeshH3 = h3 + h2 + h1;
-    double MeshH4 = h4 + h3 + h2 + h1;
-    double C0 = (h2 * h3 * (h3 + h4)) / (h1 * MeshH2 * MeshH3 * MeshH4);
-    double C1 = -1.0 * (MeshH2 * h3 * (h3 + h4)) /
-                (h1 * h2 * (h2 + h3) * (h2 + h3 + h4));
-    double C3 = MeshH2 * h2 * (h4 + h3) / (MeshH3 * (h2 + h3) * h3 * h4);
-    double C4 = -1.0 * MeshH2 * h2 * h3 / (MeshH4 * (h2 + h3 + h4) * (h3 + h4) * h4);
-    double C2 = (h2 * h3 * (h3 + h4) + MeshH2 * h3 * (h3 + h4) -
-                 MeshH2 * h2 * (h3 + h4) - MeshH2 * h2 * h3) /
-                (MeshH2 * h2 * h3 * (h3 + h4));
+    double C[5];
+    ov_glc_weights(c->dAltF[X(i - 1)], c->dAltF[X(i)], c->dAltF[X(i + 1)],
+                   c->dAltF[X(i + 2)], C);
     for (sp = 0; sp < NSP; sp++)
-      glc[sp][i] = C0 * lc[sp][X(i - 2)] + C1 * lc[sp][X(i - 1)] +
-                   C2 * lc[sp][X(i)] + C3 * lc[sp][X(i + 1)] +
-                   C4 * lc[sp][X(i + 2)];
+      glc[sp][i] = C[0] * lc[sp][X(i - 2)] + C[1] * lc[sp][X(i - 1)] +
+                   C[2] * lc[sp][X(i)] + C[3] * lc[sp][X(i + 1)] +
+                   C[4] * lc[sp][X(i + 2)];
   }
 }
 
@@ -348,6 +354,20 @@ static void bc_bwd_coefs(const double *dF, int k, double C[5]) {
   C[2] = 1.0 * (H1 * H3 * H4) / (H2 * h2 * h3 * (h3 + h4));
   C[3] = -1.0 * (H1 * H2 * H4) / (H3 * (h3 + h2) * h3 * h4);
   C[4] = 1.0 * (H1 * H2 * H3) / (H4 * (h2 + h3 + h4) * (h3 + h4) * h4);
+}
+
+void ov_bc_coefs(double h1, double h2, double h3, double h4, int backward,
+                 double C[5]) {
+  /* forward (:228-248) and backward (:505-525) one-sided weights; the
+   * point being filled is at distance h1 from the first of the five */
+  double dF[LM];
+  if (!backward) {
+    dF[X(2)] = h1; dF[X(3)] = h2; dF[X(4)] = h3; dF[X(5)] = h4;
+    bc_fwd_coefs(dF, 0, C);
+  } else {
+    dF[X(9)] = h1; dF[X(8)] = h2; dF[X(7)] = h3; dF[X(6)] = h4;
+    bc_bwd_coefs(dF, 10, C);
+  }
 }
 
 static int set_vertical_bcs(ov_col *c, const ov_params *p, ov_state *s) {

@@ -97,6 +97,9 @@ void ov_kt_facevalues(int n_alts, const double *dalt_f, const double *inv_dalt_f
                       double beta, const double *var, double *left_m12,
                       double *right_m12, double *left_p12, double *right_p12);
 void ov_species_masses(double mass[OV_NSPECIES]);
+void ov_glc_weights(double h1, double h2, double h3, double h4, double w[5]);
+void ov_bc_coefs(double h1, double h2, double h3, double h4, int backward,
+                 double C[5]);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,81 @@
+"""TEST INFRASTRUCTURE ONLY: runs the host build of the column kernels."""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from gitm_amd import _abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+LIB = os.path.join(HERE, "libgv_emu.so")
+SRC = os.path.join(HERE, "emu_vert.cpp")
+DEPS = [SRC] + [os.path.join(ROOT, "gitm_amd", "csrc", f)
+                for f in ("vert_column.h", "vert_params.h")] + \
+    [os.path.join(ROOT, "include", "gitm_vert.h")]
+
+_lib = None
+
+
+def build(force: bool = False) -> None:
+    if (not force and os.path.exists(LIB) and
+            all(os.path.getmtime(LIB) >= os.path.getmtime(d) for d in DEPS)):
+        return
+    subprocess.run(["g++", "-std=c++20", "-O2", "-fPIC", "-shared", "-ffp-contract=off",
+                    "-pthread", "-I" + os.path.join(ROOT, "include"),
+                    "-I" + os.path.join(ROOT, "gitm_amd", "csrc"), SRC, "-o", LIB],
+                   check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        _lib = ctypes.CDLL(LIB)
+        _lib.gv_emu_advance.restype = ctypes.c_int
+    return _lib
+
+
+def params_from(sp) -> _abi.Params:
+    p = _abi.Params()
+    lib().gv_emu_default_params(ctypes.byref(p))
+    p.dt = sp.dt
+    p.omega_body = sp.omega_body
+    p.max_vertical_velocity = sp.max_vert_vel
+    p.beta_limiter = sp.beta_limiter
+    p.min_tec = sp.min_tec
+    p.vert_tau = sp.vert_tau
+    p.use_neutral_friction = int(sp.use_neutral_friction)
+    p.use_coriolis = int(sp.use_coriolis)
+    p.use_ion_advection = int(sp.use_ion_advection)
+    p.use_improved_ion_advection = int(sp.use_improved_ion_advection)
+    p.use_nighttime_ion_bcs = int(sp.use_nighttime_ion_bcs)
+    return p
+
+
+def advance(cols: dict, sp, solver: int = 1, n_steps: int = 1, waves: int = 0,
+            cols_range=None) -> dict:
+    nc, L = cols["temp"].shape
+    na = L - 4
+    out = {k: np.ascontiguousarray(np.array(cols[k], dtype=np.float64))
+           for k in ("nds", "vvel", "vel", "temp", "rho")}
+    out["logns"] = np.zeros_like(out["nds"])
+    out["nden"] = np.zeros_like(out["temp"])
+    keep = {k: np.ascontiguousarray(cols[k], dtype=np.float64)
+            for k in ("tempunit", "mmm", "gamma", "cmax", "eddy", "alt", "grav", "invr",
+                      "dalt", "coslat")}
+    ptrs = {k: v.ctypes.data for k, v in out.items()}
+    ptrs.update({k: v.ctypes.data for k, v in keep.items()})
+    c = _abi.columns_struct(ptrs, nc, na)
+    p = params_from(sp)
+    e = _abi.Error()
+    c0, c1 = cols_range if cols_range else (0, nc)
+    rc = lib().gv_emu_advance(ctypes.byref(p), ctypes.byref(c), solver, n_steps, waves,
+                              c0, c1, ctypes.byref(e))
+    if rc:
+        raise RuntimeError(f"emulated kernel: {_abi.ERRORS.get(rc, rc)} "
+                           f"(column {e.column}, level {e.level})")
+    return out

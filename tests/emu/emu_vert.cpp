@@ -1,0 +1,93 @@
+// emu_vert.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// Host build of the column kernels in gitm_amd/csrc/vert_column.h: each
+// workgroup thread runs as a CPU thread, __syncthreads() is a std::barrier
+// and the LDS block is an ordinary array.  With glibc's exp/log on both
+// sides this checks the kernel logic (indexing, exchanges, operation order)
+// against the CPU oracle bit for bit, without a GPU.  Nothing in the
+// product library loads this code.
+#define GV_HOST_EMULATION 1
+#include <atomic>
+#include <barrier>
+#include <thread>
+#include <vector>
+
+#include "vert_params.h"
+
+namespace {
+
+struct Shared {
+  std::barrier<> bar;
+  std::atomic<int> orflag{0};
+  std::atomic<unsigned long long> err{~0ULL};
+  explicit Shared(int n) : bar(n) {}
+};
+
+struct EmuCtx {
+  int lane_;
+  Shared *sh;
+  int lane() const { return lane_; }
+  void sync() { sh->bar.arrive_and_wait(); }
+  bool sync_or(bool p) {
+    if (p) sh->orflag.store(1);
+    sync();
+    bool r = sh->orflag.load() != 0;
+    sync();
+    if (lane_ == 0) sh->orflag.store(0);
+    sync();
+    return r;
+  }
+  void report(unsigned long long key) {
+    unsigned long long cur = sh->err.load();
+    while (key < cur && !sh->err.compare_exchange_weak(cur, key)) {
+    }
+  }
+};
+
+template <int NT>
+unsigned long long run_all(const gv::KParams &kp, const gitm_vert_columns &io, int c0,
+                           int c1) {
+  const int solver = kp.solver;
+  Shared sh(NT);
+  std::vector<double> lds(gv::LdsSize<NT>::bytes / sizeof(double) + 16, 0.0);
+  std::vector<std::thread> th;
+  for (int t = 0; t < NT; t++)
+    th.emplace_back([&, t] {
+      EmuCtx cx{t, &sh};
+      for (int c = c0; c < c1; c++) {
+        if (solver == GITM_VERT_AUSM)
+          gv::run_column<NT, GITM_VERT_AUSM>(cx, kp, io, c, lds.data());
+        else
+          gv::run_column<NT, GITM_VERT_RUSANOV>(cx, kp, io, c, lds.data());
+        cx.sync();
+      }
+    });
+  for (auto &x : th) x.join();
+  return sh.err.load();
+}
+
+}  // namespace
+
+extern "C" int gv_emu_advance(const gitm_vert_params *p, const gitm_vert_columns *c,
+                              int solver, int n_steps, int waves, int c0, int c1,
+                              gitm_vert_error *e) {
+  int rc = gv::check_args(p, c, solver, n_steps);
+  if (rc) return rc;
+  gv::KParams kp = gv::make_kparams(p, c, solver, n_steps, nullptr);
+  if (waves <= 0) waves = gv::auto_waves(c->n_alts);
+  if (64 * waves < c->n_alts + 4) return GITM_VERT_EINVAL;
+  unsigned long long key;
+  switch (waves) {
+    case 1: key = run_all<64>(kp, *c, c0, c1); break;
+    case 2: key = run_all<128>(kp, *c, c0, c1); break;
+    case 3: key = run_all<192>(kp, *c, c0, c1); break;
+    case 4: key = run_all<256>(kp, *c, c0, c1); break;
+    default: return GITM_VERT_EINVAL;
+  }
+  gitm_vert_error tmp;
+  if (!e) e = &tmp;
+  gv::decode_error(key, e);
+  return e->code;
+}
+
+extern "C" void gv_emu_default_params(gitm_vert_params *p) { gv::default_params(p); }
