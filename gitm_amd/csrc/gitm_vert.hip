@@ -30,15 +30,18 @@ __global__ void __launch_bounds__(NT) vert_column_kernel(gv::KParams kp, gitm_ve
   gv::run_column<NT, SOLVER>(cx, kp, io, (int)blockIdx.x, lds);
 }
 
+// One thread per level: blockDim = nAlts+4 (<= NT, the compile-time bound).
 template <int NT>
 void launch(int solver, dim3 grid, hipStream_t stream, const gv::KParams &kp,
             const gitm_vert_columns &c) {
+  const dim3 block(kp.n_alts + 4);
+  const size_t lds = gv::lds_bytes(kp.n_alts);
   if (solver == GITM_VERT_AUSM)
-    hipLaunchKernelGGL((vert_column_kernel<NT, GITM_VERT_AUSM>), grid, dim3(NT),
-                       gv::LdsSize<NT>::bytes, stream, kp, c);
+    hipLaunchKernelGGL((vert_column_kernel<NT, GITM_VERT_AUSM>), grid, block, lds, stream,
+                       kp, c);
   else
-    hipLaunchKernelGGL((vert_column_kernel<NT, GITM_VERT_RUSANOV>), grid, dim3(NT),
-                       gv::LdsSize<NT>::bytes, stream, kp, c);
+    hipLaunchKernelGGL((vert_column_kernel<NT, GITM_VERT_RUSANOV>), grid, block, lds,
+                       stream, kp, c);
 }
 
 __global__ void reset_error_word(unsigned long long *e) { *e = ~0ULL; }
@@ -50,6 +53,9 @@ struct gitm_vert_handle {
   hipStream_t stream = nullptr;
   unsigned long long *d_err = nullptr;
   int waves = 0;
+  // RK4 carries of the columns in flight (grown on demand, reused)
+  double *scratch = nullptr;
+  size_t scratch_bytes = 0;
   // device staging of the host-array entry point
   std::vector<void *> stage;
   std::vector<size_t> stage_bytes;
@@ -110,6 +116,7 @@ int gitm_vert_destroy(gitm_vert_handle *h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   else (void)hipDeviceSynchronize();
   for (void *p : h->stage) (void)hipFree(p);
+  if (h->scratch) (void)hipFree(h->scratch);
   (void)hipFree(h->d_err);
   delete h;
   return GITM_VERT_OK;
@@ -137,7 +144,20 @@ int gitm_vert_advance(gitm_vert_handle *h, const gitm_vert_params *p,
   int waves = h->waves ? h->waves : gv::auto_waves(c->n_alts);
   if (64 * waves < c->n_alts + 4) return GITM_VERT_EINVAL;
   GV_HIP(hipSetDevice(h->device));
+  const size_t need = (size_t)ncols * 2 * gv::NF * (size_t)(c->n_alts + 4) * sizeof(double);
+  if (h->scratch_bytes < need) {
+    if (h->scratch) {
+      // the previous launches on this handle's stream may still use it
+      GV_HIP(hipStreamSynchronize(h->stream));
+      (void)hipFree(h->scratch);
+      h->scratch = nullptr;
+      h->scratch_bytes = 0;
+    }
+    if (hipMalloc(&h->scratch, need) != hipSuccess) return GITM_VERT_ENOMEM;
+    h->scratch_bytes = need;
+  }
   gv::KParams kp = gv::make_kparams(p, c, solver, n_steps, h->d_err);
+  kp.scratch = h->scratch;
   const dim3 grid(ncols);
   switch (waves) {
     case 1: launch<64>(solver, grid, h->stream, kp, *c); break;

@@ -30,10 +30,12 @@
 #include <cmath>
 #define GV_FN static inline
 #define GV_MEM inline
+#define GV_HD static inline
 #else
 #include <hip/hip_runtime.h>
 #define GV_FN static __device__ __forceinline__
 #define GV_MEM __device__ __forceinline__
+#define GV_HD static __host__ __device__ inline
 #endif
 
 namespace gv {
@@ -83,6 +85,7 @@ struct KParams {
   int n_alts, n_steps, solver;
   int friction, coriolis, ions, improved_ions, nighttime_ion_bcs;
   unsigned long long *err;  // first-error word, see error_key()
+  double *scratch;          // RK4 carries: n_cols x (2*NF) x (nAlts+4) doubles
 };
 
 // Error word: the lowest (column, code, level) wins an atomic min.
@@ -95,24 +98,30 @@ GV_FN unsigned long long error_key(int col, int code, int level) {
 // LDS layout of one column: NSTAT static tables and NBUF exchange buffers of
 // pitch NT+4 (two guard cells on each side, so j-2..j+2 stays inside), a
 // scalar area and an int buffer.
-enum StatArray { A_DF = 0, A_IF, A_DC, A_CMF, A_KP, A_KU, NSTAT };
-constexpr int NBUF = 16;
+enum StatArray {
+  A_DF = 0, A_IF, A_DC, A_CMF, A_KP, A_KU,  // also read at neighbouring levels
+  A_GL1, A_GL2, A_GL3, A_GR1, A_GR2, A_GR3,  // gamma factors of face k|k+1
+  A_HF, A_AP, A_AM, A_VOL, A_INVR, A_EDDY, A_GRAV,
+  A_GLC0, A_GLC1, A_GLC2, A_GLC3, A_GLC4,
+  NSTAT
+};
+constexpr int NBUF = 21;
 enum Scalar {
   SC_ALT0 = 0, SC_MMM_M1, SC_MMM_0, SC_MG0, SC_MGT1, SC_MGT2,
-  SC_CA = 8,    // bottom extrapolation coefficients for level 0 (5)
-  SC_CB = 13,   // ... for level -1 (5)
-  SC_BOT = 18,  // bottom BC results: 6 species x 6 values
-  SC_TOP = 54,  // top BC results: 6 species x 4 values
-  NSCAL = 80
+  SC_CA = 6,    // bottom extrapolation coefficients for level 0 (5)
+  SC_CB = 11,   // ... for level -1 (5)
+  SC_BOT = 16,  // bottom BC results: 6 species x 6 values
+  SC_TOP = 52,  // top BC results: 6 species x 4 values
+  NSCAL = 76
 };
 
-template <int NT>
-struct LdsSize {
-  static constexpr int P = NT + 4;
-  static constexpr unsigned long bytes =
-      ((unsigned long)(NSTAT + NBUF) * P + NSCAL) * sizeof(double) +
-      (unsigned long)P * sizeof(int);
-};
+// LDS bytes of one column: the pitch is the column height plus guards, so a
+// 100-level column needs 39 KiB and four columns fit a 160 KiB CU.
+GV_HD constexpr unsigned long lds_bytes(int n_alts) {
+  return ((unsigned long)(NSTAT + NBUF) * (unsigned long)(n_alts + 8) + NSCAL) *
+             sizeof(double) +
+         (unsigned long)(n_alts + 8) * sizeof(int);
+}
 
 // ---------------------------------------------------------------------------
 // small numerics
@@ -273,15 +282,17 @@ GV_FN void bc_fwd_coefs(double h1, double h2, double h3, double h4, double *C) {
 }
 
 // Per-thread constants of the owned level.
+// Per-thread constants of the owned level kept in registers; the rest live
+// in the LDS tables of StatArray:
+//   GL*/GR*  gamma factors 1/(g-1), g/(g-1), 2(g-1)/(g+1) of face k|k+1
+//   HF       0.35*exp(-(z - z0)/45 km)                    (ausm :950)
+//   AP/AM/VOL face areas r^2 and cell volume (r^3)/3       (ausm :2029-2040)
+//   GLC*     5-point d ln(rho_s/rho)/dz weights            (ausm :659-676)
 struct LevelStatic {
   double eg;    // EffectiveGravity = g + Centrifugal/InvR (ausm :582-589)
   double mgh;   // 0.5*(eg(k) + eg(k-1))                    (ausm :697)
   double gm1;   // Gamma - 1
-  double gl1, gl2, gl3, gr1, gr2, gr3;  // gamma factors of face k|k+1
-  double hf;    // 0.35*exp(-(z - z0)/45 km)               (ausm :950)
-  double ap, am, vol;  // face areas r^2 and volume (r^3)/3  (ausm :2029-2040)
-  double invr, eddy, tu, grav;
-  double glc[5];  // 5-point d ln(rho_s/rho)/dz weights     (ausm :659-676)
+  double tu;    // TempUnit
 };
 
 // ---------------------------------------------------------------------------
@@ -289,20 +300,20 @@ struct LevelStatic {
 // ---------------------------------------------------------------------------
 template <int NT, class Ctx>
 struct Column {
-  static constexpr int P = NT + 4;
   Ctx &cx;
   const KParams &kp;
   double *lds;
-  const int j, k, nA, L;
+  const int j, k, nA, L, P;
   const bool lv;  // owns a level
   int col;
   LevelStatic S;
   double U[NF];
+  double *scr;  // this column's RK4 carries (Y, acc) in device memory
   double centrifugal, coriolis;
 
   GV_MEM Column(Ctx &c, const KParams &p, double *l, int column)
       : cx(c), kp(p), lds(l), j(c.lane()), k(c.lane() - 1), nA(p.n_alts),
-        L(p.n_alts + 4), lv(c.lane() < p.n_alts + 4), col(column) {}
+        L(p.n_alts + 4), P(p.n_alts + 8), lv(c.lane() < p.n_alts + 4), col(column) {}
 
   GV_MEM double &st(int a, int jj) const { return lds[a * P + jj + 2]; }
   GV_MEM double &xb(int b, int jj) const { return lds[(NSTAT + b) * P + jj + 2]; }
@@ -310,6 +321,52 @@ struct Column {
   GV_MEM int &ib(int jj) const {
     return reinterpret_cast<int *>(lds + (NSTAT + NBUF) * P + NSCAL)[jj + 2];
   }
+  // RK4 carries of the owned level: the stage-1 state Y and the running
+  // combination acc.  They are read and written only at the end of a stage,
+  // so they live in device memory ([2*NF][L] per column, coalesced) instead
+  // of registers.
+  GV_MEM double &Yv(int f) const { return scr[f * L + j]; }
+  GV_MEM double &Av(int f) const { return scr[(NF + f) * L + j]; }
+
+  // advance_vertical_1d_ausm (vertical_solver_ausm.f90:167-417) for one
+  // field of an interior level: stage k+1 input = Y + (New - U), and the
+  // final (1/3)*(-Y + S2 + 2*S3 + S4 + (New_4 - U_4)) accumulated in the
+  // reference's left-to-right order.
+  GV_MEM void ausm_commit(int stg, int f, double nv) {
+    if (stg < 3) {
+      const double y = Yv(f);
+      const double nxt = y + (nv - U[f]);
+      if (stg == 0)
+        Av(f) = -1.0 * y + nxt;
+      else if (stg == 1)
+        Av(f) = Av(f) + 2.0 * nxt;
+      else
+        Av(f) = Av(f) + nxt;
+      U[f] = nxt;
+    } else {
+      U[f] = (1.0 / 3.0) * (Av(f) + (nv - U[f]));
+    }
+  }
+
+  // advance_vertical_1d_rusanov (vertical_solver_rusanov.f90:120-369):
+  // K = New - U; Y + K/2, Y + K/2, Y + K, then Y + (K1 + 2K2 + 2K3 + K4)/6.
+  GV_MEM void rusanov_commit(int stg, int f, double nv) {
+    const double K = nv - U[f];
+    const double y = Yv(f);
+    if (stg == 0) {
+      Av(f) = K;
+      U[f] = y + 0.5 * K;
+    } else if (stg == 1) {
+      Av(f) = Av(f) + 2.0 * K;
+      U[f] = y + 0.5 * K;
+    } else if (stg == 2) {
+      Av(f) = Av(f) + 2.0 * K;
+      U[f] = y + K;
+    } else {
+      U[f] = y + (1.0 / 6.0) * (Av(f) + K);
+    }
+  }
+
   GV_MEM bool face() const { return k >= 0 && k <= nA; }
   GV_MEM bool cell() const { return k >= 1 && k <= nA; }
   GV_MEM void fail(int code, int level) { cx.report(error_key(col, code, level)); }
@@ -323,10 +380,10 @@ struct Column {
     centrifugal = (coslat * kp.omega) * (coslat * kp.omega);  // :103
     coriolis = 2 * coslat * kp.omega;                         // :104
     S.eg = grav + centrifugal / invr;
-    S.invr = invr;
-    S.eddy = eddy;
+    st(A_INVR, j) = invr;
+    st(A_EDDY, j) = eddy;
     S.tu = tu;
-    S.grav = grav;
+    st(A_GRAV, j) = grav;
     S.gm1 = gam - 1.0;
     const double rd = 1.0 / invr;  // RadialDistance_C (ausm :595-597)
     xb(0, j) = alt;
@@ -364,11 +421,11 @@ struct Column {
     {
       const double LR = 0.5 * (rd + xb(4, j - 1));
       const double RR = 0.5 * (rd + xb(4, j + 1));
-      S.ap = RR * RR;
-      S.am = LR * LR;
-      S.vol = (1.0 / 3.0) * (RR * RR * RR - LR * LR * LR);
+      st(A_AP, j) = RR * RR;
+      st(A_AM, j) = LR * LR;
+      st(A_VOL, j) = (1.0 / 3.0) * (RR * RR * RR - LR * LR * LR);
     }
-    S.hf = 0.35 * exp(-1.0 * (alt - sc(SC_ALT0)) / (45.0e+03));
+    st(A_HF, j) = 0.35 * exp(-1.0 * (alt - sc(SC_ALT0)) / (45.0e+03));
     cx.sync();
     // gamma face states (calc_kt_facevalues of Gamma_1d, ausm :1535)
     double dlg = 0.0;
@@ -383,15 +440,15 @@ struct Column {
       const double MeshH2 = h2 + h1;
       const double MeshH3 = h3 + h2 + h1;
       const double MeshH4 = h4 + h3 + h2 + h1;
-      S.glc[0] = (h2 * h3 * (h3 + h4)) / (h1 * MeshH2 * MeshH3 * MeshH4);
-      S.glc[1] = -1.0 * (MeshH2 * h3 * (h3 + h4)) / (h1 * h2 * (h2 + h3) * (h2 + h3 + h4));
-      S.glc[3] = MeshH2 * h2 * (h4 + h3) / (MeshH3 * (h2 + h3) * h3 * h4);
-      S.glc[4] = -1.0 * MeshH2 * h2 * h3 / (MeshH4 * (h2 + h3 + h4) * (h3 + h4) * h4);
-      S.glc[2] = (h2 * h3 * (h3 + h4) + MeshH2 * h3 * (h3 + h4) -
+      st(A_GLC0 + 0, j) = (h2 * h3 * (h3 + h4)) / (h1 * MeshH2 * MeshH3 * MeshH4);
+      st(A_GLC0 + 1, j) = -1.0 * (MeshH2 * h3 * (h3 + h4)) / (h1 * h2 * (h2 + h3) * (h2 + h3 + h4));
+      st(A_GLC0 + 3, j) = MeshH2 * h2 * (h4 + h3) / (MeshH3 * (h2 + h3) * h3 * h4);
+      st(A_GLC0 + 4, j) = -1.0 * MeshH2 * h2 * h3 / (MeshH4 * (h2 + h3 + h4) * (h3 + h4) * h4);
+      st(A_GLC0 + 2, j) = (h2 * h3 * (h3 + h4) + MeshH2 * h3 * (h3 + h4) -
                   MeshH2 * h2 * (h3 + h4) - MeshH2 * h2 * h3) /
                  (MeshH2 * h2 * h3 * (h3 + h4));
     } else {
-      for (int q = 0; q < 5; q++) S.glc[q] = 0.0;
+      for (int q = 0; q < 5; q++) st(A_GLC0 + q, j) = 0.0;
     }
     // bottom extrapolation weights for levels 0 and -1 (bc :228-248)
     if (j == 0) {
@@ -402,14 +459,14 @@ struct Column {
     if (face()) {
       const double gL = gam + 0.5 * dlg * dF;
       const double gR = xb(2, j + 1) - 0.5 * xb(5, j + 1) * st(A_DF, j + 1);
-      S.gl1 = 1.0 / (gL - 1.0);
-      S.gl2 = gL / (gL - 1.0);
-      S.gl3 = 2.0 * ((gL - 1.0) / (gL + 1.0));
-      S.gr1 = 1.0 / (gR - 1.0);
-      S.gr2 = gR / (gR - 1.0);
-      S.gr3 = 2.0 * ((gR - 1.0) / (gR + 1.0));
+      st(A_GL1, j) = 1.0 / (gL - 1.0);
+      st(A_GL2, j) = gL / (gL - 1.0);
+      st(A_GL3, j) = 2.0 * ((gL - 1.0) / (gL + 1.0));
+      st(A_GR1, j) = 1.0 / (gR - 1.0);
+      st(A_GR2, j) = gR / (gR - 1.0);
+      st(A_GR3, j) = 2.0 * ((gR - 1.0) / (gR + 1.0));
     } else {
-      S.gl1 = S.gl2 = S.gl3 = S.gr1 = S.gr2 = S.gr3 = 0.0;
+      st(A_GL1, j) = st(A_GL2, j) = st(A_GL3, j) = st(A_GR1, j) = st(A_GR2, j) = st(A_GR3, j) = 0.0;
     }
     cx.sync();
   }
@@ -608,20 +665,21 @@ struct Column {
     cx.sync();
 #pragma unroll
     for (int f = 0; f < NSP; f++)
-      glc[f] = S.glc[0] * xb(f, j - 2) + S.glc[1] * xb(f, j - 1) + S.glc[2] * lcs[f] +
-               S.glc[3] * xb(f, j + 1) + S.glc[4] * xb(f, j + 2);
+      glc[f] = st(A_GLC0 + 0, j) * xb(f, j - 2) + st(A_GLC0 + 1, j) * xb(f, j - 1) + st(A_GLC0 + 2, j) * lcs[f] +
+               st(A_GLC0 + 3, j) * xb(f, j + 1) + st(A_GLC0 + 4, j) * xb(f, j + 2);
     cx.sync();
   }
 
   // -------------------------------------------------------------------------
   // advance_vertical_1stage_ausm, vertical_solver_ausm.f90:436-999, with
   // calc_all_fluxes_hydro (:1093-2070).  Writes New (valid on cell threads).
+  // Ordered so that few per-level values stay live across the flux phase.
   // -------------------------------------------------------------------------
-  GV_MEM void ausm_stage(double dtin, double (&New)[NF]) {
+  GV_MEM void ausm_stage(double dtin, int stg) {
     const double *m = kp.mass;
     const double T = U[F_T];
     // --- pointwise (:591-609, :649-652, :752-773)
-    double NS[NSP], rhos[NSP], lrs[NSP], lps[NSP], lcs[NSP];
+    double NS[NSP], rhos[NSP];
     const double rho = exp(U[F_LR]);
     double nt = 0.0;
 #pragma unroll
@@ -629,22 +687,32 @@ struct Column {
       NS[s] = exp(U[F_LN + s]);
       nt = nt + NS[s];
     }
-    const double press = nt * KB * T;
 #pragma unroll
-    for (int s = 0; s < NSP; s++) {
-      rhos[s] = m[s] * NS[s];
-      lrs[s] = log(rhos[s]);
-      lps[s] = log(NS[s] * T * KB);
-      lcs[s] = log(m[s] * NS[s] / rho);
+    for (int s = 0; s < NSP; s++) rhos[s] = m[s] * NS[s];
+    const double te = nt * KB * T / S.gm1 +
+                      0.5 * rho *
+                          (U[F_VU] * U[F_VU] + U[F_VN] * U[F_VN] + U[F_VE] * U[F_VE]);
+
+    // --- gradients the sources need (:611-646, :649-688): GradTemp,
+    // DiffVel(E,N) and d ln(rho_s/rho)/dz
+    double glc[NSP];
+    {
+      double lcs[NSP];
+#pragma unroll
+      for (int s = 0; s < NSP; s++) lcs[s] = log(m[s] * NS[s] / rho);
+      grad_log_cons(lcs, glc);
     }
-    const double te = press / S.gm1 + 0.5 * rho *
-                                          (U[F_VU] * U[F_VU] + U[F_VN] * U[F_VN] +
-                                           U[F_VE] * U[F_VE]);
-    const double momE = rho * U[F_VE], momN = rho * U[F_VN];
+    double gr3[3], df3[3];
+    {
+      const double gv3[3] = {T, U[F_VE], U[F_VN]};
+      rusanov_grad_diff<3>(gv3, gr3, df3);
+    }
 
     // --- hydrostatic background (:694-751): per-level factors in parallel,
     // the recurrence in altitude serially (one thread per species)
     xb(0, j) = T;
+#pragma unroll
+    for (int s = 0; s < NSP; s++) xb(8 + s, j) = NS[s];
     cx.sync();
     {
       const double tm1 = xb(0, j - 1), tp1 = xb(0, j + 1);
@@ -666,8 +734,6 @@ struct Column {
 #pragma unroll
         for (int s = 0; s < NSP; s++) xb(2 + s, j) = b;
       }
-#pragma unroll
-      for (int s = 0; s < NSP; s++) xb(8 + s, j) = NS[s];
     }
     cx.sync();
     if (j < NSP) {
@@ -682,80 +748,87 @@ struct Column {
       h[0] = ns0 * a[0] * b[0];
     }
     cx.sync();
-    double lhps[NSP], dev[NSP];
+    double dev[NSP];
     int sub = 0;  // bit s: SubtractHydrostatic(k, s)
+    {
+      double lhps[NSP];
 #pragma unroll
-    for (int s = 0; s < NSP; s++) {
-      const double hns = xb(8 + s, j);
-      lhps[s] = log(hns * KB * T);
-      const double hrhos = m[s] * hns;
-      const double ratio = fabs(rhos[s] - hrhos) / rhos[s];
-      dev[s] = rhos[s] - hrhos;
-      if (!(fabs(ratio) > 1.0)) sub |= 1 << s;
-    }
-    cx.sync();
-
-    // --- face states (calc_kt_facevalues, :2073-2117), one face k|k+1 per
-    // thread.  Batch 1: ln rho_s, ln p_s, u, v.  Batch 2: ln p_hydro,s, w_s.
-    const double iF0 = st(A_IF, j), iF1 = st(A_IF, j + 1), iF2 = st(A_IF, j + 2);
-    const double dF0 = st(A_DF, j), dF1 = st(A_DF, j + 1);
-    double RSL[NSP], RSR[NSP], PSL[NSP], PSR[NSP], HL[NSP], HR[NSP], VL[NSP], VR[NSP];
-    double VEL, VER, VNL, VNR;
-#define GV_KT(b, f0, OUTL, OUTR)                                                  \
-  do {                                                                            \
-    const double fm1 = xb(b, j - 1), f1 = xb(b, j + 1), f2 = xb(b, j + 2);         \
-    const double d0 = limiter_mc(kp.beta, (f1 - (f0)) * iF1, ((f0) - fm1) * iF0); \
-    const double d1 = limiter_mc(kp.beta, (f2 - f1) * iF2, (f1 - (f0)) * iF1);   \
-    OUTL = (f0) + 0.5 * d0 * dF0;                                                 \
-    OUTR = f1 - 0.5 * d1 * dF1;                                                   \
-  } while (0)
+      for (int s = 0; s < NSP; s++) {
+        const double hns = xb(8 + s, j);
+        lhps[s] = log(hns * KB * T);
+        const double hrhos = m[s] * hns;
+        const double ratio = fabs(rhos[s] - hrhos) / rhos[s];
+        dev[s] = rhos[s] - hrhos;
+        if (!(fabs(ratio) > 1.0)) sub |= 1 << s;
+      }
+      cx.sync();
+      // batch 2 of the face states goes first: ln p_hydro,s and w_s
 #pragma unroll
-    for (int s = 0; s < NSP; s++) {
-      xb(s, j) = lrs[s];
-      xb(6 + s, j) = lps[s];
+      for (int s = 0; s < NSP; s++) {
+        xb(s, j) = lhps[s];
+        xb(6 + s, j) = U[F_W + s];
+      }
     }
-    xb(12, j) = U[F_VE];
-    xb(13, j) = U[F_VN];
     ib(j) = sub;
     cx.sync();
+
+    // --- face states (calc_kt_facevalues, :2073-2117) of face k|k+1, one
+    // face per thread, exponentiated where the flux needs them (:1356-1436)
     const int subUp = ib(j + 1);  // flags of cell k+1
+    double RSL[NSP], RSR[NSP], PSL[NSP], PSR[NSP], HL[NSP], HR[NSP], VL[NSP], VR[NSP];
+    double VEL, VER, VNL, VNR;
+    {
+      const double iF0 = st(A_IF, j), iF1 = st(A_IF, j + 1), iF2 = st(A_IF, j + 2);
+      const double dF0 = st(A_DF, j), dF1 = st(A_DF, j + 1);
+#define GV_KT(b, OUTL, OUTR)                                                        \
+  do {                                                                              \
+    const double f0 = xb(b, j), fm1 = xb(b, j - 1), f1 = xb(b, j + 1), f2 = xb(b, j + 2); \
+    const double d0 = limiter_mc(kp.beta, (f1 - f0) * iF1, (f0 - fm1) * iF0);       \
+    const double d1 = limiter_mc(kp.beta, (f2 - f1) * iF2, (f1 - f0) * iF1);        \
+    OUTL = f0 + 0.5 * d0 * dF0;                                                     \
+    OUTR = f1 - 0.5 * d1 * dF1;                                                     \
+  } while (0)
 #pragma unroll
-    for (int s = 0; s < NSP; s++) {
-      double l, r;
-      GV_KT(s, lrs[s], l, r);
-      RSL[s] = exp(l);
-      RSR[s] = exp(r);
-      GV_KT(6 + s, lps[s], l, r);
-      PSL[s] = exp(l);
-      PSR[s] = exp(r);
-    }
-    GV_KT(12, U[F_VE], VEL, VER);
-    GV_KT(13, U[F_VN], VNL, VNR);
-    cx.sync();
+      for (int s = 0; s < NSP; s++) {
+        double l, r;
+        GV_KT(s, l, r);
+        HL[s] = exp(l);
+        HR[s] = exp(r);
+        GV_KT(6 + s, VL[s], VR[s]);
+      }
+      cx.sync();
 #pragma unroll
-    for (int s = 0; s < NSP; s++) {
-      xb(s, j) = lhps[s];
-      xb(6 + s, j) = U[F_W + s];
-    }
-    cx.sync();
+      for (int s = 0; s < NSP; s++) {
+        xb(s, j) = log(rhos[s]);
+        xb(6 + s, j) = log(NS[s] * T * KB);
+      }
+      xb(12, j) = U[F_VE];
+      xb(13, j) = U[F_VN];
+      cx.sync();
 #pragma unroll
-    for (int s = 0; s < NSP; s++) {
-      double l, r;
-      GV_KT(s, lhps[s], l, r);
-      HL[s] = exp(l);
-      HR[s] = exp(r);
-      GV_KT(6 + s, U[F_W + s], VL[s], VR[s]);
-    }
+      for (int s = 0; s < NSP; s++) {
+        double l, r;
+        GV_KT(s, l, r);
+        RSL[s] = exp(l);
+        RSR[s] = exp(r);
+        GV_KT(6 + s, l, r);
+        PSL[s] = exp(l);
+        PSR[s] = exp(r);
+      }
+      GV_KT(12, VEL, VER);
+      GV_KT(13, VNL, VNR);
 #undef GV_KT
+    }
     cx.sync();
 
     // --- AUSM+-up fluxes of face k|k+1 (:1459-1970).  The P12 side belongs
-    // to cell k, the M12 side to cell k+1 (with that cell's Kp, Ku, dAlt_C
-    // and SubtractHydrostatic flags).
-    double FRp[NSP], FMp[NSP], NPp[NSP], FRm[NSP], FMm[NSP], NPm[NSP];
-    double EFp = 0.0, MOEp = 0.0, MONp = 0.0, EFm = 0.0, MOEm = 0.0, MONm = 0.0;
+    // to cell k and stays in registers; the M12 side belongs to cell k+1 (with
+    // that cell's Kp, Ku, dAlt_C and SubtractHydrostatic flags) and goes to
+    // LDS buffers 0..20 for the thread above.
+    double FRp[NSP], FMp[NSP], NPp[NSP];
+    double EFp = 0.0, MOEp = 0.0, MONp = 0.0;
 #pragma unroll
-    for (int s = 0; s < NSP; s++) FRp[s] = FMp[s] = NPp[s] = FRm[s] = FMm[s] = NPm[s] = 0.0;
+    for (int s = 0; s < NSP; s++) FRp[s] = FMp[s] = NPp[s] = 0.0;
     if (face()) {
       double RL = 0.0, RR = 0.0, PL = 0.0, PR = 0.0;
 #pragma unroll
@@ -776,16 +849,18 @@ struct Column {
       }
       // bulk energy (:1538-1562), enthalpy (:1569-1615), Liou sound speed
       // (:1619-1655)
-      const double EL = S.gl1 * PL + 0.5 * RL * (VUL * VUL + VEL * VEL + VNL * VNL);
-      const double ER = S.gr1 * PR + 0.5 * RR * (VUR * VUR + VER * VER + VNR * VNR);
-      const double HtL = 0.5 * (VUL * VUL + VEL * VEL + VNL * VNL) + S.gl2 * PL / RL;
-      const double HtR = 0.5 * (VUR * VUR + VER * VER + VNR * VNR) + S.gr2 * PR / RR;
-      double sub_cs = sqrt(S.gl3 * HtL);
-      const double CSL = (sub_cs * sub_cs) / dmax(sub_cs, VUL);
-      sub_cs = sqrt(S.gr3 * HtR);
-      const double CSR = (sub_cs * sub_cs) / dmax(sub_cs, -1.0 * VUR);
-      const double CS = dmin(CSL, CSR);
-
+      const double EL = st(A_GL1, j) * PL + 0.5 * RL * (VUL * VUL + VEL * VEL + VNL * VNL);
+      const double ER = st(A_GR1, j) * PR + 0.5 * RR * (VUR * VUR + VER * VER + VNR * VNR);
+      double CS;
+      {
+        const double HtL = 0.5 * (VUL * VUL + VEL * VEL + VNL * VNL) + st(A_GL2, j) * PL / RL;
+        const double HtR = 0.5 * (VUR * VUR + VER * VER + VNR * VNR) + st(A_GR2, j) * PR / RR;
+        double sub_cs = sqrt(st(A_GL3, j) * HtL);
+        const double CSL = (sub_cs * sub_cs) / dmax(sub_cs, VUL);
+        sub_cs = sqrt(st(A_GR3, j) * HtR);
+        const double CSR = (sub_cs * sub_cs) / dmax(sub_cs, -1.0 * VUR);
+        CS = dmin(CSL, CSR);
+      }
       const double kp0 = st(A_KP, j), kp1 = st(A_KP, j + 1);
       const double ku0 = st(A_KU, j), ku1 = st(A_KU, j + 1);
       const double rdt0 = st(A_DC, j) / dtin, rdt1 = st(A_DC, j + 1) / dtin;
@@ -834,13 +909,13 @@ struct Column {
           const bool sh = (subUp >> s) & 1;
           const double MP = kp1 * f1 * (sh ? pdh : pd) / (mcs * (facs + rdt1));
           const double NV = CS * (MF4P + MF4N - MP);
-          NPm[s] = (sh ? (MPS - MHPS) : MPS) - 0.5 * ku1 * CS * rvd;
+          xb(12 + s, j) = (sh ? (MPS - MHPS) : MPS) - 0.5 * ku1 * CS * rvd;
           if (NV >= 0.0) {
-            FRm[s] = RSL[s] * NV;
-            FMm[s] = RSL[s] * VL[s] * NV;
+            xb(s, j) = RSL[s] * NV;
+            xb(6 + s, j) = RSL[s] * VL[s] * NV;
           } else {
-            FRm[s] = RSR[s] * NV;
-            FMm[s] = RSR[s] * VR[s] * NV;
+            xb(s, j) = RSR[s] * NV;
+            xb(6 + s, j) = RSR[s] * VR[s] * NV;
           }
           BNVm = BNVm + (RSL[s] + RSR[s]) * NV / RLR;
         }
@@ -856,54 +931,36 @@ struct Column {
         MONp = RR * VNR * BNVp;
       }
       if (BNVm >= 0.0) {
-        EFm = (EL + PL) * BNVm;
-        MOEm = RL * VEL * BNVm;
-        MONm = RL * VNL * BNVm;
+        xb(18, j) = (EL + PL) * BNVm;
+        xb(19, j) = RL * VEL * BNVm;
+        xb(20, j) = RL * VNL * BNVm;
       } else {
-        EFm = (ER + PR) * BNVm;
-        MOEm = RR * VER * BNVm;
-        MONm = RR * VNR * BNVm;
+        xb(18, j) = (ER + PR) * BNVm;
+        xb(19, j) = RR * VER * BNVm;
+        xb(20, j) = RR * VNR * BNVm;
       }
     }
+    cx.sync();
     // divergence of the face fluxes (:2042-2068); cell k takes its M12 side
     // from the face below
-#pragma unroll
-    for (int s = 0; s < NSP; s++) {
-      xb(s, j) = FRm[s];
-      xb(6 + s, j) = FMm[s];
-    }
-    xb(12, j) = EFm;
-    xb(13, j) = MOEm;
-    xb(14, j) = MONm;
-    cx.sync();
-    double rsf[NSP], msf[NSP], ef = 0.0, mfE = 0.0, mfN = 0.0;
-#pragma unroll
-    for (int s = 0; s < NSP; s++) {
-      rsf[s] = (S.ap * FRp[s] - S.am * xb(s, j - 1)) / S.vol;
-      msf[s] = (S.ap * FMp[s] - S.am * xb(6 + s, j - 1)) / S.vol;
-    }
-    ef = (S.ap * EFp - S.am * xb(12, j - 1)) / S.vol;
-    mfE = (S.ap * MOEp - S.am * xb(13, j - 1)) / S.vol;
-    mfN = (S.ap * MONp - S.am * xb(14, j - 1)) / S.vol;
-    cx.sync();
-#pragma unroll
-    for (int s = 0; s < NSP; s++) xb(s, j) = NPm[s];
-    cx.sync();
+    double rsf[NSP], msf[NSP], ef, mfE, mfN;
     {
+      const double ap = st(A_AP, j), am = st(A_AM, j), vol = st(A_VOL, j);
       const double dC = st(A_DC, j);
 #pragma unroll
-      for (int s = 0; s < NSP; s++) msf[s] = msf[s] + (NPp[s] - xb(s, j - 1)) / dC;
+      for (int s = 0; s < NSP; s++) {
+        rsf[s] = (ap * FRp[s] - am * xb(s, j - 1)) / vol;
+        msf[s] = (ap * FMp[s] - am * xb(6 + s, j - 1)) / vol;
+        msf[s] = msf[s] + (NPp[s] - xb(12 + s, j - 1)) / dC;
+      }
+      ef = (ap * EFp - am * xb(18, j - 1)) / vol;
+      mfE = (ap * MOEp - am * xb(19, j - 1)) / vol;
+      mfN = (ap * MONp - am * xb(20, j - 1)) / vol;
     }
     cx.sync();
 
-    // --- gradients the sources need (:611-646): GradTemp, DiffVel(E,N)
-    const double gv3[3] = {T, U[F_VE], U[F_VN]};
-    double gr3[3], df3[3];
-    rusanov_grad_diff<3>(gv3, gr3, df3);
-    double glc[NSP];
-    grad_log_cons(lcs, glc);
-
-    // --- update of cell k (:802-997)
+    // --- update of cell k (:802-997); each new value is folded into the RK4
+    // carries as soon as the stage has no further use of the old one
     if (cell()) {
       double nrs[NSP], nns[NSP], w[NSP];
       double nrho = 0.0, nnt = 0.0;
@@ -911,27 +968,45 @@ struct Column {
       for (int s = 0; s < NSP; s++) {
         nrs[s] = rhos[s] - dtin * (rsf[s]);
         nns[s] = nrs[s] / m[s];
-        New[F_LN + s] = log(nns[s]);
         nrho = nrho + nrs[s];
         nnt = nnt + nns[s];
       }
       const double gradT = gr3[0];
+      const double invr = st(A_INVR, j);
 #pragma unroll
       for (int s = 0; s < NSP; s++) {
-        const double moms = m[s] * NS[s] * U[F_W + s];
+        const double moms = rhos[s] * U[F_W + s];
         double nm;
         if (!((sub >> s) & 1))
           nm = moms - dtin * (msf[s]) + dtin * rhos[s] * S.eg;
         else
           nm = moms - dtin * (msf[s]) + dtin * dev[s] * S.eg;
         double ww = nm / nrs[s];
-        ww = ww + dtin * (U[F_VN] * U[F_VN] + U[F_VE] * U[F_VE]) * S.invr;
+        ww = ww + dtin * (U[F_VN] * U[F_VN] + U[F_VE] * U[F_VE]) * invr;
         if (kp.coriolis) ww = ww + dtin * (coriolis * U[F_VE]);
         ww = ww - dtin * (kp.tdc[s] * KB * gradT) / m[s];
         w[s] = ww;
       }
+      const double hf = st(A_HF, j);
+      double nvE = (rho * U[F_VE] - dtin * (mfE)) / nrho;
+      nvE = nvE + hf * dtin * df3[1];
+      double nvN = (rho * U[F_VN] - dtin * (mfN)) / nrho;
+      nvN = nvN + hf * dtin * df3[2];
+      const double nte = te - dtin * ef + dtin * rho * U[F_VU] * S.eg;
+      {
+        double sr = 0.0;
+#pragma unroll
+        for (int s = 0; s < NSP; s++) {
+          const double nl = log(nns[s]);
+          sr = sr + m[s] * exp(nl);
+          ausm_commit(stg, F_LN + s, nl);
+        }
+        ausm_commit(stg, F_LR, log(sr));
+      }
+      ausm_commit(stg, F_VE, nvE);
+      ausm_commit(stg, F_VN, nvN);
       if (kp.friction) {
-        if (!neutral_friction(kp, dtin, w, nnt, nns, glc, S.eddy, T))
+        if (!neutral_friction(kp, dtin, w, nnt, nns, glc, st(A_EDDY, j), T))
           fail(GITM_VERT_ESINGULAR, k);
       }
       double nvu = 0.0;
@@ -939,24 +1014,12 @@ struct Column {
       for (int s = 0; s < NSP; s++) {
         double ww = dmax(-kp.maxvv, w[s]);
         ww = dmin(kp.maxvv, ww);
-        New[F_W + s] = ww;
         nvu = nvu + ww * nrs[s] / nrho;
+        ausm_commit(stg, F_W + s, ww);
       }
-      New[F_VU] = nvu;
-      double nv = (momE - dtin * (mfE)) / nrho;
-      New[F_VE] = nv + S.hf * dtin * df3[1];
-      nv = (momN - dtin * (mfN)) / nrho;
-      New[F_VN] = nv + S.hf * dtin * df3[2];
-      const double nte = te - dtin * ef + dtin * rho * U[F_VU] * S.eg;
-      const double np = (nte - 0.5 * nrho *
-                                   (New[F_VU] * New[F_VU] + New[F_VN] * New[F_VN] +
-                                    New[F_VE] * New[F_VE])) *
-                        S.gm1;
-      New[F_T] = np / (KB * nnt);
-      double sr = 0.0;
-#pragma unroll
-      for (int s = 0; s < NSP; s++) sr = sr + m[s] * exp(New[F_LN + s]);
-      New[F_LR] = log(sr);
+      const double np = (nte - 0.5 * nrho * (nvu * nvu + nvN * nvN + nvE * nvE)) * S.gm1;
+      ausm_commit(stg, F_VU, nvu);
+      ausm_commit(stg, F_T, np / (KB * nnt));
     }
   }
 
@@ -964,7 +1027,7 @@ struct Column {
   // advance_vertical_1stage (Rusanov), vertical_solver_rusanov.f90:387-686.
   // Writes New (valid on cell threads).
   // -------------------------------------------------------------------------
-  GV_MEM void rusanov_stage(double (&New)[NF]) {
+  GV_MEM void rusanov_stage(int stg) {
     const double *m = kp.mass;
     const double dt = kp.dt;
     const double T = U[F_T];
@@ -996,25 +1059,25 @@ struct Column {
     double glc[NSP];
     grad_log_cons(lcs, glc);
     if (!cell()) return;
-    const double divvel = ga[3] + 2 * U[F_VU] * S.invr;  // :476
-    double w[NSP];
+    const double divvel = ga[3] + 2 * U[F_VU] * st(A_INVR, j);  // :476
+    double w[NSP], nln[NSP];
 #pragma unroll
     for (int s = 0; s < NSP; s++) {  // :550-614
       const double vv = U[F_W + s];
-      const double divvv = gw[s] + 2 * vv * S.invr;
-      New[F_LN + s] = U[F_LN + s] - dt * (divvv + vv * gln[s]) + dt * dln[s];
+      const double divvv = gw[s] + 2 * vv * st(A_INVR, j);
+      nln[s] = U[F_LN + s] - dt * (divvv + vv * gln[s]) + dt * dln[s];
       double ww = vv -
                   dt * (vv * gw[s] -
-                        (U[F_VN] * U[F_VN] + U[F_VE] * U[F_VE]) * S.invr +
-                        T * gln[s] * KB / m[s] + ga[0] * KB / m[s] - S.grav) +
+                        (U[F_VN] * U[F_VN] + U[F_VE] * U[F_VE]) * st(A_INVR, j) +
+                        T * gln[s] * KB / m[s] + ga[0] * KB / m[s] - st(A_GRAV, j)) +
                   dt * dw[s] - vv / kp.vert_tau;
       if (kp.coriolis)
-        ww = ww + dt * (centrifugal / S.invr + coriolis * U[F_VE]);
+        ww = ww + dt * (centrifugal / st(A_INVR, j) + coriolis * U[F_VE]);
       ww = ww - dt * (kp.tdc[s] * KB * ga[0]) / m[s];
       w[s] = ww;
     }
     if (kp.friction) {  // :632-641, old densities
-      if (!neutral_friction(kp, dt, w, nt, NS, glc, S.eddy, T))
+      if (!neutral_friction(kp, dt, w, nt, NS, glc, st(A_EDDY, j), T))
         fail(GITM_VERT_ESINGULAR, k);
     }
     double nvu = 0.0;
@@ -1022,18 +1085,26 @@ struct Column {
     for (int s = 0; s < NSP; s++) {  // :643-658
       double ww = dmax(-kp.maxvv, w[s]);
       ww = dmin(kp.maxvv, ww);
-      New[F_W + s] = ww;
+      w[s] = ww;
       nvu = nvu + ww * (m[s] * NS[s] / rho);
     }
-    New[F_VU] = nvu;
     const double vu = U[F_VU];  // :660-679
-    New[F_VE] = U[F_VE] - dt * vu * ga[1] + dt * da[1];
-    New[F_VN] = U[F_VN] - dt * vu * ga[2] + dt * da[2];
-    New[F_T] = T - dt * (vu * ga[0] + (S.gm1) * (T * divvel)) + dt * da[0];
+    const double nvE = U[F_VE] - dt * vu * ga[1] + dt * da[1];
+    const double nvN = U[F_VN] - dt * vu * ga[2] + dt * da[2];
+    const double nT = T - dt * (vu * ga[0] + (S.gm1) * (T * divvel)) + dt * da[0];
     double sr = 0.0;
 #pragma unroll
-    for (int s = 0; s < NSP; s++) sr = sr + m[s] * exp(New[F_LN + s]);
-    New[F_LR] = log(sr);
+    for (int s = 0; s < NSP; s++) sr = sr + m[s] * exp(nln[s]);
+#pragma unroll
+    for (int s = 0; s < NSP; s++) {
+      rusanov_commit(stg, F_LN + s, nln[s]);
+      rusanov_commit(stg, F_W + s, w[s]);
+    }
+    rusanov_commit(stg, F_LR, log(sr));
+    rusanov_commit(stg, F_VU, nvu);
+    rusanov_commit(stg, F_VE, nvE);
+    rusanov_commit(stg, F_VN, nvN);
+    rusanov_commit(stg, F_T, nT);
   }
 
   // -------------------------------------------------------------------------
@@ -1045,72 +1116,28 @@ struct Column {
   // the reference's left-to-right order; the boundary conditions follow every
   // stage (ghost levels never enter the combination).
   GV_MEM void ausm_step() {
-    double Y[NF], acc[NF], New[NF];
     bcs();
+    if (cell()) {
 #pragma unroll
-    for (int f = 0; f < NF; f++) {
-      Y[f] = U[f];
-      acc[f] = 0.0;
-      New[f] = U[f];
+      for (int f = 0; f < NF; f++) Yv(f) = U[f];
     }
-    const double dtin[4] = {0.5 * kp.dt, 0.5 * kp.dt, kp.dt, 0.5 * kp.dt};
+    const double dtin[4] = {0.5 * kp.dt, 0.5 * kp.dt, kp.dt, 0.5 * kp.dt};  // :160-377
+#pragma unroll 1
     for (int stg = 0; stg < 4; stg++) {
-      ausm_stage(dtin[stg], New);
-      if (cell()) {
-        if (stg < 3) {
-#pragma unroll
-          for (int f = 0; f < NF; f++) {
-            const double nxt = Y[f] + (New[f] - U[f]);
-            if (stg == 0)
-              acc[f] = -1.0 * Y[f] + nxt;
-            else if (stg == 1)
-              acc[f] = acc[f] + 2.0 * nxt;
-            else
-              acc[f] = acc[f] + nxt;
-            U[f] = nxt;
-          }
-        } else {
-#pragma unroll
-          for (int f = 0; f < NF; f++)
-            U[f] = (1.0 / 3.0) * (acc[f] + (New[f] - U[f]));
-        }
-      }
+      ausm_stage(dtin[stg], stg);
       bcs();
     }
   }
 
-  // advance_vertical_1d_rusanov, vertical_solver_rusanov.f90:10-384:
-  // classic RK4, K_i = New_i - U_i, Y + K/2, Y + K/2, Y + K, then
-  // Y + (K1 + 2K2 + 2K3 + K4)/6.
   GV_MEM void rusanov_step() {
-    double Y[NF], acc[NF], New[NF];
     bcs();
+    if (cell()) {
 #pragma unroll
-    for (int f = 0; f < NF; f++) {
-      Y[f] = U[f];
-      acc[f] = 0.0;
-      New[f] = U[f];
+      for (int f = 0; f < NF; f++) Yv(f) = U[f];
     }
+#pragma unroll 1
     for (int stg = 0; stg < 4; stg++) {
-      rusanov_stage(New);
-      if (cell()) {
-#pragma unroll
-        for (int f = 0; f < NF; f++) {
-          const double K = New[f] - U[f];
-          if (stg == 0) {
-            acc[f] = K;
-            U[f] = Y[f] + 0.5 * K;
-          } else if (stg == 1) {
-            acc[f] = acc[f] + 2.0 * K;
-            U[f] = Y[f] + 0.5 * K;
-          } else if (stg == 2) {
-            acc[f] = acc[f] + 2.0 * K;
-            U[f] = Y[f] + K;
-          } else {
-            U[f] = Y[f] + (1.0 / 6.0) * (acc[f] + K);
-          }
-        }
-      }
+      rusanov_stage(stg);
       bcs();
     }
   }
@@ -1130,6 +1157,7 @@ template <int NT, int SOLVER, class Ctx>
 GV_FN void run_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io,
                       int col, double *lds) {
   Column<NT, Ctx> c(cx, kp, lds, col);
+  c.scr = kp.scratch + (long long)col * (2 * NF) * (kp.n_alts + 4);
   const int nlb = io.n_lons * io.n_lats;
   const int ci = col % io.n_lons, cj = (col / io.n_lons) % io.n_lats, cb = col / nlb;
   const int k = c.k, nA = c.nA;

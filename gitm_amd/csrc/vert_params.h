@@ -43,7 +43,7 @@ inline int check_args(const gitm_vert_params *p, const gitm_vert_columns *c,
   if (!p || !c) return GITM_VERT_EINVAL;
   if (solver != GITM_VERT_AUSM && solver != GITM_VERT_RUSANOV) return GITM_VERT_EINVAL;
   if (n_steps < 0) return GITM_VERT_EINVAL;
-  if (c->n_alts < 6 || c->n_alts > GITM_VERT_MAX_NALTS) return GITM_VERT_EINVAL;
+  if (c->n_alts < 10 || c->n_alts > GITM_VERT_MAX_NALTS) return GITM_VERT_EINVAL;
   if (c->n_lons < 1 || c->n_lats < 1 || c->n_blocks < 1) return GITM_VERT_EINVAL;
   if ((long long)c->n_lons * c->n_lats * c->n_blocks > 0x7fffffffLL) return GITM_VERT_EINVAL;
   if (p->use_msis || p->use_gswm_tides || p->use_waccm_tides || p->use_bc_perturbation ||

@@ -48,10 +48,11 @@ template <int NT>
 unsigned long long run_all(const gv::KParams &kp, const gitm_vert_columns &io, int c0,
                            int c1) {
   const int solver = kp.solver;
-  Shared sh(NT);
-  std::vector<double> lds(gv::LdsSize<NT>::bytes / sizeof(double) + 16, 0.0);
+  const int nthreads = kp.n_alts + 4;  // one thread per level
+  Shared sh(nthreads);
+  std::vector<double> lds(gv::lds_bytes(kp.n_alts) / sizeof(double) + 16, 0.0);
   std::vector<std::thread> th;
-  for (int t = 0; t < NT; t++)
+  for (int t = 0; t < nthreads; t++)
     th.emplace_back([&, t] {
       EmuCtx cx{t, &sh};
       for (int c = c0; c < c1; c++) {
@@ -74,6 +75,9 @@ extern "C" int gv_emu_advance(const gitm_vert_params *p, const gitm_vert_columns
   int rc = gv::check_args(p, c, solver, n_steps);
   if (rc) return rc;
   gv::KParams kp = gv::make_kparams(p, c, solver, n_steps, nullptr);
+  std::vector<double> scratch((size_t)c->n_lons * c->n_lats * c->n_blocks * 2 * gv::NF *
+                              (c->n_alts + 4));
+  kp.scratch = scratch.data();
   if (waves <= 0) waves = gv::auto_waves(c->n_alts);
   if (64 * waves < c->n_alts + 4) return GITM_VERT_EINVAL;
   unsigned long long key;
