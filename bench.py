@@ -1,0 +1,224 @@
+"""Throughput of the fused FP64 vertical step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--solver ausm|rusanov]
+
+Workload: the 1 deg x 1 deg production grid (180 x 360 = 64800 columns),
+nAlts = 100 stretched 100-700 km, seeded synthetic atmosphere
+(gitm_amd.synth, "1deg_64800x100"), dt = 2 s.  One step is one call of the
+vertical step (src/advance_vertical.f90:8-39, one RK4 AUSM step per column)
+over every column, i.e. one kernel launch.  Inputs are resident in HBM before
+the timed region starts.
+
+Prints one JSON line.  ``value`` is column-updates/s over the K timed steps.
+``roofline`` prices the kernel's algorithmic HBM bytes per launch
+(``bytes_per_column(nAlts)`` x columns) against its mean launch duration,
+timed with HIP events on the stream the kernel is queued on; ``traffic`` is
+the measured HBM bytes per launch when a counter summary is present under
+profiles/.  ``cpu_baseline`` times the CPU restatement of the same step
+(oracle/, bit-identical to the reference Fortran) on a bounded sample of the
+same grid on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+WORKLOAD = "1deg_64800x100"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "vert_column_ausm_1deg_64800x100_traffic.txt")
+
+
+def bytes_per_column(n_alts: int) -> int:
+    """Algorithmic HBM bytes of one column-update: 42 FP64 values per level.
+
+    Reads (25 per level): log densities 6, species vertical winds 6, winds 3,
+    temperature, log rho, MeanMajorMass, Gamma, eddy coefficient, cMax,
+    altitude, gravity, 1/r, dAlt.  Writes (17): log densities 6, vertical
+    winds 6, winds 3, temperature, log rho.
+    """
+    return 336 * (n_alts + 4)
+
+
+def interface_bytes_per_column(n_alts: int) -> int:
+    """HBM bytes the library's interface moves per column-update (GITM's own
+    arrays, each value read or written once).
+
+    Read and written on all nAlts+4 levels: 6 densities, 6 vertical winds,
+    3 winds, temperature, rho (17 values).  Written only: 6 log densities and
+    the total density (7).  Read only on all levels: TempUnit, MeanMajorMass,
+    Gamma, altitude, gravity, 1/r, dAlt (7).  cMax on nAlts+2 levels, the eddy
+    coefficient on nAlts levels, cos(latitude) once.
+    """
+    L = n_alts + 4
+    return 8 * ((17 * 2 + 7 + 7) * L + (n_alts + 2) + n_alts + 1)
+
+
+def read_traffic(path: str):
+    """HBM bytes per launch from the counter summary (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration), or None."""
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(spec, solver: str, seconds: float, threads: int) -> dict:
+    """Time the oracle (CPU restatement) on chunks of the same grid until
+    ``seconds`` of wall time have passed."""
+    from gitm_amd import synth
+    from oracle import cpu
+
+    cols = synth.make_columns(spec, seed=1)
+    sid = cpu.AUSM if solver == "ausm" else cpu.RUSANOV
+    chunk = 1024 * threads
+    done = 0
+    t0 = time.perf_counter()
+    while done < spec.ncols:
+        hi = min(done + chunk, spec.ncols)
+        sub = {k: np.ascontiguousarray(v[done:hi]) for k, v in cols.items()}
+        cpu.advance(sub, synth.StepParams(), sid, 1, threads=threads)
+        done = hi
+        if time.perf_counter() - t0 >= seconds:
+            break
+    el = time.perf_counter() - t0
+    return {"value": done / el, "unit": "column-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"first {done} of {spec.ncols} columns of the {spec.nalts}-level "
+                      f"grid, one {solver.upper()} RK4 step each, {el:.1f} s wall on "
+                      f"{threads} threads (C restatement, gcc -O2)"}
+
+
+def copy_probe(torch, mib: int = 1024) -> None:
+    """A plain device copy of known size, for calibrating FETCH_SIZE/WRITE_SIZE."""
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copy probe: {mib} MiB read, {mib} MiB written", file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--solver", default="ausm", choices=("ausm", "rusanov"))
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="wall-time budget of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 1 GiB device copy (counter calibration)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
+              "supported (columns are independent: N GPUs would each run their own "
+              "share of the grid)", file=sys.stderr)
+        return 2
+    if args.steps < 1 or args.warmup < 0:
+        print("bench.py: need --steps >= 1 and --warmup >= 0", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from gitm_amd import synth, vertical
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        return 1
+    spec = synth.CONFIGS[WORKLOAD]
+    params = synth.StepParams()
+    print(f"bench: {WORKLOAD} {args.solver} steps={args.steps} warmup={args.warmup}",
+          file=sys.stderr, flush=True)
+    cols = synth.make_columns(spec, seed=1)
+    dev = vertical.to_device(cols, "cuda")
+    del cols
+    solver = vertical.VerticalSolver()
+    stream = torch.cuda.current_stream()
+
+    if args.copy_probe:
+        copy_probe(torch)
+
+    for _ in range(args.warmup):
+        solver.advance(dev, params, args.solver, 1, check=False)
+    torch.cuda.synchronize()
+    solver.check()
+
+    # Per-launch events on the launch stream (torch's current stream, which
+    # VerticalSolver.advance hands to the library).
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record(stream)
+        solver.advance(dev, params, args.solver, 1, check=False)
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    solver.check()
+    if not bool(torch.isfinite(dev["temp"]).all()) or not bool((dev["nds"] > 0).all()):
+        print("bench.py: non-finite or non-positive state after the timed steps",
+              file=sys.stderr)
+        return 1
+    launch_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+
+    ncols = spec.ncols
+    value = ncols * args.steps / wall
+    per_launch = bytes_per_column(spec.nalts) * ncols
+    achieved = per_launch / (launch_ms * 1e-3) / 1e9
+    traffic = read_traffic(TRAFFIC_FILE) if args.solver == "ausm" else None
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        print(f"bench: CPU baseline on {threads} threads", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(spec, args.solver, args.cpu_seconds, threads)
+
+    line = {
+        "metric": "FP64 AUSM RK4 column-updates/s @1°×1°×100 alts; adjoint/forward "
+                  "time ratio",
+        "value": value,
+        "unit": "column-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded hydrostatic atmosphere, gitm_amd.synth)",
+        "config": {"workload": f"{WORKLOAD}: 1°×1° grid, 64800 columns x 100 levels "
+                               f"(stretched 100-700 km), {args.solver} RK4, dt=2 s",
+                   "n_columns": ncols, "n_alts": spec.nalts, "solver": args.solver},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic,
+                     "algorithmic_bytes_per_launch": per_launch,
+                     "interface_bytes_per_launch":
+                         interface_bytes_per_column(spec.nalts) * ncols,
+                     "launch_ms": launch_ms,
+                     "kernel": f"vert_column_kernel<{64 * vertical.auto_waves(spec.nalts)},"
+                               f"{1 if args.solver == 'ausm' else 0}>"},
+        "cpu_baseline": cpu,
+        "adjoint_forward_ratio": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -94,6 +94,12 @@ def make_params(sp: StepParams | None = None) -> _abi.Params:
     return p
 
 
+def auto_waves(n_alts: int) -> int:
+    """Wavefronts per column workgroup picked by default: the fewest 64-lane
+    waves that hold nAlts+4 levels (one level per lane)."""
+    return (n_alts + 4 + 63) // 64
+
+
 def solver_id(solver) -> int:
     if isinstance(solver, str):
         return {"ausm": _abi.AUSM, "rusanov": _abi.RUSANOV}[solver.lower()]

@@ -248,3 +248,21 @@ def test_hydrostatic_column_stays_at_rest(solver):
     bad["rho"][0] = synth.species_masses() @ bad["nds"][0]
     out_b = cpu.advance(bad, p, solver=solver, n_steps=1)
     assert np.abs(out_b["vvel"][0][[1, 2], 25:45]).max() > 1e-2
+
+
+def test_wind_metric_conditioning():
+    """How far the reference algorithm itself moves after one AUSM step when
+    every input value is changed by one ulp (324 columns, nAlts 50).  The
+    elementwise wind ratio with a floor at 1e-3 of the largest wind moves by
+    more than 1e-11, so no implementation whose exp/log differ from glibc's
+    in the last bit can be held to it; the normwise per-component ratio the
+    parity tests use moves by ~3e-14."""
+    from tests.helpers import floored_errors, rel_errors, ulp_perturbed
+
+    cols = synth.make_columns(synth.CONFIGS["cpu_324x50"], seed=1)
+    p = synth.StepParams()
+    a = cpu.advance(cols, p, 1, 1, threads=4)
+    b = cpu.advance(ulp_perturbed(cols, 1), p, 1, 1, threads=4)
+    assert floored_errors(b, a)["vvel"] > 1e-11
+    e = rel_errors(b, a)
+    assert e["vvel"] < 1e-12 and e["vel"] < 1e-12
