@@ -21,27 +21,43 @@ struct GpuCtx {
   __device__ void sync() const { __syncthreads(); }
   __device__ bool sync_or(bool p) const { return __syncthreads_or(p ? 1 : 0) != 0; }
   __device__ void report(unsigned long long key) const { atomicMin(err, key); }
+  // Temp**DiffExp as exp(DiffExp*ln Temp): one log and six exp in place of
+  // fifteen pow (a pow costs five times an exp here); within a few ulp of pow.
+  __device__ void diff_powers(double temp, double (&out)[6]) const {
+    const double lt = log(temp);
+#pragma unroll
+    for (int u = 0; u < 6; u++) out[u] = exp(gv::DEXU[u] * lt);
+  }
 };
 
-template <int NT, int SOLVER>
+template <int NT, int SOLVER, bool IONS>
 __global__ void __launch_bounds__(NT) vert_column_kernel(gv::KParams kp, gitm_vert_columns io) {
   extern __shared__ double lds[];
   GpuCtx cx{kp.err};
-  gv::run_column<NT, SOLVER>(cx, kp, io, (int)blockIdx.x, lds);
+  gv::run_column<NT, SOLVER, IONS>(cx, kp, io, (int)blockIdx.x, lds);
 }
 
 // One thread per level: blockDim = nAlts+4 (<= NT, the compile-time bound).
+template <int NT, bool IONS>
+void launch_ions(int solver, dim3 grid, hipStream_t stream, const gv::KParams &kp,
+                 const gitm_vert_columns &c) {
+  const dim3 block(kp.n_alts + 4);
+  const size_t lds = gv::lds_bytes(kp.n_alts, IONS, gv::carry_in_lds(NT / 64));
+  if (solver == GITM_VERT_AUSM)
+    hipLaunchKernelGGL((vert_column_kernel<NT, GITM_VERT_AUSM, IONS>), grid, block, lds,
+                       stream, kp, c);
+  else
+    hipLaunchKernelGGL((vert_column_kernel<NT, GITM_VERT_RUSANOV, IONS>), grid, block, lds,
+                       stream, kp, c);
+}
+
 template <int NT>
 void launch(int solver, dim3 grid, hipStream_t stream, const gv::KParams &kp,
             const gitm_vert_columns &c) {
-  const dim3 block(kp.n_alts + 4);
-  const size_t lds = gv::lds_bytes(kp.n_alts);
-  if (solver == GITM_VERT_AUSM)
-    hipLaunchKernelGGL((vert_column_kernel<NT, GITM_VERT_AUSM>), grid, block, lds, stream,
-                       kp, c);
+  if (kp.ions)
+    launch_ions<NT, true>(solver, grid, stream, kp, c);
   else
-    hipLaunchKernelGGL((vert_column_kernel<NT, GITM_VERT_RUSANOV>), grid, block, lds,
-                       stream, kp, c);
+    launch_ions<NT, false>(solver, grid, stream, kp, c);
 }
 
 __global__ void reset_error_word(unsigned long long *e) { *e = ~0ULL; }
@@ -81,7 +97,7 @@ const char *gitm_vert_strerror(int code) {
     case GITM_VERT_ENEGION: return "negative ion density too close to the upper boundary";
     case GITM_VERT_EINVAL: return "invalid arguments";
     case GITM_VERT_EUNSUPPORTED:
-      return "unsupported option (UseMSIS, tides, BC perturbation, UseDamping, ions)";
+      return "unsupported option (UseMSIS, tides, BC perturbation, UseDamping)";
     case GITM_VERT_EHIP: return "HIP runtime error";
     case GITM_VERT_ENOMEM: return "out of device memory";
     default: return "unknown error";
@@ -144,7 +160,9 @@ int gitm_vert_advance(gitm_vert_handle *h, const gitm_vert_params *p,
   int waves = h->waves ? h->waves : gv::auto_waves(c->n_alts);
   if (64 * waves < c->n_alts + 4) return GITM_VERT_EINVAL;
   GV_HIP(hipSetDevice(h->device));
-  const size_t need = (size_t)ncols * 2 * gv::NF * (size_t)(c->n_alts + 4) * sizeof(double);
+  gv::KParams kp = gv::make_kparams(p, c, solver, n_steps, h->d_err);
+  kp.scr_fields = gv::scratch_fields(gv::carry_in_lds(waves), kp.ions != 0);
+  const size_t need = (size_t)ncols * kp.scr_fields * (size_t)(c->n_alts + 4) * sizeof(double);
   if (h->scratch_bytes < need) {
     if (h->scratch) {
       // the previous launches on this handle's stream may still use it
@@ -156,7 +174,6 @@ int gitm_vert_advance(gitm_vert_handle *h, const gitm_vert_params *p,
     if (hipMalloc(&h->scratch, need) != hipSuccess) return GITM_VERT_ENOMEM;
     h->scratch_bytes = need;
   }
-  gv::KParams kp = gv::make_kparams(p, c, solver, n_steps, h->d_err);
   kp.scratch = h->scratch;
   const dim3 grid(ncols);
   switch (waves) {
@@ -201,26 +218,31 @@ struct FieldSpec {
   int last_level_offset;  // last level read = nAlts + this
   bool written;
   bool per_column;
+  bool ions_only;  // staged only with UseIonAdvection
 };
 
 const FieldSpec kFields[] = {
-    {&gitm_vert_columns::ndensity_s, 6, 2, true, false},
-    {&gitm_vert_columns::vertical_velocity, 6, 2, true, false},
-    {&gitm_vert_columns::velocity, 3, 2, true, false},
-    {&gitm_vert_columns::temperature, 1, 2, true, false},
-    {&gitm_vert_columns::rho, 1, 2, true, false},
-    {&gitm_vert_columns::log_ns, 6, 2, true, false},
-    {&gitm_vert_columns::ndensity, 1, 2, true, false},
-    {&gitm_vert_columns::temp_unit, 1, 2, false, false},
-    {&gitm_vert_columns::mean_major_mass, 1, 2, false, false},
-    {&gitm_vert_columns::gamma, 1, 2, false, false},
-    {&gitm_vert_columns::cmax, 1, 1, false, false},
-    {&gitm_vert_columns::eddy, 1, 2, false, false},
-    {&gitm_vert_columns::altitude, 1, 2, false, false},
-    {&gitm_vert_columns::gravity, 1, 2, false, false},
-    {&gitm_vert_columns::inv_radial_distance, 1, 2, false, false},
-    {&gitm_vert_columns::dalt, 1, 2, false, false},
-    {&gitm_vert_columns::cos_latitude, 1, 0, false, true},
+    {&gitm_vert_columns::ndensity_s, 6, 2, true, false, false},
+    {&gitm_vert_columns::vertical_velocity, 6, 2, true, false, false},
+    {&gitm_vert_columns::velocity, 3, 2, true, false, false},
+    {&gitm_vert_columns::temperature, 1, 2, true, false, false},
+    {&gitm_vert_columns::rho, 1, 2, true, false, false},
+    {&gitm_vert_columns::log_ns, 6, 2, true, false, false},
+    {&gitm_vert_columns::ndensity, 1, 2, true, false, false},
+    {&gitm_vert_columns::temp_unit, 1, 2, false, false, false},
+    {&gitm_vert_columns::mean_major_mass, 1, 2, false, false, false},
+    {&gitm_vert_columns::gamma, 1, 2, false, false, false},
+    {&gitm_vert_columns::cmax, 1, 1, false, false, false},
+    {&gitm_vert_columns::eddy, 1, 2, false, false, false},
+    {&gitm_vert_columns::altitude, 1, 2, false, false, false},
+    {&gitm_vert_columns::gravity, 1, 2, false, false, false},
+    {&gitm_vert_columns::inv_radial_distance, 1, 2, false, false, false},
+    {&gitm_vert_columns::dalt, 1, 2, false, false, false},
+    {&gitm_vert_columns::cos_latitude, 1, 0, false, true, false},
+    {&gitm_vert_columns::idensity_s, 10, 2, true, false, true},
+    {&gitm_vert_columns::ivelocity, 3, 2, false, false, true},
+    {&gitm_vert_columns::sza, 1, 0, false, true, true},
+    {&gitm_vert_columns::mlat, 1, 0, false, true, true},
 };
 
 size_t field_span(const gitm_vert_columns &c, const gitm_vert_field &f, const FieldSpec &fs) {
@@ -249,7 +271,7 @@ extern "C" int gitm_vert_advance_host(gitm_vert_handle *h, const gitm_vert_param
   for (size_t i = 0; i < nf; i++) {
     const FieldSpec &fs = kFields[i];
     const gitm_vert_field &f = c->*fs.member;
-    if (!f.ptr) continue;
+    if (!f.ptr || (fs.ions_only && !p->use_ion_advection)) continue;
     size_t n = field_span(*c, f, fs);
     if (!n) return GITM_VERT_EINVAL;
     size_t bytes = n * sizeof(double);
@@ -268,7 +290,7 @@ extern "C" int gitm_vert_advance_host(gitm_vert_handle *h, const gitm_vert_param
   for (size_t i = 0; i < nf; i++) {
     const FieldSpec &fs = kFields[i];
     const gitm_vert_field &f = c->*fs.member;
-    if (!f.ptr || !fs.written) continue;
+    if (!f.ptr || !fs.written || (fs.ions_only && !p->use_ion_advection)) continue;
     size_t bytes = field_span(*c, f, fs) * sizeof(double);
     GV_HIP(hipMemcpyAsync(f.ptr, h->stage[i], bytes, hipMemcpyDeviceToHost, h->stream));
   }

@@ -58,6 +58,10 @@ constexpr int F_VE = 14;  // Vel_GD(iEast_)
 constexpr int F_VN = 15;  // Vel_GD(iNorth_)
 constexpr int F_VU = 16;  // Vel_GD(iUp_)
 constexpr int NF = 17;
+// ion state (UseIonAdvection): LogINS of the nIons-1 = 9 ions (electrons are
+// their sum); only the first nIonsAdvect = 2 are advected (ModEarth.f90:39)
+constexpr int NI = 9;
+constexpr int NI_ADVECT = 2;
 
 // Diff0/1e17 and DiffExp (ModEarth.f90:237-261); both symmetric
 constexpr double D0R[NSP][NSP] = {{0.000, 0.969, 0.969, 0.969, 0.715, 3.440},
@@ -73,6 +77,15 @@ constexpr double DEX[NSP][NSP] = {{0.000, 0.774, 0.774, 0.774, 0.750, 0.749},
                                   {0.750, 0.750, 0.810, 0.774, 0.000, 0.718},
                                   {0.749, 0.710, 0.718, 0.718, 0.718, 0.000}};
 
+// the distinct DiffExp values and where each pair of species takes its own
+constexpr double DEXU[6] = {0.774, 0.750, 0.749, 0.810, 0.710, 0.718};
+constexpr int DEXI[NSP][NSP] = {{0, 0, 0, 0, 1, 2},
+                                {0, 0, 1, 0, 1, 4},
+                                {0, 1, 0, 0, 3, 5},
+                                {0, 0, 0, 0, 0, 5},
+                                {1, 1, 3, 0, 0, 5},
+                                {2, 4, 5, 5, 5, 0}};
+
 // 4-point reconstruction of calc_facevalues_alts_* (vertical_solver_ausm.f90
 // :1052-1053); Factor2 is the literal 0.0416667, not 1/24.
 constexpr double FACT1 = 0.6250000, FACT2 = 0.0416667;
@@ -84,9 +97,19 @@ struct KParams {
   double log_min_kp, log_max_kp, log_min_ku, log_max_ku;  // log(1e-19), ...
   int n_alts, n_steps, solver;
   int friction, coriolis, ions, improved_ions, nighttime_ion_bcs;
+  int scr_fields;           // fields per level carried in device memory (see carry_in_lds)
   unsigned long long *err;  // first-error word, see error_key()
-  double *scratch;          // RK4 carries: n_cols x (2*NF) x (nAlts+4) doubles
+  double *scratch;          // RK4 carries: n_cols x scr_fields x (nAlts+4) doubles
 };
+
+// Where the RK4 carries (Y and acc, 2*NF per level) live: in LDS for columns
+// of up to three wavefronts (they then cost 29 KiB at nAlts = 100 and leave
+// room for two columns per CU), in device memory for the tallest columns,
+// whose LDS would not fit.  Ion carries always live in device memory.
+GV_HD constexpr bool carry_in_lds(int waves) { return waves <= 3; }
+GV_HD constexpr int scratch_fields(bool carry_lds, bool ions) {
+  return (carry_lds ? 0 : 2 * NF) + (ions ? 2 * NI : 0);
+}
 
 // Error word: the lowest (column, code, level) wins an atomic min.
 GV_FN unsigned long long error_key(int col, int code, int level) {
@@ -95,9 +118,13 @@ GV_FN unsigned long long error_key(int col, int code, int level) {
          (unsigned long long)(unsigned)(level + 1024);
 }
 
-// LDS layout of one column: NSTAT static tables and NBUF exchange buffers of
-// pitch NT+4 (two guard cells on each side, so j-2..j+2 stays inside), a
-// scalar area and an int buffer.
+// LDS layout of one column: one record per level (levels -3..nAlts+4, i.e.
+// two guard rows on each side so that j-2..j+2 stays inside) holding the
+// NSTAT static tables, the NBUF exchange buffers, the RK4 carries when they
+// live in LDS and the ion rows; then a scalar area and an int buffer.  With
+// the fields of a level side by side, every access of a lane to its own or a
+// neighbouring level is the lane's record address plus a constant offset,
+// and an odd record length keeps 64-bit accesses free of bank conflicts.
 enum StatArray {
   A_DF = 0, A_IF, A_DC, A_CMF, A_KP, A_KU,  // also read at neighbouring levels
   A_GL1, A_GL2, A_GL3, A_GR1, A_GR2, A_GR3,  // gamma factors of face k|k+1
@@ -112,15 +139,31 @@ enum Scalar {
   SC_CB = 11,   // ... for level -1 (5)
   SC_BOT = 16,  // bottom BC results: 6 species x 6 values
   SC_TOP = 52,  // top BC results: 6 species x 4 values
-  NSCAL = 76
+  SC_CTA = 76,  // top one-sided weights for level nAlts+1 (5), ions
+  SC_CTB = 81,  // ... for level nAlts+2 (5)
+  SC_TEC = 86,  // TEC of the column (ions, nighttime BCs)
+  NSCAL = 88
 };
 
-// LDS bytes of one column: the pitch is the column height plus guards, so a
-// 100-level column needs 39 KiB and four columns fit a 160 KiB CU.
-GV_HD constexpr unsigned long lds_bytes(int n_alts) {
-  return ((unsigned long)(NSTAT + NBUF) * (unsigned long)(n_alts + 8) + NSCAL) *
-             sizeof(double) +
-         (unsigned long)(n_alts + 8) * sizeof(int);
+// Doubles per level record: tables, exchange buffers, carries (when in LDS),
+// ions (9 LogINS and IVel(up)).  Odd in every configuration.
+GV_HD constexpr int record_len(bool ions, bool carry_lds) {
+  return NSTAT + NBUF + (carry_lds ? 2 * NF : 0) + (ions ? NI + 1 : 0);
+}
+static_assert(record_len(false, false) % 2 == 1 && record_len(false, true) % 2 == 1 &&
+                  record_len(true, false) % 2 == 1 && record_len(true, true) % 2 == 1,
+              "an odd record length keeps 64-bit LDS accesses conflict-free");
+
+// LDS doubles of one column: nAlts+8 level records, the scalars and the int
+// buffer.
+GV_HD constexpr unsigned long lds_doubles(int n_alts, bool ions, bool carry_lds) {
+  return (unsigned long)record_len(ions, carry_lds) * (unsigned long)(n_alts + 8) + NSCAL +
+         (unsigned long)(n_alts + 9) / 2;
+}
+// LDS bytes of one column: at nAlts = 100, 69 KiB with the carries (two
+// columns per 160 KiB CU), 77 KiB with ions as well.
+GV_HD constexpr unsigned long lds_bytes(int n_alts, bool ions, bool carry_lds) {
+  return lds_doubles(n_alts, ions, carry_lds) * sizeof(double);
 }
 
 // ---------------------------------------------------------------------------
@@ -233,9 +276,11 @@ GV_FN bool lu_solve6(double (&a)[6][6], double (&b)[6]) {
 }
 
 // calc_neutral_friction, calc_neutral_friction.f90:47-106, one level.
+// tpow[u] = Temp**DEXU[u], supplied by the caller's context.
 GV_FN bool neutral_friction(const KParams &P, double dtin, double (&vel)[NSP],
                             double nt, const double (&ns)[NSP],
-                            const double (&glc)[NSP], double eddy, double temp) {
+                            const double (&glc)[NSP], double eddy, double temp,
+                            const double (&tpow)[6]) {
   double tdij[NSP][NSP];
 #pragma unroll
   for (int i = 0; i < NSP; i++)
@@ -243,7 +288,7 @@ GV_FN bool neutral_friction(const KParams &P, double dtin, double (&vel)[NSP],
     for (int q = i + 1; q < NSP; q++) {
       // Diff0 and DiffExp are symmetric, so TempDij(i,j) == TempDij(j,i)
       const double d0 = 1.0e17 * D0R[i][q];
-      double v = (1.0e-04) * (d0 * pow(temp, DEX[i][q])) / (nt * (1.0e-06));
+      double v = (1.0e-04) * (d0 * tpow[DEXI[i][q]]) / (nt * (1.0e-06));
       tdij[i][q] = v;
       tdij[q][i] = v;
     }
@@ -281,6 +326,18 @@ GV_FN void bc_fwd_coefs(double h1, double h2, double h3, double h4, double *C) {
   C[4] = -1.0 * (H1 * H2 * H3) / (H4 * (h2 + h3 + h4) * (h3 + h4) * h4);
 }
 
+// backward weights for the top ion slopes, set_vertical_bcs.Earth.f90
+// :505-525 (h1..h4 are the spacings below the point being filled)
+GV_FN void bc_bwd_coefs(double h1, double h2, double h3, double h4, double *C) {
+  double H1 = h1, H2 = h1 + h2, H3 = h1 + h2 + h3, H4 = h1 + h2 + h3 + h4;
+  C[0] = 1.0 * (H2 * H3 * H4 + H1 * H3 * H4 + H1 * H2 * H4 + H1 * H2 * H3) /
+         (H1 * H2 * H3 * H4);
+  C[1] = -1.0 * (H2 * H3 * H4) / (h1 * h2 * (h2 + h3) * (h2 + h3 + h4));
+  C[2] = 1.0 * (H1 * H3 * H4) / (H2 * h2 * h3 * (h3 + h4));
+  C[3] = -1.0 * (H1 * H2 * H4) / (H3 * (h3 + h2) * h3 * h4);
+  C[4] = 1.0 * (H1 * H2 * H3) / (H4 * (h2 + h3 + h4) * (h3 + h4) * h4);
+}
+
 // Per-thread constants of the owned level.
 // Per-thread constants of the owned level kept in registers; the rest live
 // in the LDS tables of StatArray:
@@ -296,10 +353,13 @@ struct LevelStatic {
 };
 
 // ---------------------------------------------------------------------------
-// One column.  Ctx supplies lane(), sync(), sync_or(bool) and report(key).
+// One column.  Ctx supplies lane(), sync(), sync_or(bool), report(key) and
+// diff_powers(temp, out[6]) = temp**DEXU[0..5] (the friction coefficients'
+// temperature dependence, calc_neutral_friction.f90:77-79).
 // ---------------------------------------------------------------------------
-template <int NT, class Ctx>
+template <int NT, class Ctx, bool IONS>
 struct Column {
+  static constexpr bool CARRY_LDS = carry_in_lds(NT / 64);
   Ctx &cx;
   const KParams &kp;
   double *lds;
@@ -310,23 +370,39 @@ struct Column {
   double U[NF];
   double *scr;  // this column's RK4 carries (Y, acc) in device memory
   double centrifugal, coriolis;
+  bool plasma = false;  // nighttime plasmasphere column (bc :492-497)
 
   GV_MEM Column(Ctx &c, const KParams &p, double *l, int column)
       : cx(c), kp(p), lds(l), j(c.lane()), k(c.lane() - 1), nA(p.n_alts),
         L(p.n_alts + 4), P(p.n_alts + 8), lv(c.lane() < p.n_alts + 4), col(column) {}
 
-  GV_MEM double &st(int a, int jj) const { return lds[a * P + jj + 2]; }
-  GV_MEM double &xb(int b, int jj) const { return lds[(NSTAT + b) * P + jj + 2]; }
-  GV_MEM double &sc(int i) const { return lds[(NSTAT + NBUF) * P + i]; }
+  // record layout: tables | exchange buffers | carries Y, acc | ions
+  static constexpr int RS = record_len(IONS, CARRY_LDS);
+  static constexpr int OFF_X = NSTAT;
+  static constexpr int OFF_Y = NSTAT + NBUF;
+  static constexpr int OFF_ION = OFF_Y + (CARRY_LDS ? 2 * NF : 0);
+  GV_MEM double *rec(int jj) const { return lds + (jj + 2) * RS; }
+  GV_MEM double &st(int a, int jj) const { return rec(jj)[a]; }
+  GV_MEM double &xb(int b, int jj) const { return rec(jj)[OFF_X + b]; }
+  GV_MEM double &sc(int i) const { return lds[P * RS + i]; }
   GV_MEM int &ib(int jj) const {
-    return reinterpret_cast<int *>(lds + (NSTAT + NBUF) * P + NSCAL)[jj + 2];
+    return reinterpret_cast<int *>(lds + P * RS + NSCAL)[jj + 2];
   }
   // RK4 carries of the owned level: the stage-1 state Y and the running
   // combination acc.  They are read and written only at the end of a stage,
-  // so they live in device memory ([2*NF][L] per column, coalesced) instead
-  // of registers.
-  GV_MEM double &Yv(int f) const { return scr[f * L + j]; }
-  GV_MEM double &Av(int f) const { return scr[(NF + f) * L + j]; }
+  // so they live in LDS, or for the tallest columns in device memory
+  // ([2*NF][L] per column, coalesced), instead of registers.
+  GV_MEM double &Yv(int f) const { return CARRY_LDS ? rec(j)[OFF_Y + f] : scr[f * L + j]; }
+  GV_MEM double &Av(int f) const {
+    return CARRY_LDS ? rec(j)[OFF_Y + NF + f] : scr[(NF + f) * L + j];
+  }
+  // Ions: LogINS of level jj-1 (field NI holds IVel(up)) in LDS, so that the
+  // boundary lanes can reach the top levels; their carries follow the
+  // neutral ones in device memory.
+  static constexpr int ION_SCR = CARRY_LDS ? 0 : 2 * NF;  // scratch rows before them
+  GV_MEM double &ion(int f, int jj) const { return rec(jj)[OFF_ION + f]; }
+  GV_MEM double &Yi(int f) const { return scr[(ION_SCR + f) * L + j]; }
+  GV_MEM double &Ai(int f) const { return scr[(ION_SCR + NI + f) * L + j]; }
 
   // advance_vertical_1d_ausm (vertical_solver_ausm.f90:167-417) for one
   // field of an interior level: stage k+1 input = Y + (New - U), and the
@@ -364,6 +440,41 @@ struct Column {
       U[f] = y + K;
     } else {
       U[f] = y + (1.0 / 6.0) * (Av(f) + K);
+    }
+  }
+
+  // The same two combinations for an ion field held in LDS.
+  GV_MEM void ausm_commit_ion(int stg, int f, double nv) {
+    double &u = ion(f, j);
+    if (stg < 3) {
+      const double y = Yi(f);
+      const double nxt = y + (nv - u);
+      if (stg == 0)
+        Ai(f) = -1.0 * y + nxt;
+      else if (stg == 1)
+        Ai(f) = Ai(f) + 2.0 * nxt;
+      else
+        Ai(f) = Ai(f) + nxt;
+      u = nxt;
+    } else {
+      u = (1.0 / 3.0) * (Ai(f) + (nv - u));
+    }
+  }
+  GV_MEM void rusanov_commit_ion(int stg, int f, double nv) {
+    double &u = ion(f, j);
+    const double K = nv - u;
+    const double y = Yi(f);
+    if (stg == 0) {
+      Ai(f) = K;
+      u = y + 0.5 * K;
+    } else if (stg == 1) {
+      Ai(f) = Ai(f) + 2.0 * K;
+      u = y + 0.5 * K;
+    } else if (stg == 2) {
+      Ai(f) = Ai(f) + 2.0 * K;
+      u = y + K;
+    } else {
+      u = y + (1.0 / 6.0) * (Ai(f) + K);
     }
   }
 
@@ -454,6 +565,13 @@ struct Column {
     if (j == 0) {
       bc_fwd_coefs(st(A_DF, 3), st(A_DF, 4), st(A_DF, 5), st(A_DF, 6), &sc(SC_CA));
       bc_fwd_coefs(st(A_DF, 2), st(A_DF, 3), st(A_DF, 4), st(A_DF, 5), &sc(SC_CB));
+    }
+    if (IONS && j == 1) {
+      // top ion slopes for levels nAlts+1 and nAlts+2 (bc :505-525)
+      bc_bwd_coefs(st(A_DF, nA + 1), st(A_DF, nA), st(A_DF, nA - 1), st(A_DF, nA - 2),
+                   &sc(SC_CTA));
+      bc_bwd_coefs(st(A_DF, nA + 2), st(A_DF, nA + 1), st(A_DF, nA), st(A_DF, nA - 1),
+                   &sc(SC_CTB));
     }
     cx.sync();
     if (face()) {
@@ -562,6 +680,22 @@ struct Column {
       R[2] = w1;
       R[3] = w2;
     }
+    if (IONS) {
+      // bottom: zero-gradient ions (:277-281)
+      if (j <= 1) {
+#pragma unroll
+        for (int f = 0; f < NI; f++) ion(f, j) = ion(f, 2);
+      }
+      // TEC of the column in 1e16 m^-2 (:498-503), in the reference's order
+      if (j == 6 && plasma) {
+        double tec = 0.0;
+        for (int jj = 2; jj <= nA + 1; jj++) {
+          const double v = kp.improved_ions ? ion(0, jj) : exp(ion(0, jj));
+          tec = tec + st(A_DF, jj) * v;
+        }
+        sc(SC_TEC) = tec / 1e16;
+      }
+    }
     cx.sync();
     if (j <= 1) {
       // levels -1 and 0: ghost species, LogRho and bulk wind (:346-362)
@@ -604,7 +738,62 @@ struct Column {
       U[F_VU] = (vu < 0.0) ? -vu : vu;
       U[F_T] = xb(12, jn);
     }
+    if (IONS && j < NI) ion_top_bc(j);
     cx.sync();
+  }
+
+  // Top ghost levels of one ion (set_vertical_bcs.Earth.f90:505-596),
+  // levels nAlts+1 then nAlts+2.  With UseImprovedIonAdvection the state
+  // holds densities: negative values near the top are repaired first
+  // (check_for_negative_densities, :608-649), and the slope is taken in log
+  // space.  The slope is limited to a decay of at least 0.1% per dAlt unless
+  // the column is a thin nighttime plasmasphere.
+  GV_MEM void ion_top_bc(int q) {
+    const bool improved = kp.improved_ions != 0;
+    const bool keep = plasma && sc(SC_TEC) < kp.min_tec;
+    const double dFn = st(A_DF, nA + 1);  // dAlt_F(nAlts)
+#pragma unroll 1
+    for (int t = 0; t < 2; t++) {
+      const int jk = nA + 2 + t;  // lane of level nAlts+1+t
+      const double *C = &sc(t ? SC_CTB : SC_CTA);
+      double n0, n1, n2, n3, n4, n5;
+      if (improved) {
+        double mn = ion(q, jk - 5);
+        for (int jj = jk - 4; jj <= jk; jj++)
+          if (ion(q, jj) < mn) mn = ion(q, jj);
+        if (mn < 0.0) {
+          if (ion(q, jk - 6) < 0.0) fail(GITM_VERT_ENEGION, jk - 1);
+          for (int jj = jk - 5; jj <= jk - 1; jj++) {
+            if (ion(q, jj) < 0.0) {
+              if (ion(q, jj - 1) > 0.0 && ion(q, jj + 1) > 0.0)
+                ion(q, jj) = (ion(q, jj - 1) + ion(q, jj + 1)) / 2.0;
+              else
+                ion(q, jj) = ion(q, jj - 1) * 0.95;
+            }
+          }
+          if (ion(q, jk) < 0.0) ion(q, jk) = ion(q, jk - 1) * 0.95;
+        }
+        n0 = log(ion(q, jk));
+        n1 = log(ion(q, jk - 1));
+        n2 = log(ion(q, jk - 2));
+        n3 = log(ion(q, jk - 3));
+        n4 = log(ion(q, jk - 4));
+        n5 = log(ion(q, jk - 5));
+      } else {
+        n0 = ion(q, jk);
+        n1 = ion(q, jk - 1);
+        n2 = ion(q, jk - 2);
+        n3 = ion(q, jk - 3);
+        n4 = ion(q, jk - 4);
+        n5 = ion(q, jk - 5);
+      }
+      double dn = C[0] * n1 + C[1] * n2 + C[2] * n3 + C[3] * n4 + C[4] * n5;
+      if (!keep) {
+        if (dn > -0.001 * n0 / dFn) dn = -0.001 * n0 / dFn;
+        n0 = n1 + dn * st(A_DF, jk);
+        ion(q, jk) = improved ? exp(n0) : n0;
+      }
+    }
   }
 
   // -------------------------------------------------------------------------
@@ -707,6 +896,7 @@ struct Column {
       const double gv3[3] = {T, U[F_VE], U[F_VN]};
       rusanov_grad_diff<3>(gv3, gr3, df3);
     }
+    if (IONS) ion_stage(dtin, stg, true);  // :817-829
 
     // --- hydrostatic background (:694-751): per-level factors in parallel,
     // the recurrence in altitude serially (one thread per species)
@@ -738,14 +928,13 @@ struct Column {
     cx.sync();
     if (j < NSP) {
       const int s = j;
-      double *a = &xb(1, 0), *b = &xb(2 + s, 0), *h = &xb(8 + s, 0);
-      const double ns0 = h[1];
+      const double ns0 = xb(8 + s, 1);
       double x = ns0;
       for (int jj = 2; jj < L; jj++) {
-        x = x * a[jj] * b[jj];
-        h[jj] = x;
+        x = x * xb(1, jj) * xb(2 + s, jj);
+        xb(8 + s, jj) = x;
       }
-      h[0] = ns0 * a[0] * b[0];
+      xb(8 + s, 0) = ns0 * xb(1, 0) * xb(2 + s, 0);
     }
     cx.sync();
     double dev[NSP];
@@ -1006,7 +1195,9 @@ struct Column {
       ausm_commit(stg, F_VE, nvE);
       ausm_commit(stg, F_VN, nvN);
       if (kp.friction) {
-        if (!neutral_friction(kp, dtin, w, nnt, nns, glc, st(A_EDDY, j), T))
+        double tpow[6];
+        cx.diff_powers(T, tpow);
+        if (!neutral_friction(kp, dtin, w, nnt, nns, glc, st(A_EDDY, j), T, tpow))
           fail(GITM_VERT_ESINGULAR, k);
       }
       double nvu = 0.0;
@@ -1058,6 +1249,7 @@ struct Column {
     }
     double glc[NSP];
     grad_log_cons(lcs, glc);
+    if (IONS) ion_stage(dt, stg, false);  // :617-630
     if (!cell()) return;
     const double divvel = ga[3] + 2 * U[F_VU] * st(A_INVR, j);  // :476
     double w[NSP], nln[NSP];
@@ -1077,7 +1269,9 @@ struct Column {
       w[s] = ww;
     }
     if (kp.friction) {  // :632-641, old densities
-      if (!neutral_friction(kp, dt, w, nt, NS, glc, st(A_EDDY, j), T))
+      double tpow[6];
+      cx.diff_powers(T, tpow);
+      if (!neutral_friction(kp, dt, w, nt, NS, glc, st(A_EDDY, j), T, tpow))
         fail(GITM_VERT_ESINGULAR, k);
     }
     double nvu = 0.0;
@@ -1107,6 +1301,45 @@ struct Column {
     rusanov_commit(stg, F_T, nT);
   }
 
+  // Ion advection of one stage: the first NI_ADVECT ions move with IVel(up)
+  // (Rusanov gradient and dissipation, calc_rusanov_alts_*), the others are
+  // carried through the RK4 combination unchanged.  The values are final as
+  // soon as they are computed, so they are committed at once.
+  GV_MEM void ion_stage(double dtin, int stg, bool ausm) {
+    double vi[NI_ADVECT], gi[NI_ADVECT], di[NI_ADVECT];
+#pragma unroll
+    for (int f = 0; f < NI_ADVECT; f++) vi[f] = ion(f, j);
+    rusanov_grad_diff<NI_ADVECT>(vi, gi, di);
+    if (!cell()) return;
+    const double ivu = ion(NI, j);
+#pragma unroll
+    for (int f = 0; f < NI; f++) {
+      const double nv = (f < NI_ADVECT) ? vi[f] - dtin * (ivu * gi[f]) + dtin * di[f]
+                                        : ion(f, j);
+      if (ausm)
+        ausm_commit_ion(stg, f, nv);
+      else
+        rusanov_commit_ion(stg, f, nv);
+    }
+  }
+
+  // Stage inputs keep the ghost ion values of the first stage's input: the
+  // stage leaves ghosts unchanged, so Y + (New - U) is Y there (AUSM
+  // :167-189) and so is Y + K/2 (Rusanov :139-169).  The top ion boundary
+  // reads these old ghost values.
+  GV_MEM void ion_ghosts_from_y() {
+    if (IONS && !cell()) {
+#pragma unroll
+      for (int f = 0; f < NI; f++) ion(f, j) = Yi(f);
+    }
+  }
+  GV_MEM void ion_store_y() {
+    if (IONS) {
+#pragma unroll
+      for (int f = 0; f < NI; f++) Yi(f) = ion(f, j);
+    }
+  }
+
   // -------------------------------------------------------------------------
   // RK4 drivers
   // -------------------------------------------------------------------------
@@ -1121,10 +1354,12 @@ struct Column {
 #pragma unroll
       for (int f = 0; f < NF; f++) Yv(f) = U[f];
     }
+    ion_store_y();
     const double dtin[4] = {0.5 * kp.dt, 0.5 * kp.dt, kp.dt, 0.5 * kp.dt};  // :160-377
 #pragma unroll 1
     for (int stg = 0; stg < 4; stg++) {
       ausm_stage(dtin[stg], stg);
+      ion_ghosts_from_y();
       bcs();
     }
   }
@@ -1135,9 +1370,11 @@ struct Column {
 #pragma unroll
       for (int f = 0; f < NF; f++) Yv(f) = U[f];
     }
+    ion_store_y();
 #pragma unroll 1
     for (int stg = 0; stg < 4; stg++) {
       rusanov_stage(stg);
+      ion_ghosts_from_y();
       bcs();
     }
   }
@@ -1153,11 +1390,11 @@ GV_FN double *fld(const gitm_vert_field &f, long long base, int k, int comp) {
   return f.ptr + base + (long long)(k - f.first_level) * f.s_lev + (long long)comp * f.s_comp;
 }
 
-template <int NT, int SOLVER, class Ctx>
+template <int NT, int SOLVER, bool IONS, class Ctx>
 GV_FN void run_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io,
                       int col, double *lds) {
-  Column<NT, Ctx> c(cx, kp, lds, col);
-  c.scr = kp.scratch + (long long)col * (2 * NF) * (kp.n_alts + 4);
+  Column<NT, Ctx, IONS> c(cx, kp, lds, col);
+  c.scr = kp.scratch + (long long)col * kp.scr_fields * (kp.n_alts + 4);
   const int nlb = io.n_lons * io.n_lats;
   const int ci = col % io.n_lons, cj = (col / io.n_lons) % io.n_lats, cb = col / nlb;
   const int k = c.k, nA = c.nA;
@@ -1201,6 +1438,24 @@ GV_FN void run_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io,
     c.U[F_LR] = GV_LD(rho);
 #undef GV_LD
   }
+  if (IONS) {
+    // LogINS (linear densities with UseImprovedIonAdvection) and IVel(up)
+    // (:120-130); SZA and MLat of the column for the nighttime BCs (:84-85)
+    const long long bi = col_base(io.idensity_s, ci, cj, cb);
+#pragma unroll
+    for (int f = 0; f < NI; f++) {
+      const double v = *fld(io.idensity_s, bi, k, f);
+      c.ion(f, c.j) = kp.improved_ions ? v : log(v);
+    }
+    c.ion(NI, c.j) = (k >= 1 && k <= nA)
+                         ? *fld(io.ivelocity, col_base(io.ivelocity, ci, cj, cb), k, 2)
+                         : 0.0;
+    if (kp.nighttime_ion_bcs) {
+      const double sza = io.sza.ptr[col_base(io.sza, ci, cj, cb)];
+      const double mlat = io.mlat.ptr[col_base(io.mlat, ci, cj, cb)];
+      c.plasma = sza > PI / 2 && ((mlat > 30.0 && mlat < 70.0) || (mlat > -60.0 && mlat < -30.0));
+    }
+  }
   c.setup(alt, grav, invr, dalt, mmm, gam, eddy, cmax, tu, coslat);
 
   for (int step = 0; step < kp.n_steps; step++) {
@@ -1232,6 +1487,19 @@ GV_FN void run_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io,
     c.U[F_T] = c.U[F_T] / tu;
 #pragma unroll
     for (int s = 0; s < NSP; s++) nds[s] = exp(c.U[F_LN + s]);
+    if (IONS) {
+      // IDensityS as the scatter writes it (:187-212) and the next gather
+      // reads it back: floored at 1 on 1..nAlts, or exp(LogINS)
+#pragma unroll
+      for (int f = 0; f < NI; f++) {
+        double &u = c.ion(f, c.j);
+        if (kp.improved_ions) {
+          if (c.cell() && u < 1) u = 1.0;
+        } else if (step + 1 < kp.n_steps) {
+          u = log(exp(u));
+        }
+      }
+    }
   }
   if (lv) {
     const long long bn = col_base(io.ndensity_s, ci, cj, cb);
@@ -1255,6 +1523,19 @@ GV_FN void run_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io,
     *fld(io.velocity, bv, k, 2) = c.U[F_VU];
     *fld(io.temperature, col_base(io.temperature, ci, cj, cb), k, 0) = c.U[F_T];
     *fld(io.rho, col_base(io.rho, ci, cj, cb), k, 0) = c.U[F_LR];
+    if (IONS) {
+      // ions and electrons = their sum (:187-212)
+      const long long bi = col_base(io.idensity_s, ci, cj, cb);
+      double e = 0.0;
+#pragma unroll
+      for (int f = 0; f < NI; f++) {
+        const double u = c.ion(f, c.j);
+        const double v = kp.improved_ions ? u : exp(u);
+        *fld(io.idensity_s, bi, k, f) = v;
+        e = e + v;
+      }
+      *fld(io.idensity_s, bi, k, NI) = e;
+    }
   }
 }
 

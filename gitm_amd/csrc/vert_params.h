@@ -49,7 +49,6 @@ inline int check_args(const gitm_vert_params *p, const gitm_vert_columns *c,
   if (p->use_msis || p->use_gswm_tides || p->use_waccm_tides || p->use_bc_perturbation ||
       p->during_perturb || p->use_damping)
     return GITM_VERT_EUNSUPPORTED;
-  if (p->use_ion_advection) return GITM_VERT_EUNSUPPORTED;
   if (!(p->dt > 0.0) || !(p->vert_tau != 0.0)) return GITM_VERT_EINVAL;
   for (int s = 0; s < NSP; s++)
     if (!(p->mass[s] > 0.0)) return GITM_VERT_EINVAL;
@@ -61,6 +60,11 @@ inline int check_args(const gitm_vert_params *p, const gitm_vert_columns *c,
   for (const gitm_vert_field *f : req)
     if (!field_ok(*f)) return GITM_VERT_EINVAL;
   if (c->cmax.first_level < -1 || c->cmax.first_level > 0) return GITM_VERT_EINVAL;
+  if (p->use_ion_advection) {
+    if (!field_ok(c->idensity_s) || !field_ok(c->ivelocity)) return GITM_VERT_EINVAL;
+    if (p->use_nighttime_ion_bcs && (!field_ok(c->sza) || !field_ok(c->mlat)))
+      return GITM_VERT_EINVAL;
+  }
   return GITM_VERT_OK;
 }
 
@@ -91,6 +95,7 @@ inline KParams make_kparams(const gitm_vert_params *p, const gitm_vert_columns *
   k.ions = p->use_ion_advection != 0;
   k.improved_ions = p->use_improved_ion_advection != 0;
   k.nighttime_ion_bcs = p->use_nighttime_ion_bcs != 0;
+  k.scr_fields = 0;  // set with the tile (scratch_fields)
   k.err = err;
   return k;
 }

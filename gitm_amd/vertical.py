@@ -32,6 +32,7 @@ STATE = ("nds", "vvel", "vel", "temp", "rho")
 OUTPUTS = ("logns", "nden")
 INPUTS = ("tempunit", "mmm", "gamma", "cmax", "eddy", "alt", "grav", "invr", "dalt",
           "coslat")
+ION_FIELDS = ("ions", "ivel", "sza", "mlat")  # read/written with UseIonAdvection
 
 _lib = None
 
@@ -193,13 +194,15 @@ class VerticalSolver:
 
 
 def to_device(cols: dict, device="cuda") -> dict:
-    """Copy the state and input arrays of ``cols`` to the device, adding zeroed
-    LogNS / NDensity outputs."""
+    """Copy the state and input arrays of ``cols`` (and the ion fields when
+    present) to the device, adding zeroed LogNS / NDensity outputs."""
     import torch
 
     out = {}
-    for k in STATE + INPUTS:
-        out[k] = torch.from_numpy(np.ascontiguousarray(cols[k], dtype=np.float64)).to(device)
+    for k in STATE + INPUTS + ION_FIELDS:
+        if k in cols:
+            out[k] = torch.from_numpy(np.ascontiguousarray(cols[k],
+                                                          dtype=np.float64)).to(device)
     out["logns"] = torch.zeros_like(out["nds"])
     out["nden"] = torch.zeros_like(out["temp"])
     return out

@@ -57,7 +57,11 @@ def params_from(sp) -> _abi.Params:
 
 
 def advance(cols: dict, sp, solver: int = 1, n_steps: int = 1, waves: int = 0,
-            cols_range=None) -> dict:
+            cols_range=None, device_powers: bool = False) -> dict:
+    """Run the host build of the column kernels.  ``device_powers`` evaluates
+    the friction coefficients' Temp**DiffExp as the GPU does (exp of a
+    product with the log); by default pow, as the reference does, so that
+    the result can be compared with the oracle bit for bit."""
     nc, L = cols["temp"].shape
     na = L - 4
     out = {k: np.ascontiguousarray(np.array(cols[k], dtype=np.float64))
@@ -67,6 +71,10 @@ def advance(cols: dict, sp, solver: int = 1, n_steps: int = 1, waves: int = 0,
     keep = {k: np.ascontiguousarray(cols[k], dtype=np.float64)
             for k in ("tempunit", "mmm", "gamma", "cmax", "eddy", "alt", "grav", "invr",
                       "dalt", "coslat")}
+    if sp.use_ion_advection:
+        out["ions"] = np.ascontiguousarray(np.array(cols["ions"], dtype=np.float64))
+        for k in ("ivel", "sza", "mlat"):
+            keep[k] = np.ascontiguousarray(cols[k], dtype=np.float64)
     ptrs = {k: v.ctypes.data for k, v in out.items()}
     ptrs.update({k: v.ctypes.data for k, v in keep.items()})
     c = _abi.columns_struct(ptrs, nc, na)
@@ -74,7 +82,7 @@ def advance(cols: dict, sp, solver: int = 1, n_steps: int = 1, waves: int = 0,
     e = _abi.Error()
     c0, c1 = cols_range if cols_range else (0, nc)
     rc = lib().gv_emu_advance(ctypes.byref(p), ctypes.byref(c), solver, n_steps, waves,
-                              c0, c1, ctypes.byref(e))
+                              c0, c1, int(device_powers), ctypes.byref(e))
     if rc:
         raise RuntimeError(f"emulated kernel: {_abi.ERRORS.get(rc, rc)} "
                            f"(column {e.column}, level {e.level})")

@@ -20,7 +20,8 @@ struct Shared {
   std::barrier<> bar;
   std::atomic<int> orflag{0};
   std::atomic<unsigned long long> err{~0ULL};
-  explicit Shared(int n) : bar(n) {}
+  bool device_powers;  // the GPU's exp(e*ln T) instead of the reference's pow
+  Shared(int n, bool dp) : bar(n), device_powers(dp) {}
 };
 
 struct EmuCtx {
@@ -42,24 +43,43 @@ struct EmuCtx {
     while (key < cur && !sh->err.compare_exchange_weak(cur, key)) {
     }
   }
+  void diff_powers(double temp, double (&out)[6]) const {
+    if (sh->device_powers) {
+      const double lt = log(temp);
+      for (int u = 0; u < 6; u++) out[u] = exp(gv::DEXU[u] * lt);
+    } else {
+      for (int u = 0; u < 6; u++) out[u] = pow(temp, gv::DEXU[u]);
+    }
+  }
 };
+
+template <int NT, bool IONS>
+void run_one(EmuCtx &cx, const gv::KParams &kp, const gitm_vert_columns &io, int c,
+             double *lds) {
+  if (kp.solver == GITM_VERT_AUSM)
+    gv::run_column<NT, GITM_VERT_AUSM, IONS>(cx, kp, io, c, lds);
+  else
+    gv::run_column<NT, GITM_VERT_RUSANOV, IONS>(cx, kp, io, c, lds);
+}
 
 template <int NT>
 unsigned long long run_all(const gv::KParams &kp, const gitm_vert_columns &io, int c0,
-                           int c1) {
-  const int solver = kp.solver;
+                           int c1, bool device_powers) {
   const int nthreads = kp.n_alts + 4;  // one thread per level
-  Shared sh(nthreads);
-  std::vector<double> lds(gv::lds_bytes(kp.n_alts) / sizeof(double) + 16, 0.0);
+  Shared sh(nthreads, device_powers);
+  // exactly the dynamic LDS a launch gets, so out-of-range indexing shows up
+  // under the host sanitizers
+  std::vector<double> lds(gv::lds_doubles(kp.n_alts, kp.ions != 0, gv::carry_in_lds(NT / 64)),
+                          0.0);
   std::vector<std::thread> th;
   for (int t = 0; t < nthreads; t++)
     th.emplace_back([&, t] {
       EmuCtx cx{t, &sh};
       for (int c = c0; c < c1; c++) {
-        if (solver == GITM_VERT_AUSM)
-          gv::run_column<NT, GITM_VERT_AUSM>(cx, kp, io, c, lds.data());
+        if (kp.ions)
+          run_one<NT, true>(cx, kp, io, c, lds.data());
         else
-          gv::run_column<NT, GITM_VERT_RUSANOV>(cx, kp, io, c, lds.data());
+          run_one<NT, false>(cx, kp, io, c, lds.data());
         cx.sync();
       }
     });
@@ -69,23 +89,26 @@ unsigned long long run_all(const gv::KParams &kp, const gitm_vert_columns &io, i
 
 }  // namespace
 
+// device_powers != 0 evaluates the friction powers as the GPU does
+// (exp(e*ln T)); 0 uses pow as the reference does, for bitwise checks.
 extern "C" int gv_emu_advance(const gitm_vert_params *p, const gitm_vert_columns *c,
                               int solver, int n_steps, int waves, int c0, int c1,
-                              gitm_vert_error *e) {
+                              int device_powers, gitm_vert_error *e) {
   int rc = gv::check_args(p, c, solver, n_steps);
   if (rc) return rc;
   gv::KParams kp = gv::make_kparams(p, c, solver, n_steps, nullptr);
-  std::vector<double> scratch((size_t)c->n_lons * c->n_lats * c->n_blocks * 2 * gv::NF *
-                              (c->n_alts + 4));
-  kp.scratch = scratch.data();
   if (waves <= 0) waves = gv::auto_waves(c->n_alts);
   if (64 * waves < c->n_alts + 4) return GITM_VERT_EINVAL;
+  kp.scr_fields = gv::scratch_fields(gv::carry_in_lds(waves), kp.ions != 0);
+  std::vector<double> scratch((size_t)c->n_lons * c->n_lats * c->n_blocks * kp.scr_fields *
+                              (c->n_alts + 4));
+  kp.scratch = scratch.data();
   unsigned long long key;
   switch (waves) {
-    case 1: key = run_all<64>(kp, *c, c0, c1); break;
-    case 2: key = run_all<128>(kp, *c, c0, c1); break;
-    case 3: key = run_all<192>(kp, *c, c0, c1); break;
-    case 4: key = run_all<256>(kp, *c, c0, c1); break;
+    case 1: key = run_all<64>(kp, *c, c0, c1, device_powers != 0); break;
+    case 2: key = run_all<128>(kp, *c, c0, c1, device_powers != 0); break;
+    case 3: key = run_all<192>(kp, *c, c0, c1, device_powers != 0); break;
+    case 4: key = run_all<256>(kp, *c, c0, c1, device_powers != 0); break;
     default: return GITM_VERT_EINVAL;
   }
   gitm_vert_error tmp;

@@ -5,7 +5,7 @@ outputs of the reference Fortran vertical step (advance_vertical ->
 advance_vertical_1d_ausm / _rusanov, compiled from the reference sources by
 ``make -C oracle ref``) after n steps.  Run from the repository root:
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [name ...]
 """
 from __future__ import annotations
 
@@ -38,17 +38,44 @@ CASES = [
     ("ausm_hot_dt5", 1, 2, 5, 47, {"dt": 5.0}, {"tinf": 1400.0}),
     ("ausm_ions_log", 1, 2, 6, 59,
      {"use_ion_advection": True, "use_improved_ion_advection": False}, {"ions": True}),
+    # nighttime plasmasphere columns with a TEC below MinTEC keep their top
+    # ion ghosts (set_vertical_bcs.Earth.f90:492-497,590)
+    ("ausm_ions_low_tec", 1, 2, 7, 19, {"use_ion_advection": True}, {"ions": True}),
+    # negative ion densities near the top are repaired before the top slope
+    # (check_for_negative_densities, :608-649)
+    ("rusanov_ions_negative_top", 0, 2, 8, 31, {"use_ion_advection": True},
+     {"ions": True}),
 ]
 
 
-def main() -> None:
+def tweak(name: str, sub: dict) -> None:
+    """Edits of the generated columns that steer them into rare branches."""
+    if name == "ausm_ions_low_tec":
+        sub["ions"] *= 1e-5           # TEC ~1e-4 TECU, far below MinTEC = 2
+        sub["ions"][:, :9] = np.maximum(sub["ions"][:, :9], 1.0)
+        sub["ions"][:, 9] = sub["ions"][:, :9].sum(axis=1)
+        sub["sza"][:] = 2.5           # night ...
+        sub["mlat"][::2] = 45.0       # ... half of them inside the plasmasphere band
+    elif name == "rusanov_ions_negative_top":
+        na = sub["temp"].shape[1] - 4
+        top = na + 1                  # offset of level nAlts
+        sub["ions"][0, 0, top - 1] = -5.0e3           # one negative level: averaged
+        sub["ions"][1, 1, top - 2:top] = -1.0e3       # two in a row: 0.95 x below
+        sub["ions"][2, 3, top + 1] = -2.0             # negative ghost at nAlts+1
+        sub["ions"][3, 2, top - 4] = -7.0             # deepest repairable level
+
+
+def main(only=None) -> None:
     if not ref.available():
         raise SystemExit("reference build missing: run `make -C oracle ref` first")
     spec = synth.CONFIGS["cpu_324x50"]
     for name, solver, steps, seed, stride, pover, gover in CASES:
+        if only and name not in only:
+            continue
         cols = synth.make_columns(spec, seed=seed, **gover)
         idx = np.arange(0, spec.ncols, stride)
         sub = {k: np.ascontiguousarray(v[idx]) for k, v in cols.items()}
+        tweak(name, sub)
         params = dataclasses.replace(synth.StepParams(), **pover)
         out = ref.advance(sub, params, solver=solver, n_steps=steps)
         meta = dict(solver=solver, steps=steps, seed=seed, columns=idx.tolist(),
@@ -62,4 +89,4 @@ def main() -> None:
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

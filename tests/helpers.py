@@ -9,9 +9,9 @@ import numpy as np
 # wind components): max|a - b| / max|b| over all columns and levels.
 # ``floored_errors`` gives the stricter elementwise ratio with a floor of 1e-3
 # of the component's largest magnitude, for reporting.
-POSITIVE = ("nds", "temp", "rho", "nden")
+POSITIVE = ("nds", "temp", "rho", "nden", "ions")
 SIGNED = ("vvel", "vel")
-ALL = ("nds", "vvel", "vel", "temp", "rho", "logns", "nden")
+ALL = ("nds", "vvel", "vel", "temp", "rho", "logns", "nden", "ions")
 
 
 def _components(x: np.ndarray):
