@@ -23,6 +23,7 @@ struct GpuCtx {
   __device__ void report(unsigned long long key) const { atomicMin(err, key); }
   // Temp**DiffExp as exp(DiffExp*ln Temp): one log and six exp in place of
   // fifteen pow (a pow costs five times an exp here); within a few ulp of pow.
+  __device__ static constexpr bool device_math() { return true; }
   __device__ void diff_powers(double temp, double (&out)[6]) const {
     const double lt = log(temp);
 #pragma unroll

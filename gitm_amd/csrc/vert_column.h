@@ -94,6 +94,7 @@ constexpr double FACT1 = 0.6250000, FACT2 = 0.0416667;
 struct KParams {
   double dt, beta, maxvv, min_tec, vert_tau, omega;
   double mass[NSP], tdc[NSP];
+  double lmass[NSP];  // log(mass), for the log-space forms of the device math
   double log_min_kp, log_max_kp, log_min_ku, log_max_ku;  // log(1e-19), ...
   int n_alts, n_steps, solver;
   int friction, coriolis, ions, improved_ions, nighttime_ion_bcs;
@@ -353,9 +354,17 @@ struct LevelStatic {
 };
 
 // ---------------------------------------------------------------------------
-// One column.  Ctx supplies lane(), sync(), sync_or(bool), report(key) and
+// One column.  Ctx supplies lane(), sync(), sync_or(bool), report(key),
 // diff_powers(temp, out[6]) = temp**DEXU[0..5] (the friction coefficients'
-// temperature dependence, calc_neutral_friction.f90:77-79).
+// temperature dependence, calc_neutral_friction.f90:77-79) and
+// device_math().  With device_math() the column uses the log-space forms of
+// a few expressions: log(m n_s / rho) = log m + ln n_s - ln rho,
+// log(m n_s) = log m + ln n_s, log(n_s k T) = ln n_s + log(k T), and the new
+// rho as the sum of m n_s instead of m exp(log n_s).  They replace 18 logs
+// and 6 exps per level and stage by one log, and differ from the
+// reference's rounding by an ulp of the logarithm, as the device's own
+// exp/log do.  The GPU always uses them; the host build of the kernels runs
+// either form, so that the reference forms can be checked bit for bit.
 // ---------------------------------------------------------------------------
 template <int NT, class Ctx, bool IONS>
 struct Column {
@@ -888,7 +897,9 @@ struct Column {
     {
       double lcs[NSP];
 #pragma unroll
-      for (int s = 0; s < NSP; s++) lcs[s] = log(m[s] * NS[s] / rho);
+      for (int s = 0; s < NSP; s++)
+        lcs[s] = cx.device_math() ? (kp.lmass[s] + U[F_LN + s]) - U[F_LR]
+                                  : log(m[s] * NS[s] / rho);
       grad_log_cons(lcs, glc);
     }
     double gr3[3], df3[3];
@@ -986,10 +997,19 @@ struct Column {
         GV_KT(6 + s, VL[s], VR[s]);
       }
       cx.sync();
+      if (cx.device_math()) {
+        const double ltk = log(T * KB);
 #pragma unroll
-      for (int s = 0; s < NSP; s++) {
-        xb(s, j) = log(rhos[s]);
-        xb(6 + s, j) = log(NS[s] * T * KB);
+        for (int s = 0; s < NSP; s++) {
+          xb(s, j) = kp.lmass[s] + U[F_LN + s];
+          xb(6 + s, j) = U[F_LN + s] + ltk;
+        }
+      } else {
+#pragma unroll
+        for (int s = 0; s < NSP; s++) {
+          xb(s, j) = log(rhos[s]);
+          xb(6 + s, j) = log(NS[s] * T * KB);
+        }
       }
       xb(12, j) = U[F_VE];
       xb(13, j) = U[F_VN];
@@ -1187,7 +1207,7 @@ struct Column {
 #pragma unroll
         for (int s = 0; s < NSP; s++) {
           const double nl = log(nns[s]);
-          sr = sr + m[s] * exp(nl);
+          sr = sr + m[s] * (cx.device_math() ? nns[s] : exp(nl));
           ausm_commit(stg, F_LN + s, nl);
         }
         ausm_commit(stg, F_LR, log(sr));
@@ -1232,7 +1252,9 @@ struct Column {
     }
     const double nt = exp(log(sum));  // NT = exp(LogNum) (:458-460)
 #pragma unroll
-    for (int s = 0; s < NSP; s++) lcs[s] = log(m[s] * NS[s] / rho);
+    for (int s = 0; s < NSP; s++)
+      lcs[s] = cx.device_math() ? (kp.lmass[s] + U[F_LN + s]) - U[F_LR]
+                                : log(m[s] * NS[s] / rho);
     // gradients and dissipation (:462-498)
     const double va[4] = {T, U[F_VE], U[F_VN], U[F_VU]};
     double ga[4], da[4];

@@ -81,6 +81,7 @@ inline KParams make_kparams(const gitm_vert_params *p, const gitm_vert_columns *
   for (int s = 0; s < NSP; s++) {
     k.mass[s] = p->mass[s];
     k.tdc[s] = p->thermal_diff_coef[s];
+    k.lmass[s] = log(p->mass[s]);
   }
   // alog(MinKpS) etc., vertical_solver_ausm.f90:1317-1332
   k.log_min_kp = log(1.0e-19);

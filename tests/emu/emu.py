@@ -57,11 +57,12 @@ def params_from(sp) -> _abi.Params:
 
 
 def advance(cols: dict, sp, solver: int = 1, n_steps: int = 1, waves: int = 0,
-            cols_range=None, device_powers: bool = False) -> dict:
-    """Run the host build of the column kernels.  ``device_powers`` evaluates
-    the friction coefficients' Temp**DiffExp as the GPU does (exp of a
-    product with the log); by default pow, as the reference does, so that
-    the result can be compared with the oracle bit for bit."""
+            cols_range=None, device_math: bool = False) -> dict:
+    """Run the host build of the column kernels.  ``device_math`` evaluates
+    the expressions the GPU computes in a different form as the GPU does
+    (friction powers as exp(e ln T), log-space sums); by default the
+    reference's forms, so that the result can be compared with the oracle
+    bit for bit."""
     nc, L = cols["temp"].shape
     na = L - 4
     out = {k: np.ascontiguousarray(np.array(cols[k], dtype=np.float64))
@@ -82,7 +83,7 @@ def advance(cols: dict, sp, solver: int = 1, n_steps: int = 1, waves: int = 0,
     e = _abi.Error()
     c0, c1 = cols_range if cols_range else (0, nc)
     rc = lib().gv_emu_advance(ctypes.byref(p), ctypes.byref(c), solver, n_steps, waves,
-                              c0, c1, int(device_powers), ctypes.byref(e))
+                              c0, c1, int(device_math), ctypes.byref(e))
     if rc:
         raise RuntimeError(f"emulated kernel: {_abi.ERRORS.get(rc, rc)} "
                            f"(column {e.column}, level {e.level})")

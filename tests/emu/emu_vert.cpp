@@ -20,8 +20,8 @@ struct Shared {
   std::barrier<> bar;
   std::atomic<int> orflag{0};
   std::atomic<unsigned long long> err{~0ULL};
-  bool device_powers;  // the GPU's exp(e*ln T) instead of the reference's pow
-  Shared(int n, bool dp) : bar(n), device_powers(dp) {}
+  bool device_math;  // the GPU's forms (exp(e ln T), log-space sums) instead of the reference's
+  Shared(int n, bool dm) : bar(n), device_math(dm) {}
 };
 
 struct EmuCtx {
@@ -43,8 +43,9 @@ struct EmuCtx {
     while (key < cur && !sh->err.compare_exchange_weak(cur, key)) {
     }
   }
+  bool device_math() const { return sh->device_math; }
   void diff_powers(double temp, double (&out)[6]) const {
-    if (sh->device_powers) {
+    if (sh->device_math) {
       const double lt = log(temp);
       for (int u = 0; u < 6; u++) out[u] = exp(gv::DEXU[u] * lt);
     } else {
@@ -64,9 +65,9 @@ void run_one(EmuCtx &cx, const gv::KParams &kp, const gitm_vert_columns &io, int
 
 template <int NT>
 unsigned long long run_all(const gv::KParams &kp, const gitm_vert_columns &io, int c0,
-                           int c1, bool device_powers) {
+                           int c1, bool device_math) {
   const int nthreads = kp.n_alts + 4;  // one thread per level
-  Shared sh(nthreads, device_powers);
+  Shared sh(nthreads, device_math);
   // exactly the dynamic LDS a launch gets, so out-of-range indexing shows up
   // under the host sanitizers
   std::vector<double> lds(gv::lds_doubles(kp.n_alts, kp.ions != 0, gv::carry_in_lds(NT / 64)),
@@ -89,11 +90,12 @@ unsigned long long run_all(const gv::KParams &kp, const gitm_vert_columns &io, i
 
 }  // namespace
 
-// device_powers != 0 evaluates the friction powers as the GPU does
-// (exp(e*ln T)); 0 uses pow as the reference does, for bitwise checks.
+// device_math != 0 evaluates as the GPU does (friction powers as exp(e ln T),
+// the log-space forms of Column); 0 uses the reference's forms, for bitwise
+// checks against the oracle.
 extern "C" int gv_emu_advance(const gitm_vert_params *p, const gitm_vert_columns *c,
                               int solver, int n_steps, int waves, int c0, int c1,
-                              int device_powers, gitm_vert_error *e) {
+                              int device_math, gitm_vert_error *e) {
   int rc = gv::check_args(p, c, solver, n_steps);
   if (rc) return rc;
   gv::KParams kp = gv::make_kparams(p, c, solver, n_steps, nullptr);
@@ -105,10 +107,10 @@ extern "C" int gv_emu_advance(const gitm_vert_params *p, const gitm_vert_columns
   kp.scratch = scratch.data();
   unsigned long long key;
   switch (waves) {
-    case 1: key = run_all<64>(kp, *c, c0, c1, device_powers != 0); break;
-    case 2: key = run_all<128>(kp, *c, c0, c1, device_powers != 0); break;
-    case 3: key = run_all<192>(kp, *c, c0, c1, device_powers != 0); break;
-    case 4: key = run_all<256>(kp, *c, c0, c1, device_powers != 0); break;
+    case 1: key = run_all<64>(kp, *c, c0, c1, device_math != 0); break;
+    case 2: key = run_all<128>(kp, *c, c0, c1, device_math != 0); break;
+    case 3: key = run_all<192>(kp, *c, c0, c1, device_math != 0); break;
+    case 4: key = run_all<256>(kp, *c, c0, c1, device_math != 0); break;
     default: return GITM_VERT_EINVAL;
   }
   gitm_vert_error tmp;

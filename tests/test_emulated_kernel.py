@@ -85,17 +85,18 @@ def test_emulated_kernel_ions_100_levels(solver, waves):
 
 
 @pytest.mark.parametrize("solver", [0, 1])
-def test_emulated_kernel_device_friction_powers(solver):
-    """The GPU evaluates Temp**DiffExp as exp(DiffExp*ln Temp).  With that
-    formula the host build stays within the parity bar of the oracle on 100
-    stretched levels (per step, from the oracle's state)."""
+def test_emulated_kernel_device_math(solver):
+    """The GPU evaluates Temp**DiffExp as exp(DiffExp*ln Temp) and a few
+    logarithms in log space.  With those forms the host build stays within
+    the parity bar of the oracle on 100 stretched levels (per step, from the
+    oracle's state)."""
     from tests.helpers import assert_parity
     cols = synth.make_columns(synth.GridSpec(1, 1, 100, stretched=True), seed=17)
     cols = subset(cols, [5, 50])
     p = synth.StepParams()
     state = {k: np.array(v) for k, v in cols.items()}
     for step in range(3):
-        out = emu.advance(state, p, solver=solver, n_steps=1, device_powers=True)
+        out = emu.advance(state, p, solver=solver, n_steps=1, device_math=True)
         ref = cpu.advance(state, p, solver=solver, n_steps=1)
         errs = assert_parity(out, ref, 1e-11, what=f"step {step + 1}")
         assert max(errs.values()) > 0.0  # the formula does change the rounding
