@@ -82,6 +82,7 @@ inline KParams make_kparams(const gitm_vert_params *p, const gitm_vert_columns *
     k.mass[s] = p->mass[s];
     k.tdc[s] = p->thermal_diff_coef[s];
     k.lmass[s] = log(p->mass[s]);
+    k.imass[s] = 1.0 / p->mass[s];
   }
   // alog(MinKpS) etc., vertical_solver_ausm.f90:1317-1332
   k.log_min_kp = log(1.0e-19);
@@ -101,8 +102,16 @@ inline KParams make_kparams(const gitm_vert_params *p, const gitm_vert_columns *
   return k;
 }
 
-// Smallest workgroup (multiple of 64 threads) holding the nAlts+4 levels.
-inline int auto_waves(int n_alts) { return (n_alts + 4 + 63) / 64; }
+// Workgroup tiles: wavefronts per column.  Two lanes per level, so a tile
+// of W waves holds columns of up to 32*W levels.
+inline bool valid_waves(int w) { return w == 1 || w == 2 || w == 4 || w == 6 || w == 8; }
+inline bool tile_fits(int w, int n_alts) { return 64 * w >= 2 * (n_alts + 4); }
+// Smallest tile holding the nAlts+4 levels.
+inline int auto_waves(int n_alts) {
+  for (int w : {1, 2, 4, 6, 8})
+    if (tile_fits(w, n_alts)) return w;
+  return 0;
+}
 
 inline void decode_error(unsigned long long key, gitm_vert_error *e) {
   if (key == ~0ULL) {
