@@ -2,7 +2,7 @@
 //
 // Entry points are declared in include/gitm_vert.h.  One workgroup per
 // column (gitm_vert_columns column index = blockIdx.x), NT = 64*W threads,
-// two lanes per level; see vert_column.h for the column algorithm.
+// one level per thread; see vert_column.h for the column algorithm.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -21,23 +21,9 @@ struct GpuCtx {
   __device__ void sync() const { __syncthreads(); }
   __device__ bool sync_or(bool p) const { return __syncthreads_or(p ? 1 : 0) != 0; }
   __device__ void report(unsigned long long key) const { atomicMin(err, key); }
-  // the other lane of the level (lanes 2j, 2j+1): a DPP quad permutation
-  // [1,0,3,2] within the wavefront
-  __device__ static int pair_int(int v) {
-    return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);
-  }
-  __device__ static double pair(double v) {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    const int lo = pair_int((int)(unsigned)b), hi = pair_int((int)(unsigned)(b >> 32));
-    return __longlong_as_double(
-        (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo));
-  }
-  // the device forms of vert_column.h (log-space sums, division-free
-  // friction coefficients)
+  // Temp**DiffExp as exp(DiffExp*ln Temp): one log and six exp in place of
+  // fifteen pow (a pow costs five times an exp here); within a few ulp of pow.
   __device__ static constexpr bool device_math() { return true; }
-  // Temp**(+-DiffExp) as exp(+-DiffExp*ln Temp): one log and six exp in
-  // place of fifteen pow (a pow costs five times an exp here); within a few
-  // ulp of pow.
   __device__ void diff_powers(double temp, double sign, double (&out)[6]) const {
     const double lt = log(temp);
 #pragma unroll
@@ -45,21 +31,18 @@ struct GpuCtx {
   }
 };
 
-// At least two wavefronts per SIMD: the register budget is 256 per lane.
 template <int NT, int SOLVER, bool IONS>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2)))
-vert_column_kernel(gv::KParams kp, gitm_vert_columns io) {
+__global__ void __launch_bounds__(NT) vert_column_kernel(gv::KParams kp, gitm_vert_columns io) {
   extern __shared__ double lds[];
   GpuCtx cx{kp.err};
   gv::run_column<NT, SOLVER, IONS>(cx, kp, io, (int)blockIdx.x, lds);
 }
 
-// Two threads per level: blockDim = 2*(nAlts+4) (<= NT, the compile-time
-// bound).
+// One thread per level: blockDim = nAlts+4 (<= NT, the compile-time bound).
 template <int NT, bool IONS>
 void launch_ions(int solver, dim3 grid, hipStream_t stream, const gv::KParams &kp,
                  const gitm_vert_columns &c) {
-  const dim3 block(2 * (kp.n_alts + 4));
+  const dim3 block(kp.n_alts + 4);
   const size_t lds = gv::lds_bytes(kp.n_alts, IONS, gv::carry_in_lds(NT / 64));
   if (solver == GITM_VERT_AUSM)
     hipLaunchKernelGGL((vert_column_kernel<NT, GITM_VERT_AUSM, IONS>), grid, block, lds,
@@ -198,9 +181,8 @@ int gitm_vert_advance(gitm_vert_handle *h, const gitm_vert_params *p,
   switch (waves) {
     case 1: launch<64>(solver, grid, h->stream, kp, *c); break;
     case 2: launch<128>(solver, grid, h->stream, kp, *c); break;
+    case 3: launch<192>(solver, grid, h->stream, kp, *c); break;
     case 4: launch<256>(solver, grid, h->stream, kp, *c); break;
-    case 6: launch<384>(solver, grid, h->stream, kp, *c); break;
-    case 8: launch<512>(solver, grid, h->stream, kp, *c); break;
     default: return GITM_VERT_EINVAL;
   }
   GV_HIP(hipGetLastError());

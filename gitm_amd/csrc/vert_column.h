@@ -1,32 +1,21 @@
 // vert_column.h -- the column-step kernels of GITM's vertical dynamics,
 // written for CDNA4 (gfx950).
 //
-// Mapping: one workgroup of NT = 64*W threads advances one (lon, lat) column,
-// two lanes per level.  Lanes 2j and 2j+1 own level k = j-1 (levels
-// -1..nAlts+2, L = nAlts+4, 2L <= NT); the even lane carries species 0-2 and
-// the odd lane species 3-5, and both carry the bulk fields (LogRho, Temp and
-// the three winds), which they compute identically.  The two lanes of a level
-// sit next to each other in one wavefront, so they trade values with a DPP
-// swap, without LDS or barriers; sums over the six species keep the
-// reference's left-to-right order by handing the partial sum of species 0-2
-// to the odd lane.  Splitting the species halves each lane's working set, so
-// that two wavefronts fit per SIMD: a single wavefront cannot issue FP64
-// instructions at the SIMD's rate.
-//
-// Everything a level needs from its neighbours (the i+-2 reconstruction
-// stencils, face states, the hydrostatic recurrence, ghost cells) is staged
-// through LDS.  All RK4 stages, the AUSM+-up / Rusanov fluxes, the boundary
-// conditions and the implicit neutral-friction solve run inside one launch;
-// the column goes to HBM once on entry and once on exit of n_steps vertical
-// steps.
+// Mapping: one workgroup of NT = 64*W threads advances one (lon, lat) column.
+// Thread j owns level k = j-1 (levels -1..nAlts+2, L = nAlts+4 <= NT) and
+// keeps that level's RK4 state in registers.  Everything a level needs from
+// its neighbours (the i+-2 reconstruction stencils, face states, the
+// hydrostatic recurrence, ghost cells) is staged through LDS.  All RK4
+// stages, the AUSM+-up / Rusanov fluxes, the boundary conditions and the
+// implicit neutral-friction solve run inside one launch; the column goes to
+// HBM once on entry and once on exit of n_steps vertical steps.
 //
 // Floating-point work is FP64 throughout and keeps the reference's operation
 // order (Fortran evaluates left to right; the build disables contraction), so
 // results differ from the reference only through the last-bit behaviour of
-// exp/log/pow/tanh and the few rewritten forms listed at Column.  Face
-// quantities that the reference computes twice (once as the M12 face of cell
-// i, once as the P12 face of cell i-1, src/vertical_solver_ausm.f90:2096-2104)
-// are computed once per face here.
+// exp/log/pow/tanh.  Face quantities that the reference computes twice (once
+// as the M12 face of cell i, once as the P12 face of cell i-1,
+// src/vertical_solver_ausm.f90:2096-2104) are computed once per face here.
 //
 // The same source compiles for the host (GV_HOST_EMULATION) where a test
 // harness runs the NT threads of a workgroup as CPU threads to check the
@@ -60,21 +49,15 @@ constexpr double PI = 3.141592653589793;  // ModConstants.f90:23
 constexpr int S_O = 0, S_N4S = 3, S_NO = 4;
 GV_FN constexpr bool photochemical(int s) { return s == 3 || s == 4; }
 
-// RK4-carried fields of a level (carry ids): bulk fields, then the log
-// densities and vertical winds of the six species
-constexpr int C_LR = 0;   // LogRho
-constexpr int C_T = 1;    // Temp (= Temperature*TempUnit)
-constexpr int C_VE = 2;   // Vel_GD(iEast_)
-constexpr int C_VN = 3;   // Vel_GD(iNorth_)
-constexpr int C_VU = 4;   // Vel_GD(iUp_)
-constexpr int C_LN = 5;   // LogNS(1:6)
-constexpr int C_W = 11;   // VertVel(1:6)
+// per-level state kept in registers
+constexpr int F_LR = 0;   // LogRho
+constexpr int F_T = 1;    // Temp (= Temperature*TempUnit)
+constexpr int F_LN = 2;   // LogNS(1:6)
+constexpr int F_W = 8;    // VertVel(1:6)
+constexpr int F_VE = 14;  // Vel_GD(iEast_)
+constexpr int F_VN = 15;  // Vel_GD(iNorth_)
+constexpr int F_VU = 16;  // Vel_GD(iUp_)
 constexpr int NF = 17;
-constexpr int NB = 5;     // bulk fields
-// species per lane, and the per-lane state U: the bulk fields (same ids as
-// above) and the lane's three log densities and vertical winds
-constexpr int NSH = 3;
-constexpr int U_LN = 5, U_W = 8, NU = 11;
 // ion state (UseIonAdvection): LogINS of the nIons-1 = 9 ions (electrons are
 // their sum); only the first nIonsAdvect = 2 are advected (ModEarth.f90:39)
 constexpr int NI = 9;
@@ -122,11 +105,10 @@ struct KParams {
 };
 
 // Where the RK4 carries (Y and acc, 2*NF per level) live: in LDS for columns
-// of up to six wavefronts (nAlts <= 188; they then cost 29 KiB at nAlts = 100
-// and leave room for two columns per CU), in device memory for the tallest
-// columns, whose LDS would not fit.  Ion carries always live in device
-// memory.
-GV_HD constexpr bool carry_in_lds(int waves) { return waves <= 6; }
+// of up to three wavefronts (they then cost 29 KiB at nAlts = 100 and leave
+// room for two columns per CU), in device memory for the tallest columns,
+// whose LDS would not fit.  Ion carries always live in device memory.
+GV_HD constexpr bool carry_in_lds(int waves) { return waves <= 3; }
 GV_HD constexpr int scratch_fields(bool carry_lds, bool ions) {
   return (carry_lds ? 0 : 2 * NF) + (ions ? 2 * NI : 0);
 }
@@ -398,23 +380,17 @@ struct LevelStatic {
 
 // ---------------------------------------------------------------------------
 // One column.  Ctx supplies lane(), sync(), sync_or(bool), report(key),
-// pair(double) / pair_int(int) (the value of the other lane of the level),
 // diff_powers(temp, sign, out[6]) = temp**(sign*DEXU[0..5]) (the friction
 // coefficients' temperature dependence, calc_neutral_friction.f90:77-79) and
 // device_math().  With device_math() the column uses the log-space forms of
 // a few expressions: log(m n_s / rho) = log m + ln n_s - ln rho,
 // log(m n_s) = log m + ln n_s, log(n_s k T) = ln n_s + log(k T), the new
 // rho as the sum of m n_s instead of m exp(log n_s), and the friction
-// coefficients without divisions.  They replace 18 logs and 6 exps per level
-// and stage by one log, and differ from the reference's rounding by an ulp of
-// the logarithm, as the device's own exp/log do.  The GPU always uses them;
-// the host build of the kernels runs either form, so that the reference
-// forms can be checked bit for bit.
-//
-// pair() and pair_int() must be reached by both lanes of every level
-// (uniform control flow): the device swaps registers within the wavefront.
-// Per-level values both lanes hold (bulk fields, tables) are written to LDS
-// by the even lane only.
+// coefficients without divisions.  They replace 18 logs
+// and 6 exps per level and stage by one log, and differ from the
+// reference's rounding by an ulp of the logarithm, as the device's own
+// exp/log do.  The GPU always uses them; the host build of the kernels runs
+// either form, so that the reference forms can be checked bit for bit.
 // ---------------------------------------------------------------------------
 template <int NT, class Ctx, bool IONS>
 struct Column {
@@ -422,23 +398,18 @@ struct Column {
   Ctx &cx;
   const KParams &kp;
   double *lds;
-  const int q;   // lane
-  const int h;   // 0: species 0-2, 1: species 3-5
-  const int s0;  // first species of the lane
-  const bool w0; // the lane that writes per-level values
   const int j, k, nA, L, P;
   const bool lv;  // owns a level
   int col;
   LevelStatic S;
-  double U[NU];
-  double *scr;  // this column's device-memory carries
+  double U[NF];
+  double *scr;  // this column's RK4 carries (Y, acc) in device memory
   double centrifugal, coriolis;
   bool plasma = false;  // nighttime plasmasphere column (bc :492-497)
 
   GV_MEM Column(Ctx &c, const KParams &p, double *l, int column)
-      : cx(c), kp(p), lds(l), q(c.lane()), h(c.lane() & 1), s0(3 * (c.lane() & 1)),
-        w0((c.lane() & 1) == 0), j(c.lane() >> 1), k((c.lane() >> 1) - 1), nA(p.n_alts),
-        L(p.n_alts + 4), P(p.n_alts + 8), lv((c.lane() >> 1) < p.n_alts + 4), col(column) {}
+      : cx(c), kp(p), lds(l), j(c.lane()), k(c.lane() - 1), nA(p.n_alts),
+        L(p.n_alts + 4), P(p.n_alts + 8), lv(c.lane() < p.n_alts + 4), col(column) {}
 
   // record layout: tables | exchange buffers | carries Y, acc | ions
   static constexpr int RS = record_len(IONS, CARRY_LDS);
@@ -452,15 +423,13 @@ struct Column {
   GV_MEM int &ib(int jj) const {
     return reinterpret_cast<int *>(lds + P * RS + NSCAL)[jj + 2];
   }
-  // RK4 carries of the owned level (carry id c): the stage-1 state Y and the
-  // running combination acc.  They are read and written only at the end of a
-  // stage, so they live in LDS, or for the tallest columns in device memory
-  // ([2*NF][L] per column, coalesced), instead of registers.  Each lane
-  // touches the carries of its own species; the bulk ones are the even
-  // lane's.
-  GV_MEM double &Yv(int c) const { return CARRY_LDS ? rec(j)[OFF_Y + c] : scr[c * L + j]; }
-  GV_MEM double &Av(int c) const {
-    return CARRY_LDS ? rec(j)[OFF_Y + NF + c] : scr[(NF + c) * L + j];
+  // RK4 carries of the owned level: the stage-1 state Y and the running
+  // combination acc.  They are read and written only at the end of a stage,
+  // so they live in LDS, or for the tallest columns in device memory
+  // ([2*NF][L] per column, coalesced), instead of registers.
+  GV_MEM double &Yv(int f) const { return CARRY_LDS ? rec(j)[OFF_Y + f] : scr[f * L + j]; }
+  GV_MEM double &Av(int f) const {
+    return CARRY_LDS ? rec(j)[OFF_Y + NF + f] : scr[(NF + f) * L + j];
   }
   // Ions: LogINS of level jj-1 (field NI holds IVel(up)) in LDS, so that the
   // boundary lanes can reach the top levels; their carries follow the
@@ -470,93 +439,46 @@ struct Column {
   GV_MEM double &Yi(int f) const { return scr[(ION_SCR + f) * L + j]; }
   GV_MEM double &Ai(int f) const { return scr[(ION_SCR + NI + f) * L + j]; }
 
-  // carry id of U slot u
-  GV_MEM int cid(int u) const {
-    return u < NB ? u : (u < U_W ? C_LN + s0 + (u - U_LN) : C_W + s0 + (u - U_W));
-  }
-
-  // --- the two lanes of a level -------------------------------------------
-  // the even lane's value on both lanes
-  GV_MEM double from_even(double v) {
-    const double o = cx.pair(v);
-    return h ? o : v;
-  }
-  // ((((0 + x_0) + x_1) + ... ) + x_5) over the six species, in that order,
-  // on both lanes; each lane holds x of its own three species
-  GV_MEM double sum6(const double (&x)[NSH]) {
-    double a = 0.0;
-#pragma unroll
-    for (int u = 0; u < NSH; u++) a = a + x[u];
-    double b = cx.pair(a);  // odd lane: the sum over species 0-2
-#pragma unroll
-    for (int u = 0; u < NSH; u++) b = b + x[u];
-    const double c = cx.pair(b);  // even lane: the full sum
-    return h ? b : c;
-  }
-  // all six species of a per-species value
-  GV_MEM void gather6(const double (&x)[NSH], double (&all)[NSP]) {
-#pragma unroll
-    for (int u = 0; u < NSH; u++) {
-      const double o = cx.pair(x[u]);
-      all[u] = h ? o : x[u];
-      all[NSH + u] = h ? x[u] : o;
-    }
-  }
-
   // advance_vertical_1d_ausm (vertical_solver_ausm.f90:167-417) for one
   // field of an interior level: stage k+1 input = Y + (New - U), and the
   // final (1/3)*(-Y + S2 + 2*S3 + S4 + (New_4 - U_4)) accumulated in the
   // reference's left-to-right order.
-  GV_MEM void ausm_commit(int stg, int u, double nv) {
-    const int c = cid(u);
+  GV_MEM void ausm_commit(int stg, int f, double nv) {
     if (stg < 3) {
-      const double y = Yv(c);
-      const double nxt = y + (nv - U[u]);
+      const double y = Yv(f);
+      const double nxt = y + (nv - U[f]);
       if (stg == 0)
-        Av(c) = -1.0 * y + nxt;
+        Av(f) = -1.0 * y + nxt;
       else if (stg == 1)
-        Av(c) = Av(c) + 2.0 * nxt;
+        Av(f) = Av(f) + 2.0 * nxt;
       else
-        Av(c) = Av(c) + nxt;
-      U[u] = nxt;
+        Av(f) = Av(f) + nxt;
+      U[f] = nxt;
     } else {
-      U[u] = (1.0 / 3.0) * (Av(c) + (nv - U[u]));
+      U[f] = (1.0 / 3.0) * (Av(f) + (nv - U[f]));
     }
   }
 
   // advance_vertical_1d_rusanov (vertical_solver_rusanov.f90:120-369):
   // K = New - U; Y + K/2, Y + K/2, Y + K, then Y + (K1 + 2K2 + 2K3 + K4)/6.
-  GV_MEM void rusanov_commit(int stg, int u, double nv) {
-    const int c = cid(u);
-    const double K = nv - U[u];
-    const double y = Yv(c);
+  GV_MEM void rusanov_commit(int stg, int f, double nv) {
+    const double K = nv - U[f];
+    const double y = Yv(f);
     if (stg == 0) {
-      Av(c) = K;
-      U[u] = y + 0.5 * K;
+      Av(f) = K;
+      U[f] = y + 0.5 * K;
     } else if (stg == 1) {
-      Av(c) = Av(c) + 2.0 * K;
-      U[u] = y + 0.5 * K;
+      Av(f) = Av(f) + 2.0 * K;
+      U[f] = y + 0.5 * K;
     } else if (stg == 2) {
-      Av(c) = Av(c) + 2.0 * K;
-      U[u] = y + K;
+      Av(f) = Av(f) + 2.0 * K;
+      U[f] = y + K;
     } else {
-      U[u] = y + (1.0 / 6.0) * (Av(c) + K);
+      U[f] = y + (1.0 / 6.0) * (Av(f) + K);
     }
   }
-  // A bulk field: the even lane commits (it owns the bulk carries), the odd
-  // lane takes the result afterwards (mirror_bulk).
-  GV_MEM void ausm_commit_bulk(int stg, int u, double nv) {
-    if (w0) ausm_commit(stg, u, nv);
-  }
-  GV_MEM void rusanov_commit_bulk(int stg, int u, double nv) {
-    if (w0) rusanov_commit(stg, u, nv);
-  }
-  GV_MEM void mirror_bulk() {
-#pragma unroll
-    for (int u = 0; u < NB; u++) U[u] = from_even(U[u]);
-  }
 
-  // The same two combinations for an ion field held in LDS (even lane).
+  // The same two combinations for an ion field held in LDS.
   GV_MEM void ausm_commit_ion(int stg, int f, double nv) {
     double &u = ion(f, j);
     if (stg < 3) {
@@ -592,26 +514,16 @@ struct Column {
   }
 
   // calc_neutral_friction of the owned level, in the device or the
-  // reference form (see neutral_friction).  Both lanes solve the level's
-  // system with all six species; each keeps its own three winds.
-  GV_MEM bool friction(double dtin, double (&w)[NSH], double nt, const double (&ns)[NSH],
-                       const double (&glc)[NSH], double T) {
-    double wa[NSP], na[NSP], ga[NSP];
-    gather6(w, wa);
-    gather6(ns, na);
-    gather6(glc, ga);
+  // reference form (see neutral_friction).
+  GV_MEM bool friction(double dtin, double (&w)[NSP], double nt, const double (&ns)[NSP],
+                       const double (&glc)[NSP], double T) {
     double tp[6];
-    bool ok;
     if (cx.device_math()) {
       cx.diff_powers(T, -1.0, tp);
-      ok = neutral_friction<true>(kp, dtin, wa, nt, na, ga, st(A_EDDY, j), T, tp);
-    } else {
-      cx.diff_powers(T, 1.0, tp);
-      ok = neutral_friction<false>(kp, dtin, wa, nt, na, ga, st(A_EDDY, j), T, tp);
+      return neutral_friction<true>(kp, dtin, w, nt, ns, glc, st(A_EDDY, j), T, tp);
     }
-#pragma unroll
-    for (int u = 0; u < NSH; u++) w[u] = h ? wa[NSH + u] : wa[u];
-    return ok;
+    cx.diff_powers(T, 1.0, tp);
+    return neutral_friction<false>(kp, dtin, w, nt, ns, glc, st(A_EDDY, j), T, tp);
   }
 
   GV_MEM bool face() const { return k >= 0 && k <= nA; }
@@ -627,111 +539,100 @@ struct Column {
     centrifugal = (coslat * kp.omega) * (coslat * kp.omega);  // :103
     coriolis = 2 * coslat * kp.omega;                         // :104
     S.eg = grav + centrifugal / invr;
+    st(A_INVR, j) = invr;
+    st(A_EDDY, j) = eddy;
     S.tu = tu;
+    st(A_GRAV, j) = grav;
     S.gm1 = gam - 1.0;
     const double rd = 1.0 / invr;  // RadialDistance_C (ausm :595-597)
-    if (w0) {
-      st(A_INVR, j) = invr;
-      st(A_EDDY, j) = eddy;
-      st(A_GRAV, j) = grav;
-      xb(0, j) = alt;
-      xb(1, j) = S.eg;
-      xb(2, j) = gam;
-      xb(3, j) = cmax;
-      xb(4, j) = rd;
-      if (j == 0) sc(SC_MMM_M1) = mmm;
-      if (j == 1) {
-        sc(SC_MMM_0) = mmm;
-        sc(SC_ALT0) = alt;
-      }
+    xb(0, j) = alt;
+    xb(1, j) = S.eg;
+    xb(2, j) = gam;
+    xb(3, j) = cmax;
+    xb(4, j) = rd;
+    if (j == 0) sc(SC_MMM_M1) = mmm;
+    if (j == 1) {
+      sc(SC_MMM_0) = mmm;
+      sc(SC_ALT0) = alt;
     }
     cx.sync();
     // dAlt_F(k) = Alt(k) - Alt(k-1), dAlt_F(-1) = dAlt_F(0)  (:149-151)
     const double dF = (j >= 1) ? alt - xb(0, j - 1) : xb(0, 1) - alt;
+    st(A_DF, j) = dF;
+    st(A_IF, j) = 1.0 / dF;
+    st(A_DC, j) = dalt;
+    // Rusanov dissipation speed of face k-1|k: 0.5*max(cMax(k-1), cMax(k))
+    st(A_CMF, j) = 0.5 * dmax(xb(3, j - 1), cmax);
     S.mgh = (j >= 1) ? 0.5 * (S.eg + xb(1, j - 1)) : 0.0;
-    if (w0) {
-      st(A_DF, j) = dF;
-      st(A_IF, j) = 1.0 / dF;
-      st(A_DC, j) = dalt;
-      // Rusanov dissipation speed of face k-1|k: 0.5*max(cMax(k-1), cMax(k))
-      st(A_CMF, j) = 0.5 * dmax(xb(3, j - 1), cmax);
-      if (j == 1) sc(SC_MG0) = S.mgh;
-      if (j == nA + 2) sc(SC_MGT1) = S.mgh;
-      if (j == nA + 3) sc(SC_MGT2) = S.mgh;
-      // Kp/Ku blends, vertical_solver_ausm.f90:1320-1352 (same for all species)
-      if (k >= 1 && k <= nA) {
-        const double t = (1.0 + tanh((alt - 160.0e+03) / 15.0e+03));
-        st(A_KP, j) = exp(kp.log_min_kp + 0.5 * (kp.log_max_kp - kp.log_min_kp) * t);
-        st(A_KU, j) = exp(kp.log_min_ku + 0.5 * (kp.log_max_ku - kp.log_min_ku) * t);
-      } else {
-        st(A_KP, j) = 0.0;
-        st(A_KU, j) = 0.0;
-      }
-      // spherical areas and volume, vertical_solver_ausm.f90:2029-2040
-      {
-        const double LR = 0.5 * (rd + xb(4, j - 1));
-        const double RR = 0.5 * (rd + xb(4, j + 1));
-        st(A_AP, j) = RR * RR;
-        st(A_AM, j) = LR * LR;
-        st(A_VOL, j) = (1.0 / 3.0) * (RR * RR * RR - LR * LR * LR);
-      }
-      st(A_HF, j) = 0.35 * exp(-1.0 * (alt - sc(SC_ALT0)) / (45.0e+03));
+    if (j == 1) sc(SC_MG0) = S.mgh;
+    if (j == nA + 2) sc(SC_MGT1) = S.mgh;
+    if (j == nA + 3) sc(SC_MGT2) = S.mgh;
+    // Kp/Ku blends, vertical_solver_ausm.f90:1320-1352 (same for all species)
+    if (k >= 1 && k <= nA) {
+      const double t = (1.0 + tanh((alt - 160.0e+03) / 15.0e+03));
+      st(A_KP, j) = exp(kp.log_min_kp + 0.5 * (kp.log_max_kp - kp.log_min_kp) * t);
+      st(A_KU, j) = exp(kp.log_min_ku + 0.5 * (kp.log_max_ku - kp.log_min_ku) * t);
+    } else {
+      st(A_KP, j) = 0.0;
+      st(A_KU, j) = 0.0;
     }
+    // spherical areas and volume, vertical_solver_ausm.f90:2029-2040
+    {
+      const double LR = 0.5 * (rd + xb(4, j - 1));
+      const double RR = 0.5 * (rd + xb(4, j + 1));
+      st(A_AP, j) = RR * RR;
+      st(A_AM, j) = LR * LR;
+      st(A_VOL, j) = (1.0 / 3.0) * (RR * RR * RR - LR * LR * LR);
+    }
+    st(A_HF, j) = 0.35 * exp(-1.0 * (alt - sc(SC_ALT0)) / (45.0e+03));
     cx.sync();
     // gamma face states (calc_kt_facevalues of Gamma_1d, ausm :1535)
     double dlg = 0.0;
     if (k >= 0 && k <= nA + 1)
       dlg = limiter_mc(kp.beta, (xb(2, j + 1) - gam) * st(A_IF, j + 1),
                        (gam - xb(2, j - 1)) * st(A_IF, j));
-    if (w0) {
-      xb(5, j) = dlg;
-      // 5-point gradient weights (ausm :649-676)
-      if (k >= 1 && k <= nA) {
-        const double h1 = st(A_DF, j - 1), h2 = dF, h3 = st(A_DF, j + 1),
-                     h4 = st(A_DF, j + 2);
-        const double MeshH2 = h2 + h1;
-        const double MeshH3 = h3 + h2 + h1;
-        const double MeshH4 = h4 + h3 + h2 + h1;
-        st(A_GLC0 + 0, j) = (h2 * h3 * (h3 + h4)) / (h1 * MeshH2 * MeshH3 * MeshH4);
-        st(A_GLC0 + 1, j) =
-            -1.0 * (MeshH2 * h3 * (h3 + h4)) / (h1 * h2 * (h2 + h3) * (h2 + h3 + h4));
-        st(A_GLC0 + 3, j) = MeshH2 * h2 * (h4 + h3) / (MeshH3 * (h2 + h3) * h3 * h4);
-        st(A_GLC0 + 4, j) =
-            -1.0 * MeshH2 * h2 * h3 / (MeshH4 * (h2 + h3 + h4) * (h3 + h4) * h4);
-        st(A_GLC0 + 2, j) = (h2 * h3 * (h3 + h4) + MeshH2 * h3 * (h3 + h4) -
-                             MeshH2 * h2 * (h3 + h4) - MeshH2 * h2 * h3) /
-                            (MeshH2 * h2 * h3 * (h3 + h4));
-      } else {
-        for (int a = 0; a < 5; a++) st(A_GLC0 + a, j) = 0.0;
-      }
-      // bottom extrapolation weights for levels 0 and -1 (bc :228-248)
-      if (j == 0) {
-        bc_fwd_coefs(st(A_DF, 3), st(A_DF, 4), st(A_DF, 5), st(A_DF, 6), &sc(SC_CA));
-        bc_fwd_coefs(st(A_DF, 2), st(A_DF, 3), st(A_DF, 4), st(A_DF, 5), &sc(SC_CB));
-      }
-      if (IONS && j == 1) {
-        // top ion slopes for levels nAlts+1 and nAlts+2 (bc :505-525)
-        bc_bwd_coefs(st(A_DF, nA + 1), st(A_DF, nA), st(A_DF, nA - 1), st(A_DF, nA - 2),
-                     &sc(SC_CTA));
-        bc_bwd_coefs(st(A_DF, nA + 2), st(A_DF, nA + 1), st(A_DF, nA), st(A_DF, nA - 1),
-                     &sc(SC_CTB));
-      }
+    xb(5, j) = dlg;
+    // 5-point gradient weights (ausm :649-676)
+    if (k >= 1 && k <= nA) {
+      const double h1 = st(A_DF, j - 1), h2 = dF, h3 = st(A_DF, j + 1),
+                   h4 = st(A_DF, j + 2);
+      const double MeshH2 = h2 + h1;
+      const double MeshH3 = h3 + h2 + h1;
+      const double MeshH4 = h4 + h3 + h2 + h1;
+      st(A_GLC0 + 0, j) = (h2 * h3 * (h3 + h4)) / (h1 * MeshH2 * MeshH3 * MeshH4);
+      st(A_GLC0 + 1, j) = -1.0 * (MeshH2 * h3 * (h3 + h4)) / (h1 * h2 * (h2 + h3) * (h2 + h3 + h4));
+      st(A_GLC0 + 3, j) = MeshH2 * h2 * (h4 + h3) / (MeshH3 * (h2 + h3) * h3 * h4);
+      st(A_GLC0 + 4, j) = -1.0 * MeshH2 * h2 * h3 / (MeshH4 * (h2 + h3 + h4) * (h3 + h4) * h4);
+      st(A_GLC0 + 2, j) = (h2 * h3 * (h3 + h4) + MeshH2 * h3 * (h3 + h4) -
+                  MeshH2 * h2 * (h3 + h4) - MeshH2 * h2 * h3) /
+                 (MeshH2 * h2 * h3 * (h3 + h4));
+    } else {
+      for (int q = 0; q < 5; q++) st(A_GLC0 + q, j) = 0.0;
+    }
+    // bottom extrapolation weights for levels 0 and -1 (bc :228-248)
+    if (j == 0) {
+      bc_fwd_coefs(st(A_DF, 3), st(A_DF, 4), st(A_DF, 5), st(A_DF, 6), &sc(SC_CA));
+      bc_fwd_coefs(st(A_DF, 2), st(A_DF, 3), st(A_DF, 4), st(A_DF, 5), &sc(SC_CB));
+    }
+    if (IONS && j == 1) {
+      // top ion slopes for levels nAlts+1 and nAlts+2 (bc :505-525)
+      bc_bwd_coefs(st(A_DF, nA + 1), st(A_DF, nA), st(A_DF, nA - 1), st(A_DF, nA - 2),
+                   &sc(SC_CTA));
+      bc_bwd_coefs(st(A_DF, nA + 2), st(A_DF, nA + 1), st(A_DF, nA), st(A_DF, nA - 1),
+                   &sc(SC_CTB));
     }
     cx.sync();
-    if (w0) {
-      if (face()) {
-        const double gL = gam + 0.5 * dlg * dF;
-        const double gR = xb(2, j + 1) - 0.5 * xb(5, j + 1) * st(A_DF, j + 1);
-        st(A_GL1, j) = 1.0 / (gL - 1.0);
-        st(A_GL2, j) = gL / (gL - 1.0);
-        st(A_GL3, j) = 2.0 * ((gL - 1.0) / (gL + 1.0));
-        st(A_GR1, j) = 1.0 / (gR - 1.0);
-        st(A_GR2, j) = gR / (gR - 1.0);
-        st(A_GR3, j) = 2.0 * ((gR - 1.0) / (gR + 1.0));
-      } else {
-        st(A_GL1, j) = st(A_GL2, j) = st(A_GL3, j) = 0.0;
-        st(A_GR1, j) = st(A_GR2, j) = st(A_GR3, j) = 0.0;
-      }
+    if (face()) {
+      const double gL = gam + 0.5 * dlg * dF;
+      const double gR = xb(2, j + 1) - 0.5 * xb(5, j + 1) * st(A_DF, j + 1);
+      st(A_GL1, j) = 1.0 / (gL - 1.0);
+      st(A_GL2, j) = gL / (gL - 1.0);
+      st(A_GL3, j) = 2.0 * ((gL - 1.0) / (gL + 1.0));
+      st(A_GR1, j) = 1.0 / (gR - 1.0);
+      st(A_GR2, j) = gR / (gR - 1.0);
+      st(A_GR3, j) = 2.0 * ((gR - 1.0) / (gR + 1.0));
+    } else {
+      st(A_GL1, j) = st(A_GL2, j) = st(A_GL3, j) = st(A_GR1, j) = st(A_GR2, j) = st(A_GR3, j) = 0.0;
     }
     cx.sync();
   }
@@ -740,26 +641,20 @@ struct Column {
   // set_vertical_bcs, set_vertical_bcs.Earth.f90:8-652 (neutrals, UseMSIS=F,
   // no tides, no BC perturbation).  Ghost values depend only on the interior
   // and on Temp(0) and LogNS(0) of the non-photochemical species, which the
-  // BCs never change.  Lane q < 6 takes species q at the bottom, lane 8+q at
-  // the top.
+  // BCs never change.
   // -------------------------------------------------------------------------
   GV_MEM void bcs() {
     // stage ln(6), w(6), T, vE, vN, vU
 #pragma unroll
-    for (int u = 0; u < NSH; u++) {
-      xb(s0 + u, j) = U[U_LN + u];
-      xb(NSP + s0 + u, j) = U[U_W + u];
-    }
-    if (w0) {
-      xb(12, j) = U[C_T];
-      xb(13, j) = U[C_VE];
-      xb(14, j) = U[C_VN];
-      xb(15, j) = U[C_VU];
-    }
+    for (int f = 0; f < 12; f++) xb(f, j) = U[F_LN + f];
+    xb(12, j) = U[F_T];
+    xb(13, j) = U[F_VE];
+    xb(14, j) = U[F_VN];
+    xb(15, j) = U[F_VU];
     cx.sync();
-    if (q < NSP) {
-      // bottom, one species per lane (:198-272, :277-344, :382-395)
-      const int sp = q;
+    if (j < NSP) {
+      // bottom, one species per thread (:198-272, :277-344, :382-395)
+      const int sp = j;
       double *CA = &sc(SC_CA), *CB = &sc(SC_CB);
       const double dF1 = st(A_DF, 2), dF0 = st(A_DF, 1), dFm1 = st(A_DF, 0);
       double ln0 = xb(sp, 1), ln1 = xb(sp, 2), ln2 = xb(sp, 3), ln3 = xb(sp, 4),
@@ -788,11 +683,12 @@ struct Column {
           nm1 = log(ns);
         }
       }
-      const double w1 = xb(NSP + sp, 2), w2 = xb(NSP + sp, 3), w3 = xb(NSP + sp, 4),
-                   w4 = xb(NSP + sp, 5), w5 = xb(NSP + sp, 6);
-      const double w0v = w1 - dF1 * (w1 * CA[0] + w2 * CA[1] + w3 * CA[2] + w4 * CA[3] + w5 * CA[4]);
-      const double wm1 = w0v - dF0 * (w0v * CB[0] + w1 * CB[1] + w2 * CB[2] + w3 * CB[3] + w4 * CB[4]);
-      double w0r = w0v, wm1r = wm1;
+      const double w1 = xb(F_W - F_LN + sp, 2), w2 = xb(F_W - F_LN + sp, 3),
+                   w3 = xb(F_W - F_LN + sp, 4), w4 = xb(F_W - F_LN + sp, 5),
+                   w5 = xb(F_W - F_LN + sp, 6);
+      const double w0 = w1 - dF1 * (w1 * CA[0] + w2 * CA[1] + w3 * CA[2] + w4 * CA[3] + w5 * CA[4]);
+      const double wm1 = w0 - dF0 * (w0 * CB[0] + w1 * CB[1] + w2 * CB[2] + w3 * CB[3] + w4 * CB[4]);
+      double w0r = w0, wm1r = wm1;
       if (photochemical(sp) && w1 > 0.0) {
         w0r = -1.0 * w1;
         wm1r = -1.0 * w1;
@@ -800,16 +696,16 @@ struct Column {
       double *R = &sc(SC_BOT + 6 * sp);
       R[0] = n0;
       R[1] = nm1;
-      R[2] = w0v;
+      R[2] = w0;
       R[3] = wm1;
       R[4] = w0r;
       R[5] = wm1r;
-    } else if (q >= 8 && q < 8 + NSP) {
-      // top, one species per lane (:429-441, :471-486)
-      const int sp = q - 8;
+    } else if (j >= 8 && j < 8 + NSP) {
+      // top, one species per thread (:429-441, :471-486)
+      const int sp = j - 8;
       const int jn = nA + 1;  // offset of level nAlts
       const double tn = xb(12, jn);
-      const double wn = xb(NSP + sp, jn), wnm1 = xb(NSP + sp, jn - 1);
+      const double wn = xb(F_W - F_LN + sp, jn), wnm1 = xb(F_W - F_LN + sp, jn - 1);
       double w1 = wn, w2 = wn;
       if (photochemical(sp) && wn < 0.0) {
         w1 = -wn;
@@ -834,12 +730,12 @@ struct Column {
     }
     if (IONS) {
       // bottom: zero-gradient ions (:277-281)
-      if (w0 && j <= 1) {
+      if (j <= 1) {
 #pragma unroll
         for (int f = 0; f < NI; f++) ion(f, j) = ion(f, 2);
       }
       // TEC of the column in 1e16 m^-2 (:498-503), in the reference's order
-      if (q == 6 && plasma) {
+      if (j == 6 && plasma) {
         double tec = 0.0;
         for (int jj = 2; jj <= nA + 1; jj++) {
           const double v = kp.improved_ions ? ion(0, jj) : exp(ion(0, jj));
@@ -856,25 +752,21 @@ struct Column {
 #pragma unroll
       for (int sp = 0; sp < NSP; sp++) {
         const double *R = &sc(SC_BOT + 6 * sp);
+        U[F_LN + sp] = R[o];
+        U[F_W + sp] = R[4 + o];
         ns[sp] = exp(R[o]);
         sr = sr + kp.mass[sp] * ns[sp];
       }
-#pragma unroll
-      for (int u = 0; u < NSH; u++) {
-        const double *R = &sc(SC_BOT + 6 * (s0 + u));
-        U[U_LN + u] = R[o];
-        U[U_W + u] = R[4 + o];
-      }
-      U[C_LR] = log(sr);
-      const double er = exp(U[C_LR]);
+      U[F_LR] = log(sr);
+      const double er = exp(U[F_LR]);
       double vu = 0.0;
 #pragma unroll
       for (int sp = 0; sp < NSP; sp++)
         vu = vu + ns[sp] * kp.mass[sp] * sc(SC_BOT + 6 * sp + 2 + o) / er;
-      U[C_VU] = vu;
-      U[C_VE] = 0.0;
-      U[C_VN] = 0.0;
-      if (j == 0) U[C_T] = xb(12, 1);
+      U[F_VU] = vu;
+      U[F_VE] = 0.0;
+      U[F_VN] = 0.0;
+      if (j == 0) U[F_T] = xb(12, 1);
     } else if (j == nA + 2 || j == nA + 3) {
       // levels nAlts+1, nAlts+2 (:420-463, :598-604)
       const int o = (j == nA + 2) ? 0 : 1;
@@ -883,22 +775,18 @@ struct Column {
 #pragma unroll
       for (int sp = 0; sp < NSP; sp++) {
         const double *R = &sc(SC_TOP + 4 * sp);
+        U[F_LN + sp] = R[o];
+        U[F_W + sp] = R[2 + o];
         sr = sr + kp.mass[sp] * exp(R[o]);
       }
-#pragma unroll
-      for (int u = 0; u < NSH; u++) {
-        const double *R = &sc(SC_TOP + 4 * (s0 + u));
-        U[U_LN + u] = R[o];
-        U[U_W + u] = R[2 + o];
-      }
-      U[C_LR] = log(sr);
-      U[C_VE] = xb(13, jn);
-      U[C_VN] = xb(14, jn);
+      U[F_LR] = log(sr);
+      U[F_VE] = xb(13, jn);
+      U[F_VN] = xb(14, jn);
       const double vu = xb(15, jn);
-      U[C_VU] = (vu < 0.0) ? -vu : vu;
-      U[C_T] = xb(12, jn);
+      U[F_VU] = (vu < 0.0) ? -vu : vu;
+      U[F_T] = xb(12, jn);
     }
-    if (IONS && q < NI) ion_top_bc(q);
+    if (IONS && j < NI) ion_top_bc(j);
     cx.sync();
   }
 
@@ -908,86 +796,79 @@ struct Column {
   // (check_for_negative_densities, :608-649), and the slope is taken in log
   // space.  The slope is limited to a decay of at least 0.1% per dAlt unless
   // the column is a thin nighttime plasmasphere.
-  GV_MEM void ion_top_bc(int iq) {
+  GV_MEM void ion_top_bc(int q) {
     const bool improved = kp.improved_ions != 0;
     const bool keep = plasma && sc(SC_TEC) < kp.min_tec;
     const double dFn = st(A_DF, nA + 1);  // dAlt_F(nAlts)
 #pragma unroll 1
     for (int t = 0; t < 2; t++) {
-      const int jk = nA + 2 + t;  // record of level nAlts+1+t
+      const int jk = nA + 2 + t;  // lane of level nAlts+1+t
       const double *C = &sc(t ? SC_CTB : SC_CTA);
       double n0, n1, n2, n3, n4, n5;
       if (improved) {
-        double mn = ion(iq, jk - 5);
+        double mn = ion(q, jk - 5);
         for (int jj = jk - 4; jj <= jk; jj++)
-          if (ion(iq, jj) < mn) mn = ion(iq, jj);
+          if (ion(q, jj) < mn) mn = ion(q, jj);
         if (mn < 0.0) {
-          if (ion(iq, jk - 6) < 0.0) fail(GITM_VERT_ENEGION, jk - 1);
+          if (ion(q, jk - 6) < 0.0) fail(GITM_VERT_ENEGION, jk - 1);
           for (int jj = jk - 5; jj <= jk - 1; jj++) {
-            if (ion(iq, jj) < 0.0) {
-              if (ion(iq, jj - 1) > 0.0 && ion(iq, jj + 1) > 0.0)
-                ion(iq, jj) = (ion(iq, jj - 1) + ion(iq, jj + 1)) / 2.0;
+            if (ion(q, jj) < 0.0) {
+              if (ion(q, jj - 1) > 0.0 && ion(q, jj + 1) > 0.0)
+                ion(q, jj) = (ion(q, jj - 1) + ion(q, jj + 1)) / 2.0;
               else
-                ion(iq, jj) = ion(iq, jj - 1) * 0.95;
+                ion(q, jj) = ion(q, jj - 1) * 0.95;
             }
           }
-          if (ion(iq, jk) < 0.0) ion(iq, jk) = ion(iq, jk - 1) * 0.95;
+          if (ion(q, jk) < 0.0) ion(q, jk) = ion(q, jk - 1) * 0.95;
         }
-        n0 = log(ion(iq, jk));
-        n1 = log(ion(iq, jk - 1));
-        n2 = log(ion(iq, jk - 2));
-        n3 = log(ion(iq, jk - 3));
-        n4 = log(ion(iq, jk - 4));
-        n5 = log(ion(iq, jk - 5));
+        n0 = log(ion(q, jk));
+        n1 = log(ion(q, jk - 1));
+        n2 = log(ion(q, jk - 2));
+        n3 = log(ion(q, jk - 3));
+        n4 = log(ion(q, jk - 4));
+        n5 = log(ion(q, jk - 5));
       } else {
-        n0 = ion(iq, jk);
-        n1 = ion(iq, jk - 1);
-        n2 = ion(iq, jk - 2);
-        n3 = ion(iq, jk - 3);
-        n4 = ion(iq, jk - 4);
-        n5 = ion(iq, jk - 5);
+        n0 = ion(q, jk);
+        n1 = ion(q, jk - 1);
+        n2 = ion(q, jk - 2);
+        n3 = ion(q, jk - 3);
+        n4 = ion(q, jk - 4);
+        n5 = ion(q, jk - 5);
       }
       double dn = C[0] * n1 + C[1] * n2 + C[2] * n3 + C[3] * n4 + C[4] * n5;
       if (!keep) {
         if (dn > -0.001 * n0 / dFn) dn = -0.001 * n0 / dFn;
         n0 = n1 + dn * st(A_DF, jk);
-        ion(iq, jk) = improved ? exp(n0) : n0;
+        ion(q, jk) = improved ? exp(n0) : n0;
       }
     }
   }
 
   // -------------------------------------------------------------------------
   // 4th-order limited face values and Rusanov gradient / dissipation of up to
-  // 8 fields (calc_rusanov_alts_*, vertical_solver_ausm.f90:1007-1090), in
-  // exchange buffers b0+f (values) and b0+8+f (left face states).  Shared
-  // fields (both lanes hold the same values) are staged by the even lane;
-  // per-species fields (SPECIES) use buffers b0+s0+f of each lane.
-  // grad/diff are valid on cell levels.
+  // 8 fields (calc_rusanov_alts_*, vertical_solver_ausm.f90:1007-1090).
+  // Uses exchange buffers 0..15.  grad/diff are valid on cell threads.
   // -------------------------------------------------------------------------
-  template <int N, bool SPECIES>
-  GV_MEM void rusanov_grad_diff(const double (&v)[N], double (&grad)[N], double (&diff)[N],
-                                int b0 = 0) {
+  template <int N>
+  GV_MEM void rusanov_grad_diff(const double (&v)[N], double (&grad)[N],
+                                double (&diff)[N]) {
     static_assert(N <= 8, "at most 8 fields per pass");
-    const int bb = b0 + (SPECIES ? s0 : 0);
-    const bool wr = SPECIES || w0;
-    if (wr) {
 #pragma unroll
-      for (int f = 0; f < N; f++) xb(bb + f, j) = v[f];
-    }
+    for (int f = 0; f < N; f++) xb(f, j) = v[f];
     cx.sync();
     const double iF0 = st(A_IF, j), iF1 = st(A_IF, j + 1);
     const double dF0 = st(A_DF, j), dF1 = st(A_DF, j + 1);
     double rf[N], ln[N];
 #pragma unroll
     for (int f = 0; f < N; f++) {
-      const double vm2 = xb(bb + f, j - 2), vm1 = xb(bb + f, j - 1), v0 = v[f],
-                   v1 = xb(bb + f, j + 1), v2 = xb(bb + f, j + 2);
+      const double vm2 = xb(f, j - 2), vm1 = xb(f, j - 1), v0 = v[f],
+                   v1 = xb(f, j + 1), v2 = xb(f, j + 2);
       double dl = 0.0;
       if (k >= 1 && k <= nA) {
-        double hh = iF1 * 2.0;
-        const double up = hh * (FACT1 * (v1 - v0) - FACT2 * (v2 - vm1));
-        hh = iF0 * 2.0;
-        const double dn = hh * (FACT1 * (v0 - vm1) - FACT2 * (v1 - vm2));
+        double h = iF1 * 2.0;
+        const double up = h * (FACT1 * (v1 - v0) - FACT2 * (v2 - vm1));
+        h = iF0 * 2.0;
+        const double dn = h * (FACT1 * (v0 - vm1) - FACT2 * (v1 - vm2));
         dl = limiter_mc(kp.beta, up, dn);
       } else if (k == 0 || k == nA + 1) {
         dl = limiter_mc(kp.beta, (v1 - v0) * iF1, (v0 - vm1) * iF0);
@@ -996,19 +877,17 @@ struct Column {
       ln[f] = v0 + 0.5 * dl * dF1;  // left state of face k|k+1
     }
     cx.sync();
-    if (wr) {
 #pragma unroll
-      for (int f = 0; f < N; f++) {
-        xb(bb + f, j) = rf[f];
-        xb(bb + 8 + f, j) = ln[f];
-      }
+    for (int f = 0; f < N; f++) {
+      xb(f, j) = rf[f];
+      xb(8 + f, j) = ln[f];
     }
     cx.sync();
     const double dC = st(A_DC, j);
     const double cm0 = st(A_CMF, j), cm1 = st(A_CMF, j + 1);
 #pragma unroll
     for (int f = 0; f < N; f++) {
-      const double L0 = xb(bb + 8 + f, j - 1), R0 = rf[f], L1 = ln[f], R1 = xb(bb + f, j + 1);
+      const double L0 = xb(8 + f, j - 1), R0 = rf[f], L1 = ln[f], R1 = xb(f, j + 1);
       grad[f] = 0.5 * (L1 + R1 - L0 - R0) / dC;
       const double df0 = cm0 * (R0 - L0), df1 = cm1 * (R1 - L1);
       diff[f] = (df1 - df0) / dC;
@@ -1016,91 +895,89 @@ struct Column {
     cx.sync();
   }
 
-  // 5-point gradient of ln(rho_s/rho) of the lane's species
-  // (vertical_solver_ausm.f90:649-688)
-  GV_MEM void grad_log_cons(const double (&lcs)[NSH], double (&glc)[NSH]) {
+  // 5-point gradient of ln(rho_s/rho) (vertical_solver_ausm.f90:649-688)
+  GV_MEM void grad_log_cons(const double (&lcs)[NSP], double (&glc)[NSP]) {
 #pragma unroll
-    for (int u = 0; u < NSH; u++) xb(s0 + u, j) = lcs[u];
+    for (int f = 0; f < NSP; f++) xb(f, j) = lcs[f];
     cx.sync();
 #pragma unroll
-    for (int u = 0; u < NSH; u++) {
-      const int f = s0 + u;
-      glc[u] = st(A_GLC0 + 0, j) * xb(f, j - 2) + st(A_GLC0 + 1, j) * xb(f, j - 1) +
-               st(A_GLC0 + 2, j) * lcs[u] + st(A_GLC0 + 3, j) * xb(f, j + 1) +
-               st(A_GLC0 + 4, j) * xb(f, j + 2);
-    }
+    for (int f = 0; f < NSP; f++)
+      glc[f] = st(A_GLC0 + 0, j) * xb(f, j - 2) + st(A_GLC0 + 1, j) * xb(f, j - 1) + st(A_GLC0 + 2, j) * lcs[f] +
+               st(A_GLC0 + 3, j) * xb(f, j + 1) + st(A_GLC0 + 4, j) * xb(f, j + 2);
     cx.sync();
   }
 
   // -------------------------------------------------------------------------
   // advance_vertical_1stage_ausm, vertical_solver_ausm.f90:436-999, with
-  // calc_all_fluxes_hydro (:1093-2070).  Every lane computes (the results of
-  // the levels without a face or cell are discarded), so that the pair swaps
-  // stay in uniform control flow; the new values are committed on cells.
+  // calc_all_fluxes_hydro (:1093-2070).  Writes New (valid on cell threads).
+  // Ordered so that few per-level values stay live across the flux phase.
   // -------------------------------------------------------------------------
   GV_MEM void ausm_stage(double dtin, int stg) {
     const double *m = kp.mass;
-    const double T = U[C_T];
+    const double T = U[F_T];
     // --- pointwise (:591-609, :649-652, :752-773)
-    double NS[NSH], rhos[NSH];
-    const double rho = exp(U[C_LR]);
+    double NS[NSP], rhos[NSP];
+    const double rho = exp(U[F_LR]);
+    double nt = 0.0;
 #pragma unroll
-    for (int u = 0; u < NSH; u++) NS[u] = exp(U[U_LN + u]);
-    const double nt = sum6(NS);
+    for (int s = 0; s < NSP; s++) {
+      NS[s] = exp(U[F_LN + s]);
+      nt = nt + NS[s];
+    }
 #pragma unroll
-    for (int u = 0; u < NSH; u++) rhos[u] = m[s0 + u] * NS[u];
+    for (int s = 0; s < NSP; s++) rhos[s] = m[s] * NS[s];
     const double te = nt * KB * T / S.gm1 +
                       0.5 * rho *
-                          (U[C_VU] * U[C_VU] + U[C_VN] * U[C_VN] + U[C_VE] * U[C_VE]);
+                          (U[F_VU] * U[F_VU] + U[F_VN] * U[F_VN] + U[F_VE] * U[F_VE]);
 
     // --- gradients the sources need (:611-646, :649-688): GradTemp,
     // DiffVel(E,N) and d ln(rho_s/rho)/dz
-    double glc[NSH];
+    double glc[NSP];
     {
-      double lcs[NSH];
+      double lcs[NSP];
 #pragma unroll
-      for (int u = 0; u < NSH; u++)
-        lcs[u] = cx.device_math() ? (kp.lmass[s0 + u] + U[U_LN + u]) - U[C_LR]
-                                  : log(m[s0 + u] * NS[u] / rho);
+      for (int s = 0; s < NSP; s++)
+        lcs[s] = cx.device_math() ? (kp.lmass[s] + U[F_LN + s]) - U[F_LR]
+                                  : log(m[s] * NS[s] / rho);
       grad_log_cons(lcs, glc);
     }
     double gr3[3], df3[3];
     {
-      const double gv3[3] = {T, U[C_VE], U[C_VN]};
-      rusanov_grad_diff<3, false>(gv3, gr3, df3);
+      const double gv3[3] = {T, U[F_VE], U[F_VN]};
+      rusanov_grad_diff<3>(gv3, gr3, df3);
     }
     if (IONS) ion_stage(dtin, stg, true);  // :817-829
 
     // --- hydrostatic background (:694-751): per-level factors in parallel,
-    // the recurrence in altitude serially (one lane per species)
-    if (w0) xb(0, j) = T;
+    // the recurrence in altitude serially (one thread per species)
+    xb(0, j) = T;
 #pragma unroll
-    for (int u = 0; u < NSH; u++) xb(8 + s0 + u, j) = NS[u];
+    for (int s = 0; s < NSP; s++) xb(8 + s, j) = NS[s];
     cx.sync();
     {
       const double tm1 = xb(0, j - 1), tp1 = xb(0, j + 1);
       const double dF = st(A_DF, j);
       if (lv && j >= 2) {
-        if (w0) xb(1, j) = tm1 / T;
+        xb(1, j) = tm1 / T;
         const double MeanTemp = 0.5 * (T + tm1);
 #pragma unroll
-        for (int u = 0; u < NSH; u++) {
-          const double ish = -1.0 * m[s0 + u] * S.mgh / (KB * MeanTemp);
-          xb(2 + s0 + u, j) = exp(-1.0 * dF * ish);
+        for (int s = 0; s < NSP; s++) {
+          const double ish = -1.0 * m[s] * S.mgh / (KB * MeanTemp);
+          xb(2 + s, j) = exp(-1.0 * dF * ish);
         }
       } else if (j == 0) {
-        if (w0) xb(1, j) = tp1 / T;
+        xb(1, j) = tp1 / T;
         const double MeanMass = 0.5 * (sc(SC_MMM_M1) + sc(SC_MMM_0));
         const double MeanTemp = 0.5 * (tp1 + T);
         const double ish = -1.0 * MeanMass * sc(SC_MG0) / (KB * MeanTemp);
         const double b = exp(dF * ish);
 #pragma unroll
-        for (int u = 0; u < NSH; u++) xb(2 + s0 + u, j) = b;
+        for (int s = 0; s < NSP; s++) xb(2 + s, j) = b;
       }
     }
     cx.sync();
-    if (q < NSP) {
-      const int s = q;
+    if (j < NSP) {
+      const int s = j;
       const double ns0 = xb(8 + s, 1);
       double x = ns0;
       // The product runs level by level in the reference's order; the
@@ -1145,35 +1022,34 @@ struct Column {
       xb(8 + s, 0) = ns0 * xb(1, 0) * xb(2 + s, 0);
     }
     cx.sync();
-    double dev[NSH];
-    int sub = 0;  // bit s: SubtractHydrostatic(k, s), all six species
+    double dev[NSP];
+    int sub = 0;  // bit s: SubtractHydrostatic(k, s)
     {
-      double lhps[NSH];
+      double lhps[NSP];
 #pragma unroll
-      for (int u = 0; u < NSH; u++) {
-        const double hns = xb(8 + s0 + u, j);
-        lhps[u] = log(hns * KB * T);
-        const double hrhos = m[s0 + u] * hns;
-        const double ratio = fabs(rhos[u] - hrhos) / rhos[u];
-        dev[u] = rhos[u] - hrhos;
-        if (!(fabs(ratio) > 1.0)) sub |= 1 << (s0 + u);
+      for (int s = 0; s < NSP; s++) {
+        const double hns = xb(8 + s, j);
+        lhps[s] = log(hns * KB * T);
+        const double hrhos = m[s] * hns;
+        const double ratio = fabs(rhos[s] - hrhos) / rhos[s];
+        dev[s] = rhos[s] - hrhos;
+        if (!(fabs(ratio) > 1.0)) sub |= 1 << s;
       }
-      sub |= cx.pair_int(sub);
       cx.sync();
       // batch 2 of the face states goes first: ln p_hydro,s and w_s
 #pragma unroll
-      for (int u = 0; u < NSH; u++) {
-        xb(s0 + u, j) = lhps[u];
-        xb(6 + s0 + u, j) = U[U_W + u];
+      for (int s = 0; s < NSP; s++) {
+        xb(s, j) = lhps[s];
+        xb(6 + s, j) = U[F_W + s];
       }
     }
-    if (w0) ib(j) = sub;
+    ib(j) = sub;
     cx.sync();
 
     // --- face states (calc_kt_facevalues, :2073-2117) of face k|k+1, one
-    // face per level, exponentiated where the flux needs them (:1356-1436)
+    // face per thread, exponentiated where the flux needs them (:1356-1436)
     const int subUp = ib(j + 1);  // flags of cell k+1
-    double RSL[NSH], RSR[NSH], PSL[NSH], PSR[NSH], HL[NSH], HR[NSH], VL[NSH], VR[NSH];
+    double RSL[NSP], RSR[NSP], PSL[NSP], PSR[NSP], HL[NSP], HR[NSP], VL[NSP], VR[NSP];
     double VEL, VER, VNL, VNR;
     {
       const double iF0 = st(A_IF, j), iF1 = st(A_IF, j + 1), iF2 = st(A_IF, j + 2);
@@ -1187,42 +1063,40 @@ struct Column {
     OUTR = f1 - 0.5 * d1 * dF1;                                                     \
   } while (0)
 #pragma unroll
-      for (int u = 0; u < NSH; u++) {
+      for (int s = 0; s < NSP; s++) {
         double l, r;
-        GV_KT(s0 + u, l, r);
-        HL[u] = exp(l);
-        HR[u] = exp(r);
-        GV_KT(6 + s0 + u, VL[u], VR[u]);
+        GV_KT(s, l, r);
+        HL[s] = exp(l);
+        HR[s] = exp(r);
+        GV_KT(6 + s, VL[s], VR[s]);
       }
       cx.sync();
       if (cx.device_math()) {
         const double ltk = log(T * KB);
 #pragma unroll
-        for (int u = 0; u < NSH; u++) {
-          xb(s0 + u, j) = kp.lmass[s0 + u] + U[U_LN + u];
-          xb(6 + s0 + u, j) = U[U_LN + u] + ltk;
+        for (int s = 0; s < NSP; s++) {
+          xb(s, j) = kp.lmass[s] + U[F_LN + s];
+          xb(6 + s, j) = U[F_LN + s] + ltk;
         }
       } else {
 #pragma unroll
-        for (int u = 0; u < NSH; u++) {
-          xb(s0 + u, j) = log(rhos[u]);
-          xb(6 + s0 + u, j) = log(NS[u] * T * KB);
+        for (int s = 0; s < NSP; s++) {
+          xb(s, j) = log(rhos[s]);
+          xb(6 + s, j) = log(NS[s] * T * KB);
         }
       }
-      if (w0) {
-        xb(12, j) = U[C_VE];
-        xb(13, j) = U[C_VN];
-      }
+      xb(12, j) = U[F_VE];
+      xb(13, j) = U[F_VN];
       cx.sync();
 #pragma unroll
-      for (int u = 0; u < NSH; u++) {
+      for (int s = 0; s < NSP; s++) {
         double l, r;
-        GV_KT(s0 + u, l, r);
-        RSL[u] = exp(l);
-        RSR[u] = exp(r);
-        GV_KT(6 + s0 + u, l, r);
-        PSL[u] = exp(l);
-        PSR[u] = exp(r);
+        GV_KT(s, l, r);
+        RSL[s] = exp(l);
+        RSR[s] = exp(r);
+        GV_KT(6 + s, l, r);
+        PSL[s] = exp(l);
+        PSR[s] = exp(r);
       }
       GV_KT(12, VEL, VER);
       GV_KT(13, VNL, VNR);
@@ -1233,22 +1107,28 @@ struct Column {
     // --- AUSM+-up fluxes of face k|k+1 (:1459-1970).  The P12 side belongs
     // to cell k and stays in registers; the M12 side belongs to cell k+1 (with
     // that cell's Kp, Ku, dAlt_C and SubtractHydrostatic flags) and goes to
-    // LDS buffers 0..20 for the level above.
-    double FRp[NSH], FMp[NSH], NPp[NSH];
-    double EFp, MOEp, MONp;
-    {
-      const double RL = sum6(RSL), RR = sum6(RSR);
-      const double PL = sum6(PSL), PR = sum6(PSR);
-      double VUL, VUR;  // mass-weighted face winds (:1505-1532)
-      {
-        double tl[NSH], tr[NSH];
+    // LDS buffers 0..20 for the thread above.
+    double FRp[NSP], FMp[NSP], NPp[NSP];
+    double EFp = 0.0, MOEp = 0.0, MONp = 0.0;
 #pragma unroll
-        for (int u = 0; u < NSH; u++) {
-          tl[u] = RSL[u] * VL[u] / RL;
-          tr[u] = RSR[u] * VR[u] / RR;
-        }
-        VUL = sum6(tl);
-        VUR = sum6(tr);
+    for (int s = 0; s < NSP; s++) FRp[s] = FMp[s] = NPp[s] = 0.0;
+    if (face()) {
+      double RL = 0.0, RR = 0.0, PL = 0.0, PR = 0.0;
+#pragma unroll
+      for (int s = 0; s < NSP; s++) {
+        RL = RL + RSL[s];
+        RR = RR + RSR[s];
+      }
+#pragma unroll
+      for (int s = 0; s < NSP; s++) {
+        PL = PL + PSL[s];
+        PR = PR + PSR[s];
+      }
+      double VUL = 0.0, VUR = 0.0;  // mass-weighted face winds (:1505-1532)
+#pragma unroll
+      for (int s = 0; s < NSP; s++) {
+        VUL = VUL + RSL[s] * VL[s] / RL;
+        VUR = VUR + RSR[s] * VR[s] / RR;
       }
       // bulk energy (:1538-1562), enthalpy (:1569-1615), Liou sound speed
       // (:1619-1655)
@@ -1267,15 +1147,14 @@ struct Column {
       const double kp0 = st(A_KP, j), kp1 = st(A_KP, j + 1);
       const double ku0 = st(A_KU, j), ku1 = st(A_KU, j + 1);
       const double rdt0 = st(A_DC, j) / dtin, rdt1 = st(A_DC, j + 1) / dtin;
+      double BNVp = 0.0, BNVm = 0.0;
       const double RLR = RL + RR;
-      double bp[NSH], bm[NSH];
 #pragma unroll
-      for (int u = 0; u < NSH; u++) {
-        const int s = s0 + u;
-        const double MPS = 0.5 * (PSL[u] + PSR[u]);
-        const double MHPS = 0.5 * (HL[u] + HR[u]);
-        const double MRS = 0.5 * (RSL[u] + RSR[u]);
-        const double ML = VL[u] / CS, MR = VR[u] / CS;  // :1689-1706
+      for (int s = 0; s < NSP; s++) {
+        const double MPS = 0.5 * (PSL[s] + PSR[s]);
+        const double MHPS = 0.5 * (HL[s] + HR[s]);
+        const double MRS = 0.5 * (RSL[s] + RSR[s]);
+        const double ML = VL[s] / CS, MR = VR[s] / CS;  // :1689-1706
         const double M2 = 0.5 * (ML * ML + MR * MR);
         const double MZ = sqrt(dmin(1.0, dmax(M2, 1.0e-19)));
         const double FA = MZ * (2.0 - MZ);
@@ -1291,23 +1170,23 @@ struct Column {
           MF4N = -0.25 * ((MR - 1.0) * (MR - 1.0)) *
                  (1.0 + 2.0 * (0.25 * ((MR + 1.0) * (MR + 1.0))));
         const double f1 = dmax((1.0 - M2), 0.0);
-        const double pdh = (PSR[u] - HR[u]) - (PSL[u] - HL[u]);
-        const double pd = PSR[u] - PSL[u];
-        const double rvd = (RSR[u]) * VR[u] - (RSL[u]) * VL[u];
+        const double pdh = (PSR[s] - HR[s]) - (PSL[s] - HL[s]);
+        const double pd = PSR[s] - PSL[s];
+        const double rvd = (RSR[s]) * VR[s] - (RSL[s]) * VL[s];
         const double mcs = MRS * CS, facs = FA * CS;
         {  // P12 side of cell k (:1822-1970)
           const bool sh = (sub >> s) & 1;
           const double MP = kp0 * f1 * (sh ? pdh : pd) / (mcs * (facs + rdt0));
           const double NV = CS * (MF4P + MF4N - MP);
-          NPp[u] = (sh ? (MPS - MHPS) : MPS) - 0.5 * ku0 * CS * rvd;
+          NPp[s] = (sh ? (MPS - MHPS) : MPS) - 0.5 * ku0 * CS * rvd;
           if (NV >= 0.0) {
-            FRp[u] = RSL[u] * NV;
-            FMp[u] = RSL[u] * VL[u] * NV;
+            FRp[s] = RSL[s] * NV;
+            FMp[s] = RSL[s] * VL[s] * NV;
           } else {
-            FRp[u] = RSR[u] * NV;
-            FMp[u] = RSR[u] * VR[u] * NV;
+            FRp[s] = RSR[s] * NV;
+            FMp[s] = RSR[s] * VR[s] * NV;
           }
-          bp[u] = (RSL[u] + RSR[u]) * NV / RLR;
+          BNVp = BNVp + (RSL[s] + RSR[s]) * NV / RLR;
         }
         {  // M12 side of cell k+1
           const bool sh = (subUp >> s) & 1;
@@ -1315,16 +1194,15 @@ struct Column {
           const double NV = CS * (MF4P + MF4N - MP);
           xb(12 + s, j) = (sh ? (MPS - MHPS) : MPS) - 0.5 * ku1 * CS * rvd;
           if (NV >= 0.0) {
-            xb(s, j) = RSL[u] * NV;
-            xb(6 + s, j) = RSL[u] * VL[u] * NV;
+            xb(s, j) = RSL[s] * NV;
+            xb(6 + s, j) = RSL[s] * VL[s] * NV;
           } else {
-            xb(s, j) = RSR[u] * NV;
-            xb(6 + s, j) = RSR[u] * VR[u] * NV;
+            xb(s, j) = RSR[s] * NV;
+            xb(6 + s, j) = RSR[s] * VR[s] * NV;
           }
-          bm[u] = (RSL[u] + RSR[u]) * NV / RLR;
+          BNVm = BNVm + (RSL[s] + RSR[s]) * NV / RLR;
         }
       }
-      const double BNVp = sum6(bp), BNVm = sum6(bm);
       // energy and momentum fluxes (:1973-2026)
       if (BNVp >= 0.0) {
         EFp = (EL + PL) * BNVp;
@@ -1335,31 +1213,28 @@ struct Column {
         MOEp = RR * VER * BNVp;
         MONp = RR * VNR * BNVp;
       }
-      if (w0) {
-        if (BNVm >= 0.0) {
-          xb(18, j) = (EL + PL) * BNVm;
-          xb(19, j) = RL * VEL * BNVm;
-          xb(20, j) = RL * VNL * BNVm;
-        } else {
-          xb(18, j) = (ER + PR) * BNVm;
-          xb(19, j) = RR * VER * BNVm;
-          xb(20, j) = RR * VNR * BNVm;
-        }
+      if (BNVm >= 0.0) {
+        xb(18, j) = (EL + PL) * BNVm;
+        xb(19, j) = RL * VEL * BNVm;
+        xb(20, j) = RL * VNL * BNVm;
+      } else {
+        xb(18, j) = (ER + PR) * BNVm;
+        xb(19, j) = RR * VER * BNVm;
+        xb(20, j) = RR * VNR * BNVm;
       }
     }
     cx.sync();
     // divergence of the face fluxes (:2042-2068); cell k takes its M12 side
     // from the face below
-    double rsf[NSH], msf[NSH], ef, mfE, mfN;
+    double rsf[NSP], msf[NSP], ef, mfE, mfN;
     {
       const double ap = st(A_AP, j), am = st(A_AM, j), vol = st(A_VOL, j);
       const double dC = st(A_DC, j);
 #pragma unroll
-      for (int u = 0; u < NSH; u++) {
-        const int s = s0 + u;
-        rsf[u] = (ap * FRp[u] - am * xb(s, j - 1)) / vol;
-        msf[u] = (ap * FMp[u] - am * xb(6 + s, j - 1)) / vol;
-        msf[u] = msf[u] + (NPp[u] - xb(12 + s, j - 1)) / dC;
+      for (int s = 0; s < NSP; s++) {
+        rsf[s] = (ap * FRp[s] - am * xb(s, j - 1)) / vol;
+        msf[s] = (ap * FMp[s] - am * xb(6 + s, j - 1)) / vol;
+        msf[s] = msf[s] + (NPp[s] - xb(12 + s, j - 1)) / dC;
       }
       ef = (ap * EFp - am * xb(18, j - 1)) / vol;
       mfE = (ap * MOEp - am * xb(19, j - 1)) / vol;
@@ -1369,179 +1244,161 @@ struct Column {
 
     // --- update of cell k (:802-997); each new value is folded into the RK4
     // carries as soon as the stage has no further use of the old one
-    const bool cl = cell();
-    double nrs[NSH], nns[NSH], w[NSH];
+    if (cell()) {
+      double nrs[NSP], nns[NSP], w[NSP];
+      double nrho = 0.0, nnt = 0.0;
 #pragma unroll
-    for (int u = 0; u < NSH; u++) {
-      nrs[u] = rhos[u] - dtin * (rsf[u]);
-      nns[u] = nrs[u] / m[s0 + u];
-    }
-    const double nrho = sum6(nrs), nnt = sum6(nns);
-    {
+      for (int s = 0; s < NSP; s++) {
+        nrs[s] = rhos[s] - dtin * (rsf[s]);
+        nns[s] = nrs[s] / m[s];
+        nrho = nrho + nrs[s];
+        nnt = nnt + nns[s];
+      }
       const double gradT = gr3[0];
       const double invr = st(A_INVR, j);
 #pragma unroll
-      for (int u = 0; u < NSH; u++) {
-        const int s = s0 + u;
-        const double moms = rhos[u] * U[U_W + u];
+      for (int s = 0; s < NSP; s++) {
+        const double moms = rhos[s] * U[F_W + s];
         double nm;
         if (!((sub >> s) & 1))
-          nm = moms - dtin * (msf[u]) + dtin * rhos[u] * S.eg;
+          nm = moms - dtin * (msf[s]) + dtin * rhos[s] * S.eg;
         else
-          nm = moms - dtin * (msf[u]) + dtin * dev[u] * S.eg;
-        double ww = nm / nrs[u];
-        ww = ww + dtin * (U[C_VN] * U[C_VN] + U[C_VE] * U[C_VE]) * invr;
-        if (kp.coriolis) ww = ww + dtin * (coriolis * U[C_VE]);
+          nm = moms - dtin * (msf[s]) + dtin * dev[s] * S.eg;
+        double ww = nm / nrs[s];
+        ww = ww + dtin * (U[F_VN] * U[F_VN] + U[F_VE] * U[F_VE]) * invr;
+        if (kp.coriolis) ww = ww + dtin * (coriolis * U[F_VE]);
         ww = ww - dtin * (kp.tdc[s] * KB * gradT) / m[s];
-        w[u] = ww;
+        w[s] = ww;
       }
-    }
-    const double hf = st(A_HF, j);
-    double nvE = (rho * U[C_VE] - dtin * (mfE)) / nrho;
-    nvE = nvE + hf * dtin * df3[1];
-    double nvN = (rho * U[C_VN] - dtin * (mfN)) / nrho;
-    nvN = nvN + hf * dtin * df3[2];
-    const double nte = te - dtin * ef + dtin * rho * U[C_VU] * S.eg;
-    {
-      double nl[NSH], t[NSH];
+      const double hf = st(A_HF, j);
+      double nvE = (rho * U[F_VE] - dtin * (mfE)) / nrho;
+      nvE = nvE + hf * dtin * df3[1];
+      double nvN = (rho * U[F_VN] - dtin * (mfN)) / nrho;
+      nvN = nvN + hf * dtin * df3[2];
+      const double nte = te - dtin * ef + dtin * rho * U[F_VU] * S.eg;
+      {
+        double sr = 0.0;
 #pragma unroll
-      for (int u = 0; u < NSH; u++) {
-        nl[u] = log(nns[u]);
-        t[u] = m[s0 + u] * (cx.device_math() ? nns[u] : exp(nl[u]));
+        for (int s = 0; s < NSP; s++) {
+          const double nl = log(nns[s]);
+          sr = sr + m[s] * (cx.device_math() ? nns[s] : exp(nl));
+          ausm_commit(stg, F_LN + s, nl);
+        }
+        ausm_commit(stg, F_LR, log(sr));
       }
-      const double sr = sum6(t);
-      if (cl) {
+      ausm_commit(stg, F_VE, nvE);
+      ausm_commit(stg, F_VN, nvN);
+      if (kp.friction && !friction(dtin, w, nnt, nns, glc, T))
+        fail(GITM_VERT_ESINGULAR, k);
+      double nvu = 0.0;
 #pragma unroll
-        for (int u = 0; u < NSH; u++) ausm_commit(stg, U_LN + u, nl[u]);
+      for (int s = 0; s < NSP; s++) {
+        double ww = dmax(-kp.maxvv, w[s]);
+        ww = dmin(kp.maxvv, ww);
+        nvu = nvu + ww * nrs[s] / nrho;
+        ausm_commit(stg, F_W + s, ww);
       }
-      const double nlr = log(sr);
-      if (cl) {
-        ausm_commit_bulk(stg, C_LR, nlr);
-        ausm_commit_bulk(stg, C_VE, nvE);
-        ausm_commit_bulk(stg, C_VN, nvN);
-      }
+      const double np = (nte - 0.5 * nrho * (nvu * nvu + nvN * nvN + nvE * nvE)) * S.gm1;
+      ausm_commit(stg, F_VU, nvu);
+      ausm_commit(stg, F_T, np / (KB * nnt));
     }
-    if (kp.friction) {
-      const bool ok = friction(dtin, w, nnt, nns, glc, T);
-      if (cl && w0 && !ok) fail(GITM_VERT_ESINGULAR, k);
-    }
-    double vt[NSH];
-#pragma unroll
-    for (int u = 0; u < NSH; u++) {
-      double ww = dmax(-kp.maxvv, w[u]);
-      ww = dmin(kp.maxvv, ww);
-      vt[u] = ww * nrs[u] / nrho;
-      if (cl) ausm_commit(stg, U_W + u, ww);
-    }
-    const double nvu = sum6(vt);
-    const double np = (nte - 0.5 * nrho * (nvu * nvu + nvN * nvN + nvE * nvE)) * S.gm1;
-    if (cl) {
-      ausm_commit_bulk(stg, C_VU, nvu);
-      ausm_commit_bulk(stg, C_T, np / (KB * nnt));
-    }
-    mirror_bulk();
   }
 
   // -------------------------------------------------------------------------
   // advance_vertical_1stage (Rusanov), vertical_solver_rusanov.f90:387-686.
+  // Writes New (valid on cell threads).
   // -------------------------------------------------------------------------
   GV_MEM void rusanov_stage(int stg) {
     const double *m = kp.mass;
     const double dt = kp.dt;
-    const double T = U[C_T];
-    double NS[NSH], lcs[NSH];
-    const double rho = exp(U[C_LR]);
+    const double T = U[F_T];
+    double NS[NSP], lcs[NSP];
+    const double rho = exp(U[F_LR]);
+    double sum = 0.0;
 #pragma unroll
-    for (int u = 0; u < NSH; u++) NS[u] = exp(U[U_LN + u]);
-    const double nt = exp(log(sum6(NS)));  // NT = exp(LogNum) (:458-460)
-#pragma unroll
-    for (int u = 0; u < NSH; u++)
-      lcs[u] = cx.device_math() ? (kp.lmass[s0 + u] + U[U_LN + u]) - U[C_LR]
-                                : log(m[s0 + u] * NS[u] / rho);
-    // gradients and dissipation (:462-498)
-    const double va[4] = {T, U[C_VE], U[C_VN], U[C_VU]};
-    double ga[4], da[4];
-    rusanov_grad_diff<4, false>(va, ga, da);
-    double gln[NSH], dln[NSH], gw[NSH], dw[NSH];
-    {
-      double v3[NSH];
-#pragma unroll
-      for (int u = 0; u < NSH; u++) v3[u] = U[U_LN + u];
-      rusanov_grad_diff<NSH, true>(v3, gln, dln);
-#pragma unroll
-      for (int u = 0; u < NSH; u++) v3[u] = U[U_W + u];
-      rusanov_grad_diff<NSH, true>(v3, gw, dw);
+    for (int s = 0; s < NSP; s++) {
+      NS[s] = exp(U[F_LN + s]);
+      sum = sum + NS[s];
     }
-    double glc[NSH];
+    const double nt = exp(log(sum));  // NT = exp(LogNum) (:458-460)
+#pragma unroll
+    for (int s = 0; s < NSP; s++)
+      lcs[s] = cx.device_math() ? (kp.lmass[s] + U[F_LN + s]) - U[F_LR]
+                                : log(m[s] * NS[s] / rho);
+    // gradients and dissipation (:462-498)
+    const double va[4] = {T, U[F_VE], U[F_VN], U[F_VU]};
+    double ga[4], da[4];
+    rusanov_grad_diff<4>(va, ga, da);
+    double gln[NSP], dln[NSP], gw[NSP], dw[NSP];
+    {
+      double v6[NSP];
+#pragma unroll
+      for (int s = 0; s < NSP; s++) v6[s] = U[F_LN + s];
+      rusanov_grad_diff<NSP>(v6, gln, dln);
+#pragma unroll
+      for (int s = 0; s < NSP; s++) v6[s] = U[F_W + s];
+      rusanov_grad_diff<NSP>(v6, gw, dw);
+    }
+    double glc[NSP];
     grad_log_cons(lcs, glc);
     if (IONS) ion_stage(dt, stg, false);  // :617-630
-    const bool cl = cell();
-    const double divvel = ga[3] + 2 * U[C_VU] * st(A_INVR, j);  // :476
-    double w[NSH], nln[NSH];
+    if (!cell()) return;
+    const double divvel = ga[3] + 2 * U[F_VU] * st(A_INVR, j);  // :476
+    double w[NSP], nln[NSP];
 #pragma unroll
-    for (int u = 0; u < NSH; u++) {  // :550-614
-      const int s = s0 + u;
-      const double vv = U[U_W + u];
-      const double divvv = gw[u] + 2 * vv * st(A_INVR, j);
-      nln[u] = U[U_LN + u] - dt * (divvv + vv * gln[u]) + dt * dln[u];
+    for (int s = 0; s < NSP; s++) {  // :550-614
+      const double vv = U[F_W + s];
+      const double divvv = gw[s] + 2 * vv * st(A_INVR, j);
+      nln[s] = U[F_LN + s] - dt * (divvv + vv * gln[s]) + dt * dln[s];
       double ww = vv -
-                  dt * (vv * gw[u] -
-                        (U[C_VN] * U[C_VN] + U[C_VE] * U[C_VE]) * st(A_INVR, j) +
-                        T * gln[u] * KB / m[s] + ga[0] * KB / m[s] - st(A_GRAV, j)) +
-                  dt * dw[u] - vv / kp.vert_tau;
+                  dt * (vv * gw[s] -
+                        (U[F_VN] * U[F_VN] + U[F_VE] * U[F_VE]) * st(A_INVR, j) +
+                        T * gln[s] * KB / m[s] + ga[0] * KB / m[s] - st(A_GRAV, j)) +
+                  dt * dw[s] - vv / kp.vert_tau;
       if (kp.coriolis)
-        ww = ww + dt * (centrifugal / st(A_INVR, j) + coriolis * U[C_VE]);
+        ww = ww + dt * (centrifugal / st(A_INVR, j) + coriolis * U[F_VE]);
       ww = ww - dt * (kp.tdc[s] * KB * ga[0]) / m[s];
-      w[u] = ww;
+      w[s] = ww;
     }
-    if (kp.friction) {  // :632-641, old densities
-      const bool ok = friction(dt, w, nt, NS, glc, T);
-      if (cl && w0 && !ok) fail(GITM_VERT_ESINGULAR, k);
-    }
-    double vt[NSH];
+    if (kp.friction && !friction(dt, w, nt, NS, glc, T))  // :632-641, old densities
+      fail(GITM_VERT_ESINGULAR, k);
+    double nvu = 0.0;
 #pragma unroll
-    for (int u = 0; u < NSH; u++) {  // :643-658
-      double ww = dmax(-kp.maxvv, w[u]);
+    for (int s = 0; s < NSP; s++) {  // :643-658
+      double ww = dmax(-kp.maxvv, w[s]);
       ww = dmin(kp.maxvv, ww);
-      w[u] = ww;
-      vt[u] = ww * (m[s0 + u] * NS[u] / rho);
+      w[s] = ww;
+      nvu = nvu + ww * (m[s] * NS[s] / rho);
     }
-    const double nvu = sum6(vt);
-    const double vu = U[C_VU];  // :660-679
-    const double nvE = U[C_VE] - dt * vu * ga[1] + dt * da[1];
-    const double nvN = U[C_VN] - dt * vu * ga[2] + dt * da[2];
+    const double vu = U[F_VU];  // :660-679
+    const double nvE = U[F_VE] - dt * vu * ga[1] + dt * da[1];
+    const double nvN = U[F_VN] - dt * vu * ga[2] + dt * da[2];
     const double nT = T - dt * (vu * ga[0] + (S.gm1) * (T * divvel)) + dt * da[0];
-    double t[NSH];
+    double sr = 0.0;
 #pragma unroll
-    for (int u = 0; u < NSH; u++) t[u] = m[s0 + u] * exp(nln[u]);
-    const double sr = sum6(t);
-    const double nlr = log(sr);
-    if (cl) {
+    for (int s = 0; s < NSP; s++) sr = sr + m[s] * exp(nln[s]);
 #pragma unroll
-      for (int u = 0; u < NSH; u++) {
-        rusanov_commit(stg, U_LN + u, nln[u]);
-        rusanov_commit(stg, U_W + u, w[u]);
-      }
-      rusanov_commit_bulk(stg, C_LR, nlr);
-      rusanov_commit_bulk(stg, C_VU, nvu);
-      rusanov_commit_bulk(stg, C_VE, nvE);
-      rusanov_commit_bulk(stg, C_VN, nvN);
-      rusanov_commit_bulk(stg, C_T, nT);
+    for (int s = 0; s < NSP; s++) {
+      rusanov_commit(stg, F_LN + s, nln[s]);
+      rusanov_commit(stg, F_W + s, w[s]);
     }
-    mirror_bulk();
+    rusanov_commit(stg, F_LR, log(sr));
+    rusanov_commit(stg, F_VU, nvu);
+    rusanov_commit(stg, F_VE, nvE);
+    rusanov_commit(stg, F_VN, nvN);
+    rusanov_commit(stg, F_T, nT);
   }
 
   // Ion advection of one stage: the first NI_ADVECT ions move with IVel(up)
   // (Rusanov gradient and dissipation, calc_rusanov_alts_*), the others are
   // carried through the RK4 combination unchanged.  The values are final as
-  // soon as they are computed, so they are committed at once (by the even
-  // lane, which owns the ion state).
+  // soon as they are computed, so they are committed at once.
   GV_MEM void ion_stage(double dtin, int stg, bool ausm) {
     double vi[NI_ADVECT], gi[NI_ADVECT], di[NI_ADVECT];
 #pragma unroll
     for (int f = 0; f < NI_ADVECT; f++) vi[f] = ion(f, j);
-    rusanov_grad_diff<NI_ADVECT, false>(vi, gi, di);
-    if (!cell() || !w0) return;
+    rusanov_grad_diff<NI_ADVECT>(vi, gi, di);
+    if (!cell()) return;
     const double ivu = ion(NI, j);
 #pragma unroll
     for (int f = 0; f < NI; f++) {
@@ -1559,24 +1416,16 @@ struct Column {
   // :167-189) and so is Y + K/2 (Rusanov :139-169).  The top ion boundary
   // reads these old ghost values.
   GV_MEM void ion_ghosts_from_y() {
-    if (IONS && w0 && !cell()) {
+    if (IONS && !cell()) {
 #pragma unroll
       for (int f = 0; f < NI; f++) ion(f, j) = Yi(f);
     }
   }
   GV_MEM void ion_store_y() {
-    if (IONS && w0) {
+    if (IONS) {
 #pragma unroll
       for (int f = 0; f < NI; f++) Yi(f) = ion(f, j);
     }
-  }
-  GV_MEM void store_y() {
-    if (cell()) {
-#pragma unroll
-      for (int u = 0; u < NU; u++)
-        if (u >= NB || w0) Yv(cid(u)) = U[u];
-    }
-    ion_store_y();
   }
 
   // -------------------------------------------------------------------------
@@ -1589,7 +1438,11 @@ struct Column {
   // stage (ghost levels never enter the combination).
   GV_MEM void ausm_step() {
     bcs();
-    store_y();
+    if (cell()) {
+#pragma unroll
+      for (int f = 0; f < NF; f++) Yv(f) = U[f];
+    }
+    ion_store_y();
     const double dtin[4] = {0.5 * kp.dt, 0.5 * kp.dt, kp.dt, 0.5 * kp.dt};  // :160-377
 #pragma unroll 1
     for (int stg = 0; stg < 4; stg++) {
@@ -1601,7 +1454,11 @@ struct Column {
 
   GV_MEM void rusanov_step() {
     bcs();
-    store_y();
+    if (cell()) {
+#pragma unroll
+      for (int f = 0; f < NF; f++) Yv(f) = U[f];
+    }
+    ion_store_y();
 #pragma unroll 1
     for (int stg = 0; stg < 4; stg++) {
       rusanov_stage(stg);
@@ -1628,16 +1485,16 @@ GV_FN void run_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io,
   c.scr = kp.scratch + (long long)col * kp.scr_fields * (kp.n_alts + 4);
   const int nlb = io.n_lons * io.n_lats;
   const int ci = col % io.n_lons, cj = (col / io.n_lons) % io.n_lats, cb = col / nlb;
-  const int k = c.k, nA = c.nA, s0 = c.s0;
+  const int k = c.k, nA = c.nA;
   const bool lv = c.lv;
   double alt = 0.0, grav = 0.0, invr = 1.0, dalt = 1.0, mmm = 0.0, gam = 1.5,
          eddy = 0.0, cmax = 0.0, tu = 1.0;
-  double nds[NSH];
+  double nds[NSP];
 #pragma unroll
-  for (int u = 0; u < NU; u++) c.U[u] = 0.0;
-  c.U[C_T] = 1.0;
+  for (int f = 0; f < NF; f++) c.U[f] = 0.0;
+  c.U[F_T] = 1.0;
 #pragma unroll
-  for (int u = 0; u < NSH; u++) nds[u] = 1.0;
+  for (int s = 0; s < NSP; s++) nds[s] = 1.0;
   double coslat;
   {
     const gitm_vert_field &f = io.cos_latitude;
@@ -1658,31 +1515,29 @@ GV_FN void run_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io,
     const long long bw = col_base(io.vertical_velocity, ci, cj, cb);
     const long long bv = col_base(io.velocity, ci, cj, cb);
 #pragma unroll
-    for (int u = 0; u < NSH; u++) {
-      nds[u] = *fld(io.ndensity_s, bn, k, s0 + u);
-      c.U[U_W + u] = *fld(io.vertical_velocity, bw, k, s0 + u);
+    for (int s = 0; s < NSP; s++) {
+      nds[s] = *fld(io.ndensity_s, bn, k, s);
+      c.U[F_W + s] = *fld(io.vertical_velocity, bw, k, s);
     }
-    c.U[C_VE] = *fld(io.velocity, bv, k, 0);
-    c.U[C_VN] = *fld(io.velocity, bv, k, 1);
-    c.U[C_VU] = *fld(io.velocity, bv, k, 2);
-    c.U[C_T] = GV_LD(temperature);
-    c.U[C_LR] = GV_LD(rho);
+    c.U[F_VE] = *fld(io.velocity, bv, k, 0);
+    c.U[F_VN] = *fld(io.velocity, bv, k, 1);
+    c.U[F_VU] = *fld(io.velocity, bv, k, 2);
+    c.U[F_T] = GV_LD(temperature);
+    c.U[F_LR] = GV_LD(rho);
 #undef GV_LD
   }
   if (IONS) {
     // LogINS (linear densities with UseImprovedIonAdvection) and IVel(up)
     // (:120-130); SZA and MLat of the column for the nighttime BCs (:84-85)
-    if (c.w0) {
-      const long long bi = col_base(io.idensity_s, ci, cj, cb);
+    const long long bi = col_base(io.idensity_s, ci, cj, cb);
 #pragma unroll
-      for (int f = 0; f < NI; f++) {
-        const double v = *fld(io.idensity_s, bi, k, f);
-        c.ion(f, c.j) = kp.improved_ions ? v : log(v);
-      }
-      c.ion(NI, c.j) = (k >= 1 && k <= nA)
-                           ? *fld(io.ivelocity, col_base(io.ivelocity, ci, cj, cb), k, 2)
-                           : 0.0;
+    for (int f = 0; f < NI; f++) {
+      const double v = *fld(io.idensity_s, bi, k, f);
+      c.ion(f, c.j) = kp.improved_ions ? v : log(v);
     }
+    c.ion(NI, c.j) = (k >= 1 && k <= nA)
+                         ? *fld(io.ivelocity, col_base(io.ivelocity, ci, cj, cb), k, 2)
+                         : 0.0;
     if (kp.nighttime_ion_bcs) {
       const double sza = io.sza.ptr[col_base(io.sza, ci, cj, cb)];
       const double mlat = io.mlat.ptr[col_base(io.mlat, ci, cj, cb)];
@@ -1695,34 +1550,32 @@ GV_FN void run_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io,
     // gather (:96-116): the densities must be positive on every level
     bool bad = false;
 #pragma unroll
-    for (int u = 0; u < NSH; u++) bad = bad || (lv && !(nds[u] > 0.0));
+    for (int s = 0; s < NSP; s++) bad = bad || (lv && !(nds[s] > 0.0));
     if (cx.sync_or(bad)) {
-      const int mine = bad ? k : 1 << 20;
-      const int other = cx.pair_int(mine);
-      if (c.w0) c.ib(c.j) = mine < other ? mine : other;
+      c.ib(c.j) = bad ? k : 1 << 20;
       cx.sync();
-      if (c.q == 0) {
+      if (c.j == 0) {
         int lo = 1 << 20;
-        for (int jj = 0; jj < c.L; jj++)
-          if (c.ib(jj) < lo) lo = c.ib(jj);
+        for (int q = 0; q < c.L; q++)
+          if (c.ib(q) < lo) lo = c.ib(q);
         c.fail(GITM_VERT_ENONPOSITIVE, lo);
       }
       return;  // the column is left as it was
     }
-    c.U[C_LR] = log(c.U[C_LR]);
-    c.U[C_T] = c.U[C_T] * tu;
+    c.U[F_LR] = log(c.U[F_LR]);
+    c.U[F_T] = c.U[F_T] * tu;
 #pragma unroll
-    for (int u = 0; u < NSH; u++) c.U[U_LN + u] = log(nds[u]);
+    for (int s = 0; s < NSP; s++) c.U[F_LN + s] = log(nds[s]);
     if (SOLVER == GITM_VERT_AUSM)
       c.ausm_step();
     else
       c.rusanov_step();
     // scatter (:163-185)
-    c.U[C_LR] = exp(c.U[C_LR]);
-    c.U[C_T] = c.U[C_T] / tu;
+    c.U[F_LR] = exp(c.U[F_LR]);
+    c.U[F_T] = c.U[F_T] / tu;
 #pragma unroll
-    for (int u = 0; u < NSH; u++) nds[u] = exp(c.U[U_LN + u]);
-    if (IONS && c.w0) {
+    for (int s = 0; s < NSP; s++) nds[s] = exp(c.U[F_LN + s]);
+    if (IONS) {
       // IDensityS as the scatter writes it (:187-212) and the next gather
       // reads it back: floored at 1 on 1..nAlts, or exp(LogINS)
 #pragma unroll
@@ -1736,41 +1589,40 @@ GV_FN void run_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io,
       }
     }
   }
-  const double nsum = c.sum6(nds);
   if (lv) {
     const long long bn = col_base(io.ndensity_s, ci, cj, cb);
     const long long bw = col_base(io.vertical_velocity, ci, cj, cb);
     const long long bv = col_base(io.velocity, ci, cj, cb);
+    double nsum = 0.0;
 #pragma unroll
-    for (int u = 0; u < NSH; u++) {
-      *fld(io.ndensity_s, bn, k, s0 + u) = nds[u];
-      *fld(io.vertical_velocity, bw, k, s0 + u) = c.U[U_W + u];
+    for (int s = 0; s < NSP; s++) {
+      *fld(io.ndensity_s, bn, k, s) = nds[s];
+      *fld(io.vertical_velocity, bw, k, s) = c.U[F_W + s];
+      nsum = nsum + nds[s];
     }
     if (io.log_ns.ptr) {
       const long long bl = col_base(io.log_ns, ci, cj, cb);
 #pragma unroll
-      for (int u = 0; u < NSH; u++) *fld(io.log_ns, bl, k, s0 + u) = c.U[U_LN + u];
+      for (int s = 0; s < NSP; s++) *fld(io.log_ns, bl, k, s) = c.U[F_LN + s];
     }
-    if (c.w0) {
-      if (io.ndensity.ptr) *fld(io.ndensity, col_base(io.ndensity, ci, cj, cb), k, 0) = nsum;
-      *fld(io.velocity, bv, k, 0) = c.U[C_VE];
-      *fld(io.velocity, bv, k, 1) = c.U[C_VN];
-      *fld(io.velocity, bv, k, 2) = c.U[C_VU];
-      *fld(io.temperature, col_base(io.temperature, ci, cj, cb), k, 0) = c.U[C_T];
-      *fld(io.rho, col_base(io.rho, ci, cj, cb), k, 0) = c.U[C_LR];
-      if (IONS) {
-        // ions and electrons = their sum (:187-212)
-        const long long bi = col_base(io.idensity_s, ci, cj, cb);
-        double e = 0.0;
+    if (io.ndensity.ptr) *fld(io.ndensity, col_base(io.ndensity, ci, cj, cb), k, 0) = nsum;
+    *fld(io.velocity, bv, k, 0) = c.U[F_VE];
+    *fld(io.velocity, bv, k, 1) = c.U[F_VN];
+    *fld(io.velocity, bv, k, 2) = c.U[F_VU];
+    *fld(io.temperature, col_base(io.temperature, ci, cj, cb), k, 0) = c.U[F_T];
+    *fld(io.rho, col_base(io.rho, ci, cj, cb), k, 0) = c.U[F_LR];
+    if (IONS) {
+      // ions and electrons = their sum (:187-212)
+      const long long bi = col_base(io.idensity_s, ci, cj, cb);
+      double e = 0.0;
 #pragma unroll
-        for (int f = 0; f < NI; f++) {
-          const double u = c.ion(f, c.j);
-          const double v = kp.improved_ions ? u : exp(u);
-          *fld(io.idensity_s, bi, k, f) = v;
-          e = e + v;
-        }
-        *fld(io.idensity_s, bi, k, NI) = e;
+      for (int f = 0; f < NI; f++) {
+        const double u = c.ion(f, c.j);
+        const double v = kp.improved_ions ? u : exp(u);
+        *fld(io.idensity_s, bi, k, f) = v;
+        e = e + v;
       }
+      *fld(io.idensity_s, bi, k, NI) = e;
     }
   }
 }

@@ -102,16 +102,11 @@ inline KParams make_kparams(const gitm_vert_params *p, const gitm_vert_columns *
   return k;
 }
 
-// Workgroup tiles: wavefronts per column.  Two lanes per level, so a tile
-// of W waves holds columns of up to 32*W levels.
-inline bool valid_waves(int w) { return w == 1 || w == 2 || w == 4 || w == 6 || w == 8; }
-inline bool tile_fits(int w, int n_alts) { return 64 * w >= 2 * (n_alts + 4); }
+// Workgroup tiles: wavefronts per column, one lane per level.
+inline bool valid_waves(int w) { return w >= 1 && w <= 4; }
+inline bool tile_fits(int w, int n_alts) { return 64 * w >= n_alts + 4; }
 // Smallest tile holding the nAlts+4 levels.
-inline int auto_waves(int n_alts) {
-  for (int w : {1, 2, 4, 6, 8})
-    if (tile_fits(w, n_alts)) return w;
-  return 0;
-}
+inline int auto_waves(int n_alts) { return (n_alts + 4 + 63) / 64; }
 
 inline void decode_error(unsigned long long key, gitm_vert_error *e) {
   if (key == ~0ULL) {

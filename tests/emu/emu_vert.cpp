@@ -20,10 +20,8 @@ struct Shared {
   std::barrier<> bar;
   std::atomic<int> orflag{0};
   std::atomic<unsigned long long> err{~0ULL};
-  std::vector<double> pd;  // pair exchange slots, one per lane
-  std::vector<int> pi;
   bool device_math;  // the GPU's forms (exp(e ln T), log-space sums) instead of the reference's
-  Shared(int n, bool dm) : bar(n), pd(n), pi(n), device_math(dm) {}
+  Shared(int n, bool dm) : bar(n), device_math(dm) {}
 };
 
 struct EmuCtx {
@@ -46,22 +44,6 @@ struct EmuCtx {
     }
   }
   bool device_math() const { return sh->device_math; }
-  // the other lane of the level; every lane of the workgroup calls it at the
-  // same point, as the device's in-wavefront swap requires
-  double pair(double v) {
-    sh->pd[lane_] = v;
-    sync();
-    const double r = sh->pd[lane_ ^ 1];
-    sync();
-    return r;
-  }
-  int pair_int(int v) {
-    sh->pi[lane_] = v;
-    sync();
-    const int r = sh->pi[lane_ ^ 1];
-    sync();
-    return r;
-  }
   void diff_powers(double temp, double sign, double (&out)[6]) const {
     if (sh->device_math) {
       const double lt = log(temp);
@@ -84,7 +66,7 @@ void run_one(EmuCtx &cx, const gv::KParams &kp, const gitm_vert_columns &io, int
 template <int NT>
 unsigned long long run_all(const gv::KParams &kp, const gitm_vert_columns &io, int c0,
                            int c1, bool device_math) {
-  const int nthreads = 2 * (kp.n_alts + 4);  // two threads per level
+  const int nthreads = kp.n_alts + 4;  // one thread per level
   Shared sh(nthreads, device_math);
   // exactly the dynamic LDS a launch gets, so out-of-range indexing shows up
   // under the host sanitizers
@@ -118,7 +100,7 @@ extern "C" int gv_emu_advance(const gitm_vert_params *p, const gitm_vert_columns
   if (rc) return rc;
   gv::KParams kp = gv::make_kparams(p, c, solver, n_steps, nullptr);
   if (waves <= 0) waves = gv::auto_waves(c->n_alts);
-  if (!gv::valid_waves(waves) || !gv::tile_fits(waves, c->n_alts)) return GITM_VERT_EINVAL;
+  if (64 * waves < c->n_alts + 4) return GITM_VERT_EINVAL;
   kp.scr_fields = gv::scratch_fields(gv::carry_in_lds(waves), kp.ions != 0);
   std::vector<double> scratch((size_t)c->n_lons * c->n_lats * c->n_blocks * kp.scr_fields *
                               (c->n_alts + 4));
@@ -127,9 +109,8 @@ extern "C" int gv_emu_advance(const gitm_vert_params *p, const gitm_vert_columns
   switch (waves) {
     case 1: key = run_all<64>(kp, *c, c0, c1, device_math != 0); break;
     case 2: key = run_all<128>(kp, *c, c0, c1, device_math != 0); break;
+    case 3: key = run_all<192>(kp, *c, c0, c1, device_math != 0); break;
     case 4: key = run_all<256>(kp, *c, c0, c1, device_math != 0); break;
-    case 6: key = run_all<384>(kp, *c, c0, c1, device_math != 0); break;
-    case 8: key = run_all<512>(kp, *c, c0, c1, device_math != 0); break;
     default: return GITM_VERT_EINVAL;
   }
   gitm_vert_error tmp;
