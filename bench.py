@@ -64,6 +64,38 @@ def interface_bytes_per_column(n_alts: int) -> int:
     return 8 * ((17 * 2 + 7 + 7) * L + (n_alts + 2) + n_alts + 1)
 
 
+def prep_bytes_per_column(n_alts: int) -> int:
+    """HBM bytes of the column prep (gitm_vert_prepare) per column: reads
+    NDensityS (6), Temperature, Rho on all levels, the 6 vertical winds on
+    0..nAlts+1 and dAlt on 1..nAlts; writes NDensity, Pressure, cp, Gamma,
+    MeanMajorMass, TempUnit, the eddy coefficient on all levels and cMax on
+    0..nAlts+1."""
+    L = n_alts + 4
+    return 8 * (8 * L + 6 * (n_alts + 2) + n_alts + 7 * L + (n_alts + 2))
+
+
+def time_prep(torch, solver, dev, steps: int) -> dict:
+    """Mean launch time of the column prep on the same grid (HIP events on
+    the launch stream)."""
+    from gitm_amd import synth
+    for k in ("press", "cp"):
+        dev[k] = torch.zeros_like(dev["temp"])
+    dt = torch.empty(1, dtype=torch.float64, device="cuda")
+    pp = synth.PrepParams()
+    stream = torch.cuda.current_stream()
+    solver.prepare(dev, pp, dt_out=dt)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    for a, b in ev:
+        a.record(stream)
+        solver.prepare(dev, pp, dt_out=dt)
+        b.record(stream)
+    torch.cuda.synchronize()
+    return {"launch_ms": float(np.mean([a.elapsed_time(b) for a, b in ev])),
+            "dt": float(dt.item())}
+
+
 def read_traffic(path: str):
     """HBM bytes per launch from the counter summary (FETCH_SIZE + WRITE_SIZE,
     scaled by the copy-kernel calibration), or None."""
@@ -174,6 +206,7 @@ def main() -> int:
               file=sys.stderr)
         return 1
     launch_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    prep = time_prep(torch, solver, dev, max(args.steps, 10))
 
     ncols = spec.ncols
     value = ncols * args.steps / wall
@@ -214,6 +247,16 @@ def main() -> int:
                      "kernel": f"vert_column_kernel<{64 * vertical.auto_waves(spec.nalts)},"
                                f"{1 if args.solver == 'ausm' else 0}>"},
         "cpu_baseline": cpu,
+        "prep": {"kernel": "prep_kernel (gitm_vert_prepare: calc_pressure, calc_rates "
+                           "mean mass, eddy coefficient, calc_timestep_vertical)",
+                 "launch_ms": prep["launch_ms"], "bound": "hbm",
+                 "bytes_per_launch": prep_bytes_per_column(spec.nalts) * ncols,
+                 "achieved": prep_bytes_per_column(spec.nalts) * ncols
+                 / (prep["launch_ms"] * 1e-3) / 1e9,
+                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": prep_bytes_per_column(spec.nalts) * ncols
+                 / (prep["launch_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                 "cfl_dt": prep["dt"]},
         "adjoint_forward_ratio": None,
     }
     print(json.dumps(line), flush=True)

@@ -66,6 +66,15 @@ class Params(ctypes.Structure):
                 ("reserved", ctypes.c_int32)]
 
 
+class PrepParams(ctypes.Structure):
+    _fields_ = [("mass", ctypes.c_double * NSPECIES),
+                ("vibration", ctypes.c_double * NSPECIES),
+                ("cfl", ctypes.c_double), ("fixed_dt", ctypes.c_double),
+                ("dt_end", ctypes.c_double), ("eddy_coef", ctypes.c_double),
+                ("eddy_p0", ctypes.c_double), ("eddy_p1", ctypes.c_double),
+                ("use_ion_advection", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 class Error(ctypes.Structure):
     _fields_ = [("code", ctypes.c_int32), ("column", ctypes.c_int32),
                 ("level", ctypes.c_int32), ("reserved", ctypes.c_int32)]

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "vert_params.h"
+#include "vert_prep.h"
 
 namespace {
 
@@ -62,6 +63,44 @@ void launch(int solver, dim3 grid, hipStream_t stream, const gv::KParams &kp,
 }
 
 __global__ void reset_error_word(unsigned long long *e) { *e = ~0ULL; }
+
+// Column prep, one workgroup per column and one lane per level; lane 0 folds
+// the column's CFL limit into *dt_bits (non-negative doubles order as their
+// bit patterns, so an integer atomic min is a double min).
+template <int NT>
+__global__ void __launch_bounds__(NT) prep_kernel(gv::PrepParams pp, gitm_vert_columns io,
+                                                  gitm_vert_field pf, gitm_vert_field cpf,
+                                                  unsigned long long *dt_bits) {
+  __shared__ double red[2][NT / 64];
+  const int col = (int)blockIdx.x, j = (int)threadIdx.x;
+  const int nlb = io.n_lons * io.n_lats;
+  const int ci = col % io.n_lons, cj = (col / io.n_lons) % io.n_lats, cb = col / nlb;
+  double rate = 0.0, irate = 0.0;
+  if (j < pp.n_alts + 4) gv::prep_level(pp, io, pf, cpf, ci, cj, cb, j - 1, rate, irate);
+  // maxval over the column's levels 1..nAlts (calc_timestep.f90:122,132)
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    rate = fmax(rate, __shfl_xor(rate, o));
+    irate = fmax(irate, __shfl_xor(irate, o));
+  }
+  if ((j & 63) == 0) {
+    red[0][j >> 6] = rate;
+    red[1][j >> 6] = irate;
+  }
+  __syncthreads();
+  if (j == 0) {
+    double r = red[0][0], ri = red[1][0];
+    for (int w = 1; w < (int)((blockDim.x + 63) / 64); w++) {
+      r = fmax(r, red[0][w]);
+      ri = fmax(ri, red[1][w]);
+    }
+    double dt = pp.cfl / r;
+    if (pp.ions) dt = fmin(dt, pp.cfl / ri);
+    atomicMin(dt_bits, (unsigned long long)__double_as_longlong(dt));
+  }
+}
+
+__global__ void set_word(unsigned long long *w, unsigned long long v) { *w = v; }
 
 }  // namespace
 
@@ -183,6 +222,79 @@ int gitm_vert_advance(gitm_vert_handle *h, const gitm_vert_params *p,
     case 2: launch<128>(solver, grid, h->stream, kp, *c); break;
     case 3: launch<192>(solver, grid, h->stream, kp, *c); break;
     case 4: launch<256>(solver, grid, h->stream, kp, *c); break;
+    default: return GITM_VERT_EINVAL;
+  }
+  GV_HIP(hipGetLastError());
+  return GITM_VERT_OK;
+}
+
+void gitm_vert_default_prep_params(gitm_vert_prep_params *p) {
+  if (!p) return;
+  memset(p, 0, sizeof(*p));
+  gitm_vert_params sp;
+  gv::default_params(&sp);
+  const double vib[GITM_VERT_NSPECIES] = {5.0, 7.0, 7.0, 5.0, 7.0, 5.0};  // ModEarth.f90:307-312
+  for (int s = 0; s < GITM_VERT_NSPECIES; s++) {
+    p->mass[s] = sp.mass[s];
+    p->vibration[s] = vib[s];
+  }
+  p->cfl = 0.75;          // ModInputs.f90:123
+  p->fixed_dt = 2.0;
+  p->dt_end = 1.0e32;
+  p->eddy_coef = 50.0;    // :302
+  p->eddy_p0 = 0.01;      // :303
+  p->eddy_p1 = 0.005;     // :304
+}
+
+int gitm_vert_prepare(gitm_vert_handle *h, const gitm_vert_prep_params *p,
+                      const gitm_vert_columns *c, const gitm_vert_field *pressure,
+                      const gitm_vert_field *cp, double *dt_device) {
+  if (!h || !p || !c || !dt_device) return GITM_VERT_EINVAL;
+  if (c->n_alts < 10 || c->n_alts > GITM_VERT_MAX_NALTS) return GITM_VERT_EINVAL;
+  if (c->n_lons < 1 || c->n_lats < 1 || c->n_blocks < 1) return GITM_VERT_EINVAL;
+  if ((long long)c->n_lons * c->n_lats * c->n_blocks > 0x7fffffffLL) return GITM_VERT_EINVAL;
+  const gitm_vert_field *req[] = {&c->ndensity_s, &c->temperature, &c->rho,
+                                  &c->vertical_velocity, &c->dalt, &c->gamma,
+                                  &c->mean_major_mass, &c->temp_unit, &c->eddy, &c->cmax};
+  for (const gitm_vert_field *f : req)
+    if (!f->ptr) return GITM_VERT_EINVAL;
+  if (p->use_ion_advection && !c->ivelocity.ptr) return GITM_VERT_EINVAL;
+  if (!(p->cfl > 0.0) || !(p->fixed_dt > 0.0)) return GITM_VERT_EINVAL;
+  if (c->cmax.first_level < -1 || c->cmax.first_level > 0) return GITM_VERT_EINVAL;
+  for (int s = 0; s < GITM_VERT_NSPECIES; s++)
+    if (!(p->mass[s] > 0.0) || p->vibration[s] == 2.0) return GITM_VERT_EINVAL;
+  GV_HIP(hipSetDevice(h->device));
+  gv::PrepParams pp;
+  for (int s = 0; s < gv::NSP; s++) {
+    pp.mass[s] = p->mass[s];
+    pp.vib2[s] = p->vibration[s] - 2;
+    pp.kb_over_m[s] = gv::KB / p->mass[s];
+  }
+  pp.cfl = p->cfl;
+  pp.eddy_coef = p->eddy_coef;
+  pp.eddy_p0 = p->eddy_p0;
+  pp.eddy_p1 = p->eddy_p1;
+  pp.n_alts = c->n_alts;
+  pp.ions = p->use_ion_advection != 0;
+  const gitm_vert_field none = {};
+  const gitm_vert_field pf = pressure ? *pressure : none;
+  const gitm_vert_field cpf = cp ? *cp : none;
+  // min(FixedDt, DtEnd, 1e32) before the columns' limits (calc_timestep.f90
+  // :105,142,147)
+  double d0 = 1.0e32;
+  if (p->dt_end < d0) d0 = p->dt_end;
+  if (p->fixed_dt < d0) d0 = p->fixed_dt;
+  unsigned long long *bits = reinterpret_cast<unsigned long long *>(dt_device);
+  unsigned long long d0_bits;
+  memcpy(&d0_bits, &d0, sizeof(d0_bits));
+  hipLaunchKernelGGL(set_word, dim3(1), dim3(1), 0, h->stream, bits, d0_bits);
+  const int ncols = c->n_lons * c->n_lats * c->n_blocks;
+  const dim3 grid(ncols), block(c->n_alts + 4);
+  switch (gv::auto_waves(c->n_alts)) {
+    case 1: hipLaunchKernelGGL(prep_kernel<64>, grid, block, 0, h->stream, pp, *c, pf, cpf, bits); break;
+    case 2: hipLaunchKernelGGL(prep_kernel<128>, grid, block, 0, h->stream, pp, *c, pf, cpf, bits); break;
+    case 3: hipLaunchKernelGGL(prep_kernel<192>, grid, block, 0, h->stream, pp, *c, pf, cpf, bits); break;
+    case 4: hipLaunchKernelGGL(prep_kernel<256>, grid, block, 0, h->stream, pp, *c, pf, cpf, bits); break;
     default: return GITM_VERT_EINVAL;
   }
   GV_HIP(hipGetLastError());

@@ -41,6 +41,21 @@ COMPONENTS = {"nds": NSPECIES, "vvel": NSPECIES, "vel": 3, "logns": NSPECIES,
               "ions": NIONS, "ivel": 3}
 
 
+@dataclasses.dataclass
+class PrepParams:
+    """Inputs of the column prep and the vertical time step: the reference
+    defaults (ModInputs.f90:123,302-304; ModEarth.f90:271-281,307-312)."""
+    mass: tuple = dataclasses.field(default_factory=lambda: tuple(species_masses()))
+    vibration: tuple = VIBRATION
+    cfl: float = 0.75
+    fixed_dt: float = 2.0
+    dt_end: float = 1.0e32
+    eddy_coef: float = 50.0
+    eddy_p0: float = 0.01
+    eddy_p1: float = 0.005
+    use_ion_advection: bool = False
+
+
 def species_masses() -> np.ndarray:
     """Species masses in kg, formed as init_planet does (ModEarth.f90:271-281)."""
     mo = 16.0 * AMU

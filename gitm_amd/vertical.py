@@ -65,6 +65,11 @@ def lib() -> ctypes.CDLL:
                      "gitm_vert_set_tile"):
             getattr(_lib, name).restype = ctypes.c_int
         _lib.gitm_vert_strerror.restype = ctypes.c_char_p
+        _lib.gitm_vert_prepare.restype = ctypes.c_int
+        _lib.gitm_vert_prepare.argtypes = [ctypes.c_void_p, ctypes.POINTER(_abi.PrepParams),
+                                           ctypes.POINTER(_abi.Columns),
+                                           ctypes.POINTER(_abi.Field),
+                                           ctypes.POINTER(_abi.Field), ctypes.c_void_p]
         _lib.gitm_vert_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]
         _lib.gitm_vert_destroy.argtypes = [ctypes.c_void_p]
         _lib.gitm_vert_set_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
@@ -92,6 +97,21 @@ def make_params(sp: StepParams | None = None) -> _abi.Params:
     p.use_ion_advection = int(sp.use_ion_advection)
     p.use_improved_ion_advection = int(sp.use_improved_ion_advection)
     p.use_nighttime_ion_bcs = int(sp.use_nighttime_ion_bcs)
+    return p
+
+
+def make_prep_params(pp=None) -> _abi.PrepParams:
+    """C-ABI prep parameters from a :class:`gitm_amd.synth.PrepParams`."""
+    from .synth import PrepParams
+    pp = pp or PrepParams()
+    p = _abi.PrepParams()
+    lib().gitm_vert_default_prep_params(ctypes.byref(p))
+    for s in range(_abi.NSPECIES):
+        p.mass[s] = pp.mass[s]
+        p.vibration[s] = pp.vibration[s]
+    p.cfl, p.fixed_dt, p.dt_end = pp.cfl, pp.fixed_dt, pp.dt_end
+    p.eddy_coef, p.eddy_p0, p.eddy_p1 = pp.eddy_coef, pp.eddy_p0, pp.eddy_p1
+    p.use_ion_advection = int(pp.use_ion_advection)
     return p
 
 
@@ -171,6 +191,39 @@ class VerticalSolver:
             raise VerticalStepError(rc)
         if check:
             self.check()
+
+    def prepare(self, cols: dict, pp=None, dt_out=None):
+        """Column prep and the vertical CFL step on device tensors
+        (``gitm_vert_prepare``): fills cols["nden"], "gamma", "mmm",
+        "tempunit", "eddy", "cmax" (levels 0..nAlts+1) and, when present,
+        "press" and "cp" from "nds", "temp", "rho", "vvel", "dalt" (and
+        "ivel" with ion advection).  Returns a one-element device tensor with
+        the time step (``dt_out`` if given); queued on the current stream."""
+        import torch
+
+        ptrs = {}
+        for k, t in cols.items():
+            if k not in _abi.CANON and k not in _abi.PER_COLUMN and k not in ("press", "cp"):
+                continue
+            if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+                raise ValueError(f"{k}: need a contiguous float64 device tensor")
+            ptrs[k] = t.data_ptr()
+        nc, L = cols["temp"].shape
+        extra = {k: ptrs.pop(k) for k in ("press", "cp") if k in ptrs}
+        c = _abi.columns_struct(ptrs, nc, L - 4)
+        pf = _abi.column_field(extra["press"], 1, L) if "press" in extra else None
+        cpf = _abi.column_field(extra["cp"], 1, L) if "cp" in extra else None
+        dt = dt_out if dt_out is not None else torch.empty(1, dtype=torch.float64,
+                                                           device=cols["temp"].device)
+        p = make_prep_params(pp)
+        self.set_stream(torch.cuda.current_stream().cuda_stream)
+        rc = lib().gitm_vert_prepare(self._h, ctypes.byref(p), ctypes.byref(c),
+                                     ctypes.byref(pf) if pf is not None else None,
+                                     ctypes.byref(cpf) if cpf is not None else None,
+                                     ctypes.c_void_p(dt.data_ptr()))
+        if rc:
+            raise VerticalStepError(rc)
+        return dt
 
     # -- host numpy arrays --------------------------------------------------
     def advance_host(self, cols: dict, params: StepParams | None = None,

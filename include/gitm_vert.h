@@ -166,6 +166,46 @@ int gitm_vert_set_tile(gitm_vert_handle *h, int waves_per_column);
 
 const char *gitm_vert_strerror(int code);
 
+/* ------------------------------------------------------------------------
+ * Column prep and the vertical time step: the per-step inputs the vertical
+ * step reads, computed on the device from the state, so that a caller can
+ * keep the state in device memory across steps.
+ * ------------------------------------------------------------------------ */
+typedef struct gitm_vert_prep_params {
+  double mass[GITM_VERT_NSPECIES];      /* Mass(1:6) [kg]                     */
+  double vibration[GITM_VERT_NSPECIES]; /* Vibration(1:6)  ModEarth.f90:307-312 */
+  double cfl;                /* CFL                  ModInputs.f90:123      */
+  double fixed_dt;           /* FixedDt, the step before the CFL limits     */
+  double dt_end;             /* EndTime - CurrentTime                        */
+  double eddy_coef;          /* EddyDiffusionCoef    ModInputs.f90:302       */
+  double eddy_p0, eddy_p1;   /* EddyDiffusionPressure0/1           :303-304  */
+  int32_t use_ion_advection; /* adds the ion CFL limit (calc_timestep.f90:125-136) */
+  int32_t reserved;
+} gitm_vert_prep_params;
+
+/* Fill `p` with the reference defaults. */
+void gitm_vert_default_prep_params(gitm_vert_prep_params *p);
+
+/* For every column of `c` (device pointers), from NDensityS, Temperature,
+ * Rho, VerticalVelocity, dAlt (and IVelocity with use_ion_advection):
+ *   NDensity = sum of NDensityS, Pressure = Temperature*Rho, cp and Gamma
+ *                                          (calc_pressure.f90:17-23,56-86)
+ *   MeanMajorMass, TempUnit                (calc_rates.Earth.f90:34-50)
+ *   KappaEddyDiffusion from Pressure       (Earth.f90:376-411)
+ *   cMax(iUp_) on levels 0..nAlts+1        (calc_timestep.f90:113-118)
+ * written to c->ndensity, pressure (optional), cp (optional),
+ * c->gamma, c->mean_major_mass, c->temp_unit, c->eddy and c->cmax, all on
+ * levels -1..nAlts+2 except cmax.  *dt_device (one device double) receives
+ * min(fixed_dt, dt_end, CFL / max over 1..nAlts of cMax/dAlt over every
+ * column) (calc_timestep.f90:85-145; the MPI_AllREDUCE MIN over ranks at
+ * :144 stays with the caller).  Asynchronous on the handle's stream.
+ * Replaces: calc_pressure (neutral part), calc_rates' mean mass,
+ * calc_eddy_diffusion_coefficient and calc_timestep_vertical for the
+ * columns given. */
+int gitm_vert_prepare(gitm_vert_handle *h, const gitm_vert_prep_params *p,
+                      const gitm_vert_columns *c, const gitm_vert_field *pressure,
+                      const gitm_vert_field *cp, double *dt_device);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,3 +114,41 @@ def advance(cols: dict, params, solver: int = AUSM, n_steps: int = 1,
         raise RuntimeError(f"oracle failed: {ERRORS.get(rc, rc)} (column {st.col},"
                            f" level {st.level})")
     return out
+
+
+class _PrepParams(ctypes.Structure):
+    _fields_ = [("mass", ctypes.c_double * 6), ("vibration", ctypes.c_double * 6),
+                ("cfl", ctypes.c_double), ("fixed_dt", ctypes.c_double),
+                ("dt_end", ctypes.c_double), ("eddy_coef", ctypes.c_double),
+                ("eddy_p0", ctypes.c_double), ("eddy_p1", ctypes.c_double),
+                ("use_ion_advection", ctypes.c_int)]
+
+
+PREP_OUTPUTS = ("nden", "press", "cp", "gamma", "mmm", "tempunit", "eddy", "cmax")
+
+
+def prepare(cols: dict, pp) -> tuple[dict, float]:
+    """Column prep and the vertical time step (ov_prepare) on a copy.
+
+    ``pp`` has mass, vibration, cfl, fixed_dt, dt_end, eddy_coef, eddy_p0,
+    eddy_p1 and use_ion_advection (gitm_amd.synth.PrepParams).  Returns the
+    outputs (cmax only on levels 0..nAlts+1; its other levels are those of
+    cols["cmax"] if present, else zero) and the time step."""
+    nc, L = cols["temp"].shape
+    na = L - 4
+    inp = {k: np.ascontiguousarray(cols[k], dtype=np.float64)
+           for k in ("nds", "temp", "rho", "vvel", "dalt")}
+    ivel = np.ascontiguousarray(cols["ivel"], dtype=np.float64) if pp.use_ion_advection \
+        else None
+    out = {k: np.zeros((nc, L)) for k in PREP_OUTPUTS}
+    if "cmax" in cols:
+        out["cmax"][:] = cols["cmax"]
+    p = _PrepParams((ctypes.c_double * 6)(*pp.mass), (ctypes.c_double * 6)(*pp.vibration),
+                    pp.cfl, pp.fixed_dt, pp.dt_end, pp.eddy_coef, pp.eddy_p0, pp.eddy_p1,
+                    int(pp.use_ion_advection))
+    f = lib().ov_prepare
+    f.restype = ctypes.c_double
+    dt = f(ctypes.byref(p), nc, na, _ptr(inp["nds"]), _ptr(inp["temp"]), _ptr(inp["rho"]),
+           _ptr(inp["vvel"]), _ptr(inp["dalt"]), _ptr(ivel),
+           *[_ptr(out[k]) for k in PREP_OUTPUTS])
+    return out, float(dt)

@@ -1443,3 +1443,76 @@ int ov_advance_threaded(const ov_params *p, const ov_columns *c, int solver,
   free(th);
   return rc;
 }
+
+
+/* ------------------------------------------------------------------ */
+/* Column prep and the vertical time step                              */
+/* ------------------------------------------------------------------ */
+double ov_prepare(const ov_prep_params *p, int n_cols, int n_alts, const double *nds,
+                  const double *temp, const double *rho, const double *vvel,
+                  const double *dalt, const double *ivel, double *nden, double *press,
+                  double *cp, double *gamma, double *mmm, double *tempunit,
+                  double *eddy, double *cmax) {
+  const int L = n_alts + 4;
+  double dt_local = 1.0e32; /* calc_timestep.f90:105 */
+  for (int col = 0; col < n_cols; col++) {
+    const size_t o = (size_t)col * L;
+    for (int k = 0; k < L; k++) {
+      double n[OV_NSPECIES];
+      for (int s = 0; s < OV_NSPECIES; s++) n[s] = nds[((size_t)col * OV_NSPECIES + s) * L + k];
+      /* calc_pressure.f90:17-23 */
+      double N = 0.0;
+      for (int s = 0; s < OV_NSPECIES; s++) N = N + n[s];
+      nden[o + k] = N;
+      const double P = temp[o + k] * rho[o + k];
+      press[o + k] = P;
+      /* calc_pressure.f90:58-83 */
+      double c = 0.0, g = 0.0;
+      for (int s = 0; s < OV_NSPECIES; s++) {
+        c = c + (p->vibration[s] - 2) * n[s] * (kB / p->mass[s]);
+        g = g + n[s] / (p->vibration[s] - 2);
+      }
+      cp[o + k] = c / (2.0 * N);
+      g = g * 2.0 / N + 1;
+      gamma[o + k] = g;
+      /* calc_rates.Earth.f90:34-50 */
+      double m = 0.0;
+      for (int s = 0; s < OV_NSPECIES; s++) m = m + p->mass[s] * n[s] / (N + 1.0);
+      if (m == 0) m = p->mass[0];
+      mmm[o + k] = m;
+      tempunit[o + k] = m / kB;
+      /* Earth.f90:395-405 */
+      double e = 0.0;
+      if (P > p->eddy_p0)
+        e = p->eddy_coef;
+      else if (P > p->eddy_p1)
+        e = p->eddy_coef * (P - p->eddy_p1) / (p->eddy_p0 - p->eddy_p1);
+      eddy[o + k] = e;
+    }
+    /* calc_timestep.f90:113-136 */
+    double rmax = -1.0, imax = -1.0;
+    for (int k = 0; k <= n_alts + 1; k++) {
+      double w = 0.0;
+      for (int s = 0; s < OV_NSPECIES; s++) {
+        const double a = fabs(vvel[((size_t)col * OV_NSPECIES + s) * L + X(k)]);
+        if (a > w) w = a;
+      }
+      const double c = w + sqrt(gamma[o + X(k)] * temp[o + X(k)]);
+      cmax[o + X(k)] = c;
+      if (k >= 1 && k <= n_alts) {
+        const double r = c / dalt[o + X(k)];
+        if (r > rmax) rmax = r;
+        if (p->use_ion_advection) {
+          const double cm = fabs(ivel[((size_t)col * 3 + 2) * L + X(k)]) +
+                            sqrt(gamma[o + X(k)] * temp[o + X(k)]);
+          const double ri = cm / dalt[o + X(k)];
+          if (ri > imax) imax = ri;
+        }
+      }
+    }
+    dt_local = fmin2(dt_local, p->cfl / rmax);
+    if (p->use_ion_advection) dt_local = fmin2(dt_local, p->cfl / imax);
+  }
+  dt_local = fmin2(dt_local, p->dt_end); /* :142 */
+  return fmin2(p->fixed_dt, dt_local);   /* main.f90:62, calc_timestep.f90:147 */
+}

@@ -101,6 +101,26 @@ void ov_glc_weights(double h1, double h2, double h3, double h4, double w[5]);
 void ov_bc_coefs(double h1, double h2, double h3, double h4, int backward,
                  double C[5]);
 
+/* Column prep and the vertical time step (calc_pressure, calc_rates' mean
+ * mass, calc_eddy_diffusion_coefficient, calc_timestep_vertical). */
+typedef struct {
+  double mass[OV_NSPECIES], vibration[OV_NSPECIES];
+  double cfl, fixed_dt, dt_end, eddy_coef, eddy_p0, eddy_p1;
+  int use_ion_advection;
+} ov_prep_params;
+
+/* Per-level inputs in the column-contiguous layout ([col][comp][level],
+ * levels -1..nAlts+2): nds (6 comps), temp (normalised), rho, vvel (6),
+ * dalt, ivel (3; read only with use_ion_advection).  Outputs (one comp):
+ * nden, press, cp, gamma, mmm, tempunit, eddy on every level, cmax on levels
+ * 0..nAlts+1 (others untouched).  Returns the time step
+ * min(fixed_dt, dt_end, CFL limits of every column). */
+double ov_prepare(const ov_prep_params *p, int n_cols, int n_alts, const double *nds,
+                  const double *temp, const double *rho, const double *vvel,
+                  const double *dalt, const double *ivel, double *nden, double *press,
+                  double *cp, double *gamma, double *mmm, double *tempunit,
+                  double *eddy, double *cmax);
+
 #ifdef __cplusplus
 }
 #endif
