@@ -526,6 +526,13 @@ struct Column {
     return neutral_friction<false>(kp, dtin, w, nt, ns, glc, st(A_EDDY, j), T, tp);
   }
 
+  // a / b in the reference form; a * (1/b) in the device form, with the
+  // reciprocal ib = 1/b shared by several quotients (one division in place
+  // of several)
+  GV_MEM double dq(double a, double b, double ib) const {
+    return cx.device_math() ? a * ib : a / b;
+  }
+
   GV_MEM bool face() const { return k >= 0 && k <= nA; }
   GV_MEM bool cell() const { return k >= 1 && k <= nA; }
   GV_MEM void fail(int code, int level) { cx.report(error_key(col, code, level)); }
@@ -1124,11 +1131,12 @@ struct Column {
         PL = PL + PSL[s];
         PR = PR + PSR[s];
       }
+      const double iRL = 1.0 / RL, iRR = 1.0 / RR;
       double VUL = 0.0, VUR = 0.0;  // mass-weighted face winds (:1505-1532)
 #pragma unroll
       for (int s = 0; s < NSP; s++) {
-        VUL = VUL + RSL[s] * VL[s] / RL;
-        VUR = VUR + RSR[s] * VR[s] / RR;
+        VUL = VUL + dq(RSL[s] * VL[s], RL, iRL);
+        VUR = VUR + dq(RSR[s] * VR[s], RR, iRR);
       }
       // bulk energy (:1538-1562), enthalpy (:1569-1615), Liou sound speed
       // (:1619-1655)
@@ -1136,8 +1144,10 @@ struct Column {
       const double ER = st(A_GR1, j) * PR + 0.5 * RR * (VUR * VUR + VER * VER + VNR * VNR);
       double CS;
       {
-        const double HtL = 0.5 * (VUL * VUL + VEL * VEL + VNL * VNL) + st(A_GL2, j) * PL / RL;
-        const double HtR = 0.5 * (VUR * VUR + VER * VER + VNR * VNR) + st(A_GR2, j) * PR / RR;
+        const double HtL =
+            0.5 * (VUL * VUL + VEL * VEL + VNL * VNL) + dq(st(A_GL2, j) * PL, RL, iRL);
+        const double HtR =
+            0.5 * (VUR * VUR + VER * VER + VNR * VNR) + dq(st(A_GR2, j) * PR, RR, iRR);
         double sub_cs = sqrt(st(A_GL3, j) * HtL);
         const double CSL = (sub_cs * sub_cs) / dmax(sub_cs, VUL);
         sub_cs = sqrt(st(A_GR3, j) * HtR);
@@ -1149,12 +1159,13 @@ struct Column {
       const double rdt0 = st(A_DC, j) / dtin, rdt1 = st(A_DC, j + 1) / dtin;
       double BNVp = 0.0, BNVm = 0.0;
       const double RLR = RL + RR;
+      const double iRLR = 1.0 / RLR, icS = 1.0 / CS;
 #pragma unroll
       for (int s = 0; s < NSP; s++) {
         const double MPS = 0.5 * (PSL[s] + PSR[s]);
         const double MHPS = 0.5 * (HL[s] + HR[s]);
         const double MRS = 0.5 * (RSL[s] + RSR[s]);
-        const double ML = VL[s] / CS, MR = VR[s] / CS;  // :1689-1706
+        const double ML = dq(VL[s], CS, icS), MR = dq(VR[s], CS, icS);  // :1689-1706
         const double M2 = 0.5 * (ML * ML + MR * MR);
         const double MZ = sqrt(dmin(1.0, dmax(M2, 1.0e-19)));
         const double FA = MZ * (2.0 - MZ);
@@ -1186,7 +1197,7 @@ struct Column {
             FRp[s] = RSR[s] * NV;
             FMp[s] = RSR[s] * VR[s] * NV;
           }
-          BNVp = BNVp + (RSL[s] + RSR[s]) * NV / RLR;
+          BNVp = BNVp + dq((RSL[s] + RSR[s]) * NV, RLR, iRLR);
         }
         {  // M12 side of cell k+1
           const bool sh = (subUp >> s) & 1;
@@ -1200,7 +1211,7 @@ struct Column {
             xb(s, j) = RSR[s] * NV;
             xb(6 + s, j) = RSR[s] * VR[s] * NV;
           }
-          BNVm = BNVm + (RSL[s] + RSR[s]) * NV / RLR;
+          BNVm = BNVm + dq((RSL[s] + RSR[s]) * NV, RLR, iRLR);
         }
       }
       // energy and momentum fluxes (:1973-2026)
@@ -1230,15 +1241,16 @@ struct Column {
     {
       const double ap = st(A_AP, j), am = st(A_AM, j), vol = st(A_VOL, j);
       const double dC = st(A_DC, j);
+      const double ivol = 1.0 / vol, idC = 1.0 / dC;
 #pragma unroll
       for (int s = 0; s < NSP; s++) {
-        rsf[s] = (ap * FRp[s] - am * xb(s, j - 1)) / vol;
-        msf[s] = (ap * FMp[s] - am * xb(6 + s, j - 1)) / vol;
-        msf[s] = msf[s] + (NPp[s] - xb(12 + s, j - 1)) / dC;
+        rsf[s] = dq(ap * FRp[s] - am * xb(s, j - 1), vol, ivol);
+        msf[s] = dq(ap * FMp[s] - am * xb(6 + s, j - 1), vol, ivol);
+        msf[s] = msf[s] + dq(NPp[s] - xb(12 + s, j - 1), dC, idC);
       }
-      ef = (ap * EFp - am * xb(18, j - 1)) / vol;
-      mfE = (ap * MOEp - am * xb(19, j - 1)) / vol;
-      mfN = (ap * MONp - am * xb(20, j - 1)) / vol;
+      ef = dq(ap * EFp - am * xb(18, j - 1), vol, ivol);
+      mfE = dq(ap * MOEp - am * xb(19, j - 1), vol, ivol);
+      mfN = dq(ap * MONp - am * xb(20, j - 1), vol, ivol);
     }
     cx.sync();
 
@@ -1250,7 +1262,7 @@ struct Column {
 #pragma unroll
       for (int s = 0; s < NSP; s++) {
         nrs[s] = rhos[s] - dtin * (rsf[s]);
-        nns[s] = nrs[s] / m[s];
+        nns[s] = dq(nrs[s], m[s], kp.imass[s]);
         nrho = nrho + nrs[s];
         nnt = nnt + nns[s];
       }
@@ -1267,13 +1279,14 @@ struct Column {
         double ww = nm / nrs[s];
         ww = ww + dtin * (U[F_VN] * U[F_VN] + U[F_VE] * U[F_VE]) * invr;
         if (kp.coriolis) ww = ww + dtin * (coriolis * U[F_VE]);
-        ww = ww - dtin * (kp.tdc[s] * KB * gradT) / m[s];
+        ww = ww - dq(dtin * (kp.tdc[s] * KB * gradT), m[s], kp.imass[s]);
         w[s] = ww;
       }
       const double hf = st(A_HF, j);
-      double nvE = (rho * U[F_VE] - dtin * (mfE)) / nrho;
+      const double inrho = 1.0 / nrho;
+      double nvE = dq(rho * U[F_VE] - dtin * (mfE), nrho, inrho);
       nvE = nvE + hf * dtin * df3[1];
-      double nvN = (rho * U[F_VN] - dtin * (mfN)) / nrho;
+      double nvN = dq(rho * U[F_VN] - dtin * (mfN), nrho, inrho);
       nvN = nvN + hf * dtin * df3[2];
       const double nte = te - dtin * ef + dtin * rho * U[F_VU] * S.eg;
       {
@@ -1295,7 +1308,7 @@ struct Column {
       for (int s = 0; s < NSP; s++) {
         double ww = dmax(-kp.maxvv, w[s]);
         ww = dmin(kp.maxvv, ww);
-        nvu = nvu + ww * nrs[s] / nrho;
+        nvu = nvu + dq(ww * nrs[s], nrho, inrho);
         ausm_commit(stg, F_W + s, ww);
       }
       const double np = (nte - 0.5 * nrho * (nvu * nvu + nvN * nvN + nvE * nvE)) * S.gm1;
