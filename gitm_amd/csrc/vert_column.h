@@ -400,16 +400,18 @@ struct Column {
   double *lds;
   const int j, k, nA, L, P;
   const bool lv;  // owns a level
-  int col;
+  int col;        // column number in error reports
+  const double dt;  // Dt of this column's run
   LevelStatic S;
   double U[NF];
   double *scr;  // this column's RK4 carries (Y, acc) in device memory
   double centrifugal, coriolis;
   bool plasma = false;  // nighttime plasmasphere column (bc :492-497)
 
-  GV_MEM Column(Ctx &c, const KParams &p, double *l, int column)
+  GV_MEM Column(Ctx &c, const KParams &p, double *l, int column, double step_dt)
       : cx(c), kp(p), lds(l), j(c.lane()), k(c.lane() - 1), nA(p.n_alts),
-        L(p.n_alts + 4), P(p.n_alts + 8), lv(c.lane() < p.n_alts + 4), col(column) {}
+        L(p.n_alts + 4), P(p.n_alts + 8), lv(c.lane() < p.n_alts + 4), col(column),
+        dt(step_dt) {}
 
   // record layout: tables | exchange buffers | carries Y, acc | ions
   static constexpr int RS = record_len(IONS, CARRY_LDS);
@@ -1323,7 +1325,6 @@ struct Column {
   // -------------------------------------------------------------------------
   GV_MEM void rusanov_stage(int stg) {
     const double *m = kp.mass;
-    const double dt = kp.dt;
     const double T = U[F_T];
     double NS[NSP], lcs[NSP];
     const double rho = exp(U[F_LR]);
@@ -1456,7 +1457,7 @@ struct Column {
       for (int f = 0; f < NF; f++) Yv(f) = U[f];
     }
     ion_store_y();
-    const double dtin[4] = {0.5 * kp.dt, 0.5 * kp.dt, kp.dt, 0.5 * kp.dt};  // :160-377
+    const double dtin[4] = {0.5 * dt, 0.5 * dt, dt, 0.5 * dt};  // :160-377
 #pragma unroll 1
     for (int stg = 0; stg < 4; stg++) {
       ausm_stage(dtin[stg], stg);
@@ -1491,11 +1492,14 @@ GV_FN double *fld(const gitm_vert_field &f, long long base, int k, int comp) {
   return f.ptr + base + (long long)(k - f.first_level) * f.s_lev + (long long)comp * f.s_comp;
 }
 
+// One column of `io` (col = its index there): gather, n_steps steps of Dt =
+// dt, scatter.  scratch: the RK4 carries of io's columns in device memory;
+// err_col: the column number that error reports carry.
 template <int NT, int SOLVER, bool IONS, class Ctx>
-GV_FN void run_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io,
-                      int col, double *lds) {
-  Column<NT, Ctx, IONS> c(cx, kp, lds, col);
-  c.scr = kp.scratch + (long long)col * kp.scr_fields * (kp.n_alts + 4);
+GV_FN void run_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io, int col,
+                      double *lds, double *scratch, double dt, int err_col) {
+  Column<NT, Ctx, IONS> c(cx, kp, lds, err_col, dt);
+  c.scr = scratch + (long long)col * kp.scr_fields * (kp.n_alts + 4);
   const int nlb = io.n_lons * io.n_lats;
   const int ci = col % io.n_lons, cj = (col / io.n_lons) % io.n_lats, cb = col / nlb;
   const int k = c.k, nA = c.nA;
@@ -1638,6 +1642,13 @@ GV_FN void run_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io,
       *fld(io.idensity_s, bi, k, NI) = e;
     }
   }
+}
+
+// One column of a single-grid launch: the run's Dt, the handle's scratch.
+template <int NT, int SOLVER, bool IONS, class Ctx>
+GV_FN void run_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io, int col,
+                      double *lds) {
+  run_column<NT, SOLVER, IONS>(cx, kp, io, col, lds, kp.scratch, kp.dt, col);
 }
 
 }  // namespace gv

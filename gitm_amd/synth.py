@@ -285,3 +285,52 @@ class StepParams:
     use_nighttime_ion_bcs: bool = True       # ModInputs.f90:260
     min_tec: float = 2.0               # ModInputs.f90:261
     vert_tau: float = 1.0e9            # ModEarth.f90:325
+
+
+# ---------------------------------------------------------------------------
+# ensembles of members with grids of their own
+# ---------------------------------------------------------------------------
+ENSEMBLE_NALTS = (50, 60, 70, 80, 90, 100, 110, 120)
+# 2592, 10368 and 25920 columns: 8x4, 16x8 and 20x16 blocks of 9x9
+ENSEMBLE_BLOCKS = ((8, 4), (16, 8), (20, 16))
+
+
+@dataclasses.dataclass
+class Member:
+    """One ensemble member: its grid (uniform 100-600 km levels), exospheric
+    temperature and seed (the seed also draws its +-5% perturbations)."""
+
+    spec: GridSpec
+    tinf: float
+    seed: int
+
+
+def ensemble(n_members: int = 64, seed: int = 4, nalts=ENSEMBLE_NALTS,
+             blocks=ENSEMBLE_BLOCKS, tinf_range=(700.0, 1400.0)) -> list:
+    """Members with nAlts drawn uniformly from ``nalts``, block layouts from
+    ``blocks`` and the exospheric temperature uniform in ``tinf_range``,
+    from counter-based draws of ``seed``."""
+    ua = _uniform(seed, 101, n_members)
+    ub = _uniform(seed, 102, n_members)
+    ut = _uniform(seed, 103, n_members)
+    out = []
+    for m in range(n_members):
+        na = nalts[min(int(ua[m] * len(nalts)), len(nalts) - 1)]
+        bl = blocks[min(int(ub[m] * len(blocks)), len(blocks) - 1)]
+        tinf = tinf_range[0] + (tinf_range[1] - tinf_range[0]) * ut[m]
+        out.append(Member(GridSpec(bl[0], bl[1], na), float(tinf), seed * 1000 + m))
+    return out
+
+
+def member_columns(member: Member, ncols: int | None = None, ions: bool = False) -> dict:
+    """The member's columns (all of them, or its first ``ncols``)."""
+    spec = member.spec
+    if ncols is not None and ncols < spec.ncols:
+        # the first whole blocks that hold ncols columns, then cut
+        per = spec.nlons * spec.nlats
+        nb = -(-ncols // per)
+        spec = dataclasses.replace(spec, nblocks_lon=nb, nblocks_lat=1)
+    cols = make_columns(spec, seed=member.seed, tinf=member.tinf, ions=ions)
+    if ncols is not None and ncols < spec.ncols:
+        cols = {k: v[:ncols] for k, v in cols.items()}
+    return cols

@@ -70,6 +70,19 @@ def lib() -> ctypes.CDLL:
                                            ctypes.POINTER(_abi.Columns),
                                            ctypes.POINTER(_abi.Field),
                                            ctypes.POINTER(_abi.Field), ctypes.c_void_p]
+        vp, ci = ctypes.c_void_p, ctypes.c_int
+        _lib.gitm_vert_batch_create.argtypes = [vp, ctypes.POINTER(_abi.Columns), ci, ci,
+                                                ctypes.POINTER(vp)]
+        _lib.gitm_vert_batch_destroy.argtypes = [vp]
+        _lib.gitm_vert_batch_set_tile.argtypes = [vp, ci, ci]
+        _lib.gitm_vert_batch_advance.argtypes = [vp, ctypes.POINTER(_abi.Params), ci, ci, vp]
+        _lib.gitm_vert_batch_prepare.argtypes = [vp, ctypes.POINTER(_abi.PrepParams), vp]
+        _lib.gitm_vert_batch_locate.argtypes = [vp, ci, ctypes.POINTER(ci),
+                                                ctypes.POINTER(ci)]
+        for name in ("gitm_vert_batch_create", "gitm_vert_batch_destroy",
+                     "gitm_vert_batch_set_tile", "gitm_vert_batch_advance",
+                     "gitm_vert_batch_prepare", "gitm_vert_batch_locate"):
+            getattr(_lib, name).restype = ctypes.c_int
         _lib.gitm_vert_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]
         _lib.gitm_vert_destroy.argtypes = [ctypes.c_void_p]
         _lib.gitm_vert_set_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
@@ -127,6 +140,23 @@ def solver_id(solver) -> int:
     return int(solver)
 
 
+def _device_columns(cols: dict, extra: tuple = ()):
+    """C-ABI descriptor of device-resident columns (torch FP64 tensors in the
+    column-contiguous layout) and the pointers of the ``extra`` fields."""
+    import torch
+
+    ptrs = {}
+    for k, t in cols.items():
+        if k not in _abi.CANON and k not in _abi.PER_COLUMN and k not in extra:
+            continue
+        if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+            raise ValueError(f"{k}: need a contiguous float64 device tensor")
+        ptrs[k] = t.data_ptr()
+    nc, L = cols["temp"].shape
+    more = {k: ptrs.pop(k) for k in extra if k in ptrs}
+    return _abi.columns_struct(ptrs, nc, L - 4), more
+
+
 class VerticalSolver:
     """A handle on one HIP device (``gitm_vert_create``)."""
 
@@ -174,15 +204,7 @@ class VerticalSolver:
         """
         import torch
 
-        ptrs = {}
-        for k, t in cols.items():
-            if k not in _abi.CANON and k not in _abi.PER_COLUMN:
-                continue
-            if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
-                raise ValueError(f"{k}: need a contiguous float64 device tensor")
-            ptrs[k] = t.data_ptr()
-        nc, L = cols["temp"].shape
-        c = _abi.columns_struct(ptrs, nc, L - 4)
+        c, _ = _device_columns(cols)
         p = make_params(params)
         self.set_stream(torch.cuda.current_stream().cuda_stream)
         rc = lib().gitm_vert_advance(self._h, ctypes.byref(p), ctypes.byref(c),
@@ -201,16 +223,8 @@ class VerticalSolver:
         the time step (``dt_out`` if given); queued on the current stream."""
         import torch
 
-        ptrs = {}
-        for k, t in cols.items():
-            if k not in _abi.CANON and k not in _abi.PER_COLUMN and k not in ("press", "cp"):
-                continue
-            if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
-                raise ValueError(f"{k}: need a contiguous float64 device tensor")
-            ptrs[k] = t.data_ptr()
-        nc, L = cols["temp"].shape
-        extra = {k: ptrs.pop(k) for k in ("press", "cp") if k in ptrs}
-        c = _abi.columns_struct(ptrs, nc, L - 4)
+        c, extra = _device_columns(cols, ("press", "cp"))
+        L = cols["temp"].shape[1]
         pf = _abi.column_field(extra["press"], 1, L) if "press" in extra else None
         cpf = _abi.column_field(extra["cp"], 1, L) if "cp" in extra else None
         dt = dt_out if dt_out is not None else torch.empty(1, dtype=torch.float64,
@@ -244,6 +258,143 @@ class VerticalSolver:
         if rc:
             e = _abi.Error()
             raise VerticalStepError(rc)
+
+
+class VerticalBatch:
+    """Ensemble members with grids of their own, stepped together
+    (``gitm_vert_batch_*``).
+
+    ``members`` is a list of device column dicts (as for
+    :meth:`VerticalSolver.advance`); their tensors must stay alive and in
+    place for the life of the batch (the batch keeps references).  Members
+    with the same nAlts share one launch; a step is one launch per distinct
+    nAlts, and ``advance``/``prepare`` allocate nothing, so a loop of them can
+    be captured in a CUDA (HIP) graph.
+    """
+
+    def __init__(self, solver: VerticalSolver, members: list, ions: bool = False):
+        self.solver = solver
+        self.members = list(members)
+        n = len(self.members)
+        descs = (_abi.Columns * n)()
+        for i, cols in enumerate(self.members):
+            descs[i], _ = _device_columns(cols)
+        self.n_alts = sorted({int(m["temp"].shape[1]) - 4 for m in self.members})
+        self._b = ctypes.c_void_p()
+        rc = lib().gitm_vert_batch_create(solver._h, descs, n, int(ions),
+                                          ctypes.byref(self._b))
+        if rc:
+            raise VerticalStepError(rc)
+
+    def close(self) -> None:
+        if self._b:
+            lib().gitm_vert_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tile(self, n_alts: int, waves_per_column: int) -> None:
+        """Waves per column for the members with ``n_alts`` (0: all)."""
+        rc = lib().gitm_vert_batch_set_tile(self._b, n_alts, waves_per_column)
+        if rc:
+            raise VerticalStepError(rc)
+
+    def locate(self, column: int) -> tuple:
+        m, c = ctypes.c_int(), ctypes.c_int()
+        rc = lib().gitm_vert_batch_locate(self._b, column, ctypes.byref(m), ctypes.byref(c))
+        if rc:
+            raise VerticalStepError(rc)
+        return m.value, c.value
+
+    def check(self) -> None:
+        """Synchronise and raise the first stepping error, naming the member
+        (``err.member``) and its column."""
+        e = _abi.Error()
+        rc = lib().gitm_vert_get_error(self.solver._h, ctypes.byref(e))
+        if rc:
+            member, col = self.locate(e.column) if e.column >= 0 else (-1, -1)
+            err = VerticalStepError(rc, col, e.level)
+            err.member = member
+            raise err
+
+    def advance(self, params: StepParams | None = None, solver="ausm", n_steps: int = 1,
+                dt=None, check: bool = True) -> None:
+        """Advance every member in place.  ``dt``: None (params.dt for all)
+        or a device tensor of one step per member (e.g. from ``prepare``)."""
+        import torch
+
+        p = make_params(params)
+        if dt is not None:
+            if dt.dtype != torch.float64 or not dt.is_cuda or dt.numel() < len(self.members):
+                raise ValueError("dt: need a float64 device tensor with one step per member")
+        self.solver.set_stream(torch.cuda.current_stream().cuda_stream)
+        rc = lib().gitm_vert_batch_advance(self._b, ctypes.byref(p), solver_id(solver),
+                                           n_steps,
+                                           ctypes.c_void_p(dt.data_ptr()) if dt is not None
+                                           else None)
+        if rc:
+            raise VerticalStepError(rc)
+        if check:
+            self.check()
+
+    def prepare(self, pp=None, dt_out=None):
+        """Column prep of every member (``gitm_vert_batch_prepare``); returns
+        a device tensor with each member's step (``dt_out`` if given)."""
+        import torch
+
+        n = len(self.members)
+        dt = dt_out if dt_out is not None else torch.empty(
+            n, dtype=torch.float64, device=self.members[0]["temp"].device)
+        p = make_prep_params(pp)
+        self.solver.set_stream(torch.cuda.current_stream().cuda_stream)
+        rc = lib().gitm_vert_batch_prepare(self._b, ctypes.byref(p),
+                                           ctypes.c_void_p(dt.data_ptr()))
+        if rc:
+            raise VerticalStepError(rc)
+        return dt
+
+    def autotune(self, params: StepParams | None = None, solver="ausm",
+                 probe_columns: int = 4096, repeats: int = 3) -> dict:
+        """Pick the waves per column of every nAlts group by timing.
+
+        For each distinct nAlts, a probe batch over a copy of (at most
+        ``probe_columns``) columns of the group's largest member is stepped
+        with every tile that holds the column (the smallest wave count up to
+        four); the fastest is set on this batch.  The members' own state is
+        not touched.  Returns {nAlts: {waves: ms per step}}.
+        """
+        import torch
+
+        out = {}
+        for na in self.n_alts:
+            group = [m for m in self.members if int(m["temp"].shape[1]) - 4 == na]
+            big = max(group, key=lambda m: int(m["temp"].shape[0]))
+            n = min(int(big["temp"].shape[0]), probe_columns)
+            times = {}
+            for w in range(auto_waves(na), 5):
+                probe = {k: v[:n].clone() for k, v in big.items()}
+                b = VerticalBatch(self.solver, [probe], ions=bool(params and
+                                                                  params.use_ion_advection))
+                b.set_tile(na, w)
+                b.advance(params, solver, 1)
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(repeats):
+                    b.advance(params, solver, 1, check=False)
+                e1.record()
+                torch.cuda.synchronize()
+                b.check()
+                times[w] = e0.elapsed_time(e1) / repeats
+                b.close()
+            best = min(times, key=times.get)
+            self.set_tile(na, best)
+            out[na] = times
+        return out
 
 
 def to_device(cols: dict, device="cuda") -> dict:

@@ -206,6 +206,53 @@ int gitm_vert_prepare(gitm_vert_handle *h, const gitm_vert_prep_params *p,
                       const gitm_vert_columns *c, const gitm_vert_field *pressure,
                       const gitm_vert_field *cp, double *dt_device);
 
+/* ------------------------------------------------------------------------
+ * Batches: many ensemble members, each a whole GITM grid of its own
+ * (n_lons, n_lats, n_blocks, n_alts and arrays), advanced together.  The
+ * reference runs one model per MPI job and has no counterpart; each member
+ * is stepped exactly as gitm_vert_advance would step it alone.
+ *
+ * Members with the same nAlts share one launch (one workgroup per column,
+ * one lane per level), so a batch step is one launch per distinct nAlts.
+ * gitm_vert_batch_create builds the device member table and reserves the
+ * scratch memory; afterwards gitm_vert_batch_advance and
+ * gitm_vert_batch_prepare only queue kernels on the handle's stream (no
+ * allocation, no copy, no synchronisation), so a multi-step loop of them can
+ * be captured in a hipGraph.  The member arrays must stay valid (and at the
+ * same addresses) for the life of the batch.
+ * ------------------------------------------------------------------------ */
+typedef struct gitm_vert_batch gitm_vert_batch;
+
+/* Batch of `n_members` members (device pointers, as for gitm_vert_advance).
+ * use_ion_advection reserves the ion carries, so that steps with ion
+ * advection can run. */
+int gitm_vert_batch_create(gitm_vert_handle *h, const gitm_vert_columns *members,
+                           int n_members, int use_ion_advection, gitm_vert_batch **out);
+int gitm_vert_batch_destroy(gitm_vert_batch *b);
+
+/* Waves per column (1..4; 0 = automatic) for the members with nAlts =
+ * n_alts, or for every member when n_alts = 0.  May grow the scratch memory
+ * (synchronises the handle's stream). */
+int gitm_vert_batch_set_tile(gitm_vert_batch *b, int n_alts, int waves_per_column);
+
+/* Advance every member by n_steps steps.  dt_device: NULL to use p->dt for
+ * all members, or n_members device doubles, member m stepping with
+ * dt_device[m] (e.g. from gitm_vert_batch_prepare).  Errors as for
+ * gitm_vert_advance, reported through gitm_vert_get_error of the handle with
+ * the column numbered across the batch (members in the caller's order); see
+ * gitm_vert_batch_locate. */
+int gitm_vert_batch_advance(gitm_vert_batch *b, const gitm_vert_params *p, int solver,
+                            int n_steps, const double *dt_device);
+
+/* gitm_vert_prepare for every member (no pressure or cp outputs);
+ * dt_device[m] receives member m's step. */
+int gitm_vert_batch_prepare(gitm_vert_batch *b, const gitm_vert_prep_params *p,
+                            double *dt_device);
+
+/* Member and member column of a batch-wide column number. */
+int gitm_vert_batch_locate(const gitm_vert_batch *b, int column, int *member,
+                           int *member_column);
+
 #ifdef __cplusplus
 }
 #endif
