@@ -141,6 +141,68 @@ def copy_probe(torch, mib: int = 1024) -> None:
     print(f"copy probe: {mib} MiB read, {mib} MiB written", file=sys.stderr, flush=True)
 
 
+def bench_ensemble(torch, solver, params, steps: int = 20, seed: int = 4) -> dict:
+    """The ragged ensemble: 64 members with nAlts drawn from 50..120 and 2592,
+    10368 or 25920 columns each (gitm_amd.synth.ensemble), advanced as one
+    batch.  Each member's columns tile the first min(n, 2592) columns of its
+    own seeded atmosphere.  Tiles are autotuned per nAlts; then a hipGraph
+    holding ``steps`` x (batch prep with per-member CFL steps + batch step)
+    is replayed once and timed.  Also timed: the same steps at a fixed dt
+    as one launch per distinct nAlts."""
+    from gitm_amd import synth, vertical
+
+    mem = synth.ensemble(64, seed=seed)
+    dev = []
+    for m in mem:
+        n = m.spec.ncols
+        t = vertical.to_device(synth.member_columns(m, min(n, 2592)))
+        reps = -(-n // t["temp"].shape[0])
+        dev.append({k: v.repeat((reps,) + (1,) * (v.dim() - 1))[:n].contiguous()
+                    for k, v in t.items()})
+        del t
+    ncols = sum(int(d["temp"].shape[0]) for d in dev)
+    levels = sum(int(d["temp"].shape[0]) * (int(d["temp"].shape[1]) - 4) for d in dev)
+    b = vertical.VerticalBatch(solver, dev)
+    tune = b.autotune(params, "ausm", probe_columns=4096)
+    pp = synth.PrepParams()
+    dt = torch.empty(len(dev), dtype=torch.float64, device="cuda")
+    b.prepare(pp, dt_out=dt)
+    b.advance(params, "ausm", 1, dt=dt)  # eager: loads every kernel before capture
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for _ in range(steps):
+            b.prepare(pp, dt_out=dt)
+            b.advance(params, "ausm", 1, dt=dt, check=False)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    b.check()
+    graph_ms = e0.elapsed_time(e1) / steps
+    e0.record()
+    b.advance(params, "ausm", steps, check=False)
+    e1.record()
+    torch.cuda.synchronize()
+    b.check()
+    fixed_ms = e0.elapsed_time(e1) / steps
+    ok = all(bool(torch.isfinite(d["temp"]).all()) and bool((d["nds"] > 0).all())
+             for d in dev)
+    del g
+    b.close()
+    return {"members": len(dev), "columns": ncols, "mean_n_alts": levels / ncols,
+            "distinct_n_alts": sorted({int(d["temp"].shape[1]) - 4 for d in dev}),
+            "tiles": {str(k): min(v, key=v.get) for k, v in tune.items()},
+            "graph_steps": steps,
+            "ms_per_step_graph_prep_and_step": graph_ms,
+            "column_updates_per_s_graph": ncols / (graph_ms * 1e-3),
+            "ms_per_step_fixed_dt": fixed_ms,
+            "column_updates_per_s_fixed_dt": ncols / (fixed_ms * 1e-3),
+            "state_finite_and_positive": ok}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +212,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="wall-time budget of the CPU baseline sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the 64-member ragged-ensemble figure")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -214,6 +278,14 @@ def main() -> int:
     achieved = per_launch / (launch_ms * 1e-3) / 1e9
     traffic = read_traffic(TRAFFIC_FILE) if args.solver == "ausm" else None
 
+    ens = None
+    if not args.no_ensemble:
+        del dev
+        torch.cuda.empty_cache()
+        print("bench: 64-member ensemble", file=sys.stderr, flush=True)
+        ens = bench_ensemble(torch, solver, params)
+        torch.cuda.empty_cache()
+
     cpu = None
     if args.cpu_seconds > 0:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -257,6 +329,7 @@ def main() -> int:
                  "frac": prep_bytes_per_column(spec.nalts) * ncols
                  / (prep["launch_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
                  "cfl_dt": prep["dt"]},
+        "ensemble": ens,
         "adjoint_forward_ratio": None,
     }
     print(json.dumps(line), flush=True)
