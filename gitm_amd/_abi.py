@@ -24,6 +24,7 @@ ERRORS = {
     5: "unsupported option (UseMSIS, tides, BC perturbation, UseDamping)",
     6: "HIP runtime error",
     7: "out of device memory",
+    8: "adjoint tape too small for the column",
 }
 
 

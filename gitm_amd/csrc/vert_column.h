@@ -171,11 +171,20 @@ GV_HD constexpr unsigned long lds_bytes(int n_alts, bool ions, bool carry_lds) {
 // ---------------------------------------------------------------------------
 // small numerics
 // ---------------------------------------------------------------------------
-GV_FN double dmax(double a, double b) { return a > b ? a : b; }
-GV_FN double dmin(double a, double b) { return a < b ? a : b; }
+// min/max/limiter/LU below are templates over the column code's number type
+// (double, or the taped type of the adjoint build, gitm_amd/csrc/vert_ad.h)
+template <class A, class B>
+GV_FN auto dmax(A a, B b) -> decltype(a + b) {
+  return a > b ? a : b;
+}
+template <class A, class B>
+GV_FN auto dmin(A a, B b) -> decltype(a + b) {
+  return a < b ? a : b;
+}
 
 // Limiter_mc, ModLimiter.f90:16-36
-GV_FN double limiter_mc(double beta, double up, double dn) {
+template <class R>
+GV_FN R limiter_mc(double beta, R up, R dn) {
   if (up > 0.0) {
     if (dn > 0.0) return dmin(dmin(beta * up, beta * dn), (up + dn) * 0.5);
     return 0.0;
@@ -190,15 +199,16 @@ GV_FN double limiter_mc(double beta, double up, double dn) {
 // leading-zero skip of src/lubksb.f.  Fully unrolled: every array index is a
 // compile-time constant and row interchanges are selects, so the system stays
 // in registers.  Returns false for a zero row (singular matrix).
-GV_FN bool lu_solve6(double (&a)[6][6], double (&b)[6]) {
-  double sc[6];
+template <class R>
+GV_FN bool lu_solve6(R (&a)[6][6], R (&b)[6]) {
+  R sc[6];
   bool ok = true;
 #pragma unroll
   for (int i = 0; i < 6; i++) {
-    double big = 0.0;
+    R big = 0.0;
 #pragma unroll
     for (int j = 0; j < 6; j++) {
-      double v = fabs(a[i][j]);
+      R v = fabs(a[i][j]);
       if (v > big) big = v;
     }
     if (big == 0.0) ok = false;
@@ -209,20 +219,20 @@ GV_FN bool lu_solve6(double (&a)[6][6], double (&b)[6]) {
   for (int j = 0; j < 6; j++) {
 #pragma unroll
     for (int i = 0; i < j; i++) {
-      double s = a[i][j];
+      R s = a[i][j];
 #pragma unroll
       for (int q = 0; q < i; q++) s = s - a[i][q] * a[q][j];
       a[i][j] = s;
     }
-    double best = 0.0;
+    R best = 0.0;
     int piv = j;
 #pragma unroll
     for (int i = j; i < 6; i++) {
-      double s = a[i][j];
+      R s = a[i][j];
 #pragma unroll
       for (int q = 0; q < j; q++) s = s - a[i][q] * a[q][j];
       a[i][j] = s;
-      double t = sc[i] * fabs(s);
+      R t = sc[i] * fabs(s);
       if (t >= best) {
         piv = i;
         best = t;
@@ -233,7 +243,7 @@ GV_FN bool lu_solve6(double (&a)[6][6], double (&b)[6]) {
       if (piv == r) {
 #pragma unroll
         for (int q = 0; q < 6; q++) {
-          double t = a[r][q];
+          R t = a[r][q];
           a[r][q] = a[j][q];
           a[j][q] = t;
         }
@@ -243,7 +253,7 @@ GV_FN bool lu_solve6(double (&a)[6][6], double (&b)[6]) {
     perm[j] = piv;
     if (a[j][j] == 0.) a[j][j] = 1.0e-20;
     if (j != 5) {
-      double r = 1. / a[j][j];
+      R r = 1. / a[j][j];
 #pragma unroll
       for (int i = j + 1; i < 6; i++) a[i][j] = a[i][j] * r;
     }
@@ -251,7 +261,7 @@ GV_FN bool lu_solve6(double (&a)[6][6], double (&b)[6]) {
   int first = -1;
 #pragma unroll
   for (int i = 0; i < 6; i++) {
-    double s = b[i];
+    R s = b[i];
 #pragma unroll
     for (int r = i + 1; r < 6; r++)
       if (perm[i] == r) {
@@ -269,7 +279,7 @@ GV_FN bool lu_solve6(double (&a)[6][6], double (&b)[6]) {
   }
 #pragma unroll
   for (int i = 5; i >= 0; i--) {
-    double s = b[i];
+    R s = b[i];
 #pragma unroll
     for (int q = i + 1; q < 6; q++) s = s - a[i][q] * b[q];
     b[i] = s / a[i][i];
@@ -283,23 +293,22 @@ GV_FN bool lu_solve6(double (&a)[6][6], double (&b)[6]) {
 // cancels: C_iq = kT/m_i n_q/N / (1e-4 Diff0 T^e / (N 1e-6))
 //              = kT/m_i n_q (1e-2/Diff0) T^-e,
 // which needs no division (52 per level in the reference form).
-template <bool DEVICE>
-GV_FN bool neutral_friction(const KParams &P, double dtin, double (&vel)[NSP],
-                            double nt, const double (&ns)[NSP],
-                            const double (&glc)[NSP], double eddy, double temp,
-                            const double (&tp)[6]) {
-  double m[NSP][NSP], ec[NSP];
+template <bool DEVICE, class R>
+GV_FN bool neutral_friction(const KParams &P, R dtin, R (&vel)[NSP], R nt,
+                            const R (&ns)[NSP], const R (&glc)[NSP], R eddy, R temp,
+                            const R (&tp)[6]) {
+  R m[NSP][NSP], ec[NSP];
   if (DEVICE) {
-    const double kT = KB * temp;
+    const R kT = KB * temp;
 #pragma unroll
     for (int i = 0; i < NSP; i++) {
-      const double kTOverM = kT * P.imass[i];
-      double sum = 0.0;
+      const R kTOverM = kT * P.imass[i];
+      R sum = 0.0;
 #pragma unroll
       for (int q = 0; q < NSP; q++) {
         if (q == i) continue;
         const double g = 1.0e-19 / D0R[i][q];  // 1e-2 / Diff0, a constant
-        double c = kTOverM * ns[q] * (g * tp[DEXI[i][q]]);
+        R c = kTOverM * ns[q] * (g * tp[DEXI[i][q]]);
         sum = sum + c;
         m[i][q] = -dtin * c;
       }
@@ -307,26 +316,26 @@ GV_FN bool neutral_friction(const KParams &P, double dtin, double (&vel)[NSP],
       ec[i] = -1.0 * eddy * glc[i];
     }
   } else {
-    double tdij[NSP][NSP];
+    R tdij[NSP][NSP];
 #pragma unroll
     for (int i = 0; i < NSP; i++)
 #pragma unroll
       for (int q = i + 1; q < NSP; q++) {
         // Diff0 and DiffExp are symmetric, so TempDij(i,j) == TempDij(j,i)
         const double d0 = 1.0e17 * D0R[i][q];
-        double v = (1.0e-04) * (d0 * tp[DEXI[i][q]]) / (nt * (1.0e-06));
+        R v = (1.0e-04) * (d0 * tp[DEXI[i][q]]) / (nt * (1.0e-06));
         tdij[i][q] = v;
         tdij[q][i] = v;
       }
-    const double denscale = 1.0 / nt;
+    const R denscale = 1.0 / nt;
 #pragma unroll
     for (int i = 0; i < NSP; i++) {
-      const double kTOverM = KB * temp / P.mass[i];
-      double sum = 0.0;
+      const R kTOverM = KB * temp / P.mass[i];
+      R sum = 0.0;
 #pragma unroll
       for (int q = 0; q < NSP; q++) {
         if (q == i) continue;
-        double c = kTOverM * denscale * ns[q] / tdij[i][q];
+        R c = kTOverM * denscale * ns[q] / tdij[i][q];
         sum = sum + c;
         m[i][q] = -dtin * c;
       }
@@ -342,8 +351,9 @@ GV_FN bool neutral_friction(const KParams &P, double dtin, double (&vel)[NSP],
 
 // one-sided 5-point extrapolation coefficients, set_vertical_bcs.Earth.f90
 // :228-248 (h1..h4 are the spacings above the point being filled)
-GV_FN void bc_fwd_coefs(double h1, double h2, double h3, double h4, double *C) {
-  double H1 = h1, H2 = h1 + h2, H3 = h1 + h2 + h3, H4 = h1 + h2 + h3 + h4;
+template <class R>
+GV_FN void bc_fwd_coefs(R h1, R h2, R h3, R h4, R *C) {
+  R H1 = h1, H2 = h1 + h2, H3 = h1 + h2 + h3, H4 = h1 + h2 + h3 + h4;
   C[0] = -1.0 * (H2 * H3 * H4 + H1 * H3 * H4 + H1 * H2 * H4 + H1 * H2 * H3) /
          (H1 * H2 * H3 * H4);
   C[1] = 1.0 * (H2 * H3 * H4) / (h1 * h2 * (h2 + h3) * (h2 + h3 + h4));
@@ -354,8 +364,9 @@ GV_FN void bc_fwd_coefs(double h1, double h2, double h3, double h4, double *C) {
 
 // backward weights for the top ion slopes, set_vertical_bcs.Earth.f90
 // :505-525 (h1..h4 are the spacings below the point being filled)
-GV_FN void bc_bwd_coefs(double h1, double h2, double h3, double h4, double *C) {
-  double H1 = h1, H2 = h1 + h2, H3 = h1 + h2 + h3, H4 = h1 + h2 + h3 + h4;
+template <class R>
+GV_FN void bc_bwd_coefs(R h1, R h2, R h3, R h4, R *C) {
+  R H1 = h1, H2 = h1 + h2, H3 = h1 + h2 + h3, H4 = h1 + h2 + h3 + h4;
   C[0] = 1.0 * (H2 * H3 * H4 + H1 * H3 * H4 + H1 * H2 * H4 + H1 * H2 * H3) /
          (H1 * H2 * H3 * H4);
   C[1] = -1.0 * (H2 * H3 * H4) / (h1 * h2 * (h2 + h3) * (h2 + h3 + h4));
@@ -371,11 +382,12 @@ GV_FN void bc_bwd_coefs(double h1, double h2, double h3, double h4, double *C) {
 //   HF       0.35*exp(-(z - z0)/45 km)                    (ausm :950)
 //   AP/AM/VOL face areas r^2 and cell volume (r^3)/3       (ausm :2029-2040)
 //   GLC*     5-point d ln(rho_s/rho)/dz weights            (ausm :659-676)
+template <class R>
 struct LevelStatic {
-  double eg;    // EffectiveGravity = g + Centrifugal/InvR (ausm :582-589)
-  double mgh;   // 0.5*(eg(k) + eg(k-1))                    (ausm :697)
-  double gm1;   // Gamma - 1
-  double tu;    // TempUnit
+  R eg;   // EffectiveGravity = g + Centrifugal/InvR (ausm :582-589)
+  R mgh;  // 0.5*(eg(k) + eg(k-1))                    (ausm :697)
+  R gm1;  // Gamma - 1
+  R tu;   // TempUnit
 };
 
 // ---------------------------------------------------------------------------
@@ -394,21 +406,22 @@ struct LevelStatic {
 // ---------------------------------------------------------------------------
 template <int NT, class Ctx, bool IONS>
 struct Column {
+  using R = typename Ctx::real;  // double, or the adjoint build's taped type
   static constexpr bool CARRY_LDS = carry_in_lds(NT / 64);
   Ctx &cx;
   const KParams &kp;
-  double *lds;
+  R *lds;
   const int j, k, nA, L, P;
   const bool lv;  // owns a level
   int col;        // column number in error reports
-  const double dt;  // Dt of this column's run
-  LevelStatic S;
-  double U[NF];
-  double *scr;  // this column's RK4 carries (Y, acc) in device memory
-  double centrifugal, coriolis;
+  const R dt;  // Dt of this column's run
+  LevelStatic<R> S;
+  R U[NF];
+  R *scr;  // this column's RK4 carries (Y, acc) in device memory
+  R centrifugal, coriolis;
   bool plasma = false;  // nighttime plasmasphere column (bc :492-497)
 
-  GV_MEM Column(Ctx &c, const KParams &p, double *l, int column, double step_dt)
+  GV_MEM Column(Ctx &c, const KParams &p, R *l, int column, R step_dt)
       : cx(c), kp(p), lds(l), j(c.lane()), k(c.lane() - 1), nA(p.n_alts),
         L(p.n_alts + 4), P(p.n_alts + 8), lv(c.lane() < p.n_alts + 4), col(column),
         dt(step_dt) {}
@@ -418,10 +431,10 @@ struct Column {
   static constexpr int OFF_X = NSTAT;
   static constexpr int OFF_Y = NSTAT + NBUF;
   static constexpr int OFF_ION = OFF_Y + (CARRY_LDS ? 2 * NF : 0);
-  GV_MEM double *rec(int jj) const { return lds + (jj + 2) * RS; }
-  GV_MEM double &st(int a, int jj) const { return rec(jj)[a]; }
-  GV_MEM double &xb(int b, int jj) const { return rec(jj)[OFF_X + b]; }
-  GV_MEM double &sc(int i) const { return lds[P * RS + i]; }
+  GV_MEM R *rec(int jj) const { return lds + (jj + 2) * RS; }
+  GV_MEM R &st(int a, int jj) const { return rec(jj)[a]; }
+  GV_MEM R &xb(int b, int jj) const { return rec(jj)[OFF_X + b]; }
+  GV_MEM R &sc(int i) const { return lds[P * RS + i]; }
   GV_MEM int &ib(int jj) const {
     return reinterpret_cast<int *>(lds + P * RS + NSCAL)[jj + 2];
   }
@@ -429,26 +442,26 @@ struct Column {
   // combination acc.  They are read and written only at the end of a stage,
   // so they live in LDS, or for the tallest columns in device memory
   // ([2*NF][L] per column, coalesced), instead of registers.
-  GV_MEM double &Yv(int f) const { return CARRY_LDS ? rec(j)[OFF_Y + f] : scr[f * L + j]; }
-  GV_MEM double &Av(int f) const {
+  GV_MEM R &Yv(int f) const { return CARRY_LDS ? rec(j)[OFF_Y + f] : scr[f * L + j]; }
+  GV_MEM R &Av(int f) const {
     return CARRY_LDS ? rec(j)[OFF_Y + NF + f] : scr[(NF + f) * L + j];
   }
   // Ions: LogINS of level jj-1 (field NI holds IVel(up)) in LDS, so that the
   // boundary lanes can reach the top levels; their carries follow the
   // neutral ones in device memory.
   static constexpr int ION_SCR = CARRY_LDS ? 0 : 2 * NF;  // scratch rows before them
-  GV_MEM double &ion(int f, int jj) const { return rec(jj)[OFF_ION + f]; }
-  GV_MEM double &Yi(int f) const { return scr[(ION_SCR + f) * L + j]; }
-  GV_MEM double &Ai(int f) const { return scr[(ION_SCR + NI + f) * L + j]; }
+  GV_MEM R &ion(int f, int jj) const { return rec(jj)[OFF_ION + f]; }
+  GV_MEM R &Yi(int f) const { return scr[(ION_SCR + f) * L + j]; }
+  GV_MEM R &Ai(int f) const { return scr[(ION_SCR + NI + f) * L + j]; }
 
   // advance_vertical_1d_ausm (vertical_solver_ausm.f90:167-417) for one
   // field of an interior level: stage k+1 input = Y + (New - U), and the
   // final (1/3)*(-Y + S2 + 2*S3 + S4 + (New_4 - U_4)) accumulated in the
   // reference's left-to-right order.
-  GV_MEM void ausm_commit(int stg, int f, double nv) {
+  GV_MEM void ausm_commit(int stg, int f, R nv) {
     if (stg < 3) {
-      const double y = Yv(f);
-      const double nxt = y + (nv - U[f]);
+      const R y = Yv(f);
+      const R nxt = y + (nv - U[f]);
       if (stg == 0)
         Av(f) = -1.0 * y + nxt;
       else if (stg == 1)
@@ -463,9 +476,9 @@ struct Column {
 
   // advance_vertical_1d_rusanov (vertical_solver_rusanov.f90:120-369):
   // K = New - U; Y + K/2, Y + K/2, Y + K, then Y + (K1 + 2K2 + 2K3 + K4)/6.
-  GV_MEM void rusanov_commit(int stg, int f, double nv) {
-    const double K = nv - U[f];
-    const double y = Yv(f);
+  GV_MEM void rusanov_commit(int stg, int f, R nv) {
+    const R K = nv - U[f];
+    const R y = Yv(f);
     if (stg == 0) {
       Av(f) = K;
       U[f] = y + 0.5 * K;
@@ -481,11 +494,11 @@ struct Column {
   }
 
   // The same two combinations for an ion field held in LDS.
-  GV_MEM void ausm_commit_ion(int stg, int f, double nv) {
-    double &u = ion(f, j);
+  GV_MEM void ausm_commit_ion(int stg, int f, R nv) {
+    R &u = ion(f, j);
     if (stg < 3) {
-      const double y = Yi(f);
-      const double nxt = y + (nv - u);
+      const R y = Yi(f);
+      const R nxt = y + (nv - u);
       if (stg == 0)
         Ai(f) = -1.0 * y + nxt;
       else if (stg == 1)
@@ -497,10 +510,10 @@ struct Column {
       u = (1.0 / 3.0) * (Ai(f) + (nv - u));
     }
   }
-  GV_MEM void rusanov_commit_ion(int stg, int f, double nv) {
-    double &u = ion(f, j);
-    const double K = nv - u;
-    const double y = Yi(f);
+  GV_MEM void rusanov_commit_ion(int stg, int f, R nv) {
+    R &u = ion(f, j);
+    const R K = nv - u;
+    const R y = Yi(f);
     if (stg == 0) {
       Ai(f) = K;
       u = y + 0.5 * K;
@@ -517,9 +530,9 @@ struct Column {
 
   // calc_neutral_friction of the owned level, in the device or the
   // reference form (see neutral_friction).
-  GV_MEM bool friction(double dtin, double (&w)[NSP], double nt, const double (&ns)[NSP],
-                       const double (&glc)[NSP], double T) {
-    double tp[6];
+  GV_MEM bool friction(R dtin, R (&w)[NSP], R nt, const R (&ns)[NSP],
+                       const R (&glc)[NSP], R T) {
+    R tp[6];
     if (cx.device_math()) {
       cx.diff_powers(T, -1.0, tp);
       return neutral_friction<true>(kp, dtin, w, nt, ns, glc, st(A_EDDY, j), T, tp);
@@ -531,7 +544,7 @@ struct Column {
   // a / b in the reference form; a * (1/b) in the device form, with the
   // reciprocal ib = 1/b shared by several quotients (one division in place
   // of several)
-  GV_MEM double dq(double a, double b, double ib) const {
+  GV_MEM R dq(R a, R b, R ib) const {
     return cx.device_math() ? a * ib : a / b;
   }
 
@@ -543,8 +556,8 @@ struct Column {
   // Column prep: geometry and per-level constants (advance_vertical.f90
   // :84-151 plus the state-independent parts of the stage functions).
   // -------------------------------------------------------------------------
-  GV_MEM void setup(double alt, double grav, double invr, double dalt, double mmm,
-                    double gam, double eddy, double cmax, double tu, double coslat) {
+  GV_MEM void setup(R alt, R grav, R invr, R dalt, R mmm,
+                    R gam, R eddy, R cmax, R tu, R coslat) {
     centrifugal = (coslat * kp.omega) * (coslat * kp.omega);  // :103
     coriolis = 2 * coslat * kp.omega;                         // :104
     S.eg = grav + centrifugal / invr;
@@ -553,7 +566,7 @@ struct Column {
     S.tu = tu;
     st(A_GRAV, j) = grav;
     S.gm1 = gam - 1.0;
-    const double rd = 1.0 / invr;  // RadialDistance_C (ausm :595-597)
+    const R rd = 1.0 / invr;  // RadialDistance_C (ausm :595-597)
     xb(0, j) = alt;
     xb(1, j) = S.eg;
     xb(2, j) = gam;
@@ -566,7 +579,7 @@ struct Column {
     }
     cx.sync();
     // dAlt_F(k) = Alt(k) - Alt(k-1), dAlt_F(-1) = dAlt_F(0)  (:149-151)
-    const double dF = (j >= 1) ? alt - xb(0, j - 1) : xb(0, 1) - alt;
+    const R dF = (j >= 1) ? alt - xb(0, j - 1) : xb(0, 1) - alt;
     st(A_DF, j) = dF;
     st(A_IF, j) = 1.0 / dF;
     st(A_DC, j) = dalt;
@@ -578,7 +591,7 @@ struct Column {
     if (j == nA + 3) sc(SC_MGT2) = S.mgh;
     // Kp/Ku blends, vertical_solver_ausm.f90:1320-1352 (same for all species)
     if (k >= 1 && k <= nA) {
-      const double t = (1.0 + tanh((alt - 160.0e+03) / 15.0e+03));
+      const R t = (1.0 + tanh((alt - 160.0e+03) / 15.0e+03));
       st(A_KP, j) = exp(kp.log_min_kp + 0.5 * (kp.log_max_kp - kp.log_min_kp) * t);
       st(A_KU, j) = exp(kp.log_min_ku + 0.5 * (kp.log_max_ku - kp.log_min_ku) * t);
     } else {
@@ -587,8 +600,8 @@ struct Column {
     }
     // spherical areas and volume, vertical_solver_ausm.f90:2029-2040
     {
-      const double LR = 0.5 * (rd + xb(4, j - 1));
-      const double RR = 0.5 * (rd + xb(4, j + 1));
+      const R LR = 0.5 * (rd + xb(4, j - 1));
+      const R RR = 0.5 * (rd + xb(4, j + 1));
       st(A_AP, j) = RR * RR;
       st(A_AM, j) = LR * LR;
       st(A_VOL, j) = (1.0 / 3.0) * (RR * RR * RR - LR * LR * LR);
@@ -596,18 +609,18 @@ struct Column {
     st(A_HF, j) = 0.35 * exp(-1.0 * (alt - sc(SC_ALT0)) / (45.0e+03));
     cx.sync();
     // gamma face states (calc_kt_facevalues of Gamma_1d, ausm :1535)
-    double dlg = 0.0;
+    R dlg = 0.0;
     if (k >= 0 && k <= nA + 1)
       dlg = limiter_mc(kp.beta, (xb(2, j + 1) - gam) * st(A_IF, j + 1),
                        (gam - xb(2, j - 1)) * st(A_IF, j));
     xb(5, j) = dlg;
     // 5-point gradient weights (ausm :649-676)
     if (k >= 1 && k <= nA) {
-      const double h1 = st(A_DF, j - 1), h2 = dF, h3 = st(A_DF, j + 1),
+      const R h1 = st(A_DF, j - 1), h2 = dF, h3 = st(A_DF, j + 1),
                    h4 = st(A_DF, j + 2);
-      const double MeshH2 = h2 + h1;
-      const double MeshH3 = h3 + h2 + h1;
-      const double MeshH4 = h4 + h3 + h2 + h1;
+      const R MeshH2 = h2 + h1;
+      const R MeshH3 = h3 + h2 + h1;
+      const R MeshH4 = h4 + h3 + h2 + h1;
       st(A_GLC0 + 0, j) = (h2 * h3 * (h3 + h4)) / (h1 * MeshH2 * MeshH3 * MeshH4);
       st(A_GLC0 + 1, j) = -1.0 * (MeshH2 * h3 * (h3 + h4)) / (h1 * h2 * (h2 + h3) * (h2 + h3 + h4));
       st(A_GLC0 + 3, j) = MeshH2 * h2 * (h4 + h3) / (MeshH3 * (h2 + h3) * h3 * h4);
@@ -632,8 +645,8 @@ struct Column {
     }
     cx.sync();
     if (face()) {
-      const double gL = gam + 0.5 * dlg * dF;
-      const double gR = xb(2, j + 1) - 0.5 * xb(5, j + 1) * st(A_DF, j + 1);
+      const R gL = gam + 0.5 * dlg * dF;
+      const R gR = xb(2, j + 1) - 0.5 * xb(5, j + 1) * st(A_DF, j + 1);
       st(A_GL1, j) = 1.0 / (gL - 1.0);
       st(A_GL2, j) = gL / (gL - 1.0);
       st(A_GL3, j) = 2.0 * ((gL - 1.0) / (gL + 1.0));
@@ -664,13 +677,13 @@ struct Column {
     if (j < NSP) {
       // bottom, one species per thread (:198-272, :277-344, :382-395)
       const int sp = j;
-      double *CA = &sc(SC_CA), *CB = &sc(SC_CB);
-      const double dF1 = st(A_DF, 2), dF0 = st(A_DF, 1), dFm1 = st(A_DF, 0);
-      double ln0 = xb(sp, 1), ln1 = xb(sp, 2), ln2 = xb(sp, 3), ln3 = xb(sp, 4),
+      R *CA = &sc(SC_CA), *CB = &sc(SC_CB);
+      const R dF1 = st(A_DF, 2), dF0 = st(A_DF, 1), dFm1 = st(A_DF, 0);
+      R ln0 = xb(sp, 1), ln1 = xb(sp, 2), ln2 = xb(sp, 3), ln3 = xb(sp, 4),
              ln4 = xb(sp, 5), ln5 = xb(sp, 6);
-      double n0, nm1;
+      R n0, nm1;
       if (photochemical(sp)) {
-        double d = ln1 * CA[0] + ln2 * CA[1] + ln3 * CA[2] + ln4 * CA[3] + ln5 * CA[4];
+        R d = ln1 * CA[0] + ln2 * CA[1] + ln3 * CA[2] + ln4 * CA[3] + ln5 * CA[4];
         n0 = ln1 - dF1 * d;
         if (d <= 0.0) n0 = ln1;
         if (sp == S_NO && n0 < ln1 - 0.333) n0 = ln1 - 0.333;
@@ -683,59 +696,59 @@ struct Column {
         if (sp == S_O) {
           nm1 = ln0;  // zero gradient below the boundary (:219-222)
         } else {
-          const double t0 = xb(12, 1), tm1 = t0;  // Temp(-1) = Temp(0)
-          const double MeanGravity = -sc(SC_MG0);
-          const double MeanTemp = 0.5 * (t0 + tm1);
-          const double MeanMass = 0.5 * (sc(SC_MMM_0) + sc(SC_MMM_M1));
-          const double ish = MeanGravity * MeanMass / (MeanTemp * KB);
-          const double ns = exp(ln0) * (t0 / tm1) * exp(1.0 * ish * dFm1);
+          const R t0 = xb(12, 1), tm1 = t0;  // Temp(-1) = Temp(0)
+          const R MeanGravity = -sc(SC_MG0);
+          const R MeanTemp = 0.5 * (t0 + tm1);
+          const R MeanMass = 0.5 * (sc(SC_MMM_0) + sc(SC_MMM_M1));
+          const R ish = MeanGravity * MeanMass / (MeanTemp * KB);
+          const R ns = exp(ln0) * (t0 / tm1) * exp(1.0 * ish * dFm1);
           nm1 = log(ns);
         }
       }
-      const double w1 = xb(F_W - F_LN + sp, 2), w2 = xb(F_W - F_LN + sp, 3),
+      const R w1 = xb(F_W - F_LN + sp, 2), w2 = xb(F_W - F_LN + sp, 3),
                    w3 = xb(F_W - F_LN + sp, 4), w4 = xb(F_W - F_LN + sp, 5),
                    w5 = xb(F_W - F_LN + sp, 6);
-      const double w0 = w1 - dF1 * (w1 * CA[0] + w2 * CA[1] + w3 * CA[2] + w4 * CA[3] + w5 * CA[4]);
-      const double wm1 = w0 - dF0 * (w0 * CB[0] + w1 * CB[1] + w2 * CB[2] + w3 * CB[3] + w4 * CB[4]);
-      double w0r = w0, wm1r = wm1;
+      const R w0 = w1 - dF1 * (w1 * CA[0] + w2 * CA[1] + w3 * CA[2] + w4 * CA[3] + w5 * CA[4]);
+      const R wm1 = w0 - dF0 * (w0 * CB[0] + w1 * CB[1] + w2 * CB[2] + w3 * CB[3] + w4 * CB[4]);
+      R w0r = w0, wm1r = wm1;
       if (photochemical(sp) && w1 > 0.0) {
         w0r = -1.0 * w1;
         wm1r = -1.0 * w1;
       }
-      double *R = &sc(SC_BOT + 6 * sp);
-      R[0] = n0;
-      R[1] = nm1;
-      R[2] = w0;
-      R[3] = wm1;
-      R[4] = w0r;
-      R[5] = wm1r;
+      R *Rb = &sc(SC_BOT + 6 * sp);
+      Rb[0] = n0;
+      Rb[1] = nm1;
+      Rb[2] = w0;
+      Rb[3] = wm1;
+      Rb[4] = w0r;
+      Rb[5] = wm1r;
     } else if (j >= 8 && j < 8 + NSP) {
       // top, one species per thread (:429-441, :471-486)
       const int sp = j - 8;
       const int jn = nA + 1;  // offset of level nAlts
-      const double tn = xb(12, jn);
-      const double wn = xb(F_W - F_LN + sp, jn), wnm1 = xb(F_W - F_LN + sp, jn - 1);
-      double w1 = wn, w2 = wn;
+      const R tn = xb(12, jn);
+      const R wn = xb(F_W - F_LN + sp, jn), wnm1 = xb(F_W - F_LN + sp, jn - 1);
+      R w1 = wn, w2 = wn;
       if (photochemical(sp) && wn < 0.0) {
         w1 = -wn;
         w2 = -wnm1;
       }
-      const double m = kp.mass[sp];
-      double ns = exp(xb(sp, jn));
-      double MeanGravity = -sc(SC_MGT1);
-      double MeanTemp = 0.5 * (tn + tn);
-      double ish = MeanGravity * m / (MeanTemp * KB);
+      const R m = kp.mass[sp];
+      R ns = exp(xb(sp, jn));
+      R MeanGravity = -sc(SC_MGT1);
+      R MeanTemp = 0.5 * (tn + tn);
+      R ish = MeanGravity * m / (MeanTemp * KB);
       ns = ns * (tn / tn) * exp(-ish * st(A_DF, jn + 1));
-      const double l1 = log(ns);
+      const R l1 = log(ns);
       MeanGravity = -sc(SC_MGT2);
       ish = MeanGravity * m / (MeanTemp * KB);
       ns = ns * (tn / tn) * exp(-ish * st(A_DF, jn + 2));
-      const double l2 = log(ns);
-      double *R = &sc(SC_TOP + 4 * sp);
-      R[0] = l1;
-      R[1] = l2;
-      R[2] = w1;
-      R[3] = w2;
+      const R l2 = log(ns);
+      R *Rb = &sc(SC_TOP + 4 * sp);
+      Rb[0] = l1;
+      Rb[1] = l2;
+      Rb[2] = w1;
+      Rb[3] = w2;
     }
     if (IONS) {
       // bottom: zero-gradient ions (:277-281)
@@ -745,9 +758,9 @@ struct Column {
       }
       // TEC of the column in 1e16 m^-2 (:498-503), in the reference's order
       if (j == 6 && plasma) {
-        double tec = 0.0;
+        R tec = 0.0;
         for (int jj = 2; jj <= nA + 1; jj++) {
-          const double v = kp.improved_ions ? ion(0, jj) : exp(ion(0, jj));
+          const R v = kp.improved_ions ? ion(0, jj) : exp(ion(0, jj));
           tec = tec + st(A_DF, jj) * v;
         }
         sc(SC_TEC) = tec / 1e16;
@@ -757,18 +770,18 @@ struct Column {
     if (j <= 1) {
       // levels -1 and 0: ghost species, LogRho and bulk wind (:346-362)
       const int o = (j == 1) ? 0 : 1;
-      double sr = 0.0, ns[NSP];
+      R sr = 0.0, ns[NSP];
 #pragma unroll
       for (int sp = 0; sp < NSP; sp++) {
-        const double *R = &sc(SC_BOT + 6 * sp);
-        U[F_LN + sp] = R[o];
-        U[F_W + sp] = R[4 + o];
-        ns[sp] = exp(R[o]);
+        const R *Rb = &sc(SC_BOT + 6 * sp);
+        U[F_LN + sp] = Rb[o];
+        U[F_W + sp] = Rb[4 + o];
+        ns[sp] = exp(Rb[o]);
         sr = sr + kp.mass[sp] * ns[sp];
       }
       U[F_LR] = log(sr);
-      const double er = exp(U[F_LR]);
-      double vu = 0.0;
+      const R er = exp(U[F_LR]);
+      R vu = 0.0;
 #pragma unroll
       for (int sp = 0; sp < NSP; sp++)
         vu = vu + ns[sp] * kp.mass[sp] * sc(SC_BOT + 6 * sp + 2 + o) / er;
@@ -780,18 +793,18 @@ struct Column {
       // levels nAlts+1, nAlts+2 (:420-463, :598-604)
       const int o = (j == nA + 2) ? 0 : 1;
       const int jn = nA + 1;
-      double sr = 0.0;
+      R sr = 0.0;
 #pragma unroll
       for (int sp = 0; sp < NSP; sp++) {
-        const double *R = &sc(SC_TOP + 4 * sp);
-        U[F_LN + sp] = R[o];
-        U[F_W + sp] = R[2 + o];
-        sr = sr + kp.mass[sp] * exp(R[o]);
+        const R *Rb = &sc(SC_TOP + 4 * sp);
+        U[F_LN + sp] = Rb[o];
+        U[F_W + sp] = Rb[2 + o];
+        sr = sr + kp.mass[sp] * exp(Rb[o]);
       }
       U[F_LR] = log(sr);
       U[F_VE] = xb(13, jn);
       U[F_VN] = xb(14, jn);
-      const double vu = xb(15, jn);
+      const R vu = xb(15, jn);
       U[F_VU] = (vu < 0.0) ? -vu : vu;
       U[F_T] = xb(12, jn);
     }
@@ -808,14 +821,14 @@ struct Column {
   GV_MEM void ion_top_bc(int q) {
     const bool improved = kp.improved_ions != 0;
     const bool keep = plasma && sc(SC_TEC) < kp.min_tec;
-    const double dFn = st(A_DF, nA + 1);  // dAlt_F(nAlts)
+    const R dFn = st(A_DF, nA + 1);  // dAlt_F(nAlts)
 #pragma unroll 1
     for (int t = 0; t < 2; t++) {
       const int jk = nA + 2 + t;  // lane of level nAlts+1+t
-      const double *C = &sc(t ? SC_CTB : SC_CTA);
-      double n0, n1, n2, n3, n4, n5;
+      const R *C = &sc(t ? SC_CTB : SC_CTA);
+      R n0, n1, n2, n3, n4, n5;
       if (improved) {
-        double mn = ion(q, jk - 5);
+        R mn = ion(q, jk - 5);
         for (int jj = jk - 4; jj <= jk; jj++)
           if (ion(q, jj) < mn) mn = ion(q, jj);
         if (mn < 0.0) {
@@ -844,7 +857,7 @@ struct Column {
         n4 = ion(q, jk - 4);
         n5 = ion(q, jk - 5);
       }
-      double dn = C[0] * n1 + C[1] * n2 + C[2] * n3 + C[3] * n4 + C[4] * n5;
+      R dn = C[0] * n1 + C[1] * n2 + C[2] * n3 + C[3] * n4 + C[4] * n5;
       if (!keep) {
         if (dn > -0.001 * n0 / dFn) dn = -0.001 * n0 / dFn;
         n0 = n1 + dn * st(A_DF, jk);
@@ -859,25 +872,25 @@ struct Column {
   // Uses exchange buffers 0..15.  grad/diff are valid on cell threads.
   // -------------------------------------------------------------------------
   template <int N>
-  GV_MEM void rusanov_grad_diff(const double (&v)[N], double (&grad)[N],
-                                double (&diff)[N]) {
+  GV_MEM void rusanov_grad_diff(const R (&v)[N], R (&grad)[N],
+                                R (&diff)[N]) {
     static_assert(N <= 8, "at most 8 fields per pass");
 #pragma unroll
     for (int f = 0; f < N; f++) xb(f, j) = v[f];
     cx.sync();
-    const double iF0 = st(A_IF, j), iF1 = st(A_IF, j + 1);
-    const double dF0 = st(A_DF, j), dF1 = st(A_DF, j + 1);
-    double rf[N], ln[N];
+    const R iF0 = st(A_IF, j), iF1 = st(A_IF, j + 1);
+    const R dF0 = st(A_DF, j), dF1 = st(A_DF, j + 1);
+    R rf[N], ln[N];
 #pragma unroll
     for (int f = 0; f < N; f++) {
-      const double vm2 = xb(f, j - 2), vm1 = xb(f, j - 1), v0 = v[f],
+      const R vm2 = xb(f, j - 2), vm1 = xb(f, j - 1), v0 = v[f],
                    v1 = xb(f, j + 1), v2 = xb(f, j + 2);
-      double dl = 0.0;
+      R dl = 0.0;
       if (k >= 1 && k <= nA) {
-        double h = iF1 * 2.0;
-        const double up = h * (FACT1 * (v1 - v0) - FACT2 * (v2 - vm1));
+        R h = iF1 * 2.0;
+        const R up = h * (FACT1 * (v1 - v0) - FACT2 * (v2 - vm1));
         h = iF0 * 2.0;
-        const double dn = h * (FACT1 * (v0 - vm1) - FACT2 * (v1 - vm2));
+        const R dn = h * (FACT1 * (v0 - vm1) - FACT2 * (v1 - vm2));
         dl = limiter_mc(kp.beta, up, dn);
       } else if (k == 0 || k == nA + 1) {
         dl = limiter_mc(kp.beta, (v1 - v0) * iF1, (v0 - vm1) * iF0);
@@ -892,20 +905,20 @@ struct Column {
       xb(8 + f, j) = ln[f];
     }
     cx.sync();
-    const double dC = st(A_DC, j);
-    const double cm0 = st(A_CMF, j), cm1 = st(A_CMF, j + 1);
+    const R dC = st(A_DC, j);
+    const R cm0 = st(A_CMF, j), cm1 = st(A_CMF, j + 1);
 #pragma unroll
     for (int f = 0; f < N; f++) {
-      const double L0 = xb(8 + f, j - 1), R0 = rf[f], L1 = ln[f], R1 = xb(f, j + 1);
+      const R L0 = xb(8 + f, j - 1), R0 = rf[f], L1 = ln[f], R1 = xb(f, j + 1);
       grad[f] = 0.5 * (L1 + R1 - L0 - R0) / dC;
-      const double df0 = cm0 * (R0 - L0), df1 = cm1 * (R1 - L1);
+      const R df0 = cm0 * (R0 - L0), df1 = cm1 * (R1 - L1);
       diff[f] = (df1 - df0) / dC;
     }
     cx.sync();
   }
 
   // 5-point gradient of ln(rho_s/rho) (vertical_solver_ausm.f90:649-688)
-  GV_MEM void grad_log_cons(const double (&lcs)[NSP], double (&glc)[NSP]) {
+  GV_MEM void grad_log_cons(const R (&lcs)[NSP], R (&glc)[NSP]) {
 #pragma unroll
     for (int f = 0; f < NSP; f++) xb(f, j) = lcs[f];
     cx.sync();
@@ -921,13 +934,13 @@ struct Column {
   // calc_all_fluxes_hydro (:1093-2070).  Writes New (valid on cell threads).
   // Ordered so that few per-level values stay live across the flux phase.
   // -------------------------------------------------------------------------
-  GV_MEM void ausm_stage(double dtin, int stg) {
+  GV_MEM void ausm_stage(R dtin, int stg) {
     const double *m = kp.mass;
-    const double T = U[F_T];
+    const R T = U[F_T];
     // --- pointwise (:591-609, :649-652, :752-773)
-    double NS[NSP], rhos[NSP];
-    const double rho = exp(U[F_LR]);
-    double nt = 0.0;
+    R NS[NSP], rhos[NSP];
+    const R rho = exp(U[F_LR]);
+    R nt = 0.0;
 #pragma unroll
     for (int s = 0; s < NSP; s++) {
       NS[s] = exp(U[F_LN + s]);
@@ -935,24 +948,24 @@ struct Column {
     }
 #pragma unroll
     for (int s = 0; s < NSP; s++) rhos[s] = m[s] * NS[s];
-    const double te = nt * KB * T / S.gm1 +
+    const R te = nt * KB * T / S.gm1 +
                       0.5 * rho *
                           (U[F_VU] * U[F_VU] + U[F_VN] * U[F_VN] + U[F_VE] * U[F_VE]);
 
     // --- gradients the sources need (:611-646, :649-688): GradTemp,
     // DiffVel(E,N) and d ln(rho_s/rho)/dz
-    double glc[NSP];
+    R glc[NSP];
     {
-      double lcs[NSP];
+      R lcs[NSP];
 #pragma unroll
       for (int s = 0; s < NSP; s++)
         lcs[s] = cx.device_math() ? (kp.lmass[s] + U[F_LN + s]) - U[F_LR]
                                   : log(m[s] * NS[s] / rho);
       grad_log_cons(lcs, glc);
     }
-    double gr3[3], df3[3];
+    R gr3[3], df3[3];
     {
-      const double gv3[3] = {T, U[F_VE], U[F_VN]};
+      const R gv3[3] = {T, U[F_VE], U[F_VN]};
       rusanov_grad_diff<3>(gv3, gr3, df3);
     }
     if (IONS) ion_stage(dtin, stg, true);  // :817-829
@@ -964,22 +977,22 @@ struct Column {
     for (int s = 0; s < NSP; s++) xb(8 + s, j) = NS[s];
     cx.sync();
     {
-      const double tm1 = xb(0, j - 1), tp1 = xb(0, j + 1);
-      const double dF = st(A_DF, j);
+      const R tm1 = xb(0, j - 1), tp1 = xb(0, j + 1);
+      const R dF = st(A_DF, j);
       if (lv && j >= 2) {
         xb(1, j) = tm1 / T;
-        const double MeanTemp = 0.5 * (T + tm1);
+        const R MeanTemp = 0.5 * (T + tm1);
 #pragma unroll
         for (int s = 0; s < NSP; s++) {
-          const double ish = -1.0 * m[s] * S.mgh / (KB * MeanTemp);
+          const R ish = -1.0 * m[s] * S.mgh / (KB * MeanTemp);
           xb(2 + s, j) = exp(-1.0 * dF * ish);
         }
       } else if (j == 0) {
         xb(1, j) = tp1 / T;
-        const double MeanMass = 0.5 * (sc(SC_MMM_M1) + sc(SC_MMM_0));
-        const double MeanTemp = 0.5 * (tp1 + T);
-        const double ish = -1.0 * MeanMass * sc(SC_MG0) / (KB * MeanTemp);
-        const double b = exp(dF * ish);
+        const R MeanMass = 0.5 * (sc(SC_MMM_M1) + sc(SC_MMM_0));
+        const R MeanTemp = 0.5 * (tp1 + T);
+        const R ish = -1.0 * MeanMass * sc(SC_MG0) / (KB * MeanTemp);
+        const R b = exp(dF * ish);
 #pragma unroll
         for (int s = 0; s < NSP; s++) xb(2 + s, j) = b;
       }
@@ -987,22 +1000,22 @@ struct Column {
     cx.sync();
     if (j < NSP) {
       const int s = j;
-      const double ns0 = xb(8 + s, 1);
-      double x = ns0;
+      const R ns0 = xb(8 + s, 1);
+      R x = ns0;
       // The product runs level by level in the reference's order; the
       // factors of the next CH levels are read while the current ones are
       // multiplied, so that the LDS latency is not paid at every level.
       constexpr int CH = 8;
       const int nch = (L - 2) / CH;
       int jj = 2;
-      double a[CH], b[CH];
+      R a[CH], b[CH];
 #pragma unroll
       for (int u = 0; u < CH; u++) {
         a[u] = xb(1, jj + u);
         b[u] = xb(2 + s, jj + u);
       }
       for (int c = 0; c < nch; c++, jj += CH) {
-        double an[CH], bn[CH];
+        R an[CH], bn[CH];
         const bool more = c + 1 < nch;
         if (more) {
 #pragma unroll
@@ -1031,16 +1044,16 @@ struct Column {
       xb(8 + s, 0) = ns0 * xb(1, 0) * xb(2 + s, 0);
     }
     cx.sync();
-    double dev[NSP];
+    R dev[NSP];
     int sub = 0;  // bit s: SubtractHydrostatic(k, s)
     {
-      double lhps[NSP];
+      R lhps[NSP];
 #pragma unroll
       for (int s = 0; s < NSP; s++) {
-        const double hns = xb(8 + s, j);
+        const R hns = xb(8 + s, j);
         lhps[s] = log(hns * KB * T);
-        const double hrhos = m[s] * hns;
-        const double ratio = fabs(rhos[s] - hrhos) / rhos[s];
+        const R hrhos = m[s] * hns;
+        const R ratio = fabs(rhos[s] - hrhos) / rhos[s];
         dev[s] = rhos[s] - hrhos;
         if (!(fabs(ratio) > 1.0)) sub |= 1 << s;
       }
@@ -1058,22 +1071,22 @@ struct Column {
     // --- face states (calc_kt_facevalues, :2073-2117) of face k|k+1, one
     // face per thread, exponentiated where the flux needs them (:1356-1436)
     const int subUp = ib(j + 1);  // flags of cell k+1
-    double RSL[NSP], RSR[NSP], PSL[NSP], PSR[NSP], HL[NSP], HR[NSP], VL[NSP], VR[NSP];
-    double VEL, VER, VNL, VNR;
+    R RSL[NSP], RSR[NSP], PSL[NSP], PSR[NSP], HL[NSP], HR[NSP], VL[NSP], VR[NSP];
+    R VEL, VER, VNL, VNR;
     {
-      const double iF0 = st(A_IF, j), iF1 = st(A_IF, j + 1), iF2 = st(A_IF, j + 2);
-      const double dF0 = st(A_DF, j), dF1 = st(A_DF, j + 1);
+      const R iF0 = st(A_IF, j), iF1 = st(A_IF, j + 1), iF2 = st(A_IF, j + 2);
+      const R dF0 = st(A_DF, j), dF1 = st(A_DF, j + 1);
 #define GV_KT(b, OUTL, OUTR)                                                        \
   do {                                                                              \
-    const double f0 = xb(b, j), fm1 = xb(b, j - 1), f1 = xb(b, j + 1), f2 = xb(b, j + 2); \
-    const double d0 = limiter_mc(kp.beta, (f1 - f0) * iF1, (f0 - fm1) * iF0);       \
-    const double d1 = limiter_mc(kp.beta, (f2 - f1) * iF2, (f1 - f0) * iF1);        \
+    const R f0 = xb(b, j), fm1 = xb(b, j - 1), f1 = xb(b, j + 1), f2 = xb(b, j + 2); \
+    const R d0 = limiter_mc(kp.beta, (f1 - f0) * iF1, (f0 - fm1) * iF0);       \
+    const R d1 = limiter_mc(kp.beta, (f2 - f1) * iF2, (f1 - f0) * iF1);        \
     OUTL = f0 + 0.5 * d0 * dF0;                                                     \
     OUTR = f1 - 0.5 * d1 * dF1;                                                     \
   } while (0)
 #pragma unroll
       for (int s = 0; s < NSP; s++) {
-        double l, r;
+        R l, r;
         GV_KT(s, l, r);
         HL[s] = exp(l);
         HR[s] = exp(r);
@@ -1081,7 +1094,7 @@ struct Column {
       }
       cx.sync();
       if (cx.device_math()) {
-        const double ltk = log(T * KB);
+        const R ltk = log(T * KB);
 #pragma unroll
         for (int s = 0; s < NSP; s++) {
           xb(s, j) = kp.lmass[s] + U[F_LN + s];
@@ -1099,7 +1112,7 @@ struct Column {
       cx.sync();
 #pragma unroll
       for (int s = 0; s < NSP; s++) {
-        double l, r;
+        R l, r;
         GV_KT(s, l, r);
         RSL[s] = exp(l);
         RSR[s] = exp(r);
@@ -1117,12 +1130,12 @@ struct Column {
     // to cell k and stays in registers; the M12 side belongs to cell k+1 (with
     // that cell's Kp, Ku, dAlt_C and SubtractHydrostatic flags) and goes to
     // LDS buffers 0..20 for the thread above.
-    double FRp[NSP], FMp[NSP], NPp[NSP];
-    double EFp = 0.0, MOEp = 0.0, MONp = 0.0;
+    R FRp[NSP], FMp[NSP], NPp[NSP];
+    R EFp = 0.0, MOEp = 0.0, MONp = 0.0;
 #pragma unroll
     for (int s = 0; s < NSP; s++) FRp[s] = FMp[s] = NPp[s] = 0.0;
     if (face()) {
-      double RL = 0.0, RR = 0.0, PL = 0.0, PR = 0.0;
+      R RL = 0.0, RR = 0.0, PL = 0.0, PR = 0.0;
 #pragma unroll
       for (int s = 0; s < NSP; s++) {
         RL = RL + RSL[s];
@@ -1133,8 +1146,8 @@ struct Column {
         PL = PL + PSL[s];
         PR = PR + PSR[s];
       }
-      const double iRL = 1.0 / RL, iRR = 1.0 / RR;
-      double VUL = 0.0, VUR = 0.0;  // mass-weighted face winds (:1505-1532)
+      const R iRL = 1.0 / RL, iRR = 1.0 / RR;
+      R VUL = 0.0, VUR = 0.0;  // mass-weighted face winds (:1505-1532)
 #pragma unroll
       for (int s = 0; s < NSP; s++) {
         VUL = VUL + dq(RSL[s] * VL[s], RL, iRL);
@@ -1142,36 +1155,36 @@ struct Column {
       }
       // bulk energy (:1538-1562), enthalpy (:1569-1615), Liou sound speed
       // (:1619-1655)
-      const double EL = st(A_GL1, j) * PL + 0.5 * RL * (VUL * VUL + VEL * VEL + VNL * VNL);
-      const double ER = st(A_GR1, j) * PR + 0.5 * RR * (VUR * VUR + VER * VER + VNR * VNR);
-      double CS;
+      const R EL = st(A_GL1, j) * PL + 0.5 * RL * (VUL * VUL + VEL * VEL + VNL * VNL);
+      const R ER = st(A_GR1, j) * PR + 0.5 * RR * (VUR * VUR + VER * VER + VNR * VNR);
+      R CS;
       {
-        const double HtL =
+        const R HtL =
             0.5 * (VUL * VUL + VEL * VEL + VNL * VNL) + dq(st(A_GL2, j) * PL, RL, iRL);
-        const double HtR =
+        const R HtR =
             0.5 * (VUR * VUR + VER * VER + VNR * VNR) + dq(st(A_GR2, j) * PR, RR, iRR);
-        double sub_cs = sqrt(st(A_GL3, j) * HtL);
-        const double CSL = (sub_cs * sub_cs) / dmax(sub_cs, VUL);
+        R sub_cs = sqrt(st(A_GL3, j) * HtL);
+        const R CSL = (sub_cs * sub_cs) / dmax(sub_cs, VUL);
         sub_cs = sqrt(st(A_GR3, j) * HtR);
-        const double CSR = (sub_cs * sub_cs) / dmax(sub_cs, -1.0 * VUR);
+        const R CSR = (sub_cs * sub_cs) / dmax(sub_cs, -1.0 * VUR);
         CS = dmin(CSL, CSR);
       }
-      const double kp0 = st(A_KP, j), kp1 = st(A_KP, j + 1);
-      const double ku0 = st(A_KU, j), ku1 = st(A_KU, j + 1);
-      const double rdt0 = st(A_DC, j) / dtin, rdt1 = st(A_DC, j + 1) / dtin;
-      double BNVp = 0.0, BNVm = 0.0;
-      const double RLR = RL + RR;
-      const double iRLR = 1.0 / RLR, icS = 1.0 / CS;
+      const R kp0 = st(A_KP, j), kp1 = st(A_KP, j + 1);
+      const R ku0 = st(A_KU, j), ku1 = st(A_KU, j + 1);
+      const R rdt0 = st(A_DC, j) / dtin, rdt1 = st(A_DC, j + 1) / dtin;
+      R BNVp = 0.0, BNVm = 0.0;
+      const R RLR = RL + RR;
+      const R iRLR = 1.0 / RLR, icS = 1.0 / CS;
 #pragma unroll
       for (int s = 0; s < NSP; s++) {
-        const double MPS = 0.5 * (PSL[s] + PSR[s]);
-        const double MHPS = 0.5 * (HL[s] + HR[s]);
-        const double MRS = 0.5 * (RSL[s] + RSR[s]);
-        const double ML = dq(VL[s], CS, icS), MR = dq(VR[s], CS, icS);  // :1689-1706
-        const double M2 = 0.5 * (ML * ML + MR * MR);
-        const double MZ = sqrt(dmin(1.0, dmax(M2, 1.0e-19)));
-        const double FA = MZ * (2.0 - MZ);
-        double MF4P, MF4N;  // :1789-1819 (16*LiouBeta = 2)
+        const R MPS = 0.5 * (PSL[s] + PSR[s]);
+        const R MHPS = 0.5 * (HL[s] + HR[s]);
+        const R MRS = 0.5 * (RSL[s] + RSR[s]);
+        const R ML = dq(VL[s], CS, icS), MR = dq(VR[s], CS, icS);  // :1689-1706
+        const R M2 = 0.5 * (ML * ML + MR * MR);
+        const R MZ = sqrt(dmin(1.0, dmax(M2, 1.0e-19)));
+        const R FA = MZ * (2.0 - MZ);
+        R MF4P, MF4N;  // :1789-1819 (16*LiouBeta = 2)
         if (fabs(ML) >= 1.0)
           MF4P = 0.5 * (ML + fabs(ML));
         else
@@ -1182,15 +1195,15 @@ struct Column {
         else
           MF4N = -0.25 * ((MR - 1.0) * (MR - 1.0)) *
                  (1.0 + 2.0 * (0.25 * ((MR + 1.0) * (MR + 1.0))));
-        const double f1 = dmax((1.0 - M2), 0.0);
-        const double pdh = (PSR[s] - HR[s]) - (PSL[s] - HL[s]);
-        const double pd = PSR[s] - PSL[s];
-        const double rvd = (RSR[s]) * VR[s] - (RSL[s]) * VL[s];
-        const double mcs = MRS * CS, facs = FA * CS;
+        const R f1 = dmax((1.0 - M2), 0.0);
+        const R pdh = (PSR[s] - HR[s]) - (PSL[s] - HL[s]);
+        const R pd = PSR[s] - PSL[s];
+        const R rvd = (RSR[s]) * VR[s] - (RSL[s]) * VL[s];
+        const R mcs = MRS * CS, facs = FA * CS;
         {  // P12 side of cell k (:1822-1970)
           const bool sh = (sub >> s) & 1;
-          const double MP = kp0 * f1 * (sh ? pdh : pd) / (mcs * (facs + rdt0));
-          const double NV = CS * (MF4P + MF4N - MP);
+          const R MP = kp0 * f1 * (sh ? pdh : pd) / (mcs * (facs + rdt0));
+          const R NV = CS * (MF4P + MF4N - MP);
           NPp[s] = (sh ? (MPS - MHPS) : MPS) - 0.5 * ku0 * CS * rvd;
           if (NV >= 0.0) {
             FRp[s] = RSL[s] * NV;
@@ -1203,8 +1216,8 @@ struct Column {
         }
         {  // M12 side of cell k+1
           const bool sh = (subUp >> s) & 1;
-          const double MP = kp1 * f1 * (sh ? pdh : pd) / (mcs * (facs + rdt1));
-          const double NV = CS * (MF4P + MF4N - MP);
+          const R MP = kp1 * f1 * (sh ? pdh : pd) / (mcs * (facs + rdt1));
+          const R NV = CS * (MF4P + MF4N - MP);
           xb(12 + s, j) = (sh ? (MPS - MHPS) : MPS) - 0.5 * ku1 * CS * rvd;
           if (NV >= 0.0) {
             xb(s, j) = RSL[s] * NV;
@@ -1239,11 +1252,11 @@ struct Column {
     cx.sync();
     // divergence of the face fluxes (:2042-2068); cell k takes its M12 side
     // from the face below
-    double rsf[NSP], msf[NSP], ef, mfE, mfN;
+    R rsf[NSP], msf[NSP], ef, mfE, mfN;
     {
-      const double ap = st(A_AP, j), am = st(A_AM, j), vol = st(A_VOL, j);
-      const double dC = st(A_DC, j);
-      const double ivol = 1.0 / vol, idC = 1.0 / dC;
+      const R ap = st(A_AP, j), am = st(A_AM, j), vol = st(A_VOL, j);
+      const R dC = st(A_DC, j);
+      const R ivol = 1.0 / vol, idC = 1.0 / dC;
 #pragma unroll
       for (int s = 0; s < NSP; s++) {
         rsf[s] = dq(ap * FRp[s] - am * xb(s, j - 1), vol, ivol);
@@ -1259,8 +1272,8 @@ struct Column {
     // --- update of cell k (:802-997); each new value is folded into the RK4
     // carries as soon as the stage has no further use of the old one
     if (cell()) {
-      double nrs[NSP], nns[NSP], w[NSP];
-      double nrho = 0.0, nnt = 0.0;
+      R nrs[NSP], nns[NSP], w[NSP];
+      R nrho = 0.0, nnt = 0.0;
 #pragma unroll
       for (int s = 0; s < NSP; s++) {
         nrs[s] = rhos[s] - dtin * (rsf[s]);
@@ -1268,34 +1281,34 @@ struct Column {
         nrho = nrho + nrs[s];
         nnt = nnt + nns[s];
       }
-      const double gradT = gr3[0];
-      const double invr = st(A_INVR, j);
+      const R gradT = gr3[0];
+      const R invr = st(A_INVR, j);
 #pragma unroll
       for (int s = 0; s < NSP; s++) {
-        const double moms = rhos[s] * U[F_W + s];
-        double nm;
+        const R moms = rhos[s] * U[F_W + s];
+        R nm;
         if (!((sub >> s) & 1))
           nm = moms - dtin * (msf[s]) + dtin * rhos[s] * S.eg;
         else
           nm = moms - dtin * (msf[s]) + dtin * dev[s] * S.eg;
-        double ww = nm / nrs[s];
+        R ww = nm / nrs[s];
         ww = ww + dtin * (U[F_VN] * U[F_VN] + U[F_VE] * U[F_VE]) * invr;
         if (kp.coriolis) ww = ww + dtin * (coriolis * U[F_VE]);
         ww = ww - dq(dtin * (kp.tdc[s] * KB * gradT), m[s], kp.imass[s]);
         w[s] = ww;
       }
-      const double hf = st(A_HF, j);
-      const double inrho = 1.0 / nrho;
-      double nvE = dq(rho * U[F_VE] - dtin * (mfE), nrho, inrho);
+      const R hf = st(A_HF, j);
+      const R inrho = 1.0 / nrho;
+      R nvE = dq(rho * U[F_VE] - dtin * (mfE), nrho, inrho);
       nvE = nvE + hf * dtin * df3[1];
-      double nvN = dq(rho * U[F_VN] - dtin * (mfN), nrho, inrho);
+      R nvN = dq(rho * U[F_VN] - dtin * (mfN), nrho, inrho);
       nvN = nvN + hf * dtin * df3[2];
-      const double nte = te - dtin * ef + dtin * rho * U[F_VU] * S.eg;
+      const R nte = te - dtin * ef + dtin * rho * U[F_VU] * S.eg;
       {
-        double sr = 0.0;
+        R sr = 0.0;
 #pragma unroll
         for (int s = 0; s < NSP; s++) {
-          const double nl = log(nns[s]);
+          const R nl = log(nns[s]);
           sr = sr + m[s] * (cx.device_math() ? nns[s] : exp(nl));
           ausm_commit(stg, F_LN + s, nl);
         }
@@ -1305,15 +1318,15 @@ struct Column {
       ausm_commit(stg, F_VN, nvN);
       if (kp.friction && !friction(dtin, w, nnt, nns, glc, T))
         fail(GITM_VERT_ESINGULAR, k);
-      double nvu = 0.0;
+      R nvu = 0.0;
 #pragma unroll
       for (int s = 0; s < NSP; s++) {
-        double ww = dmax(-kp.maxvv, w[s]);
+        R ww = dmax(-kp.maxvv, w[s]);
         ww = dmin(kp.maxvv, ww);
         nvu = nvu + dq(ww * nrs[s], nrho, inrho);
         ausm_commit(stg, F_W + s, ww);
       }
-      const double np = (nte - 0.5 * nrho * (nvu * nvu + nvN * nvN + nvE * nvE)) * S.gm1;
+      const R np = (nte - 0.5 * nrho * (nvu * nvu + nvN * nvN + nvE * nvE)) * S.gm1;
       ausm_commit(stg, F_VU, nvu);
       ausm_commit(stg, F_T, np / (KB * nnt));
     }
@@ -1325,27 +1338,27 @@ struct Column {
   // -------------------------------------------------------------------------
   GV_MEM void rusanov_stage(int stg) {
     const double *m = kp.mass;
-    const double T = U[F_T];
-    double NS[NSP], lcs[NSP];
-    const double rho = exp(U[F_LR]);
-    double sum = 0.0;
+    const R T = U[F_T];
+    R NS[NSP], lcs[NSP];
+    const R rho = exp(U[F_LR]);
+    R sum = 0.0;
 #pragma unroll
     for (int s = 0; s < NSP; s++) {
       NS[s] = exp(U[F_LN + s]);
       sum = sum + NS[s];
     }
-    const double nt = exp(log(sum));  // NT = exp(LogNum) (:458-460)
+    const R nt = exp(log(sum));  // NT = exp(LogNum) (:458-460)
 #pragma unroll
     for (int s = 0; s < NSP; s++)
       lcs[s] = cx.device_math() ? (kp.lmass[s] + U[F_LN + s]) - U[F_LR]
                                 : log(m[s] * NS[s] / rho);
     // gradients and dissipation (:462-498)
-    const double va[4] = {T, U[F_VE], U[F_VN], U[F_VU]};
-    double ga[4], da[4];
+    const R va[4] = {T, U[F_VE], U[F_VN], U[F_VU]};
+    R ga[4], da[4];
     rusanov_grad_diff<4>(va, ga, da);
-    double gln[NSP], dln[NSP], gw[NSP], dw[NSP];
+    R gln[NSP], dln[NSP], gw[NSP], dw[NSP];
     {
-      double v6[NSP];
+      R v6[NSP];
 #pragma unroll
       for (int s = 0; s < NSP; s++) v6[s] = U[F_LN + s];
       rusanov_grad_diff<NSP>(v6, gln, dln);
@@ -1353,18 +1366,18 @@ struct Column {
       for (int s = 0; s < NSP; s++) v6[s] = U[F_W + s];
       rusanov_grad_diff<NSP>(v6, gw, dw);
     }
-    double glc[NSP];
+    R glc[NSP];
     grad_log_cons(lcs, glc);
     if (IONS) ion_stage(dt, stg, false);  // :617-630
     if (!cell()) return;
-    const double divvel = ga[3] + 2 * U[F_VU] * st(A_INVR, j);  // :476
-    double w[NSP], nln[NSP];
+    const R divvel = ga[3] + 2 * U[F_VU] * st(A_INVR, j);  // :476
+    R w[NSP], nln[NSP];
 #pragma unroll
     for (int s = 0; s < NSP; s++) {  // :550-614
-      const double vv = U[F_W + s];
-      const double divvv = gw[s] + 2 * vv * st(A_INVR, j);
+      const R vv = U[F_W + s];
+      const R divvv = gw[s] + 2 * vv * st(A_INVR, j);
       nln[s] = U[F_LN + s] - dt * (divvv + vv * gln[s]) + dt * dln[s];
-      double ww = vv -
+      R ww = vv -
                   dt * (vv * gw[s] -
                         (U[F_VN] * U[F_VN] + U[F_VE] * U[F_VE]) * st(A_INVR, j) +
                         T * gln[s] * KB / m[s] + ga[0] * KB / m[s] - st(A_GRAV, j)) +
@@ -1376,19 +1389,19 @@ struct Column {
     }
     if (kp.friction && !friction(dt, w, nt, NS, glc, T))  // :632-641, old densities
       fail(GITM_VERT_ESINGULAR, k);
-    double nvu = 0.0;
+    R nvu = 0.0;
 #pragma unroll
     for (int s = 0; s < NSP; s++) {  // :643-658
-      double ww = dmax(-kp.maxvv, w[s]);
+      R ww = dmax(-kp.maxvv, w[s]);
       ww = dmin(kp.maxvv, ww);
       w[s] = ww;
       nvu = nvu + ww * (m[s] * NS[s] / rho);
     }
-    const double vu = U[F_VU];  // :660-679
-    const double nvE = U[F_VE] - dt * vu * ga[1] + dt * da[1];
-    const double nvN = U[F_VN] - dt * vu * ga[2] + dt * da[2];
-    const double nT = T - dt * (vu * ga[0] + (S.gm1) * (T * divvel)) + dt * da[0];
-    double sr = 0.0;
+    const R vu = U[F_VU];  // :660-679
+    const R nvE = U[F_VE] - dt * vu * ga[1] + dt * da[1];
+    const R nvN = U[F_VN] - dt * vu * ga[2] + dt * da[2];
+    const R nT = T - dt * (vu * ga[0] + (S.gm1) * (T * divvel)) + dt * da[0];
+    R sr = 0.0;
 #pragma unroll
     for (int s = 0; s < NSP; s++) sr = sr + m[s] * exp(nln[s]);
 #pragma unroll
@@ -1407,16 +1420,16 @@ struct Column {
   // (Rusanov gradient and dissipation, calc_rusanov_alts_*), the others are
   // carried through the RK4 combination unchanged.  The values are final as
   // soon as they are computed, so they are committed at once.
-  GV_MEM void ion_stage(double dtin, int stg, bool ausm) {
-    double vi[NI_ADVECT], gi[NI_ADVECT], di[NI_ADVECT];
+  GV_MEM void ion_stage(R dtin, int stg, bool ausm) {
+    R vi[NI_ADVECT], gi[NI_ADVECT], di[NI_ADVECT];
 #pragma unroll
     for (int f = 0; f < NI_ADVECT; f++) vi[f] = ion(f, j);
     rusanov_grad_diff<NI_ADVECT>(vi, gi, di);
     if (!cell()) return;
-    const double ivu = ion(NI, j);
+    const R ivu = ion(NI, j);
 #pragma unroll
     for (int f = 0; f < NI; f++) {
-      const double nv = (f < NI_ADVECT) ? vi[f] - dtin * (ivu * gi[f]) + dtin * di[f]
+      const R nv = (f < NI_ADVECT) ? vi[f] - dtin * (ivu * gi[f]) + dtin * di[f]
                                         : ion(f, j);
       if (ausm)
         ausm_commit_ion(stg, f, nv);
@@ -1457,7 +1470,7 @@ struct Column {
       for (int f = 0; f < NF; f++) Yv(f) = U[f];
     }
     ion_store_y();
-    const double dtin[4] = {0.5 * dt, 0.5 * dt, dt, 0.5 * dt};  // :160-377
+    const R dtin[4] = {0.5 * dt, 0.5 * dt, dt, 0.5 * dt};  // :160-377
 #pragma unroll 1
     for (int stg = 0; stg < 4; stg++) {
       ausm_stage(dtin[stg], stg);

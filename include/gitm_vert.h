@@ -22,6 +22,7 @@
 #ifndef GITM_VERT_H
 #define GITM_VERT_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -50,7 +51,8 @@ enum {
                                  (UseMSIS, tides, BC perturbation,
                                  UseDamping): run the Fortran path         */
   GITM_VERT_EHIP = 6,         /* HIP runtime error                         */
-  GITM_VERT_ENOMEM = 7
+  GITM_VERT_ENOMEM = 7,
+  GITM_VERT_ETAPE = 8         /* adjoint tape too small for the column     */
 };
 
 /* One array of the column working set.  Element (column c, level k,
@@ -248,6 +250,38 @@ int gitm_vert_batch_advance(gitm_vert_batch *b, const gitm_vert_params *p, int s
  * dt_device[m] receives member m's step. */
 int gitm_vert_batch_prepare(gitm_vert_batch *b, const gitm_vert_prep_params *p,
                             double *dt_device);
+
+/* ------------------------------------------------------------------------
+ * Adjoint of the AUSM step (reverse-mode differentiation).  The reference
+ * has no counterpart; it serves sensitivity studies (e.g. of the upper
+ * thermosphere's density to the temperature profile and to heating), driven
+ * step by step from checkpoints by the caller (gitm_amd.adjoint in Python).
+ * ------------------------------------------------------------------------ */
+typedef struct gitm_vert_adjoint_fields {
+  gitm_vert_field ndensity_s;        /* d/d NDensityS(.., 1:6, ..)           */
+  gitm_vert_field vertical_velocity; /* d/d VerticalVelocity(.., 1:6, ..)    */
+  gitm_vert_field velocity;          /* d/d Velocity(.., 1:3, ..)            */
+  gitm_vert_field temperature;       /* d/d Temperature                      */
+  gitm_vert_field rho;               /* d/d Rho                              */
+} gitm_vert_adjoint_fields;
+
+/* One AUSM vertical step of every column of `c` (the step's input state,
+ * device pointers, not modified) is recomputed with a taped number type, the
+ * adjoints `gout` of the step's outputs (NDensityS, VerticalVelocity,
+ * Velocity, Temperature and Rho as advance_vertical scatters them, levels
+ * -1..nAlts+2) are propagated back, and `gin` receives the adjoints of the
+ * step's inputs (the same fields before the step).  gin may be gout (in
+ * place).  The column prep inputs (TempUnit, Gamma, MeanMajorMass, cMax,
+ * eddy coefficient, geometry) are held fixed.  Neutrals only
+ * (use_ion_advection must be 0), nAlts <= 124.  Synchronous: returns the
+ * status (GITM_VERT_ETAPE if a column outgrew its tape). */
+int gitm_vert_adjoint(gitm_vert_handle *h, const gitm_vert_params *p,
+                      const gitm_vert_columns *c, const gitm_vert_adjoint_fields *gout,
+                      const gitm_vert_adjoint_fields *gin);
+
+/* Device memory the adjoint's tape may use (default 16 GiB; about 22 MB
+ * per column at nAlts = 50); larger grids run in chunks of columns. */
+int gitm_vert_set_adjoint_memory(gitm_vert_handle *h, size_t bytes);
 
 /* Member and member column of a batch-wide column number. */
 int gitm_vert_batch_locate(const gitm_vert_batch *b, int column, int *member,

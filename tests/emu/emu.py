@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 LIB = os.path.join(HERE, "libgv_emu.so")
 SRC = os.path.join(HERE, "emu_vert.cpp")
 DEPS = [SRC] + [os.path.join(ROOT, "gitm_amd", "csrc", f)
-                for f in ("vert_column.h", "vert_params.h")] + \
+                for f in ("vert_column.h", "vert_params.h", "vert_ad.h")] + \
     [os.path.join(ROOT, "include", "gitm_vert.h")]
 
 _lib = None
@@ -36,6 +36,7 @@ def lib():
         build()
         _lib = ctypes.CDLL(LIB)
         _lib.gv_emu_advance.restype = ctypes.c_int
+        _lib.gv_emu_adjoint.restype = ctypes.c_int
     return _lib
 
 
@@ -88,3 +89,44 @@ def advance(cols: dict, sp, solver: int = 1, n_steps: int = 1, waves: int = 0,
         raise RuntimeError(f"emulated kernel: {_abi.ERRORS.get(rc, rc)} "
                            f"(column {e.column}, level {e.level})")
     return out
+
+
+class AdjFields(ctypes.Structure):
+    _fields_ = [(n, _abi.Field) for n in ("nds", "vvel", "vel", "temp", "rho")]
+
+
+ADJ = (("nds", 6), ("vvel", 6), ("vel", 3), ("temp", 1), ("rho", 1))
+
+
+def adj_fields(arrays: dict, L: int) -> AdjFields:
+    a = AdjFields()
+    for name, ncomp in ADJ:
+        setattr(a, name, _abi.column_field(arrays[name].ctypes.data, ncomp, L))
+    return a
+
+
+def adjoint(cols: dict, sp, gout: dict, cols_range=None, nmax: int = 1 << 15,
+            maxph: int = 1024):
+    """Adjoint of one AUSM step (host build of the taped column code):
+    ``gout`` holds the adjoints of the step's outputs (nds, vvel, vel, temp,
+    rho); returns the adjoints of its inputs and the largest node and phase
+    counts a column used."""
+    nc, L = cols["temp"].shape
+    keep = {k: np.ascontiguousarray(cols[k], dtype=np.float64)
+            for k in ("nds", "vvel", "vel", "temp", "rho", "tempunit", "mmm", "gamma",
+                      "cmax", "eddy", "alt", "grav", "invr", "dalt", "coslat")}
+    c = _abi.columns_struct({k: v.ctypes.data for k, v in keep.items()}, nc, L - 4)
+    go = {k: np.ascontiguousarray(gout[k], dtype=np.float64) for k, _ in ADJ}
+    gi = {k: np.zeros_like(go[k]) for k, _ in ADJ}
+    p = params_from(sp)
+    e = _abi.Error()
+    nodes, phases = ctypes.c_int(), ctypes.c_int()
+    c0, c1 = cols_range if cols_range else (0, nc)
+    rc = lib().gv_emu_adjoint(ctypes.byref(p), ctypes.byref(c),
+                              ctypes.byref(adj_fields(go, L)), ctypes.byref(adj_fields(gi, L)),
+                              c0, c1, nmax, maxph, ctypes.byref(nodes), ctypes.byref(phases),
+                              ctypes.byref(e))
+    if rc:
+        raise RuntimeError(f"emulated adjoint: {_abi.ERRORS.get(rc, rc)} "
+                           f"(column {e.column}, level {e.level})")
+    return gi, nodes.value, phases.value

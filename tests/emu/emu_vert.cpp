@@ -12,6 +12,7 @@
 #include <thread>
 #include <vector>
 
+#include "vert_ad.h"
 #include "vert_params.h"
 
 namespace {
@@ -25,6 +26,7 @@ struct Shared {
 };
 
 struct EmuCtx {
+  using real = double;
   int lane_;
   Shared *sh;
   int lane() const { return lane_; }
@@ -88,7 +90,139 @@ unsigned long long run_all(const gv::KParams &kp, const gitm_vert_columns &io, i
   return sh.err.load();
 }
 
+// The adjoint build: taped arithmetic (vert_ad.h), lanes as CPU threads.
+struct AdShared {
+  std::barrier<> bar;
+  std::atomic<int> orflag{0};
+  std::atomic<unsigned long long> err{~0ULL};
+  gv::AdTape tape;
+  explicit AdShared(int n) : bar(n) {}
+};
+
+struct EmuAdCtx {
+  using real = gv::adm::ad;
+  int lane_;
+  AdShared *sh;
+  int nph = 0;
+  int lane() const { return lane_; }
+  void boundary() {
+    const gv::AdTape &t = sh->tape;
+    if (nph >= t.maxph)
+      *t.overflow = 1;
+    else
+      t.phase[(long long)nph * t.stride + lane_] = gv::ad_count();
+    nph++;
+  }
+  void sync() {
+    boundary();
+    sh->bar.arrive_and_wait();
+  }
+  bool sync_or(bool p) {
+    boundary();
+    if (p) sh->orflag.store(1);
+    sh->bar.arrive_and_wait();
+    bool r = sh->orflag.load() != 0;
+    sh->bar.arrive_and_wait();
+    if (lane_ == 0) sh->orflag.store(0);
+    sh->bar.arrive_and_wait();
+    return r;
+  }
+  void report(unsigned long long key) {
+    unsigned long long cur = sh->err.load();
+    while (key < cur && !sh->err.compare_exchange_weak(cur, key)) {
+    }
+  }
+  bool device_math() const { return false; }
+  void diff_powers(real temp, double sign, real (&out)[6]) const {
+    for (int u = 0; u < 6; u++) out[u] = pow(temp, sign * gv::DEXU[u]);
+  }
+  void end_forward() { boundary(); }
+  void add(int id, double v) { std::atomic_ref<double>(sh->tape.adj[id]).fetch_add(v); }
+  double adjoint(int id) {
+    return id >= 0 ? std::atomic_ref<double>(sh->tape.adj[id]).load() : 0.0;
+  }
+  void sweep() {
+    const gv::AdTape &t = sh->tape;
+    sh->bar.arrive_and_wait();  // every lane's seeds are in
+    const int np = nph < t.maxph ? nph : t.maxph;
+    for (int ph = np - 1; ph >= 0; ph--) {
+      const int hi = t.phase[(long long)ph * t.stride + lane_];
+      const int lo = ph ? t.phase[(long long)(ph - 1) * t.stride + lane_] : 0;
+      for (int n = hi - 1; n >= lo; n--) {
+        const long long id = (long long)n * t.stride + lane_;
+        const double a = std::atomic_ref<double>(t.adj[id]).load();
+        if (a == 0.0) continue;
+        if (t.p1[id] >= 0) add(t.p1[id], a * t.d1[id]);
+        if (t.p2[id] >= 0) add(t.p2[id], a * t.d2[id]);
+      }
+      sh->bar.arrive_and_wait();
+    }
+  }
+};
+
 }  // namespace
+
+// Adjoint of one AUSM step (no ions) of columns c0..c1-1 of `c`: gout holds
+// the adjoints of the step's outputs, gin receives those of its inputs.
+// nmax/maxph size the tape of one column; the numbers of nodes and phases
+// a column used are returned through used_nodes/used_phases.
+extern "C" int gv_emu_adjoint(const gitm_vert_params *p, const gitm_vert_columns *c,
+                              const gv::AdjFields *gout, const gv::AdjFields *gin, int c0,
+                              int c1, int nmax, int maxph, int *used_nodes, int *used_phases,
+                              gitm_vert_error *e) {
+  int rc = gv::check_args(p, c, GITM_VERT_AUSM, 1);
+  if (rc) return rc;
+  if (p->use_ion_advection) return GITM_VERT_EUNSUPPORTED;
+  gv::KParams kp = gv::make_kparams(p, c, GITM_VERT_AUSM, 1, nullptr);
+  const int waves = gv::auto_waves(c->n_alts);
+  kp.scr_fields = gv::scratch_fields(gv::carry_in_lds(waves), false);
+  const int L = c->n_alts + 4;
+  std::vector<gv::adm::ad> scratch((size_t)(c1 > 0 ? c1 : 1) * kp.scr_fields * L + 1);
+  AdShared sh(L);
+  std::vector<double> d1((size_t)nmax * L), d2((size_t)nmax * L), adj((size_t)nmax * L);
+  std::vector<int> p1((size_t)nmax * L), p2((size_t)nmax * L), phase((size_t)maxph * L);
+  int overflow = 0;
+  sh.tape = gv::AdTape{d1.data(), d2.data(), p1.data(), p2.data(), adj.data(), phase.data(),
+                       L, nmax, maxph, &overflow};
+  std::vector<gv::adm::ad> lds(gv::lds_doubles(c->n_alts, false, gv::carry_in_lds(waves)));
+  std::atomic<int> max_nodes{0}, max_phases{0};
+  std::vector<std::thread> th;
+  for (int t = 0; t < L; t++)
+    th.emplace_back([&, t] {
+      gv::ad_lane.t = &sh.tape;
+      gv::ad_lane.gid = t;
+      for (int col = c0; col < c1; col++) {
+        EmuAdCtx cx{t, &sh};
+        gv::ad_lane.n = 0;
+        auto run = [&](auto nt) {
+          gv::adjoint_column<decltype(nt)::value>(cx, kp, *c, *gout, *gin, col, lds.data(),
+                                                   reinterpret_cast<double *>(scratch.data()),
+                                                   kp.dt, col);
+        };
+        switch (waves) {
+          case 1: run(std::integral_constant<int, 64>()); break;
+          case 2: run(std::integral_constant<int, 128>()); break;
+          case 3: run(std::integral_constant<int, 192>()); break;
+          default: run(std::integral_constant<int, 256>()); break;
+        }
+        int v = max_nodes.load();
+        while (gv::ad_lane.n > v && !max_nodes.compare_exchange_weak(v, gv::ad_lane.n)) {
+        }
+        v = max_phases.load();
+        while (cx.nph > v && !max_phases.compare_exchange_weak(v, cx.nph)) {
+        }
+        sh.bar.arrive_and_wait();
+      }
+    });
+  for (auto &x : th) x.join();
+  if (used_nodes) *used_nodes = max_nodes.load();
+  if (used_phases) *used_phases = max_phases.load();
+  gitm_vert_error tmp;
+  if (!e) e = &tmp;
+  gv::decode_error(sh.err.load(), e);
+  if (overflow && e->code == GITM_VERT_OK) e->code = GITM_VERT_ENOMEM;
+  return e->code;
+}
 
 // device_math != 0 evaluates as the GPU does (friction powers as exp(e ln T),
 // the log-space forms of Column); 0 uses the reference's forms, for bitwise

@@ -1,0 +1,100 @@
+"""The adjoint of the AUSM step on the CPU: the host build of the taped
+column code (gitm_amd/csrc/vert_ad.h through tests/emu) against central
+finite differences of the oracle, which is an independent restatement of the
+reference step.
+
+A loss J = sum of w . (state after one step) is differentiated with respect
+to single input cells.  Central differences of the oracle at two step sizes
+bound their own error (truncation and the rounding of the perturbed run):
+the adjoint must agree with the finer difference to within 1e-6 relative or
+to within twice the two differences' disagreement, whichever is larger.
+Cells are drawn where the loss is sensitive (|x dJ/dx| above 1e-6 of the
+largest one), so that the differences resolve the derivative.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from gitm_amd import synth
+from gitm_amd.adjoint import level_weights, heating_profile
+from oracle import cpu
+from tests.helpers import subset
+
+STATE = ("nds", "vvel", "vel", "temp", "rho")
+
+
+@pytest.fixture(scope="module")
+def case():
+    from tests.emu import emu
+    cols = subset(synth.make_columns(synth.CONFIGS["cpu_324x50"], seed=3), [17])
+    p = synth.StepParams()
+    rng = np.random.default_rng(7)
+    # a loss touching every output field: rho near 400 km plus random weights
+    w = {k: rng.standard_normal(cols[k].shape) for k in STATE}
+    scale = {k: 1.0 / np.max(np.abs(cols[k])) for k in STATE}
+    for k in STATE:
+        w[k] *= 1e-3 * scale[k]
+    w["rho"][0, 31] += 0.5 * scale["rho"]
+    w["rho"][0, 32] += 0.5 * scale["rho"]
+    gin, nodes, phases = emu.adjoint(cols, p, w)
+    return cols, p, w, gin, nodes, phases
+
+
+def loss(cols, p, w):
+    out = cpu.advance(cols, p, solver=1, threads=1)
+    return sum(float(np.sum(w[k] * out[k])) for k in STATE)
+
+
+def central(cols, p, w, field, idx, rel):
+    x0 = cols[field][idx]
+    h = rel * abs(x0) if x0 != 0 else rel
+    a = {k: np.array(v) for k, v in cols.items()}
+    a[field][idx] = x0 + h
+    b = {k: np.array(v) for k, v in cols.items()}
+    b[field][idx] = x0 - h
+    return (loss(a, p, w) - loss(b, p, w)) / (2 * h)
+
+
+def test_tape_size(case):
+    _, _, _, _, nodes, phases = case
+    # one step of a 54-level column: about 10k nodes and 85 barriers per lane
+    assert 5000 < nodes < 12288 + 8 * 50
+    assert 40 < phases < 128
+
+
+@pytest.mark.parametrize("field", STATE)
+def test_adjoint_matches_oracle_differences(case, field):
+    cols, p, w, gin, _, _ = case
+    g = gin[field]
+    sens = np.abs(g * cols[field])
+    cand = np.argwhere(sens > 1e-6 * sens.max())
+    rng = np.random.default_rng(hash(field) % 1000)
+    picks = cand[rng.choice(len(cand), size=min(4, len(cand)), replace=False)]
+    for idx in map(tuple, picks):
+        f1 = central(cols, p, w, field, idx, 1e-6)
+        f2 = central(cols, p, w, field, idx, 1e-5)
+        err = abs(g[idx] - f1) / abs(g[idx])
+        bound = max(1e-6, 2.0 * abs(f1 - f2) / abs(g[idx]))
+        assert err <= bound, (field, idx, g[idx], f1, f2, err, bound)
+
+
+def test_adjoint_is_linear_in_the_seed(case):
+    """The adjoint map is linear: seeds w and 3w give 3x the adjoints, and a
+    zero seed gives zero."""
+    from tests.emu import emu
+    cols, p, w, gin, _, _ = case
+    g3, _, _ = emu.adjoint(cols, p, {k: 3.0 * w[k] for k in STATE})
+    for k in STATE:
+        np.testing.assert_allclose(g3[k], 3.0 * gin[k], rtol=1e-13, atol=0)
+    g0, _, _ = emu.adjoint(cols, p, {k: np.zeros_like(w[k]) for k in STATE})
+    assert all(np.all(g0[k] == 0.0) for k in STATE)
+
+
+def test_level_weights_and_heating_profile():
+    alt = np.array([[90e3, 100e3, 110e3, 395e3, 405e3, 415e3, 500e3, 510e3]])
+    w = level_weights(alt, 400e3)
+    assert w[0, 3] == pytest.approx(0.5) and w[0, 4] == pytest.approx(0.5)
+    assert w.sum() == pytest.approx(1.0)
+    q = heating_profile(np.array([150e3, 170e3]))
+    assert q[0] == pytest.approx(1e-5) and q[1] == pytest.approx(1e-5 * np.exp(-1.0))
