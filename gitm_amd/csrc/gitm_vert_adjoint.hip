@@ -1,0 +1,249 @@
+// gitm_vert_adjoint.hip -- the adjoint of the AUSM step (gitm_vert_adjoint
+// of include/gitm_vert.h; the taped column code is in vert_ad.h).
+#include "vert_ad.h"
+#include "vert_gpu.h"
+
+namespace {
+
+// The adjoint build (vert_ad.h): the column code over the taped number type.
+// Every barrier closes a phase of the tape; sweep() runs the phases
+// backwards with a barrier between them.
+struct GpuAdCtx {
+  using real = gv::adm::ad;
+  unsigned long long *err;
+  int nph = 0;
+  __device__ int lane() const { return (int)threadIdx.x; }
+  __device__ void boundary() {
+    const gv::AdTape &t = gv::ad_tape();
+    if (nph >= t.maxph)
+      *t.overflow = 1;
+    else
+      t.phase[(long long)nph * t.stride + gv::ad_gid()] = gv::ad_count();
+    nph++;
+  }
+  __device__ void sync() {
+    boundary();
+    __syncthreads();
+  }
+  __device__ bool sync_or(bool p) {
+    boundary();
+    return __syncthreads_or(p ? 1 : 0) != 0;
+  }
+  __device__ void report(unsigned long long key) const { atomicMin(err, key); }
+  __device__ static constexpr bool device_math() { return false; }
+  __device__ void diff_powers(real temp, double sign, real (&out)[6]) const {
+#pragma unroll
+    for (int u = 0; u < 6; u++) out[u] = pow(temp, sign * gv::DEXU[u]);
+  }
+  __device__ void end_forward() { boundary(); }
+  __device__ void add(int id, double v) { unsafeAtomicAdd(gv::ad_tape().adj + id, v); }
+  // adjoints are summed in L2 by atomics: read them past the CU's L1
+  __device__ double adjoint(int id) const {
+    return id >= 0 ? __hip_atomic_load(gv::ad_tape().adj + id, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT)
+                   : 0.0;
+  }
+  __device__ void sweep() {
+    const gv::AdTape &t = gv::ad_tape();
+    const int g = gv::ad_gid();
+    __threadfence();
+    __syncthreads();  // every lane's seeds are in
+    const int np = nph < t.maxph ? nph : t.maxph;
+    for (int ph = np - 1; ph >= 0; ph--) {
+      const int hi = t.phase[(long long)ph * t.stride + g];
+      const int lo = ph ? t.phase[(long long)(ph - 1) * t.stride + g] : 0;
+      for (int n = hi - 1; n >= lo; n--) {
+        const long long id = (long long)n * t.stride + g;
+        const double a = adjoint((int)id);
+        if (a == 0.0) continue;
+        const int p1 = t.p1[id], p2 = t.p2[id];
+        if (p1 >= 0) add(p1, a * t.d1[id]);
+        if (p2 >= 0) add(p2, a * t.d2[id]);
+      }
+      __threadfence();
+      __syncthreads();
+    }
+  }
+};
+
+// Adjoint of one AUSM step of columns col0 .. col0+gridDim.x-1, forward
+// part: one workgroup per column, one lane per level, as the forward kernel.
+template <int NT>
+__global__ void __launch_bounds__(NT)
+    vert_adjoint_fwd_kernel(gv::KParams kp, gitm_vert_columns io, gv::AdjFields gout,
+                            gv::AdTape tape, int col0) {
+  extern __shared__ double lds[];
+  if (threadIdx.x == 0) gv::ad_tape_sh = tape;
+  gv::ad_count_sh[threadIdx.x] = 0;
+  __syncthreads();
+  GpuAdCtx cx{kp.err};
+  const int col = col0 + (int)blockIdx.x;
+  gv::adjoint_forward<NT>(cx, kp, io, gout, col, reinterpret_cast<gv::adm::ad *>(lds),
+                          kp.scratch, kp.dt, col);
+  tape.nphase[gv::ad_gid()] = cx.nph < tape.maxph ? cx.nph : tape.maxph;
+}
+
+// Backward sweep of the tape the forward part wrote, then the input
+// adjoints.  It needs no LDS, so many more columns are in flight than in
+// the forward part: the sweep is a chain of dependent memory accesses per
+// lane, and its speed is the number of lanes that wait at once.
+template <int NT>
+__global__ void __launch_bounds__(NT)
+    vert_adjoint_sweep_kernel(gitm_vert_columns io, gv::AdjFields gin, gv::AdTape t,
+                              int col0) {
+  const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int nph = t.nphase[g];  // the same on every lane of the column
+  double *adj = t.adj;
+  for (int ph = nph - 1; ph >= 0; ph--) {
+    const int hi = t.phase[(long long)ph * t.stride + g];
+    const int lo = ph ? t.phase[(long long)(ph - 1) * t.stride + g] : 0;
+    for (int n = hi - 1; n >= lo; n--) {
+      const long long id = (long long)n * t.stride + g;
+      const int p1 = t.p1[id], p2 = t.p2[id];
+      const double d1 = t.d1[id], d2 = t.d2[id];
+      const double a = __hip_atomic_load(adj + id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (a == 0.0) continue;
+      if (p1 >= 0) unsafeAtomicAdd(adj + p1, a * d1);
+      if (p2 >= 0) unsafeAtomicAdd(adj + p2, a * d2);
+    }
+    __threadfence();
+    __syncthreads();
+  }
+  gv::adjoint_store(t, g, io, gin, col0 + (int)blockIdx.x, (int)threadIdx.x - 1,
+                    [&](long long id) {
+                      return __hip_atomic_load(adj + id, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                    });
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Adjoint of one AUSM step (vert_ad.h).  The columns run in chunks whose
+// tapes fit the handle's adjoint memory budget.
+// ---------------------------------------------------------------------------
+namespace {
+
+// Default tape budget: 40% of the device's memory, at most 96 GiB (the
+// 5-degree grid's 2592 columns at nAlts = 50 take 58 GB, one chunk; the sweep
+// needs many columns in flight to hide its memory latency).
+size_t default_adjoint_budget(int device) {
+  hipDeviceProp_t prop;
+  size_t total = (size_t)16 << 30;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) total = prop.totalGlobalMem;
+  return std::min((size_t)(0.4 * (double)total), (size_t)96 << 30);
+}
+
+// Tape entries per lane: about 10k nodes per level-lane and step, plus the
+// serial loops of the boundary lanes, which grow with the column.
+int adjoint_nmax(int n_alts) { return 12288 + 8 * n_alts; }
+constexpr int kAdjMaxPhases = 128;
+constexpr size_t kNodeBytes = 2 * sizeof(double) + sizeof(double) + 2 * sizeof(int);
+
+template <int NT>
+void launch_adjoint(int ncols, int col0, hipStream_t stream, const gv::KParams &kp,
+                    const gitm_vert_columns &c, const gv::AdjFields &go,
+                    const gv::AdjFields &gi, const gv::AdTape &tape) {
+  const size_t lds =
+      gv::lds_doubles(kp.n_alts, false, gv::carry_in_lds(NT / 64)) * sizeof(gv::adm::ad);
+  hipLaunchKernelGGL(vert_adjoint_fwd_kernel<NT>, dim3(ncols), dim3(kp.n_alts + 4), lds,
+                     stream, kp, c, go, tape, col0);
+  hipLaunchKernelGGL(vert_adjoint_sweep_kernel<NT>, dim3(ncols), dim3(kp.n_alts + 4), 0,
+                     stream, c, gi, tape, col0);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gitm_vert_set_adjoint_memory(gitm_vert_handle *h, size_t bytes) {
+  if (!h || (bytes != 0 && bytes < ((size_t)64 << 20))) return GITM_VERT_EINVAL;
+  h->adj_budget = bytes;
+  return GITM_VERT_OK;
+}
+
+int gitm_vert_adjoint(gitm_vert_handle *h, const gitm_vert_params *p,
+                      const gitm_vert_columns *c, const gitm_vert_adjoint_fields *gout,
+                      const gitm_vert_adjoint_fields *gin) {
+  if (!h || !gout || !gin) return GITM_VERT_EINVAL;
+  int rc = gv::check_args(p, c, GITM_VERT_AUSM, 1);
+  if (rc) return rc;
+  if (p->use_ion_advection) return GITM_VERT_EUNSUPPORTED;
+  const gitm_vert_field *req[] = {&gout->ndensity_s, &gout->vertical_velocity, &gout->velocity,
+                                  &gout->temperature, &gout->rho, &gin->ndensity_s,
+                                  &gin->vertical_velocity, &gin->velocity, &gin->temperature,
+                                  &gin->rho};
+  for (const gitm_vert_field *f : req)
+    if (!f->ptr) return GITM_VERT_EINVAL;
+  const int waves = gv::auto_waves(c->n_alts);
+  if (waves > 2) return GITM_VERT_EUNSUPPORTED;  // the taped LDS holds up to 124 levels
+  GV_HIP(hipSetDevice(h->device));
+  const int L = c->n_alts + 4;
+  const long long ncols = (long long)c->n_lons * c->n_lats * c->n_blocks;
+  const int nmax = adjoint_nmax(c->n_alts);
+  const size_t lane_bytes =
+      (size_t)nmax * kNodeBytes + (size_t)(kAdjMaxPhases + 1) * sizeof(int);
+  const size_t budget = h->adj_budget ? h->adj_budget : default_adjoint_budget(h->device);
+  long long chunk = (long long)(budget / (lane_bytes * (size_t)L));
+  chunk = std::min(chunk, (long long)(0x7fffffffLL / ((long long)nmax * L)));
+  chunk = std::min(chunk, ncols);
+  if (chunk < 1) return GITM_VERT_ENOMEM;
+  const size_t stride = (size_t)chunk * L;
+  const size_t need = lane_bytes * stride + 256;
+  if (h->tape_bytes < need) {
+    GV_HIP(hipStreamSynchronize(h->stream));
+    if (h->tape) (void)hipFree(h->tape);
+    h->tape = nullptr;
+    h->tape_bytes = 0;
+    if (hipMalloc(&h->tape, need) != hipSuccess) return GITM_VERT_ENOMEM;
+    h->tape_bytes = need;
+  }
+  if (!h->d_overflow) {
+    if (hipMalloc(&h->d_overflow, sizeof(int)) != hipSuccess) return GITM_VERT_ENOMEM;
+  }
+  hipLaunchKernelGGL(set_int, dim3(1), dim3(1), 0, h->stream, h->d_overflow, 0);
+  char *base = static_cast<char *>(h->tape);
+  const size_t nn = (size_t)nmax * stride;
+  gv::AdTape tape;
+  tape.d1 = reinterpret_cast<double *>(base);
+  tape.d2 = tape.d1 + nn;
+  tape.adj = tape.d2 + nn;
+  tape.p1 = reinterpret_cast<int *>(tape.adj + nn);
+  tape.p2 = tape.p1 + nn;
+  tape.phase = tape.p2 + nn;
+  tape.nphase = tape.phase + (size_t)kAdjMaxPhases * stride;
+  tape.stride = (int)stride;
+  tape.nmax = nmax;
+  tape.maxph = kAdjMaxPhases;
+  tape.overflow = h->d_overflow;
+  gv::KParams kp = gv::make_kparams(p, c, GITM_VERT_AUSM, 1, h->d_err);
+  kp.scr_fields = 0;  // the carries live in LDS (waves <= 2)
+  kp.scratch = nullptr;
+  gv::AdjFields go, gi;
+  go.nds = gout->ndensity_s;
+  go.vvel = gout->vertical_velocity;
+  go.vel = gout->velocity;
+  go.temp = gout->temperature;
+  go.rho = gout->rho;
+  gi.nds = gin->ndensity_s;
+  gi.vvel = gin->vertical_velocity;
+  gi.vel = gin->velocity;
+  gi.temp = gin->temperature;
+  gi.rho = gin->rho;
+  for (long long c0 = 0; c0 < ncols; c0 += chunk) {
+    const int n = (int)std::min(chunk, ncols - c0);
+    if (waves == 1)
+      launch_adjoint<64>(n, (int)c0, h->stream, kp, *c, go, gi, tape);
+    else
+      launch_adjoint<128>(n, (int)c0, h->stream, kp, *c, go, gi, tape);
+    GV_HIP(hipGetLastError());
+  }
+  int overflow = 0;
+  GV_HIP(hipMemcpyAsync(&overflow, h->d_overflow, sizeof(int), hipMemcpyDeviceToHost,
+                        h->stream));
+  GV_HIP(hipStreamSynchronize(h->stream));
+  if (overflow) return GITM_VERT_ETAPE;
+  return gitm_vert_get_error(h, nullptr);
+}
+
+}  // extern "C"

@@ -44,6 +44,7 @@ struct AdTape {
   int *p1, *p2;     // parent nodes (-1: none)
   double *adj;      // adjoints, zeroed when the node is made
   int *phase;       // [phase * stride + lane] node count at the barrier
+  int *nphase;      // [lane] barriers of the lane's forward pass
   int stride;       // lanes in the launch
   int nmax;         // nodes per lane
   int maxph;        // barriers per lane
@@ -168,20 +169,19 @@ struct AdjFields {
   gitm_vert_field nds, vvel, vel, temp, rho;
 };
 
-// Adjoint of one AUSM step of column `col` of io (no ions): the forward
-// step is recomputed from io's state with taping, the adjoints `gout` of the
-// step's outputs (NDensityS, VerticalVelocity, Velocity, Temperature, Rho
-// after the step) are seeded, the tape is swept, and the adjoints of the
-// step's inputs (the same fields before the step) are written to `gin`.
+// Adjoint of one AUSM step of column `col` of io (no ions), forward part:
+// the step is recomputed from io's state with taping and the adjoints
+// `gout` of the step's outputs (NDensityS, VerticalVelocity, Velocity,
+// Temperature, Rho after the step) are seeded.  The 17 state inputs of the
+// lane's level are its first nodes (node q = input q of adjoint_store).
 // The prep inputs (TempUnit, Gamma, MeanMajorMass, cMax, eddy coefficient,
 // geometry) are held fixed.  Ctx: lane(), sync() (which closes a phase),
 // sync_or(), report(), device_math() == false, diff_powers(), end_forward()
-// (closes the last phase), sweep() (the backward pass) and add(node, value)
-// (an atomic adjoint add).
+// (closes the last phase) and add(node, value) (an atomic adjoint add).
 template <int NT, class Ctx>
-GV_FN void adjoint_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io,
-                          const AdjFields &gout, const AdjFields &gin, int col,
-                          adm::ad *lds, double *scratch, double dt, int err_col) {
+GV_FN void adjoint_forward(Ctx &cx, const KParams &kp, const gitm_vert_columns &io,
+                           const AdjFields &gout, int col, adm::ad *lds, double *scratch,
+                           double dt, int err_col) {
   using adm::ad;
   Column<NT, Ctx, false> c(cx, kp, lds, err_col, dt);
   c.scr = reinterpret_cast<ad *>(scratch) + (long long)col * kp.scr_fields * (kp.n_alts + 4);
@@ -267,21 +267,40 @@ GV_FN void adjoint_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &i
     for (int q = 0; q < 17; q++)
       if (out[q].i >= 0 && g[q] != 0.0) cx.add(out[q].i, g[q]);
   }
-  cx.sweep();
-  {
-    const long long bn = col_base(gin.nds, ci, cj, cb);
-    const long long bw = col_base(gin.vvel, ci, cj, cb);
-    const long long bv = col_base(gin.vel, ci, cj, cb);
-    double g[17];
-    for (int q = 0; q < 17; q++) g[q] = cx.adjoint(in[q].i);
-    for (int s = 0; s < NSP; s++) {
-      *fld(gin.nds, bn, k, s) = g[s];
-      *fld(gin.vvel, bw, k, s) = g[6 + s];
-    }
-    for (int d = 0; d < 3; d++) *fld(gin.vel, bv, k, d) = g[12 + d];
-    *fld(gin.temp, col_base(gin.temp, ci, cj, cb), k, 0) = g[15];
-    *fld(gin.rho, col_base(gin.rho, ci, cj, cb), k, 0) = g[16];
+}
+
+// After the backward sweep: the adjoints of the lane's 17 inputs (its nodes
+// 0..16; zero when the forward pass stopped at a non-positive density) to
+// `gin`.  `adjoint(id)` reads a swept adjoint.
+template <class Load>
+GV_FN void adjoint_store(const AdTape &t, int gid, const gitm_vert_columns &io,
+                         const AdjFields &gin, int col, int k, Load adjoint) {
+  const int nlb = io.n_lons * io.n_lats;
+  const int ci = col % io.n_lons, cj = (col / io.n_lons) % io.n_lats, cb = col / nlb;
+  const long long bn = col_base(gin.nds, ci, cj, cb);
+  const long long bw = col_base(gin.vvel, ci, cj, cb);
+  const long long bv = col_base(gin.vel, ci, cj, cb);
+  double g[17];
+  for (int q = 0; q < 17; q++) g[q] = adjoint((long long)q * t.stride + gid);
+  for (int s = 0; s < NSP; s++) {
+    *fld(gin.nds, bn, k, s) = g[s];
+    *fld(gin.vvel, bw, k, s) = g[6 + s];
   }
+  for (int d = 0; d < 3; d++) *fld(gin.vel, bv, k, d) = g[12 + d];
+  *fld(gin.temp, col_base(gin.temp, ci, cj, cb), k, 0) = g[15];
+  *fld(gin.rho, col_base(gin.rho, ci, cj, cb), k, 0) = g[16];
+}
+
+// Forward pass, sweep and store in one (the host build): Ctx also provides
+// sweep() and adjoint(id).
+template <int NT, class Ctx>
+GV_FN void adjoint_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io,
+                          const AdjFields &gout, const AdjFields &gin, int col,
+                          adm::ad *lds, double *scratch, double dt, int err_col) {
+  adjoint_forward<NT>(cx, kp, io, gout, col, lds, scratch, dt, err_col);
+  cx.sweep();
+  adjoint_store(ad_tape(), ad_gid(), io, gin, col, cx.lane() - 1,
+                [&](long long id) { return cx.adjoint((int)id); });
 }
 
 }  // namespace gv

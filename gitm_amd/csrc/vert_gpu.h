@@ -1,0 +1,180 @@
+// vert_gpu.h -- pieces shared by the library's HIP translation units
+// (gitm_vert.hip: single-grid step, prep and C ABI; gitm_vert_batch.hip:
+// ensemble batches; gitm_vert_adjoint.hip: the adjoint): the kernel context
+// of the plain build, the batch member table, the column prep, small
+// kernels, the handle and argument helpers.  Internal; not installed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "vert_params.h"
+#include "vert_prep.h"
+
+namespace {
+
+struct GpuCtx {
+  using real = double;
+  unsigned long long *err;
+  __device__ int lane() const { return (int)threadIdx.x; }
+  __device__ void sync() const { __syncthreads(); }
+  __device__ bool sync_or(bool p) const { return __syncthreads_or(p ? 1 : 0) != 0; }
+  __device__ void report(unsigned long long key) const { atomicMin(err, key); }
+  // Temp**DiffExp as exp(DiffExp*ln Temp): one log and six exp in place of
+  // fifteen pow (a pow costs five times an exp here); within a few ulp of pow.
+  __device__ static constexpr bool device_math() { return true; }
+  __device__ void diff_powers(double temp, double sign, double (&out)[6]) const {
+    const double lt = log(temp);
+#pragma unroll
+    for (int u = 0; u < 6; u++) out[u] = exp((sign * gv::DEXU[u]) * lt);
+  }
+};
+
+// One member of a batch as the kernels see it.  A group of members with the
+// same nAlts runs as one launch; workgroup b of the launch belongs to the
+// member with the largest first_block <= b.
+struct BatchMember {
+  gitm_vert_columns io;
+  double *scratch;  // RK4 carries of the member's columns
+  int first_block;  // the member's first workgroup in its group's launch
+  int member;       // index in the caller's member list (dt_device)
+  int col_offset;   // first column of the member in the batch-wide numbering
+  int reserved;
+};
+
+__device__ __forceinline__ int find_member(const BatchMember *__restrict__ m, int n,
+                                           int block) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (m[mid].first_block <= block)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Column prep, one workgroup per column and one lane per level; lane 0 folds
+// the column's CFL limit into *dt_bits (non-negative doubles order as their
+// bit patterns, so an integer atomic min is a double min).
+template <int NT>
+__device__ __forceinline__ void prep_column(const gv::PrepParams &pp,
+                                            const gitm_vert_columns &io,
+                                            const gitm_vert_field &pf,
+                                            const gitm_vert_field &cpf, int col,
+                                            unsigned long long *dt_bits) {
+  __shared__ double red[2][NT / 64];
+  const int j = (int)threadIdx.x;
+  const int nlb = io.n_lons * io.n_lats;
+  const int ci = col % io.n_lons, cj = (col / io.n_lons) % io.n_lats, cb = col / nlb;
+  double rate = 0.0, irate = 0.0;
+  if (j < pp.n_alts + 4) gv::prep_level(pp, io, pf, cpf, ci, cj, cb, j - 1, rate, irate);
+  // maxval over the column's levels 1..nAlts (calc_timestep.f90:122,132)
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    rate = fmax(rate, __shfl_xor(rate, o));
+    irate = fmax(irate, __shfl_xor(irate, o));
+  }
+  if ((j & 63) == 0) {
+    red[0][j >> 6] = rate;
+    red[1][j >> 6] = irate;
+  }
+  __syncthreads();
+  if (j == 0) {
+    double r = red[0][0], ri = red[1][0];
+    for (int w = 1; w < (int)((blockDim.x + 63) / 64); w++) {
+      r = fmax(r, red[0][w]);
+      ri = fmax(ri, red[1][w]);
+    }
+    double dt = pp.cfl / r;
+    if (pp.ions) dt = fmin(dt, pp.cfl / ri);
+    atomicMin(dt_bits, (unsigned long long)__double_as_longlong(dt));
+  }
+}
+
+__global__ void reset_error_word(unsigned long long *e) { *e = ~0ULL; }
+__global__ void set_word(unsigned long long *w, unsigned long long v) { *w = v; }
+__global__ void set_int(int *w, int v) { *w = v; }
+__global__ void fill_words(unsigned long long *w, int n, unsigned long long v) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) w[i] = v;
+}
+
+}  // namespace
+
+// Arguments of the column prep (gitm_vert_prepare and the batch form).
+static inline int check_prep(const gitm_vert_prep_params *p, const gitm_vert_columns *c) {
+  if (!p || !c) return GITM_VERT_EINVAL;
+  if (c->n_alts < 10 || c->n_alts > GITM_VERT_MAX_NALTS) return GITM_VERT_EINVAL;
+  if (c->n_lons < 1 || c->n_lats < 1 || c->n_blocks < 1) return GITM_VERT_EINVAL;
+  if ((long long)c->n_lons * c->n_lats * c->n_blocks > 0x7fffffffLL) return GITM_VERT_EINVAL;
+  const gitm_vert_field *req[] = {&c->ndensity_s, &c->temperature, &c->rho,
+                                  &c->vertical_velocity, &c->dalt, &c->gamma,
+                                  &c->mean_major_mass, &c->temp_unit, &c->eddy, &c->cmax};
+  for (const gitm_vert_field *f : req)
+    if (!f->ptr) return GITM_VERT_EINVAL;
+  if (p->use_ion_advection && !c->ivelocity.ptr) return GITM_VERT_EINVAL;
+  if (!(p->cfl > 0.0) || !(p->fixed_dt > 0.0)) return GITM_VERT_EINVAL;
+  if (c->cmax.first_level < -1 || c->cmax.first_level > 0) return GITM_VERT_EINVAL;
+  for (int s = 0; s < GITM_VERT_NSPECIES; s++)
+    if (!(p->mass[s] > 0.0) || p->vibration[s] == 2.0) return GITM_VERT_EINVAL;
+  return GITM_VERT_OK;
+}
+
+static inline gv::PrepParams make_prep(const gitm_vert_prep_params *p, int n_alts) {
+  gv::PrepParams pp;
+  for (int s = 0; s < gv::NSP; s++) {
+    pp.mass[s] = p->mass[s];
+    pp.vib2[s] = p->vibration[s] - 2;
+    pp.kb_over_m[s] = gv::KB / p->mass[s];
+  }
+  pp.cfl = p->cfl;
+  pp.eddy_coef = p->eddy_coef;
+  pp.eddy_p0 = p->eddy_p0;
+  pp.eddy_p1 = p->eddy_p1;
+  pp.n_alts = n_alts;
+  pp.ions = p->use_ion_advection != 0;
+  return pp;
+}
+
+// min(FixedDt, DtEnd, 1e32) before the columns' limits (calc_timestep.f90
+// :105,142,147), as the bits the device MIN starts from.
+static inline unsigned long long prep_dt0_bits(const gitm_vert_prep_params *p) {
+  double d0 = 1.0e32;
+  if (p->dt_end < d0) d0 = p->dt_end;
+  if (p->fixed_dt < d0) d0 = p->fixed_dt;
+  unsigned long long bits;
+  memcpy(&bits, &d0, sizeof(bits));
+  return bits;
+}
+
+struct gitm_vert_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  unsigned long long *d_err = nullptr;
+  int waves = 0;
+  // RK4 carries of the columns in flight (grown on demand, reused)
+  double *scratch = nullptr;
+  size_t scratch_bytes = 0;
+  // device staging of the host-array entry point
+  std::vector<void *> stage;
+  std::vector<size_t> stage_bytes;
+  // adjoint tape (grown on demand up to adj_budget bytes, reused)
+  void *tape = nullptr;
+  size_t tape_bytes = 0;
+  size_t adj_budget = 0;  // 0: default_adjoint_budget()
+  int *d_overflow = nullptr;
+};
+
+#define GV_HIP(call)                         \
+  do {                                       \
+    hipError_t e_ = (call);                  \
+    if (e_ != hipSuccess) return GITM_VERT_EHIP; \
+  } while (0)
+

@@ -279,8 +279,9 @@ int gitm_vert_adjoint(gitm_vert_handle *h, const gitm_vert_params *p,
                       const gitm_vert_columns *c, const gitm_vert_adjoint_fields *gout,
                       const gitm_vert_adjoint_fields *gin);
 
-/* Device memory the adjoint's tape may use (default 16 GiB; about 22 MB
- * per column at nAlts = 50); larger grids run in chunks of columns. */
+/* Device memory the adjoint's tape may use (0: the default, 40% of the
+ * device's memory up to 96 GiB).  A column takes about 22 MB at nAlts = 50;
+ * larger grids run in chunks of columns. */
 int gitm_vert_set_adjoint_memory(gitm_vert_handle *h, size_t bytes);
 
 /* Member and member column of a batch-wide column number. */

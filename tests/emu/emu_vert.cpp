@@ -180,10 +180,11 @@ extern "C" int gv_emu_adjoint(const gitm_vert_params *p, const gitm_vert_columns
   std::vector<gv::adm::ad> scratch((size_t)(c1 > 0 ? c1 : 1) * kp.scr_fields * L + 1);
   AdShared sh(L);
   std::vector<double> d1((size_t)nmax * L), d2((size_t)nmax * L), adj((size_t)nmax * L);
-  std::vector<int> p1((size_t)nmax * L), p2((size_t)nmax * L), phase((size_t)maxph * L);
+  std::vector<int> p1((size_t)nmax * L), p2((size_t)nmax * L), phase((size_t)maxph * L),
+      nphase(L);
   int overflow = 0;
   sh.tape = gv::AdTape{d1.data(), d2.data(), p1.data(), p2.data(), adj.data(), phase.data(),
-                       L, nmax, maxph, &overflow};
+                       nphase.data(), L, nmax, maxph, &overflow};
   std::vector<gv::adm::ad> lds(gv::lds_doubles(c->n_alts, false, gv::carry_in_lds(waves)));
   std::atomic<int> max_nodes{0}, max_phases{0};
   std::vector<std::thread> th;

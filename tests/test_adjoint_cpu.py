@@ -80,13 +80,15 @@ def test_adjoint_matches_oracle_differences(case, field):
 
 
 def test_adjoint_is_linear_in_the_seed(case):
-    """The adjoint map is linear: seeds w and 3w give 3x the adjoints, and a
+    """The adjoint map is linear: seeds w and 3w give 3x the adjoints (to
+    rounding: the order of the atomic adds varies from run to run), and a
     zero seed gives zero."""
     from tests.emu import emu
     cols, p, w, gin, _, _ = case
     g3, _, _ = emu.adjoint(cols, p, {k: 3.0 * w[k] for k in STATE})
     for k in STATE:
-        np.testing.assert_allclose(g3[k], 3.0 * gin[k], rtol=1e-13, atol=0)
+        np.testing.assert_allclose(g3[k], 3.0 * gin[k], rtol=1e-12,
+                                   atol=1e-14 * float(np.max(np.abs(3.0 * gin[k]))))
     g0, _, _ = emu.adjoint(cols, p, {k: np.zeros_like(w[k]) for k in STATE})
     assert all(np.all(g0[k] == 0.0) for k in STATE)
 

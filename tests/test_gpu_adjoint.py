@@ -63,7 +63,7 @@ def test_adjoint_chunks(solver):
     try:
         b = adjoint.adjoint_step(solver, d, go, p)
     finally:
-        adjoint.set_adjoint_memory(solver, 16 << 30)
+        adjoint.set_adjoint_memory(solver, 0)  # the default again
     for k in STATE:
         scale = float(a[k].abs().max())
         assert float((a[k] - b[k]).abs().max()) <= 1e-13 * scale, k
