@@ -17,6 +17,12 @@ the measured HBM bytes per launch when a counter summary is present under
 profiles/.  ``cpu_baseline`` times the CPU restatement of the same step
 (oracle/, bit-identical to the reference Fortran) on a bounded sample of the
 same grid on this host's cores.
+
+Two further figures ride on the same line: ``ensemble`` (64 members with
+their own nAlts and column counts as one batch, a hipGraph of 20 prep+step
+iterations; ``--no-ensemble`` skips it) and ``adjoint`` with
+``adjoint_forward_ratio`` (a 10-step window on the 5 deg grid, backward pass
+with recompute over forward pass; ``--no-adjoint`` skips it).
 """
 from __future__ import annotations
 
@@ -203,6 +209,49 @@ def bench_ensemble(torch, solver, params, steps: int = 20, seed: int = 4) -> dic
             "state_finite_and_positive": ok}
 
 
+def bench_adjoint(torch, solver, steps: int = 10, seed: int = 5) -> dict:
+    """Differentiation workload: the 5 deg x 5 deg grid (2592 columns x 50
+    levels), a window of ``steps`` x (AUSM step, Gaussian heating source
+    peaked at 150 km, 1e-5 W/kg), loss = mean mass density at 400 km.  The
+    forward pass (plain HIP step, one checkpoint per step) and the backward
+    pass (per step: the taped recompute and the sweep of gitm_vert_adjoint)
+    are timed after one untimed pass.  The ratio is backward / forward."""
+    from gitm_amd import adjoint, synth, vertical
+
+    spec = synth.CONFIGS["5deg_2592x50"]
+    cols = synth.make_columns(spec, seed=seed)
+    d = vertical.to_device(cols)
+    for k in ("press", "cp"):
+        d[k] = torch.zeros_like(d["temp"])
+    solver.prepare(d, synth.PrepParams())
+    fixed = {k: d[k] for k in adjoint.FIXED}
+    fixed["cp"] = d["cp"]
+    heat = torch.from_numpy(adjoint.heating_profile(cols["alt"])).cuda()
+    w = torch.from_numpy(adjoint.level_weights(cols["alt"], 400e3)).cuda()
+    p = synth.StepParams()
+    fwd_ms = bwd_ms = 0.0
+    loss = 0.0
+    for rep in range(2):
+        state = {k: d[k].clone().requires_grad_(True) for k in adjoint.STATE}
+        q = heat.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = adjoint.window(state, q, solver, fixed, p, steps)
+        J = (out["rho"] * w).sum() / w.shape[0]
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        J.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        fwd_ms, bwd_ms, loss = (t1 - t0) * 1e3, (t2 - t1) * 1e3, float(J.detach())
+    ok = bool(torch.isfinite(state["temp"].grad).all()) and bool(torch.isfinite(q.grad).all())
+    return {"workload": f"5deg_2592x50: 2592 columns x 50 levels, {steps} x (AUSM step + "
+                        "heating source), loss = mean mass density at 400 km",
+            "forward_ms": fwd_ms, "backward_ms": bwd_ms, "ratio": bwd_ms / fwd_ms,
+            "forward_ms_per_step": fwd_ms / steps, "backward_ms_per_step": bwd_ms / steps,
+            "loss": loss, "gradients_finite": ok}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,6 +263,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 64-member ragged-ensemble figure")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint/forward ratio")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -286,6 +337,12 @@ def main() -> int:
         ens = bench_ensemble(torch, solver, params)
         torch.cuda.empty_cache()
 
+    adj = None
+    if not args.no_adjoint and args.solver == "ausm":
+        print("bench: adjoint window", file=sys.stderr, flush=True)
+        adj = bench_adjoint(torch, solver)
+        torch.cuda.empty_cache()
+
     cpu = None
     if args.cpu_seconds > 0:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -330,7 +387,8 @@ def main() -> int:
                  / (prep["launch_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
                  "cfl_dt": prep["dt"]},
         "ensemble": ens,
-        "adjoint_forward_ratio": None,
+        "adjoint_forward_ratio": adj["ratio"] if adj else None,
+        "adjoint": adj,
     }
     print(json.dumps(line), flush=True)
     return 0
