@@ -21,7 +21,7 @@ ERRORS = {
     2: "singular neutral-friction matrix",
     3: "negative ion density too close to the upper boundary",
     4: "invalid arguments",
-    5: "unsupported option (UseMSIS, tides, BC perturbation, UseDamping)",
+    5: "unsupported option (tides, BC perturbation, UseDamping)",
     6: "HIP runtime error",
     7: "out of device memory",
     8: "adjoint tape too small for the column",
@@ -39,7 +39,7 @@ class Field(ctypes.Structure):
 FIELD_NAMES = ("ndensity_s", "vertical_velocity", "velocity", "temperature", "rho",
                "log_ns", "ndensity", "temp_unit", "mean_major_mass", "gamma",
                "cmax", "eddy", "altitude", "gravity", "inv_radial_distance", "dalt",
-               "cos_latitude", "idensity_s", "ivelocity", "sza", "mlat")
+               "cos_latitude", "idensity_s", "ivelocity", "sza", "mlat", "msis_bc")
 
 
 class Columns(ctypes.Structure):
@@ -93,6 +93,8 @@ CANON = {
     "ions": ("idensity_s", NIONS), "ivel": ("ivelocity", 3),
 }
 PER_COLUMN = {"coslat": "cos_latitude", "sza": "sza", "mlat": "mlat"}
+# per-column fields with components ([column][component])
+PER_COLUMN_MULTI = {"msis": ("msis_bc", 9)}
 
 
 def column_field(ptr: int, ncomp: int, nlev: int) -> Field:
@@ -118,6 +120,9 @@ def columns_struct(ptrs: dict, n_cols: int, n_alts: int) -> Columns:
             setattr(c, fname, column_field(addr, ncomp, L))
         elif name in PER_COLUMN:
             setattr(c, PER_COLUMN[name], per_column_field(addr))
+        elif name in PER_COLUMN_MULTI:
+            fname, ncomp = PER_COLUMN_MULTI[name]
+            setattr(c, fname, Field(addr, ncomp, 0, 0, 0, 1, 0, 0))
         else:
             raise KeyError(name)
     return c

@@ -61,7 +61,7 @@ const char *gitm_vert_strerror(int code) {
     case GITM_VERT_ENEGION: return "negative ion density too close to the upper boundary";
     case GITM_VERT_EINVAL: return "invalid arguments";
     case GITM_VERT_EUNSUPPORTED:
-      return "unsupported option (UseMSIS, tides, BC perturbation, UseDamping)";
+      return "unsupported option (tides, BC perturbation, UseDamping)";
     case GITM_VERT_EHIP: return "HIP runtime error";
     case GITM_VERT_ENOMEM: return "out of device memory";
     case GITM_VERT_ETAPE: return "adjoint tape too small for the column";
@@ -255,6 +255,7 @@ const FieldSpec kFields[] = {
     {&gitm_vert_columns::ivelocity, 3, 2, false, false, true},
     {&gitm_vert_columns::sza, 1, 0, false, true, true},
     {&gitm_vert_columns::mlat, 1, 0, false, true, true},
+    {&gitm_vert_columns::msis_bc, 9, 0, false, true, false},
 };
 
 size_t field_span(const gitm_vert_columns &c, const gitm_vert_field &f, const FieldSpec &fs) {

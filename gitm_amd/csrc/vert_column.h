@@ -99,6 +99,7 @@ struct KParams {
   double log_min_kp, log_max_kp, log_min_ku, log_max_ku;  // log(1e-19), ...
   int n_alts, n_steps, solver;
   int friction, coriolis, ions, improved_ions, nighttime_ion_bcs;
+  int msis;                 // UseMSIS: level-0 ghost values from the caller's MSIS
   int scr_fields;           // fields per level carried in device memory (see carry_in_lds)
   unsigned long long *err;  // first-error word, see error_key()
   double *scratch;          // RK4 carries: n_cols x scr_fields x (nAlts+4) doubles
@@ -420,6 +421,10 @@ struct Column {
   R *scr;  // this column's RK4 carries (Y, acc) in device memory
   R centrifugal, coriolis;
   bool plasma = false;  // nighttime plasmasphere column (bc :492-497)
+  // UseMSIS: the caller's msis_bcs values of level 0, on the lanes that use
+  // them: LogNS(0) of species j on lane j < 6; Temp(0) and the east and
+  // north winds on lane 1 (level 0)
+  R msis_ln = 0.0, msis_t = 0.0, msis_ve = 0.0, msis_vn = 0.0;
 
   GV_MEM Column(Ctx &c, const KParams &p, R *l, int column, R step_dt)
       : cx(c), kp(p), lds(l), j(c.lane()), k(c.lane() - 1), nA(p.n_alts),
@@ -666,6 +671,15 @@ struct Column {
   // BCs never change.
   // -------------------------------------------------------------------------
   GV_MEM void bcs() {
+    // UseMSIS (set_vertical_bcs.Earth.f90:113-137): Temp(0) and the horizontal
+    // winds of level 0 take the MSIS values before anything reads them; the
+    // MSIS LogNS(0) replace the non-photochemical densities below, after the
+    // hydrostatic fill of level -1 has used the entry values (NS, :93)
+    if (kp.msis && j == 1) {
+      U[F_T] = msis_t;
+      U[F_VE] = msis_ve;
+      U[F_VN] = msis_vn;
+    }
     // stage ln(6), w(6), T, vE, vN, vU
 #pragma unroll
     for (int f = 0; f < 12; f++) xb(f, j) = U[F_LN + f];
@@ -692,9 +706,9 @@ struct Column {
         if (d <= 0.0) nm1 = n0;
         if (sp == S_NO && nm1 < n0 - 0.333) nm1 = n0 - 0.333;
       } else {
-        n0 = ln0;
+        n0 = kp.msis ? msis_ln : ln0;  // lane sp holds species sp's value
         if (sp == S_O) {
-          nm1 = ln0;  // zero gradient below the boundary (:219-222)
+          nm1 = n0;  // zero gradient below the boundary (:219-222)
         } else {
           const R t0 = xb(12, 1), tm1 = t0;  // Temp(-1) = Temp(0)
           const R MeanGravity = -sc(SC_MG0);
@@ -786,8 +800,21 @@ struct Column {
       for (int sp = 0; sp < NSP; sp++)
         vu = vu + ns[sp] * kp.mass[sp] * sc(SC_BOT + 6 * sp + 2 + o) / er;
       U[F_VU] = vu;
-      U[F_VE] = 0.0;
-      U[F_VN] = 0.0;
+      if (!kp.msis) {  // :146-148
+        U[F_VE] = 0.0;
+        U[F_VN] = 0.0;
+      } else if (j == 0) {
+        // Vel(-1) east and north extrapolated from levels 0..4 (:364-375)
+        const R *CB = &sc(SC_CB);
+        const R dF0 = st(A_DF, 1);
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+          const R v0 = xb(13 + d, 1), v1 = xb(13 + d, 2), v2 = xb(13 + d, 3),
+                  v3 = xb(13 + d, 4), v4 = xb(13 + d, 5);
+          const R dv = v0 * CB[0] + v1 * CB[1] + v2 * CB[2] + v3 * CB[3] + v4 * CB[4];
+          U[F_VE + d] = v0 - dF0 * dv;
+        }
+      }
       if (j == 0) U[F_T] = xb(12, 1);
     } else if (j == nA + 2 || j == nA + 3) {
       // levels nAlts+1, nAlts+2 (:420-463, :598-604)
@@ -1572,6 +1599,17 @@ GV_FN void run_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io, i
       const double sza = io.sza.ptr[col_base(io.sza, ci, cj, cb)];
       const double mlat = io.mlat.ptr[col_base(io.mlat, ci, cj, cb)];
       c.plasma = sza > PI / 2 && ((mlat > 30.0 && mlat < 70.0) || (mlat > -60.0 && mlat < -30.0));
+    }
+  }
+  if (kp.msis) {
+    // the caller's MSIS values of the column's level 0 (msis_bcs, :113-137)
+    const gitm_vert_field &f = io.msis_bc;
+    const double *m = f.ptr + col_base(f, ci, cj, cb);
+    if (c.j < NSP) c.msis_ln = m[(long long)c.j * f.s_comp];
+    if (c.j == 1) {
+      c.msis_t = m[6 * f.s_comp];
+      c.msis_ve = m[7 * f.s_comp];
+      c.msis_vn = m[8 * f.s_comp];
     }
   }
   c.setup(alt, grav, invr, dalt, mmm, gam, eddy, cmax, tu, coslat);

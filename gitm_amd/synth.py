@@ -283,6 +283,7 @@ class StepParams:
     use_ion_advection: bool = False    # reference default T (ModInputs.f90:311)
     use_improved_ion_advection: bool = True  # ModInputs.f90:259
     use_nighttime_ion_bcs: bool = True       # ModInputs.f90:260
+    use_msis: bool = False             # ModInputs.f90:190 (needs cols["msis"])
     min_tec: float = 2.0               # ModInputs.f90:261
     vert_tau: float = 1.0e9            # ModEarth.f90:325
 
@@ -334,3 +335,18 @@ def member_columns(member: Member, ncols: int | None = None, ions: bool = False)
     if ncols is not None and ncols < spec.ncols:
         cols = {k: v[:ncols] for k, v in cols.items()}
     return cols
+
+
+def msis_bc_values(cols: dict, seed: int = 1, dlog: float = 0.05, dtemp: float = 0.02,
+                   wind: float = 20.0) -> np.ndarray:
+    """Stand-in values of what msis_bcs returns for each column's level 0
+    ([column][9]: LogNS(0,1:6), Temp(0) in K, east and north wind), drawn
+    around the column's own level 0 (seeded): log densities +-dlog, the
+    temperature within +-dtemp relative, winds N(0, wind) m/s."""
+    nc = cols["temp"].shape[0]
+    out = np.empty((nc, 9))
+    z = normal(seed, 301, (nc, 9))
+    out[:, :6] = np.log(cols["nds"][:, :, 1]) + dlog * z[:, :6]
+    out[:, 6] = cols["temp"][:, 1] * cols["tempunit"][:, 1] * (1.0 + dtemp * z[:, 6])
+    out[:, 7:] = wind * z[:, 7:]
+    return out

@@ -20,6 +20,7 @@ from __future__ import annotations
 import ctypes
 import dataclasses
 import os
+import weakref
 
 import numpy as np
 
@@ -33,6 +34,7 @@ OUTPUTS = ("logns", "nden")
 INPUTS = ("tempunit", "mmm", "gamma", "cmax", "eddy", "alt", "grav", "invr", "dalt",
           "coslat")
 ION_FIELDS = ("ions", "ivel", "sza", "mlat")  # read/written with UseIonAdvection
+MSIS_FIELDS = ("msis",)  # UseMSIS: level-0 ghost values per column
 
 _lib = None
 
@@ -110,6 +112,7 @@ def make_params(sp: StepParams | None = None) -> _abi.Params:
     p.use_ion_advection = int(sp.use_ion_advection)
     p.use_improved_ion_advection = int(sp.use_improved_ion_advection)
     p.use_nighttime_ion_bcs = int(sp.use_nighttime_ion_bcs)
+    p.use_msis = int(getattr(sp, "use_msis", False))
     return p
 
 
@@ -147,7 +150,8 @@ def _device_columns(cols: dict, extra: tuple = ()):
 
     ptrs = {}
     for k, t in cols.items():
-        if k not in _abi.CANON and k not in _abi.PER_COLUMN and k not in extra:
+        if (k not in _abi.CANON and k not in _abi.PER_COLUMN and k not in _abi.PER_COLUMN_MULTI
+                and k not in extra):
             continue
         if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
             raise ValueError(f"{k}: need a contiguous float64 device tensor")
@@ -161,6 +165,7 @@ class VerticalSolver:
     """A handle on one HIP device (``gitm_vert_create``)."""
 
     def __init__(self, device: int = -1, waves_per_column: int = 0):
+        self._batches = weakref.WeakSet()  # destroyed before the handle
         self._h = ctypes.c_void_p()
         rc = lib().gitm_vert_create(ctypes.byref(self._h), device)
         if rc:
@@ -169,6 +174,8 @@ class VerticalSolver:
             self.set_tile(waves_per_column)
 
     def close(self) -> None:
+        for b in list(self._batches):
+            b.close()
         if self._h:
             lib().gitm_vert_destroy(self._h)
             self._h = ctypes.c_void_p()
@@ -245,7 +252,8 @@ class VerticalSolver:
         """Advance host columns in place (numpy FP64, C-contiguous)."""
         ptrs = {}
         for k, a in cols.items():
-            if k not in _abi.CANON and k not in _abi.PER_COLUMN:
+            if (k not in _abi.CANON and k not in _abi.PER_COLUMN
+                    and k not in _abi.PER_COLUMN_MULTI):
                 continue
             if a.dtype != np.float64 or not a.flags.c_contiguous:
                 raise ValueError(f"{k}: need a C-contiguous float64 array")
@@ -285,6 +293,7 @@ class VerticalBatch:
                                           ctypes.byref(self._b))
         if rc:
             raise VerticalStepError(rc)
+        solver._batches.add(self)
 
     def close(self) -> None:
         if self._b:
@@ -403,7 +412,7 @@ def to_device(cols: dict, device="cuda") -> dict:
     import torch
 
     out = {}
-    for k in STATE + INPUTS + ION_FIELDS:
+    for k in STATE + INPUTS + ION_FIELDS + MSIS_FIELDS:
         if k in cols:
             out[k] = torch.from_numpy(np.ascontiguousarray(cols[k],
                                                           dtype=np.float64)).to(device)

@@ -48,8 +48,8 @@ enum {
                                  (set_vertical_bcs.Earth.f90:617-620)      */
   GITM_VERT_EINVAL = 4,       /* bad arguments                             */
   GITM_VERT_EUNSUPPORTED = 5, /* a flag this library does not implement
-                                 (UseMSIS, tides, BC perturbation,
-                                 UseDamping): run the Fortran path         */
+                                 (tides, BC perturbation, UseDamping):
+                                 run the Fortran path                      */
   GITM_VERT_EHIP = 6,         /* HIP runtime error                         */
   GITM_VERT_ENOMEM = 7,
   GITM_VERT_ETAPE = 8         /* adjoint tape too small for the column     */
@@ -98,6 +98,13 @@ typedef struct gitm_vert_columns {
   gitm_vert_field ivelocity;         /* IVelocity(.., 1:3, ..)  read        */
   gitm_vert_field sza;               /* SZA(iLon,iLat,iBlock)               */
   gitm_vert_field mlat;              /* MLatitude(iLon,iLat,nAlts,iBlock)   */
+  /* UseMSIS only: per column, what msis_bcs (init_msis.Earth.f90:367-494)
+   * returns for level 0: components 0..5 LogNS(0,1:6) (ln m^-3), 6 Temp(0)
+   * in K, 7 and 8 the east and north winds in m/s; s_lev unused.  Only
+   * these enter: the BCs overwrite every level -1 value, and the MSIS
+   * LogNS(0) of N(4S) and NO, by extrapolation
+   * (set_vertical_bcs.Earth.f90:113-137,198-272,364-375). */
+  gitm_vert_field msis_bc;
 } gitm_vert_columns;
 
 /* Run parameters (ModInputs / ModPlanet values the path reads). */
@@ -115,8 +122,8 @@ typedef struct gitm_vert_params {
   int32_t use_ion_advection;          /* :311                               */
   int32_t use_improved_ion_advection; /* :259                               */
   int32_t use_nighttime_ion_bcs;      /* :260                               */
+  int32_t use_msis;                   /* :190; needs c->msis_bc          */
   /* Must be 0 (otherwise GITM_VERT_EUNSUPPORTED): */
-  int32_t use_msis;                   /* :190                               */
   int32_t use_gswm_tides, use_waccm_tides; /* :195-196                      */
   int32_t use_bc_perturbation;        /* :402                               */
   int32_t during_perturb;             /* :121                               */

@@ -25,7 +25,7 @@ class _Params(ctypes.Structure):
                 ("use_ion_advection", ctypes.c_int),
                 ("use_improved_ion_advection", ctypes.c_int),
                 ("use_nighttime_ion_bcs", ctypes.c_int), ("min_tec", ctypes.c_double),
-                ("vert_tau", ctypes.c_double)]
+                ("vert_tau", ctypes.c_double), ("use_msis", ctypes.c_int)]
 
 
 _P = ctypes.POINTER(ctypes.c_double)
@@ -35,7 +35,8 @@ class _Cols(ctypes.Structure):
     _fields_ = [("n_cols", ctypes.c_int), ("n_alts", ctypes.c_int)] + [
         (n, _P) for n in ("nds", "vvel", "vel", "temp", "rho", "logns", "nden",
                           "tempunit", "mmm", "gamma", "cmax", "eddy", "alt", "grav",
-                          "invr", "dalt", "coslat", "ions", "ivel", "sza", "mlat")]
+                          "invr", "dalt", "coslat", "ions", "ivel", "sza", "mlat",
+                          "msis")]
 
 
 class _Status(ctypes.Structure):
@@ -91,11 +92,15 @@ def advance(cols: dict, params, solver: int = AUSM, n_steps: int = 1,
     if ions:
         for k in ("ivel", "sza", "mlat"):
             keep[k] = np.ascontiguousarray(cols[k], dtype=np.float64)
+    msis = bool(getattr(params, "use_msis", False))
+    if msis:
+        keep["msis"] = np.ascontiguousarray(cols["msis"], dtype=np.float64)
     p = _Params(na, params.dt, params.omega_body, params.max_vert_vel,
                 params.beta_limiter, int(params.use_neutral_friction),
                 int(params.use_coriolis), int(ions),
                 int(params.use_improved_ion_advection),
-                int(params.use_nighttime_ion_bcs), params.min_tec, params.vert_tau)
+                int(params.use_nighttime_ion_bcs), params.min_tec, params.vert_tau,
+                int(msis))
     c = _Cols()
     c.n_cols = nc
     c.n_alts = na

@@ -89,6 +89,7 @@ typedef struct {
   double ivel[3][LM]; /* IVel: mutated in place by set_vertical_bcs */
   double kp[LM], ku[LM]; /* LiouKpS, Ku, vertical_solver_ausm.f90:1320-1352 */
   double mass[NSP];
+  double msis[9]; /* UseMSIS: msis_bcs' LogNS(0,1:6), Temp(0), east, north wind */
 } ov_col;
 
 typedef struct {
@@ -382,10 +383,26 @@ static int set_vertical_bcs(ov_col *c, const ov_params *p, ov_state *s) {
   for (sp = 0; sp < NSP; sp++) /* :93 */
     for (k = -1; k <= nA + 2; k++) NS[sp][X(k)] = exp(s->logns[sp][X(k)]);
 
-  /* Bottom, UseMsis = F: :138-148 */
+  if (p->use_msis) {
+    /* :113-137: msis_bcs at levels -1 and 0.  The caller supplies level 0;
+     * every level -1 value set here is overwritten below (Temp(-1) = Temp(0),
+     * the hydrostatic or extrapolated LogNS(-1), Vel(-1) from levels 0..4,
+     * LogRho(-1) from the species), so level 0's values stand in for both. */
+    for (k = -1; k <= 0; k++) {
+      for (sp = 0; sp < NSP; sp++) s->logns[sp][X(k)] = c->msis[sp];
+      T[X(k)] = c->msis[6];
+      s->vel[iE_][X(k)] = c->msis[7];
+      s->vel[iN_][X(k)] = c->msis[8];
+    }
+  } else {
+    /* Bottom, UseMsis = F: :138-143 */
+    for (k = -1; k <= 0; k++) {
+      s->vel[iE_][X(k)] = 0.0;
+      s->vel[iN_][X(k)] = 0.0;
+    }
+  }
+  /* :145-148 */
   for (k = -1; k <= 0; k++) {
-    s->vel[iE_][X(k)] = 0.0;
-    s->vel[iN_][X(k)] = 0.0;
     s->vel[iU_][X(k)] = 0.0;
     for (sp = 0; sp < NSP; sp++) s->vv[sp][X(k)] = 0.0;
   }
@@ -448,6 +465,14 @@ static int set_vertical_bcs(ov_col *c, const ov_params *p, ov_state *s) {
       s->vel[iU_][X(k)] = s->vel[iU_][X(k)] + NS[sp][X(k)] * c->mass[sp] *
                                                   s->vv[sp][X(k)] /
                                                   exp(s->logrho[X(k)]);
+    if (k == -1 && p->use_msis) { /* :364-375 */
+      for (d = iE_; d <= iN_; d++) {
+        double *v = s->vel[d];
+        double dVV = v[X(k + 1)] * C[0] + v[X(k + 2)] * C[1] + v[X(k + 3)] * C[2] +
+                     v[X(k + 4)] * C[3] + v[X(k + 5)] * C[4];
+        v[X(k)] = v[X(k + 1)] - dF[X(k + 1)] * dVV;
+      }
+    }
   }
 
   for (sp = 0; sp < NSP; sp++) { /* :382-395 */
@@ -1244,6 +1269,8 @@ static void static_column(ov_col *c, const ov_params *p, const ov_columns *cs,
   c->coriolis = 2 * cs->coslat[col] * p->omega_body;
   c->sza = cs->sza ? cs->sza[col] : 0.0;
   c->mlat = cs->mlat ? cs->mlat[col] : 0.0;
+  if (p->use_msis)
+    for (k = 0; k < 9; k++) c->msis[k] = cs->msis[(size_t)col * 9 + k];
   /* :149-151 */
   for (k = 0; k <= nA + 2; k++) c->dAltF[X(k)] = c->altG[X(k)] - c->altG[X(k - 1)];
   c->dAltF[X(-1)] = c->dAltF[X(0)];

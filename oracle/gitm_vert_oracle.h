@@ -47,6 +47,7 @@ typedef struct {
   int use_nighttime_ion_bcs;  /* ModInputs.f90:260                         */
   double min_tec;             /* ModInputs.f90:261                         */
   double vert_tau;            /* VertTau(:) (init_planet, ModEarth.f90:325) */
+  int use_msis;               /* ModInputs.f90:190: level-0 ghosts from msis */
 } ov_params;
 
 typedef struct {
@@ -70,6 +71,9 @@ typedef struct {
   const double *ivel;  /* IVelocity                 [col][3][L]        */
   const double *sza;   /* SZA                       [col]              */
   const double *mlat;  /* MLatitude(:,:,nAlts,:)    [col]              */
+  /* UseMSIS: msis_bcs' level-0 values [col][9]: LogNS(0,1:6), Temp(0) in
+   * K, east and north wind */
+  const double *msis;
 } ov_columns;
 
 typedef struct {

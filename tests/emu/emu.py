@@ -54,6 +54,7 @@ def params_from(sp) -> _abi.Params:
     p.use_ion_advection = int(sp.use_ion_advection)
     p.use_improved_ion_advection = int(sp.use_improved_ion_advection)
     p.use_nighttime_ion_bcs = int(sp.use_nighttime_ion_bcs)
+    p.use_msis = int(getattr(sp, "use_msis", False))
     return p
 
 
@@ -77,6 +78,8 @@ def advance(cols: dict, sp, solver: int = 1, n_steps: int = 1, waves: int = 0,
         out["ions"] = np.ascontiguousarray(np.array(cols["ions"], dtype=np.float64))
         for k in ("ivel", "sza", "mlat"):
             keep[k] = np.ascontiguousarray(cols[k], dtype=np.float64)
+    if getattr(sp, "use_msis", False) and "msis" in cols:
+        keep["msis"] = np.ascontiguousarray(cols["msis"], dtype=np.float64)
     ptrs = {k: v.ctypes.data for k, v in out.items()}
     ptrs.update({k: v.ctypes.data for k, v in keep.items()})
     c = _abi.columns_struct(ptrs, nc, na)

@@ -233,3 +233,29 @@ def test_nonpositive_density_is_reported(solver):
     out = vertical.to_host(d)
     assert np.array_equal(out["nds"][13], cols["nds"][13])  # left untouched
     solver.check()  # the error word was cleared
+
+
+@pytest.mark.parametrize("which", ["ausm", "rusanov"])
+def test_msis_lower_boundary(solver, which):
+    """UseMSIS with the caller's level-0 MSIS values: per step against the
+    oracle on 100 stretched levels, 5 steps, and the host-array entry point
+    equal to the device one."""
+    from gitm_amd import vertical
+    cols = subset(synth.make_columns(synth.GridSpec(1, 1, 100, stretched=True), seed=41),
+                  np.arange(0, 81, 3))
+    cols["msis"] = synth.msis_bc_values(cols, seed=41)
+    p = dataclasses.replace(synth.StepParams(), use_msis=True)
+    sid = 1 if which == "ausm" else 0
+    state = {k: np.array(v) for k, v in cols.items()}
+    for step in range(5):
+        out = gpu_run(solver, state, p, which)
+        ref = cpu.advance(state, p, solver=sid, threads=THREADS)
+        assert_parity(out, ref, RTOL, what=f"msis {which} step {step + 1}")
+        state.update({k: ref[k] for k in ("nds", "vvel", "vel", "temp", "rho")})
+    dev = gpu_run(solver, cols, p, which, n_steps=2)
+    host = {k: np.array(v) for k, v in cols.items()}  # copies: advanced in place
+    host["logns"] = np.zeros_like(host["nds"])
+    host["nden"] = np.zeros_like(host["temp"])
+    solver.advance_host(host, p, which, 2)
+    for k in ("nds", "vvel", "vel", "temp", "rho"):
+        assert np.array_equal(host[k], dev[k]), k
