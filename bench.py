@@ -18,9 +18,11 @@ profiles/.  ``cpu_baseline`` times the CPU restatement of the same step
 (oracle/, bit-identical to the reference Fortran) on a bounded sample of the
 same grid on this host's cores.
 
-Two further figures ride on the same line: ``ensemble`` (64 members with
-their own nAlts and column counts as one batch, a hipGraph of 20 prep+step
-iterations; ``--no-ensemble`` skips it) and ``adjoint`` with
+Further figures ride on the same line: ``prep`` (the column prep),
+``sources`` (the sources step's conduction and viscosity on the same grid),
+``ensemble`` (64 members with their own nAlts and column counts as one
+batch, a hipGraph of 20 prep+step iterations; ``--no-ensemble`` skips it)
+and ``adjoint`` with
 ``adjoint_forward_ratio`` (a 10-step window on the 5 deg grid, backward pass
 with recompute over forward pass; ``--no-adjoint`` skips it).
 """
@@ -209,6 +211,43 @@ def bench_ensemble(torch, solver, params, steps: int = 20, seed: int = 4) -> dic
             "state_finite_and_positive": ok}
 
 
+def sources_bytes_per_column(n_alts: int) -> int:
+    """Algorithmic HBM bytes of one column of the sources step (outputs on,
+    state not updated): per level 31 FP64 values read (Temperature,
+    TempUnit, Rho, eddy, cp, KappaTemp, ViscCoef, ViscCoefS 6, NDensityS 6,
+    Velocity 3, VerticalVelocity 6, Altitude, RadialDistance, heating) and
+    10 written (Conduction, Viscosity 3, VerticalViscosityS 6)."""
+    return 41 * 8 * (n_alts + 4)
+
+
+def bench_sources(torch, solver, dev, spec, reps: int = 10) -> dict:
+    """The sources step (gitm_vert_sources) on the same grid, increments
+    only (the state is left as it is), timed with HIP events."""
+    from gitm_amd import synth
+
+    host = {k: dev[k].cpu().numpy() for k in ("nds", "temp", "tempunit", "alt", "invr")}
+    src = {k: torch.from_numpy(v).cuda() for k, v in synth.source_fields(host).items()}
+    src["conduction"] = torch.zeros_like(dev["temp"])
+    src["viscosity"] = torch.zeros_like(dev["vel"])
+    src["vert_viscosity"] = torch.zeros_like(dev["vvel"])
+    solver.sources(dev, src, 2.0, apply=False)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        solver.sources(dev, src, 2.0, apply=False)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = sources_bytes_per_column(spec.nalts) * spec.ncols
+    return {"kernel": "sources_kernel (gitm_vert_sources: calc_thermal_conduction, "
+                      "calc_viscosity, add_sources)",
+            "launch_ms": ms, "bound": "hbm", "bytes_per_launch": nbytes,
+            "achieved": nbytes / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "column_updates_per_s": spec.ncols / (ms * 1e-3)}
+
+
 def bench_adjoint(torch, solver, steps: int = 10, seed: int = 5) -> dict:
     """Differentiation workload: the 5 deg x 5 deg grid (2592 columns x 50
     levels), a window of ``steps`` x (AUSM step, Gaussian heating source
@@ -322,6 +361,7 @@ def main() -> int:
         return 1
     launch_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
     prep = time_prep(torch, solver, dev, max(args.steps, 10))
+    src = bench_sources(torch, solver, dev, spec)
 
     ncols = spec.ncols
     value = ncols * args.steps / wall
@@ -386,6 +426,7 @@ def main() -> int:
                  "frac": prep_bytes_per_column(spec.nalts) * ncols
                  / (prep["launch_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
                  "cfl_dt": prep["dt"]},
+        "sources": src,
         "ensemble": ens,
         "adjoint_forward_ratio": adj["ratio"] if adj else None,
         "adjoint": adj,

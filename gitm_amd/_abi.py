@@ -149,3 +149,18 @@ def gitm_block_field(ptr: int, n_lons: int, n_lats: int, n_alts: int, ncomp_allo
     s_blk = s_comp * max(ncomp_alloc, 1) if blocks else 0
     first = ptr + 8 * ((1 - lon_lo) * s_lon + (1 - lat_lo) * s_lat + comp0 * s_comp)
     return Field(first, s_lon, s_lat, s_blk, s_lev, s_comp, alt_lo, 0)
+
+
+class SourcesParams(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("mass", ctypes.c_double * NSPECIES),
+                ("use_turbulent_cond", ctypes.c_int32), ("apply", ctypes.c_int32)]
+
+
+SOURCES_FIELDS = (("kappa_temp", 1), ("visc_coef", 1), ("visc_coef_s", NSPECIES), ("cp", 1),
+                  ("radial_distance", 1), ("heating", 1), ("chem_heating", 1),
+                  ("user_heating", 1), ("conduction", 1), ("viscosity", 3),
+                  ("vertical_viscosity", NSPECIES))
+
+
+class SourcesFields(ctypes.Structure):
+    _fields_ = [(n, Field) for n, _ in SOURCES_FIELDS]

@@ -350,3 +350,33 @@ def msis_bc_values(cols: dict, seed: int = 1, dlog: float = 0.05, dtemp: float =
     out[:, 6] = cols["temp"][:, 1] * cols["tempunit"][:, 1] * (1.0 + dtemp * z[:, 6])
     out[:, 7:] = wind * z[:, 7:]
     return out
+
+
+def source_fields(cols: dict, seed: int = 1, heating_peak: float = 150.0e3,
+                  heating_width: float = 20.0e3, heating_wkg: float = 1.0e-5) -> dict:
+    """Coefficients of the sources step for ``cols`` (synthetic stand-ins of
+    GITM's own, seeded): cp as calc_pressure forms it (calc_pressure.f90
+    :58-83); KappaTemp = 3.6e-4 T^0.75 W/(m K) and ViscCoef = 1.3e-5
+    (T/300)^0.69 kg/(m s), each species' ViscCoefS its density share of
+    ViscCoef, all within +-5% (seeded); RadialDistance = R + Altitude (the
+    radius the inverse radius of make_columns was formed from); a Gaussian
+    heating of
+    ``heating_wkg`` W/kg at ``heating_peak`` as the Dt bracket of
+    add_sources in Temperature units per second (Q / (cp TempUnit))."""
+    nds, temp, tu = cols["nds"], cols["temp"], cols["tempunit"]
+    nc, L = temp.shape
+    mass = species_masses()
+    ntot = nds.sum(axis=1)
+    c = np.zeros((nc, L))
+    for s in range(NSPECIES):
+        c = c + (VIBRATION[s] - 2) * nds[:, s, :] * (KB / mass[s])
+    cp = c / (2.0 * ntot)
+    tk = temp * tu
+    jit = np.exp(0.05 * normal(seed, 401, (nc, 2, L)))
+    kappa = 3.6e-4 * tk ** 0.75 * jit[:, 0]
+    visc = 1.3e-5 * (tk / 300.0) ** 0.69 * jit[:, 1]
+    visc_s = visc[:, None, :] * nds / ntot[:, None, :]
+    raddist = R_EARTH + cols["alt"]
+    q = heating_wkg * np.exp(-((cols["alt"] - heating_peak) / heating_width) ** 2)
+    return dict(cp=cp, kappa_temp=kappa, visc_coef=visc, visc_coef_s=visc_s,
+                raddist=raddist, heating=q / (cp * tu))

@@ -85,6 +85,10 @@ def lib() -> ctypes.CDLL:
                      "gitm_vert_batch_set_tile", "gitm_vert_batch_advance",
                      "gitm_vert_batch_prepare", "gitm_vert_batch_locate"):
             getattr(_lib, name).restype = ctypes.c_int
+        _lib.gitm_vert_sources.restype = ctypes.c_int
+        _lib.gitm_vert_sources.argtypes = [vp, ctypes.POINTER(_abi.SourcesParams),
+                                           ctypes.POINTER(_abi.Columns),
+                                           ctypes.POINTER(_abi.SourcesFields)]
         _lib.gitm_vert_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]
         _lib.gitm_vert_destroy.argtypes = [ctypes.c_void_p]
         _lib.gitm_vert_set_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
@@ -245,6 +249,44 @@ class VerticalSolver:
         if rc:
             raise VerticalStepError(rc)
         return dt
+
+    def sources(self, cols: dict, src: dict, dt: float, apply: bool = True,
+                use_turbulent_cond: bool = False, mass=None) -> None:
+        """The sources step's neutral terms (``gitm_vert_sources``) on device
+        tensors: thermal conduction and viscosity from the state in ``cols``
+        ("temp", "tempunit", "rho", "eddy", "nds", "vel", "vvel", "alt") and
+        the coefficients in ``src`` ("kappa_temp", "visc_coef",
+        "visc_coef_s", "cp", "raddist"; optional "heating", "chem_heating",
+        "user_heating"); with ``apply`` the state is updated as add_sources
+        does.  Increments land in src["conduction"], src["viscosity"],
+        src["vert_viscosity"] when those tensors are present.  Queued on the
+        current stream."""
+        import torch
+        from .synth import species_masses
+
+        c, _ = _device_columns(cols)
+        L = int(cols["temp"].shape[1])
+        names = {"raddist": "radial_distance", "vert_viscosity": "vertical_viscosity"}
+        f = _abi.SourcesFields()
+        for key, t in src.items():
+            name = names.get(key, key)
+            ncomp = dict(_abi.SOURCES_FIELDS).get(name)
+            if ncomp is None:
+                continue
+            if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+                raise ValueError(f"{key}: need a contiguous float64 device tensor")
+            setattr(f, name, _abi.column_field(t.data_ptr(), ncomp, L))
+        p = _abi.SourcesParams()
+        p.dt = dt
+        for s, m in enumerate(species_masses() if mass is None else mass):
+            p.mass[s] = m
+        p.use_turbulent_cond = int(use_turbulent_cond)
+        p.apply = int(apply)
+        self.set_stream(torch.cuda.current_stream().cuda_stream)
+        rc = lib().gitm_vert_sources(self._h, ctypes.byref(p), ctypes.byref(c),
+                                     ctypes.byref(f))
+        if rc:
+            raise VerticalStepError(rc)
 
     # -- host numpy arrays --------------------------------------------------
     def advance_host(self, cols: dict, params: StepParams | None = None,

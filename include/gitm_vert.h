@@ -291,6 +291,53 @@ int gitm_vert_adjoint(gitm_vert_handle *h, const gitm_vert_params *p,
  * larger grids run in chunks of columns. */
 int gitm_vert_set_adjoint_memory(gitm_vert_handle *h, size_t bytes);
 
+/* ------------------------------------------------------------------------
+ * The operator-split sources step's neutral terms (add_sources.f90:53-70,
+ * 94-107): thermal conduction (calc_thermal_conduction,
+ * calc_conduction.f90:19-125) and viscosity (calc_viscosity.f90:4-189), each
+ * three implicit solves of calc_conduction_1d (calc_conduction.f90:132-244)
+ * combined to second order, then (with apply) the updates of Temperature,
+ * Velocity and VerticalVelocity.  The coefficients are the caller's:
+ * KappaTemp from calc_rates, ViscCoef and ViscCoefS from
+ * calc_viscosity_coef, cp from calc_pressure (or gitm_vert_prepare).  Ion
+ * drag, gravity-wave acceleration and the friction of the sources step are
+ * other physics; the caller adds them (add_sources.f90:96-98,110-112).
+ * ------------------------------------------------------------------------ */
+typedef struct gitm_vert_sources_params {
+  double dt;                          /* Dt                                  */
+  double mass[GITM_VERT_NSPECIES];    /* Mass(1:6) [kg]                      */
+  int32_t use_turbulent_cond;         /* UseTurbulentCond                    */
+  int32_t apply;                      /* also update the state               */
+} gitm_vert_sources_params;
+
+typedef struct gitm_vert_sources_fields {
+  /* inputs, read on levels 0..nAlts+1 (heating terms on 1..nAlts) */
+  gitm_vert_field kappa_temp;     /* KappaTemp                               */
+  gitm_vert_field visc_coef;      /* ViscCoef                                */
+  gitm_vert_field visc_coef_s;    /* ViscCoefS(.., 1:6)                      */
+  gitm_vert_field cp;             /* cp                                      */
+  gitm_vert_field radial_distance; /* RadialDistance_GB                      */
+  gitm_vert_field heating;        /* the bracket add_sources multiplies by Dt
+                                     (:54-64), in Temperature units per s;
+                                     NULL: none                             */
+  gitm_vert_field chem_heating;   /* ChemicalHeatingRate, or NULL           */
+  gitm_vert_field user_heating;   /* UserHeatingRate, or NULL               */
+  /* outputs on levels 0..nAlts+1; NULL to skip */
+  gitm_vert_field conduction;     /* Conduction                              */
+  gitm_vert_field viscosity;      /* Viscosity(.., 1:3): the up component is
+                                     what calc_viscosity leaves in it (the
+                                     last species' full-step solve, :161),
+                                     as add_sources adds it                 */
+  gitm_vert_field vertical_viscosity; /* VerticalViscosityS(.., 1:6)        */
+} gitm_vert_sources_fields;
+
+/* From c's Temperature, TempUnit, Rho, KappaEddyDiffusion, NDensityS,
+ * Velocity, VerticalVelocity and Altitude (device pointers) and the fields
+ * of `f`.  nAlts <= 231 (a column's record fits the CU's LDS).
+ * Asynchronous on the handle's stream. */
+int gitm_vert_sources(gitm_vert_handle *h, const gitm_vert_sources_params *p,
+                      const gitm_vert_columns *c, const gitm_vert_sources_fields *f);
+
 /* Member and member column of a batch-wide column number. */
 int gitm_vert_batch_locate(const gitm_vert_batch *b, int column, int *member,
                            int *member_column);

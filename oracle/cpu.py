@@ -157,3 +157,60 @@ def prepare(cols: dict, pp) -> tuple[dict, float]:
            _ptr(inp["vvel"]), _ptr(inp["dalt"]), _ptr(ivel),
            *[_ptr(out[k]) for k in PREP_OUTPUTS])
     return out, float(dt)
+
+
+# --- the sources step (ov_sources_step) ------------------------------------
+class _Sources(ctypes.Structure):
+    _fields_ = [("n_cols", ctypes.c_int), ("n_alts", ctypes.c_int), ("dt", ctypes.c_double),
+                ("use_turbulent_cond", ctypes.c_int), ("apply", ctypes.c_int),
+                ("mass", ctypes.c_double * 6)] + [
+        (n, _P) for n in ("temp", "vel", "vvel", "tempunit", "rho", "cp", "eddy", "nds",
+                          "kappa_temp", "visc_coef", "visc_coef_s", "alt", "raddist",
+                          "heating", "chem_heating", "user_heating", "conduction",
+                          "viscosity", "vert_viscosity")]
+
+
+MAX_LEVELS = 260  # OV_LMAX of gitm_vert_oracle.h
+
+SOURCE_INPUTS = ("tempunit", "rho", "cp", "eddy", "nds", "kappa_temp", "visc_coef",
+                 "visc_coef_s", "alt", "raddist")
+
+
+def sources(cols: dict, dt: float, mass, use_turbulent_cond: bool = False,
+            apply: bool = True) -> dict:
+    """The oracle's sources step on a copy of ``cols``; returns the updated
+    temp, vel, vvel and the increments conduction, viscosity,
+    vert_viscosity (levels 0..nAlts+1 set)."""
+    nc, L = cols["temp"].shape
+    out = {k: np.ascontiguousarray(np.array(cols[k], dtype=np.float64))
+           for k in ("temp", "vel", "vvel")}
+    out["conduction"] = np.zeros_like(out["temp"])
+    out["viscosity"] = np.zeros_like(out["vel"])
+    out["vert_viscosity"] = np.zeros_like(out["vvel"])
+    keep = {k: np.ascontiguousarray(cols[k], dtype=np.float64) for k in SOURCE_INPUTS}
+    for k in ("heating", "chem_heating", "user_heating"):
+        if cols.get(k) is not None:
+            keep[k] = np.ascontiguousarray(cols[k], dtype=np.float64)
+    s = _Sources()
+    s.n_cols, s.n_alts, s.dt = nc, L - 4, dt
+    s.use_turbulent_cond, s.apply = int(use_turbulent_cond), int(apply)
+    for i in range(6):
+        s.mass[i] = mass[i]
+    for k, v in list(out.items()) + list(keep.items()):
+        setattr(s, k, _ptr(v))
+    lib().ov_sources_step(ctypes.byref(s))
+    return out
+
+
+def conduction_1d(n_alts: int, dt: float, neumann: bool, quantity, diff, mulfac, alt,
+                  raddist) -> np.ndarray:
+    """calc_conduction_1d of one column (arrays of nAlts+4 levels)."""
+    if n_alts + 4 > MAX_LEVELS:
+        raise ValueError(f"n_alts {n_alts}: the oracle holds at most {MAX_LEVELS - 4}")
+    L = n_alts + 4
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in
+            (quantity, diff, mulfac, alt, raddist)]
+    res = np.zeros(L)
+    lib().ov_conduction_1d(ctypes.c_int(n_alts), ctypes.c_double(dt), ctypes.c_int(int(neumann)),
+                           *[_ptr(a) for a in args], _ptr(res))
+    return res

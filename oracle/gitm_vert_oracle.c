@@ -1543,3 +1543,165 @@ double ov_prepare(const ov_prep_params *p, int n_cols, int n_alts, const double 
   dt_local = fmin2(dt_local, p->dt_end); /* :142 */
   return fmin2(p->fixed_dt, dt_local);   /* main.f90:62, calc_timestep.f90:147 */
 }
+
+
+/* ------------------------------------------------------------------ */
+/* Sources step: conduction and viscosity (calc_conduction.f90,        */
+/* calc_viscosity.f90) and the neutral updates of add_sources.f90      */
+/* ------------------------------------------------------------------ */
+void ov_conduction_1d(int nA, double DtIn, int NeuBCS, const double *Quantity,
+                      const double *Diff, const double *MulFac, const double *Alt,
+                      const double *RadDist, double *dTdt) {
+  /* calc_conduction.f90:132-244; arrays indexed by level + 1 (X()) */
+  double r2[LM], di[LM], m[LM], du[LM], dl[LM], r[LM], du12[LM], du22[LM];
+  double a[LM], b[LM], c[LM], d[LM], cp[LM], dp[LM], temp[LM], told[LM];
+  int i;
+  if (nA < 1 || nA + 4 > LM) return;
+  for (i = 0; i <= nA + 1; i++) {
+    told[X(i)] = Quantity[X(i)];
+    r2[X(i)] = RadDist[X(i)] * RadDist[X(i)];
+    di[X(i)] = Diff[X(i)] * r2[X(i)];
+    m[X(i)] = DtIn / (MulFac[X(i)] * r2[X(i)]);
+    du[X(i)] = Alt[X(i + 1)] - Alt[X(i)];
+    dl[X(i)] = Alt[X(i)] - Alt[X(i - 1)];
+    r[X(i)] = du[X(i)] / dl[X(i)];
+    du12[X(i)] = du[X(i)] * du[X(i)] * (1 + r[X(i)]) * (1 + r[X(i)]);
+    du22[X(i)] = 0.5 * (dl[X(i)] * du[X(i)] + du[X(i)] * du[X(i)]);
+  }
+  /* :188-194 (dl reused) */
+  for (i = 1; i <= nA; i++)
+    dl[X(i)] = di[X(i + 1)] - r[X(i)] * r[X(i)] * di[X(i - 1)] -
+               (1 - r[X(i)] * r[X(i)]) * di[X(i)];
+  dl[X(0)] = dl[X(1)];
+  dl[X(nA + 1)] = dl[X(nA)];
+  for (i = 0; i <= nA + 1; i++) { /* :198-206 */
+    a[X(i)] = di[X(i)] / du22[X(i)] * r[X(i)] - dl[X(i)] / du12[X(i)] * r[X(i)] * r[X(i)];
+    c[X(i)] = di[X(i)] / du22[X(i)] + dl[X(i)] / du12[X(i)];
+    b[X(i)] = -1 / m[X(i)] - di[X(i)] / du22[X(i)] * (1 + r[X(i)]) -
+              dl[X(i)] / du12[X(i)] * (1 - r[X(i)] * r[X(i)]);
+    d[X(i)] = -told[X(i)] / m[X(i)];
+  }
+  d[X(0)] = -told[X(0)]; /* :210-213 */
+  a[X(0)] = 0.0;
+  b[X(0)] = -1.0;
+  c[X(0)] = 0.0;
+  i = nA + 1;
+  if (NeuBCS) { /* :221-230 */
+    a[X(i)] = 1.0 * (r[X(i)] * (1.0 + r[X(i)]) * di[X(i)] * m[X(i)] / du22[X(i)]);
+    b[X(i)] = -1.0 * (1.0 + r[X(i)] * (1 + r[X(i)]) * di[X(i)] * m[X(i)] / du22[X(i)]);
+    c[X(i)] = 0.0;
+    d[X(i)] = -told[X(i)];
+  } else {
+    a[X(i)] = 0.0;
+    b[X(i)] = -1.0;
+    c[X(i)] = 0.0;
+    d[X(i)] = -told[X(i)];
+  }
+  cp[X(0)] = c[X(0)] / b[X(0)]; /* :237-247 */
+  for (i = 1; i <= nA + 1; i++) cp[X(i)] = c[X(i)] / (b[X(i)] - cp[X(i - 1)] * a[X(i)]);
+  dp[X(0)] = d[X(0)] / b[X(0)];
+  for (i = 1; i <= nA + 1; i++)
+    dp[X(i)] = (d[X(i)] - dp[X(i - 1)] * a[X(i)]) / (b[X(i)] - cp[X(i - 1)] * a[X(i)]);
+  temp[X(nA + 1)] = dp[X(nA + 1)];
+  for (i = nA; i >= 0; i--) temp[X(i)] = dp[X(i)] - cp[X(i)] * temp[X(i + 1)];
+  for (i = 0; i <= nA + 1; i++) dTdt[X(i)] = temp[X(i)] - Quantity[X(i)];
+}
+
+/* three solves (Dt/2, Dt/2 from the first result, Dt) and the second-order
+ * combination (2H - F) - Q, as calc_thermal_conduction and calc_viscosity
+ * do; `unit` scales the solved quantity (TempUnit for the temperature, NULL
+ * for the winds); returns the last solve's result in `last` */
+static void two_stage(int nA, double Dt, const double *Q, const double *unit,
+                      const double *diff, const double *mulfac, const double *alt,
+                      const double *rd, double *inc, double *last) {
+  double q[LM], res[LM], s1[LM], h[LM], f[LM];
+  int i;
+  for (i = 0; i < nA + 4; i++) q[i] = unit ? Q[i] * unit[i] : Q[i];
+  ov_conduction_1d(nA, Dt / 2.0, 1, q, diff, mulfac, alt, rd, res);
+  for (i = 0; i <= nA + 1; i++)
+    s1[X(i)] = unit ? Q[X(i)] + res[X(i)] / unit[X(i)] : Q[X(i)] + res[X(i)];
+  s1[X(-1)] = s1[X(nA + 2)] = 0.0; /* not read */
+  for (i = 0; i < nA + 4; i++) q[i] = unit ? s1[i] * unit[i] : s1[i];
+  ov_conduction_1d(nA, Dt / 2.0, 1, q, diff, mulfac, alt, rd, res);
+  for (i = 0; i <= nA + 1; i++)
+    h[X(i)] = unit ? s1[X(i)] + res[X(i)] / unit[X(i)] : s1[X(i)] + res[X(i)];
+  for (i = 0; i < nA + 4; i++) q[i] = unit ? Q[i] * unit[i] : Q[i];
+  ov_conduction_1d(nA, Dt, 1, q, diff, mulfac, alt, rd, res);
+  for (i = 0; i <= nA + 1; i++) {
+    f[X(i)] = unit ? Q[X(i)] + res[X(i)] / unit[X(i)] : Q[X(i)] + res[X(i)];
+    inc[X(i)] = (2.0 * h[X(i)] - f[X(i)]) - Q[X(i)];
+    if (last) last[X(i)] = res[X(i)];
+  }
+}
+
+void ov_sources_step(const ov_sources *s) {
+  const int nA = s->n_alts, L = nA + 4;
+  if (nA < 1 || L > LM) return;
+  for (int col = 0; col < s->n_cols; col++) {
+    const size_t o = (size_t)col * L;
+    double diff[LM], mulfac[LM], cond[LM], visc[3][LM], vvisc[NSP][LM], last[LM];
+    const double *alt = s->alt + o, *rd = s->raddist + o;
+    const double *rho = s->rho + o, *eddy = s->eddy + o;
+    int i, d, sp;
+    for (i = 0; i < L; i++) diff[i] = mulfac[i] = 0.0;
+    /* calc_thermal_conduction, calc_conduction.f90:48-123 */
+    for (i = 0; i <= nA + 1; i++) {
+      const double prandtl =
+          s->use_turbulent_cond ? eddy[X(i)] * rho[X(i)] * s->cp[o + X(i)] : 0.0;
+      mulfac[X(i)] = rho[X(i)] * s->cp[o + X(i)];
+      diff[X(i)] = s->kappa_temp[o + X(i)] + prandtl;
+    }
+    two_stage(nA, s->dt, s->temp + o, s->tempunit + o, diff, mulfac, alt, rd, cond, NULL);
+    /* calc_viscosity.f90:53-117, east and north */
+    for (d = 0; d < 2; d++) {
+      for (i = 0; i <= nA + 1; i++) {
+        mulfac[X(i)] = rho[X(i)];
+        diff[X(i)] = s->visc_coef[o + X(i)] + rho[X(i)] * eddy[X(i)];
+      }
+      two_stage(nA, s->dt, s->vel + (size_t)(col * 3 + d) * L, NULL, diff, mulfac, alt, rd,
+                visc[d], NULL);
+    }
+    /* :119-186, each species; Viscosity(:,:,:,iUp_) keeps the last
+     * species' full-step result (:161) */
+    for (sp = 0; sp < NSP; sp++) {
+      const double *n = s->nds + (size_t)(col * NSP + sp) * L;
+      const double *vc = s->visc_coef_s + (size_t)(col * NSP + sp) * L;
+      for (i = 0; i <= nA + 1; i++) {
+        diff[X(i)] = (4.0 / 3.0) * vc[X(i)] + s->mass[sp] * n[X(i)] * eddy[X(i)];
+        mulfac[X(i)] = s->mass[sp] * n[X(i)];
+      }
+      two_stage(nA, s->dt, s->vvel + (size_t)(col * NSP + sp) * L, NULL, diff, mulfac, alt,
+                rd, vvisc[sp], last);
+    }
+    for (i = 0; i <= nA + 1; i++) visc[2][X(i)] = last[X(i)];
+    if (s->conduction)
+      for (i = 0; i <= nA + 1; i++) s->conduction[o + X(i)] = cond[X(i)];
+    if (s->viscosity)
+      for (d = 0; d < 3; d++)
+        for (i = 0; i <= nA + 1; i++) s->viscosity[(size_t)(col * 3 + d) * L + X(i)] = visc[d][X(i)];
+    if (s->vert_viscosity)
+      for (sp = 0; sp < NSP; sp++)
+        for (i = 0; i <= nA + 1; i++)
+          s->vert_viscosity[(size_t)(col * NSP + sp) * L + X(i)] = vvisc[sp][X(i)];
+    if (!s->apply) continue;
+    /* add_sources.f90:53-70 */
+    double *T = s->temp + o;
+    for (i = 1; i <= nA; i++) {
+      double t = T[X(i)] + s->dt * (s->heating ? s->heating[o + X(i)] : 0.0);
+      if (s->chem_heating) t = t + s->chem_heating[o + X(i)];
+      if (s->user_heating) t = t + s->user_heating[o + X(i)];
+      T[X(i)] = t;
+    }
+    for (i = 0; i <= nA + 1; i++) T[X(i)] = T[X(i)] + cond[X(i)];
+    /* :94-107 (ion drag, gravity waves and the sources-step friction are
+     * other physics, added by the caller) */
+    for (d = 0; d < 3; d++) {
+      double *v = s->vel + (size_t)(col * 3 + d) * L;
+      for (i = 0; i <= nA + 1; i++) v[X(i)] = v[X(i)] + visc[d][X(i)];
+    }
+    for (sp = 0; sp < NSP; sp++) {
+      double *w = s->vvel + (size_t)(col * NSP + sp) * L;
+      for (i = 0; i <= nA + 1; i++) w[X(i)] = w[X(i)] + vvisc[sp][X(i)];
+    }
+  }
+}

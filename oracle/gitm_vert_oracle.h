@@ -129,3 +129,47 @@ double ov_prepare(const ov_prep_params *p, int n_cols, int n_alts, const double 
 }
 #endif
 #endif
+
+/* The operator-split sources step's neutral terms (add_sources.f90:53-66,
+ * 94-107): thermal conduction (calc_conduction.f90:19-125), viscosity
+ * (calc_viscosity.f90:4-189), both through calc_conduction_1d
+ * (calc_conduction.f90:132-244), then the updates of Temperature, Velocity
+ * and VerticalVelocity.  Layout as ov_columns ([col][comp][level], levels
+ * -1..nAlts+2); the coefficient arrays are read on levels 0..nAlts+1. */
+typedef struct {
+  int n_cols, n_alts;
+  double dt;
+  int use_turbulent_cond;     /* ModInputs UseTurbulentCond                 */
+  int apply;                  /* update the state (add_sources) too         */
+  double mass[OV_NSPECIES];
+  /* state (updated when apply) */
+  double *temp;               /* Temperature (normalised)                  */
+  double *vel;                /* Velocity [3]                              */
+  double *vvel;               /* VerticalVelocity [6]                      */
+  /* inputs */
+  const double *tempunit, *rho, *cp, *eddy;
+  const double *nds;          /* NDensityS [6]                             */
+  const double *kappa_temp;   /* KappaTemp                                 */
+  const double *visc_coef;    /* ViscCoef                                  */
+  const double *visc_coef_s;  /* ViscCoefS [6]                             */
+  const double *alt;          /* Altitude_GB                               */
+  const double *raddist;      /* RadialDistance_GB                         */
+  const double *heating;      /* the Dt-multiplied heating bracket of
+                                 add_sources.f90:54-65 (1/s), or NULL      */
+  const double *chem_heating; /* ChemicalHeatingRate, or NULL              */
+  const double *user_heating; /* UserHeatingRate, or NULL                  */
+  /* outputs, levels 0..nAlts+1 (NULL to skip) */
+  double *conduction;         /* Conduction                                */
+  double *viscosity;          /* Viscosity [3]; the up component is what
+                                 calc_viscosity leaves there              */
+  double *vert_viscosity;     /* VerticalViscosityS [6]                     */
+} ov_sources;
+
+void ov_sources_step(const ov_sources *s);
+
+/* calc_conduction_1d for one column: quantity on levels -1..nAlts+2 (only
+ * 0..nAlts+1 read), diff and mulfac on 0..nAlts+1 (arrays of L with level
+ * -1 at index 0), alt and raddist on -1..nAlts+2; result on 0..nAlts+1. */
+void ov_conduction_1d(int n_alts, double dt, int neumann, const double *quantity,
+                      const double *diff, const double *mulfac, const double *alt,
+                      const double *raddist, double *result);
