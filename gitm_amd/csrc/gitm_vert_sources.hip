@@ -6,15 +6,14 @@
 // (2H - F) - Q, then the updates of add_sources.f90:53-70,94-107.
 //
 // Mapping: one workgroup per column, one lane per level (blockDim =
-// nAlts+4), as the vertical step.  The lanes stage the column in LDS, one
-// level each with coalesced loads: per quantity (0 the temperature, 1-2 the
-// east and north winds, 3-8 the species' vertical winds) the diffusion
-// coefficient and MulFac times r^2 and the solved quantity, and the
-// non-uniform grid factors.  The three tridiagonal (Thomas) solves of each
-// quantity are serial in altitude: the Dt/2 and Dt solves from the state
-// run at once on 18 lanes, then the second Dt/2 solve from the first's
-// result on 9.  The lanes then write the increments and the updated state,
-// again one level each.  Operations keep the reference's order: the
+// nAlts+4), as the vertical step.  The tridiagonal (Thomas) solves are
+// serial in altitude, so everything that is not is taken out of them: the
+// lanes stage the column one level each (coalesced loads), form every
+// solve's coefficients one level each, and the serial lanes (one per solve
+// and quantity: 0 the temperature, 1-2 the east and north winds, 3-8 the
+// species' vertical winds) run only the recurrences.  The two Dt/2 solves
+// have the same matrix, so the second reuses the first's pivots and cp and
+// runs only its dp recurrence.  Operations keep the reference's order: the
 // results equal the oracle's bit for bit.
 #include "vert_gpu.h"
 
@@ -34,19 +33,28 @@ struct SrcFields {
 
 constexpr int NQ = 3 + gv::NSP;  // temperature, two winds, six species winds
 
-// LDS record of one level (doubles): per quantity di = Diff r^2, mr = MulFac
-// r^2 and the solved input (the quantity, times TempUnit for the
-// temperature); the grid factors; the Thomas coefficients of the two
-// concurrent solves (A: Dt/2, C: Dt); the first stage's result (then the
-// increment), the full step's value F, and Velocity(up)'s increment.
+// LDS record of one level (doubles), per quantity q:
+//   R_A, R_C   the tridiagonal a and c, shared by the three solves (they do
+//              not depend on the step); at the top level, where a does,
+//              R_C holds the Dt solve's a (c is 0 there);
+//   R_BA       b of the Dt/2 solves, then the pivots b - cp a, which the
+//              two Dt/2 solves share (same a, b and c, so the same cp);
+//   R_BC       b of the Dt solve, then its cp; then the second Dt/2
+//              solve's right side, its dp and its solution;
+//   R_DA, R_DC d of the first Dt/2 solve and of the Dt solve, then their
+//              dp, then their solutions;
+//   R_CPA      the Dt/2 solves' cp.  Until the coefficients are formed this
+//              region holds the diffusion coefficients times r^2 (one slot
+//              for the two winds, whose coefficients are equal) and the
+//              altitude.
 enum {
-  R_DI = 0, R_MR = R_DI + NQ, R_QIN = R_MR + NQ,
-  R_ALT = R_QIN + NQ, R_R, R_DU12, R_DU22, R_UNIT,
-  R_CPA = R_UNIT + 1, R_DPA = R_CPA + NQ, R_CPC = R_DPA + NQ, R_DPC = R_CPC + NQ,
-  R_S1 = R_DPC + NQ, R_F = R_S1 + NQ, R_VUP = R_F + NQ,
-  R_LEN = R_VUP + 1  // odd: 64-bit LDS accesses stay conflict-free
+  R_A = 0, R_C = R_A + NQ, R_BA = R_C + NQ, R_BC = R_BA + NQ, R_DA = R_BC + NQ,
+  R_DC = R_DA + NQ, R_CPA = R_DC + NQ, R_LEN = R_CPA + NQ,
+  R_DI = R_CPA, R_ALT = R_CPA + NQ - 1
 };
-static_assert(R_LEN % 2 == 1, "odd record length");
+static_assert(R_LEN % 2 == 1, "odd record length: 64-bit LDS accesses stay conflict-free");
+constexpr int di_slot(int q) { return q == 0 ? 0 : (q < 3 ? 1 : q - 1); }
+static_assert(di_slot(NQ - 1) < NQ - 1, "diffusion slots below the altitude's");
 
 struct SrcCol {
   const SrcParams &p;
@@ -67,59 +75,22 @@ struct SrcCol {
     if (q < 3) return at(io.velocity, k, q - 1);
     return at(io.vertical_velocity, k, q - 3);
   }
-
-  // one Thomas solve of quantity q (calc_conduction.f90:176-247) on levels
-  // 0..nAlts+1: input from the record slot `in` (scaled by the unit for the
-  // temperature when `scale`), coefficients to the slots cpa/dpa; returns
-  // nothing, the back substitution is done by the caller through `emit`.
-  template <class Emit>
-  __device__ void solve(int q, double dts, int in_slot, bool scale, int cps, int dps,
-                        Emit emit) const {
-    const int nA = p.n_alts;
-    auto input = [&](int k) {
-      const double v = rec(k, in_slot + q);
-      return scale ? v * rec(k, R_UNIT) : v;
-    };
-    // level 0: a = 0, b = -1, c = 0, d = -told (:210-213)
-    double cpv = 0.0 / -1.0;
-    double dpv = (-input(0)) / -1.0;
-    rec(0, cps + q) = cpv;
-    rec(0, dps + q) = dpv;
-    for (int i = 1; i <= nA + 1; i++) {
-      const double di0 = rec(i, R_DI + q), r = rec(i, R_R), du22 = rec(i, R_DU22);
-      const double m = dts / rec(i, R_MR + q);
-      const double told = input(i);
-      double a, b, c, d;
-      if (i <= nA) {
-        const double du12 = rec(i, R_DU12);
-        const double dl = rec(i + 1, R_DI + q) - r * r * rec(i - 1, R_DI + q) -
-                          (1 - r * r) * di0;  // :188-190
-        a = di0 / du22 * r - dl / du12 * r * r;
-        c = di0 / du22 + dl / du12;
-        b = -1 / m - di0 / du22 * (1 + r) - dl / du12 * (1 - r * r);
-        d = -told / m;
-      } else {  // Neumann top, :221-230
-        a = 1.0 * (r * (1.0 + r) * di0 * m / du22);
-        b = -1.0 * (1.0 + r * (1 + r) * di0 * m / du22);
-        c = 0.0;
-        d = -told;
-      }
-      const double cpn = c / (b - cpv * a);
-      const double dpn = (d - dpv * a) / (b - cpv * a);
-      rec(i, cps + q) = cpn;
-      rec(i, dps + q) = dpn;
-      cpv = cpn;
-      dpv = dpn;
-    }
-    double tnext = dpv;  // temp(nAlts+1) = dp(nAlts+1)
-    emit(nA + 1, tnext - input(nA + 1));
-    for (int i = nA; i >= 0; i--) {
-      tnext = rec(i, dps + q) - rec(i, cps + q) * tnext;
-      emit(i, tnext - input(i));  // dTdt_cond (:249-250)
-    }
-  }
 };
 
+// Phases of one column (barriers between them):
+//  1. stage: one lane per level -1..nAlts+2 loads its level; the diffusion
+//     coefficients and the altitude go to LDS, the rest stays in registers;
+//  2. coefficients: one lane per level 0..nAlts+1 forms the grid factors
+//     and every solve's a, b, c and d (calc_conduction.f90:181-230);
+//  3. the forward sweeps and back substitutions of the first Dt/2 solve and
+//     the Dt solve, one lane per (solve, quantity), serial in altitude
+//     (:232-247);
+//  4. one lane per level: the first stage's value, the full step's, and
+//     the second Dt/2 solve's right side;
+//  5. the second Dt/2 solve on the first's pivots and cp: only its dp
+//     recurrence and back substitution, one lane per quantity;
+//  6. one lane per level: the combination (2H - F) - Q (:117-121,
+//     calc_viscosity.f90:110-114), the increments and add_sources' updates.
 template <int NT>
 __global__ void __launch_bounds__(NT)
     sources_kernel(SrcParams p, gitm_vert_columns io, SrcFields f) {
@@ -127,109 +98,186 @@ __global__ void __launch_bounds__(NT)
   const int col = (int)blockIdx.x, j = (int)threadIdx.x, k = j - 1;
   const int nA = p.n_alts;
   const int nlb = io.n_lons * io.n_lats;
+  const bool lev = k >= 0 && k <= nA + 1;
   SrcCol c{p, io, f, lds, col % io.n_lons, (col / io.n_lons) % io.n_lats, col / nlb};
-  // stage the level (coefficients on levels 0..nAlts+1)
+  const double dtA = p.dt / 2.0, dtC = p.dt;
+  double qin[NQ], mr[NQ], unit = 1.0;
+  // 1. stage
   c.rec(k, R_ALT) = c.at(io.altitude, k);
-  const double unit = c.at(io.temp_unit, k);
-  c.rec(k, R_UNIT) = unit;
-  if (k >= 0 && k <= nA + 1) {
+  if (lev) {
+    unit = c.at(io.temp_unit, k);
     const double rd = c.at(f.raddist, k);
     const double r2 = rd * rd;
     const double rho = c.at(io.rho, k), eddy = c.at(io.eddy, k);
     {  // calc_conduction.f90:59-73
       const double cpk = c.at(f.cp, k);
       const double prandtl = p.use_turbulent_cond ? eddy * rho * cpk : 0.0;
-      c.rec(k, R_DI + 0) = (c.at(f.kappa_temp, k) + prandtl) * r2;
-      c.rec(k, R_MR + 0) = (rho * cpk) * r2;
+      c.rec(k, R_DI + di_slot(0)) = (c.at(f.kappa_temp, k) + prandtl) * r2;
+      mr[0] = (rho * cpk) * r2;
     }
-    const double vd = (c.at(f.visc_coef, k) + rho * eddy) * r2;  // calc_viscosity.f90:66-67
-    c.rec(k, R_DI + 1) = vd;
-    c.rec(k, R_DI + 2) = vd;
-    c.rec(k, R_MR + 1) = rho * r2;
-    c.rec(k, R_MR + 2) = rho * r2;
+    // calc_viscosity.f90:66-67
+    c.rec(k, R_DI + di_slot(1)) = (c.at(f.visc_coef, k) + rho * eddy) * r2;
+    mr[1] = rho * r2;
+    mr[2] = rho * r2;
 #pragma unroll
     for (int s = 0; s < gv::NSP; s++) {  // :125-128
       const double n = c.at(io.ndensity_s, k, s);
       const double mn = p.mass[s] * n;
-      c.rec(k, R_DI + 3 + s) = ((4.0 / 3.0) * c.at(f.visc_coef_s, k, s) + mn * eddy) * r2;
-      c.rec(k, R_MR + 3 + s) = (p.mass[s] * n) * r2;
+      c.rec(k, R_DI + di_slot(3 + s)) =
+          ((4.0 / 3.0) * c.at(f.visc_coef_s, k, s) + mn * eddy) * r2;
+      mr[3 + s] = (p.mass[s] * n) * r2;
     }
 #pragma unroll
-    for (int q = 0; q < NQ; q++) c.rec(k, R_QIN + q) = c.quantity(q, k);
+    for (int q = 0; q < NQ; q++) qin[q] = c.quantity(q, k);
   }
   __syncthreads();
-  if (k >= 0 && k <= nA + 1) {  // grid factors (:181-186)
+  // 2. coefficients; they read only R_CPA's region and write the others
+  if (lev) {
     const double altm = c.rec(k - 1, R_ALT), alt0 = c.rec(k, R_ALT),
                  altp = c.rec(k + 1, R_ALT);
-    const double du = altp - alt0, dl = alt0 - altm;
-    const double r = du / dl;
-    c.rec(k, R_R) = r;
-    c.rec(k, R_DU12) = du * du * (1 + r) * (1 + r);
-    c.rec(k, R_DU22) = 0.5 * (dl * du + du * du);
-  }
-  __syncthreads();
-  // the first Dt/2 solve (A) and the Dt solve (C) from the state at once
-  // (columns of fewer than 18 levels take some in turn)
-  for (int task = j; task < 2 * NQ; task += (int)blockDim.x) {
-    const int q = task % NQ;
-    const bool isT = q == 0;
-    if (task < NQ) {
-      c.solve(q, p.dt / 2.0, R_QIN, isT, R_CPA, R_DPA, [&](int i, double res) {
-        const double q0 = c.rec(i, R_QIN + q);
-        c.rec(i, R_S1 + q) = isT ? q0 + res / c.rec(i, R_UNIT) : q0 + res;
-      });
-    } else {
-      c.solve(q, p.dt, R_QIN, isT, R_CPC, R_DPC, [&](int i, double res) {
-        const double q0 = c.rec(i, R_QIN + q);
-        c.rec(i, R_F + q) = isT ? q0 + res / c.rec(i, R_UNIT) : q0 + res;
-        // Viscosity(:,:,:,iUp_) keeps the last species' full-step result
-        // (calc_viscosity.f90:161)
-        if (q == NQ - 1) c.rec(i, R_VUP) = res;
-      });
+    const double du = altp - alt0, dl0 = alt0 - altm;  // :181-186
+    const double r = du / dl0;
+    const double du12 = du * du * (1 + r) * (1 + r);
+    const double du22 = 0.5 * (dl0 * du + du * du);
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      const double told = q == 0 ? qin[0] * unit : qin[q];
+      const double di0 = c.rec(k, R_DI + di_slot(q));
+      if (k == 0) {  // a = 0, b = -1, c = 0, d = -told (:210-213)
+        c.rec(k, R_A + q) = 0.0;
+        c.rec(k, R_C + q) = 0.0;
+        c.rec(k, R_BA + q) = -1.0;
+        c.rec(k, R_BC + q) = -1.0;
+        c.rec(k, R_DA + q) = -told;
+        c.rec(k, R_DC + q) = -told;
+      } else if (k <= nA) {
+        const double mA = dtA / mr[q], mC = dtC / mr[q];
+        const double dl = c.rec(k + 1, R_DI + di_slot(q)) -
+                          r * r * c.rec(k - 1, R_DI + di_slot(q)) -
+                          (1 - r * r) * di0;  // :188-190
+        c.rec(k, R_A + q) = di0 / du22 * r - dl / du12 * r * r;
+        c.rec(k, R_C + q) = di0 / du22 + dl / du12;
+        c.rec(k, R_BA + q) = -1 / mA - di0 / du22 * (1 + r) - dl / du12 * (1 - r * r);
+        c.rec(k, R_BC + q) = -1 / mC - di0 / du22 * (1 + r) - dl / du12 * (1 - r * r);
+        c.rec(k, R_DA + q) = -told / mA;
+        c.rec(k, R_DC + q) = -told / mC;
+      } else {  // Neumann top, :221-230
+        const double mA = dtA / mr[q], mC = dtC / mr[q];
+        c.rec(k, R_A + q) = 1.0 * (r * (1.0 + r) * di0 * mA / du22);
+        c.rec(k, R_C + q) = 1.0 * (r * (1.0 + r) * di0 * mC / du22);
+        c.rec(k, R_BA + q) = -1.0 * (1.0 + r * (1 + r) * di0 * mA / du22);
+        c.rec(k, R_BC + q) = -1.0 * (1.0 + r * (1 + r) * di0 * mC / du22);
+        c.rec(k, R_DA + q) = -told;
+        c.rec(k, R_DC + q) = -told;
+      }
     }
   }
   __syncthreads();
-  // the second Dt/2 solve (B) from the first's result, and the combination
-  // (2H - F) - Q (calc_conduction.f90:117-121, calc_viscosity.f90:110-114)
-  for (int q = j; q < NQ; q += (int)blockDim.x) {
-    const bool isT = q == 0;
-    c.solve(q, p.dt / 2.0, R_S1, isT, R_CPA, R_DPA, [&](int i, double res) {
-      const double s1 = c.rec(i, R_S1 + q);
-      const double h = isT ? s1 + res / c.rec(i, R_UNIT) : s1 + res;
-      c.rec(i, R_S1 + q) = (2.0 * h - c.rec(i, R_F + q)) - c.rec(i, R_QIN + q);
-    });
+  // 3. the first Dt/2 solve (A) and the Dt solve (C), at once (columns of
+  // fewer than 18 levels take some in turn)
+  for (int task = j; task < 2 * NQ; task += (int)blockDim.x) {
+    const int q = task % NQ;
+    const bool isA = task < NQ;
+    // b's slot takes the pivot, then (for C, where cs == bs) cp
+    const int bs = (isA ? R_BA : R_BC) + q, ds = (isA ? R_DA : R_DC) + q,
+              cs = (isA ? R_CPA : R_BC) + q;
+    double cpv = 0.0 / -1.0;
+    double dpv = c.rec(0, ds) / -1.0;
+    c.rec(0, cs) = cpv;
+    c.rec(0, ds) = dpv;
+    for (int i = 1; i <= nA; i++) {
+      const double a = c.rec(i, R_A + q), cc = c.rec(i, R_C + q);
+      const double den = c.rec(i, bs) - cpv * a;
+      const double cpn = cc / den;
+      const double dpn = (c.rec(i, ds) - dpv * a) / den;
+      c.rec(i, bs) = den;
+      c.rec(i, cs) = cpn;
+      c.rec(i, ds) = dpn;
+      cpv = cpn;
+      dpv = dpn;
+    }
+    const int top = nA + 1;
+    const double a = c.rec(top, (isA ? R_A : R_C) + q);
+    const double den = c.rec(top, bs) - cpv * a;
+    double t = (c.rec(top, ds) - dpv * a) / den;  // temp(nAlts+1) = dp(nAlts+1)
+    c.rec(top, bs) = den;
+    c.rec(top, ds) = t;
+    for (int i = nA; i >= 0; i--) {
+      t = c.rec(i, ds) - c.rec(i, cs) * t;
+      c.rec(i, ds) = t;
+    }
   }
   __syncthreads();
-  if (k < 0 || k > nA + 1) return;
-  // the increments and add_sources' updates (:53-70, :94-114), one level per
-  // lane
-  const double cond = c.rec(k, R_S1 + 0);
+  // 4. the first stage's value s1 and the full step's fv (dTdt_cond,
+  // :249-250), and the second Dt/2 solve's right side
+  double s1[NQ], fv[NQ], vup = 0.0;
+  if (lev) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      const double in = q == 0 ? qin[0] * unit : qin[q];
+      const double ra = c.rec(k, R_DA + q) - in, rc = c.rec(k, R_DC + q) - in;
+      s1[q] = q == 0 ? qin[0] + ra / unit : qin[q] + ra;
+      fv[q] = q == 0 ? qin[0] + rc / unit : qin[q] + rc;
+      // Viscosity(:,:,:,iUp_) keeps the last species' full-step result
+      // (calc_viscosity.f90:161)
+      if (q == NQ - 1) vup = rc;
+      const double told = q == 0 ? s1[0] * unit : s1[q];
+      c.rec(k, R_BC + q) = (k >= 1 && k <= nA) ? -told / (dtA / mr[q]) : -told;
+    }
+  }
+  __syncthreads();
+  // 5. the second Dt/2 solve (B): A's pivots and cp, its own dp
+  for (int q = j; q < NQ; q += (int)blockDim.x) {
+    double dpv = c.rec(0, R_BC + q) / -1.0;
+    c.rec(0, R_BC + q) = dpv;
+    for (int i = 1; i <= nA + 1; i++) {
+      dpv = (c.rec(i, R_BC + q) - dpv * c.rec(i, R_A + q)) / c.rec(i, R_BA + q);
+      c.rec(i, R_BC + q) = dpv;
+    }
+    double t = dpv;
+    for (int i = nA; i >= 0; i--) {
+      t = c.rec(i, R_BC + q) - c.rec(i, R_CPA + q) * t;
+      c.rec(i, R_BC + q) = t;
+    }
+  }
+  __syncthreads();
+  if (!lev) return;
+  // 6. the combination, the increments and add_sources' updates
+  // (:53-70, :94-114), one level per lane
+  double out[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const double in = q == 0 ? s1[0] * unit : s1[q];
+    const double rb = c.rec(k, R_BC + q) - in;
+    const double h = q == 0 ? s1[0] + rb / unit : s1[q] + rb;
+    out[q] = (2.0 * h - fv[q]) - qin[q];
+  }
+  const double cond = out[0];
   if (f.conduction.ptr) *c.ptr(f.conduction, k) = cond;
   if (f.viscosity.ptr) {
-    *c.ptr(f.viscosity, k, 0) = c.rec(k, R_S1 + 1);
-    *c.ptr(f.viscosity, k, 1) = c.rec(k, R_S1 + 2);
-    *c.ptr(f.viscosity, k, 2) = c.rec(k, R_VUP);
+    *c.ptr(f.viscosity, k, 0) = out[1];
+    *c.ptr(f.viscosity, k, 1) = out[2];
+    *c.ptr(f.viscosity, k, 2) = vup;
   }
   if (f.vvisc.ptr)
 #pragma unroll
-    for (int s = 0; s < gv::NSP; s++) *c.ptr(f.vvisc, k, s) = c.rec(k, R_S1 + 3 + s);
+    for (int s = 0; s < gv::NSP; s++) *c.ptr(f.vvisc, k, s) = out[3 + s];
   if (!p.apply) return;
-  double tt = c.rec(k, R_QIN + 0);
+  double tt = qin[0];
   if (k >= 1 && k <= nA) {
     tt = tt + p.dt * (f.heating.ptr ? c.at(f.heating, k) : 0.0);
     if (f.chem_heating.ptr) tt = tt + c.at(f.chem_heating, k);
     if (f.user_heating.ptr) tt = tt + c.at(f.user_heating, k);
   }
   *c.ptr(io.temperature, k) = tt + cond;
-  *c.ptr(io.velocity, k, 0) = c.rec(k, R_QIN + 1) + c.rec(k, R_S1 + 1);
-  *c.ptr(io.velocity, k, 1) = c.rec(k, R_QIN + 2) + c.rec(k, R_S1 + 2);
+  *c.ptr(io.velocity, k, 0) = qin[1] + out[1];
+  *c.ptr(io.velocity, k, 1) = qin[2] + out[2];
   {
     double *vu = c.ptr(io.velocity, k, 2);
-    *vu = *vu + c.rec(k, R_VUP);
+    *vu = *vu + vup;
   }
 #pragma unroll
-  for (int s = 0; s < gv::NSP; s++)
-    *c.ptr(io.vertical_velocity, k, s) = c.rec(k, R_QIN + 3 + s) + c.rec(k, R_S1 + 3 + s);
+  for (int s = 0; s < gv::NSP; s++) *c.ptr(io.vertical_velocity, k, s) = qin[3 + s] + out[3 + s];
 }
 
 }  // namespace
@@ -272,7 +320,7 @@ extern "C" int gitm_vert_sources(gitm_vert_handle *h, const gitm_vert_sources_pa
   f.viscosity = sf->viscosity;
   f.vvisc = sf->vertical_viscosity;
   const size_t lds = (size_t)R_LEN * (size_t)(c->n_alts + 4) * sizeof(double);
-  if (lds > (size_t)160 * 1024) return GITM_VERT_EUNSUPPORTED;  // nAlts > 231
+  if (lds > (size_t)160 * 1024) return GITM_VERT_EUNSUPPORTED;  // 129 KB at nAlts 252
   const dim3 grid((unsigned)ncols), block(c->n_alts + 4);
   switch (gv::auto_waves(c->n_alts)) {
     case 1: hipLaunchKernelGGL(sources_kernel<64>, grid, block, lds, h->stream, p, *c, f); break;

@@ -125,10 +125,6 @@ double ov_prepare(const ov_prep_params *p, int n_cols, int n_alts, const double 
                   double *cp, double *gamma, double *mmm, double *tempunit,
                   double *eddy, double *cmax);
 
-#ifdef __cplusplus
-}
-#endif
-#endif
 
 /* The operator-split sources step's neutral terms (add_sources.f90:53-66,
  * 94-107): thermal conduction (calc_conduction.f90:19-125), viscosity
@@ -173,3 +169,8 @@ void ov_sources_step(const ov_sources *s);
 void ov_conduction_1d(int n_alts, double dt, int neumann, const double *quantity,
                       const double *diff, const double *mulfac, const double *alt,
                       const double *raddist, double *result);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -36,10 +36,13 @@ def gpu_sources(solver, cols, src, dt, apply, turbulent):
 
 @pytest.mark.parametrize("config,apply,turbulent", [
     ("cpu_324x50", True, False), ("cpu_324x50", False, True), ("1deg_64800x100", True, True),
-    ("short", True, False)])
+    ("short", True, False), ("max", True, True)])
 def test_sources_match_oracle(solver, config, apply, turbulent):
-    # "short": 10 levels, fewer lanes than the 18 concurrent solves
-    spec = synth.GridSpec(1, 1, 10) if config == "short" else synth.CONFIGS[config]
+    # "short": 10 levels, fewer lanes than the 18 concurrent solves; "max":
+    # the largest column (252 levels, four wavefronts, stretched grid)
+    small = {"short": synth.GridSpec(1, 1, 10),
+             "max": synth.GridSpec(1, 1, 252, nlons=3, nlats=2, stretched=True)}
+    spec = small[config] if config in small else synth.CONFIGS[config]
     cols = synth.make_columns(spec, seed=51)
     if spec.ncols > 4096:
         cols = subset(cols, np.random.default_rng(5).choice(spec.ncols, 300, replace=False))
