@@ -380,3 +380,142 @@ def source_fields(cols: dict, seed: int = 1, heating_peak: float = 150.0e3,
     q = heating_wkg * np.exp(-((cols["alt"] - heating_peak) / heating_width) ** 2)
     return dict(cp=cp, kappa_temp=kappa, visc_coef=visc, visc_coef_s=visc_s,
                 raddist=raddist, heating=q / (cp * tu))
+
+
+# --------------------------------------------------------------------------
+# Block grids with halo cells, for the horizontal step.
+
+HORIZ_STATE = ("rho", "temp", "vel", "vvel", "nds")
+HORIZ_GEOMETRY = ("inv_dlon_gb", "inv_dlon_fb", "dlon_fb", "inv_dlat_gb", "inv_dlat_fb",
+                  "dlat_fb", "invr", "raddist")
+
+
+@dataclasses.dataclass(frozen=True)
+class BlockGrid:
+    """A global lat/lon grid of ``nblocks_lon x nblocks_lat`` blocks of
+    ``nlons x nlats`` cells, uniform altitudes from 100 km."""
+
+    nblocks_lon: int
+    nblocks_lat: int
+    nalts: int
+    nlons: int = 9
+    nlats: int = 9
+    alt_min: float = 100.0e3
+    dz: float = 5.0e3
+
+    @property
+    def nblocks(self) -> int:
+        return self.nblocks_lon * self.nblocks_lat
+
+    @property
+    def ncells(self) -> int:
+        return self.nblocks * self.nlons * self.nlats
+
+
+BLOCK_CONFIGS = {
+    # 1 degree: 360 x 180 cells in 9 x 9 blocks, 100 levels
+    "1deg_blocks_800x100": BlockGrid(40, 20, 100),
+    # 5 degrees: 72 x 36 cells, 50 levels
+    "5deg_blocks_32x50": BlockGrid(8, 4, 50),
+}
+
+
+def block_grid(spec: BlockGrid, seed: int = 1, ions: bool = False) -> dict:
+    """Block-layout state, geometry and cMax of a smooth synthetic
+    atmosphere.  Every per-cell field is [block][component][level][lat][lon]
+    with levels -1..nAlts+2 and lat/lon -1..n+2 (halo cells hold the
+    neighbouring blocks' values, as after GITM's message exchange; cells
+    past a pole continue the analytic fields).  The geometry follows
+    initialize.f90:185-267 (cell sizes half way between neighbours, face
+    distances between centres, cos(latitude) floored at 0.01, tangent
+    capped at 100); cMax(east, north) = |v| + sqrt(gamma T) on cells 0..n+1
+    as calc_timestep_horizontal (calc_timestep.f90:34-41), up as the
+    vertical CFL speed."""
+    nb, nlo, nla, nA = spec.nblocks, spec.nlons, spec.nlats, spec.nalts
+    L, NLo, NLg = nA + 4, nlo + 4, nla + 4
+    dlon = 2.0 * np.pi / (spec.nblocks_lon * nlo)
+    dlat = np.pi / (spec.nblocks_lat * nla)
+    bi = np.arange(nb) % spec.nblocks_lon
+    bj = np.arange(nb) // spec.nblocks_lon
+    ii = np.arange(-1, nlo + 3)   # lon index -1..nLons+2
+    jj = np.arange(-1, nla + 3)
+    lon = (bi[:, None] * nlo + ii[None, :] - 0.5) * dlon           # [nb][NLo]
+    lat = -0.5 * np.pi + (bj[:, None] * nla + jj[None, :] - 0.5) * dlat  # [nb][NLg]
+    alt = spec.alt_min + (np.arange(L) - 1.5) * spec.dz
+    r = R_EARTH + alt
+    shp = (nb, L, NLg, NLo)
+    LON = np.broadcast_to(lon[:, None, None, :], shp)
+    LAT = np.broadcast_to(lat[:, None, :, None], shp)
+    Z = np.broadcast_to(alt[None, :, None, None], shp)
+    R = np.broadcast_to(r[None, :, None, None], shp)
+    ph = 0.7 * seed
+    T = (1000.0 - 800.0 * np.exp(-(Z - 100.0e3) / 50.0e3)) * (
+        1.0 + 0.05 * np.cos(LAT) * np.sin(LON + ph) + 0.02 * np.sin(3 * LAT + ph))
+    m = species_masses()
+    nds = np.empty((nb, NSPECIES) + shp[1:])
+    for s in range(NSPECIES):
+        H = KB * 600.0 / (m[s] * 9.5)
+        nds[:, s] = BASE_DENSITY[s] * np.exp(-(Z - 100.0e3) / H) * (
+            1.0 + 0.1 * np.sin(2 * LON + s + ph) * np.cos(LAT) + 0.03 * np.cos(LAT * (s + 1)))
+    rho = np.einsum("s,bs...->b...", m, nds)
+    ntot = nds.sum(axis=1)
+    mmm = rho / ntot
+    temp = T / (mmm / KB)                       # Temperature = T / TempUnit
+    vel = np.empty((nb, 3) + shp[1:])
+    vel[:, 0] = 80.0 * np.cos(LAT) * np.sin(LON + 2 * LAT + ph) + 20.0 * np.sin(3 * LON)
+    vel[:, 1] = 60.0 * np.sin(2 * LAT) * np.cos(LON - ph) + 10.0 * np.cos(2 * LON + LAT)
+    vel[:, 2] = 3.0 * np.sin(LON + LAT + ph) * (Z / 400.0e3)
+    vvel = np.empty((nb, NSPECIES) + shp[1:])
+    for s in range(NSPECIES):
+        vvel[:, s] = vel[:, 2] + (1.0 + 0.3 * s) * np.cos(LON * (s % 3 + 1) + LAT + ph)
+    gamma = 1.0 + 2.0 / ntot * sum(nds[:, s] / (VIBRATION[s] - 2.0) for s in range(NSPECIES))
+    out = {"rho": rho, "temp": temp, "vel": vel, "vvel": vvel, "nds": nds, "gamma": gamma}
+    if ions:
+        base = (2e11, 1e11, 5e10, 1e9, 1e10, 1e8, 1e8, 1e9, 1e8)
+        idens = np.empty((nb, NIONS) + shp[1:])
+        for s in range(NIONS - 1):
+            idens[:, s] = base[s] * np.exp(-((Z - 300.0e3) / 150.0e3) ** 2) * (
+                1.0 + 0.2 * np.cos(LON + s * 0.3 + ph) * np.cos(LAT)) + 1.0e3
+        idens[:, NIONS - 1] = idens[:, :NIONS - 1].sum(axis=1)
+        ivel = np.empty((nb, 3) + shp[1:])
+        ivel[:, 0] = 150.0 * np.sin(LON + ph) * np.cos(LAT)
+        ivel[:, 1] = 100.0 * np.cos(2 * LON) * np.sin(LAT)
+        ivel[:, 2] = 10.0 * np.sin(LAT + ph)
+        out["ions"], out["ivel"] = idens, ivel
+    # geometry (initialize.f90:185-267)
+    cosl = np.cos(lat)
+    fl = np.maximum(np.abs(cosl), 0.01)[:, None, :, None]
+    lonp = np.concatenate([lon[:, :1] - dlon, lon, lon[:, -1:] + dlon], axis=1)  # -2..n+3
+    latp = np.concatenate([lat[:, :1] - dlat, lat, lat[:, -1:] + dlat], axis=1)
+    Rm_lon = np.concatenate([R[..., :1], R[..., :-1]], axis=3)  # r at lon-1
+    Rm_lat = np.concatenate([R[:, :, :1], R[:, :, :-1]], axis=2)
+    dlat_gb = 0.5 * (latp[:, 2:] - latp[:, :-2])[:, None, :, None] * R
+    dlat_fb = (latp[:, 1:-1] - latp[:, :-2])[:, None, :, None] * (0.5 * (R + Rm_lat))
+    dlon_gb = 0.5 * (lonp[:, 2:] - lonp[:, :-2])[:, None, None, :] * R * fl
+    dlon_fb = (lonp[:, 1:-1] - lonp[:, :-2])[:, None, None, :] * (0.5 * (R + Rm_lon)) * fl
+    out.update({"inv_dlon_gb": 1.0 / dlon_gb, "inv_dlon_fb": 1.0 / dlon_fb,
+                "dlon_fb": np.ascontiguousarray(dlon_fb),
+                "inv_dlat_gb": 1.0 / dlat_gb, "inv_dlat_fb": 1.0 / dlat_fb,
+                "dlat_fb": np.ascontiguousarray(dlat_fb),
+                "invr": 1.0 / np.ascontiguousarray(R), "raddist": np.ascontiguousarray(R),
+                "latitude": np.ascontiguousarray(lat),
+                "tan_lat": np.minimum(np.abs(np.tan(lat)), 100.0) * np.sign(lat),
+                "cos_lat": np.ascontiguousarray(cosl)})
+    # cMax_GDB [nb][3][0..nAlts+1][0..nLats+1][0..nLons+1] (iEast_, iNorth_, iUp_)
+    cs = np.sqrt(gamma * temp)[:, 1:nA + 3, 1:nla + 3, 1:nlo + 3]
+    cmax = np.empty((nb, 3, nA + 2, nla + 2, nlo + 2))
+    cmax[:, 0] = np.abs(vel[:, 0, 1:nA + 3, 1:nla + 3, 1:nlo + 3]) + cs
+    cmax[:, 1] = np.abs(vel[:, 1, 1:nA + 3, 1:nla + 3, 1:nlo + 3]) + cs
+    cmax[:, 2] = np.abs(vvel[:, :, 1:nA + 3, 1:nla + 3, 1:nlo + 3]).max(axis=1) + cs
+    out["cmax"] = cmax
+    return {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in out.items()}
+
+
+def horizontal_dt(grid: dict, cfl: float = 0.25) -> float:
+    """The horizontal CFL step of calc_timestep_horizontal
+    (calc_timestep.f90:43-50): cfl / max(cMax_E / dLon + cMax_N / dLat) over
+    interior cells and levels 1..nAlts."""
+    c = grid["cmax"][:, :, 1:-1, 1:-1, 1:-1]
+    igl = grid["inv_dlon_gb"][:, 2:-2, 2:-2, 2:-2]
+    iga = grid["inv_dlat_gb"][:, 2:-2, 2:-2, 2:-2]
+    return float(cfl / np.max(c[:, 0] * igl + c[:, 1] * iga))

@@ -170,6 +170,32 @@ void ov_conduction_1d(int n_alts, double dt, int neumann, const double *quantity
                       const double *diff, const double *mulfac, const double *alt,
                       const double *raddist, double *result);
 
+/* The horizontal step (advance_horizontal(iBlock), advance_horizontal.f90:
+ * 58-924) over every block and altitude 1..nAlts.  Block layout: state and
+ * geometry fields [block][component][level][lat][lon] with level
+ * -1..nAlts+2, lat -1..nLats+2, lon -1..nLons+2; cmax [block][3][0..nAlts+1]
+ * [0..nLats+1][0..nLons+1] (components east, north, up); latitude, tan_lat
+ * and cos_lat [block][-1..nLats+2].  Halo cells are read, never written.
+ * Returns OV_OK, OV_ERR_NONPOSITIVE_DENSITY for a negative neutral density
+ * after the step (the reference stops, :358-383) or OV_ERR_BAD_ARGS. */
+typedef struct {
+  int n_lons, n_lats, n_alts, n_blocks;
+  double dt, omega_body, beta_limiter;
+  int use_coriolis, use_ion_advection;
+  /* state, updated on interior cells of levels 1..nAlts */
+  double *rho, *temp, *vel, *vvel, *nds; /* vel [3], vvel and nds [6]     */
+  double *ions;                          /* IDensityS [10], ion advection */
+  /* inputs */
+  const double *gamma, *ivel, *invr, *raddist; /* ivel [3]                 */
+  const double *inv_dlon_gb, *inv_dlon_fb, *dlon_fb;
+  const double *inv_dlat_gb, *inv_dlat_fb, *dlat_fb;
+  const double *latitude, *tan_lat, *cos_lat;
+  const double *cmax;
+  double *temp_source; /* HorizontalTempSource (interior), or NULL        */
+} oh_grid;
+
+int oh_advance_horizontal(const oh_grid *p, ov_status *status);
+
 #ifdef __cplusplus
 }
 #endif

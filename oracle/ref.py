@@ -70,3 +70,58 @@ def advance(cols: dict, params, solver: int = 1, n_steps: int = 1) -> dict:
         out[k] = vals[pos:pos + n].reshape(shp).copy()
         pos += n
     return out
+
+
+HEXE = os.path.join(HERE, "_ref", "gitm_horiz_ref")
+HMAGIC = 1196840531
+REF_BLOCK = (9, 9, 4)  # nLons, nLats, nBlocksMax of src/ModSize.f90.orig
+
+
+def horizontal_available() -> bool:
+    return os.path.exists(HEXE)
+
+
+def horizontal(grid: dict, dt: float, omega_body: float, beta_limiter: float,
+               use_coriolis: bool = True, use_ion_advection: bool = False) -> dict:
+    """The reference's advance_horizontal on every block of the block-layout
+    ``grid`` (gitm_amd.synth.block_grid; [block][comp][level][lat][lon] is
+    the reference's own (lon, lat, level, comp, block) order)."""
+    nb, L, NLg, NLo = grid["rho"].shape
+    nlo, nla, na = NLo - 4, NLg - 4, L - 4
+    if (nlo, nla, na) != (REF_BLOCK[0], REF_BLOCK[1], REF_NALTS) or nb > REF_BLOCK[2]:
+        raise ValueError("reference build is compiled for 9x9 blocks, nAlts=50, <= 4 blocks")
+    hdr = np.zeros(16, dtype=np.int32)
+    hdr[:7] = [HMAGIC, nb, nlo, nla, na, int(use_coriolis), int(use_ion_advection)]
+    par = np.zeros(8)
+    par[:3] = [dt, omega_body, beta_limiter]
+    order = ["rho", "temp", "vel", "vvel", "nds", "gamma", "invr", "raddist", "inv_dlon_gb",
+             "inv_dlon_fb", "dlon_fb", "inv_dlat_gb", "inv_dlat_fb", "dlat_fb", "latitude",
+             "tan_lat", "cos_lat", "cmax"]
+    if use_ion_advection:
+        order += ["ions", "ivel"]
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        fout = os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(hdr.tobytes())
+            f.write(par.tobytes())
+            for k in order:
+                f.write(_fortran_order(grid[k]).tobytes())
+        subprocess.run([HEXE, fin, fout], check=True, capture_output=True)
+        raw = np.fromfile(fout, dtype=np.uint8)
+    status = raw[:16].view(np.int32)
+    if status[0]:
+        raise RuntimeError(f"reference status {status}")
+    vals = raw[16:].view(np.float64)
+    shp = (L, NLg, NLo)
+    shapes = [("rho", (nb,) + shp), ("temp", (nb,) + shp), ("vel", (nb, 3) + shp),
+              ("vvel", (nb, 6) + shp), ("nds", (nb, 6) + shp),
+              ("temp_source", (nb, na, nla, nlo))]
+    if use_ion_advection:
+        shapes.append(("ions", (nb, 10) + shp))
+    out, pos = {}, 0
+    for k, s in shapes:
+        n = int(np.prod(s))
+        out[k] = vals[pos:pos + n].reshape(s).copy()
+        pos += n
+    return out
