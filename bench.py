@@ -248,6 +248,62 @@ def bench_sources(torch, solver, dev, spec, reps: int = 10) -> dict:
             "column_updates_per_s": spec.ncols / (ms * 1e-3)}
 
 
+def horizontal_bytes_per_slab(nlo: int, nla: int, ions: bool = False) -> int:
+    """Algorithmic HBM bytes of one altitude of one block of the horizontal
+    step: every advected field read on the block's cells and halo cells
+    ((n+4)^2 each: Rho, Temperature, Velocity 3, NDensityS 6,
+    VerticalVelocity 6, with ions IDensityS 2) and written on its cells,
+    plus HorizontalTempSource; Gamma, InvRadialDistance and
+    RadialDistance on the cells; the face inverse distances on faces
+    0..n+2 and the face distances on 1..n+1 of each row or column, the cell
+    inverse distances, and cMax(east, north) on cells 0..n+1 of each row or
+    column; the three latitude arrays."""
+    nv = 17 + (2 if ions else 0)
+    cells, halo = nlo * nla, (nlo + 4) * (nla + 4)
+    reads = (nv * halo + 3 * cells + (nlo + 3) * nla + nlo * (nla + 3) + (nlo + 1) * nla
+             + nlo * (nla + 1) + 2 * cells + (nlo + 2) * nla + nlo * (nla + 2) + 3 * nla)
+    writes = (nv + 1) * cells
+    return 8 * (reads + writes)
+
+
+def bench_horizontal(torch, solver, reps: int = 10) -> dict:
+    """The horizontal step (gitm_horiz_advance) on the 1-degree block grid
+    (800 blocks of 9 x 9 cells, 100 levels), timed with HIP events."""
+    from gitm_amd import synth
+
+    spec = synth.BLOCK_CONFIGS["1deg_blocks_800x100"]
+    host = synth.block_grid(spec, seed=1)
+    dt = synth.horizontal_dt(host, cfl=0.4)
+    g = {k: torch.from_numpy(v).cuda() for k, v in host.items()}
+    del host
+    g["temp_source"] = torch.zeros_like(g["rho"])
+    solver.horizontal(g, dt)
+    torch.cuda.synchronize()
+    solver.check()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        solver.horizontal(g, dt)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    solver.check()
+    ms = e0.elapsed_time(e1) / reps
+    ok = bool(torch.isfinite(g["temp"]).all()) and bool((g["nds"] > 0).all())
+    slabs = spec.nblocks * spec.nalts
+    nbytes = horizontal_bytes_per_slab(spec.nlons, spec.nlats) * slabs
+    del g
+    return {"kernel": "horiz_kernel<17> (gitm_horiz_advance: advance_horizontal, RK4 of "
+                      "the Rusanov/MC lat-lon solver)",
+            "workload": "1deg_blocks_800x100: 360x180 cells in 800 blocks of 9x9, "
+                        "100 levels, Coriolis on, ions off",
+            "dt": dt, "launch_ms": ms, "bound": "hbm", "bytes_per_launch": nbytes,
+            "achieved": nbytes / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "cell_updates_per_s": spec.ncells * spec.nalts / (ms * 1e-3),
+            "state_finite_and_positive": ok}
+
+
 def bench_adjoint(torch, solver, steps: int = 10, seed: int = 5) -> dict:
     """Differentiation workload: the 5 deg x 5 deg grid (2592 columns x 50
     levels), a window of ``steps`` x (AUSM step, Gaussian heating source
@@ -302,6 +358,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 64-member ragged-ensemble figure")
+    ap.add_argument("--no-horizontal", action="store_true",
+                    help="skip the horizontal-step figure")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint/forward ratio")
     ap.add_argument("--copy-probe", action="store_true",
@@ -369,9 +427,17 @@ def main() -> int:
     achieved = per_launch / (launch_ms * 1e-3) / 1e9
     traffic = read_traffic(TRAFFIC_FILE) if args.solver == "ausm" else None
 
+    hor = None
+    if not args.no_horizontal:
+        dev = None  # the 1-degree columns are no longer needed
+        torch.cuda.empty_cache()
+        print("bench: horizontal step", file=sys.stderr, flush=True)
+        hor = bench_horizontal(torch, solver)
+        torch.cuda.empty_cache()
+
     ens = None
     if not args.no_ensemble:
-        del dev
+        dev = None
         torch.cuda.empty_cache()
         print("bench: 64-member ensemble", file=sys.stderr, flush=True)
         ens = bench_ensemble(torch, solver, params)
@@ -427,6 +493,7 @@ def main() -> int:
                  / (prep["launch_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
                  "cfl_dt": prep["dt"]},
         "sources": src,
+        "horizontal": hor,
         "ensemble": ens,
         "adjoint_forward_ratio": adj["ratio"] if adj else None,
         "adjoint": adj,

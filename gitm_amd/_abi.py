@@ -164,3 +164,26 @@ SOURCES_FIELDS = (("kappa_temp", 1), ("visc_coef", 1), ("visc_coef_s", NSPECIES)
 
 class SourcesFields(ctypes.Structure):
     _fields_ = [(n, Field) for n, _ in SOURCES_FIELDS]
+
+
+class HorizParams(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("omega_body", ctypes.c_double),
+                ("beta_limiter", ctypes.c_double), ("use_coriolis", ctypes.c_int32),
+                ("use_ion_advection", ctypes.c_int32),
+                ("use_improved_ion_advection", ctypes.c_int32),
+                ("use_topography", ctypes.c_int32),
+                ("altitudes_per_workgroup", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+HORIZ_FIELDS = ("inv_dlon_gb", "inv_dlon_fb", "dlon_fb", "inv_dlat_gb", "inv_dlat_fb",
+                "dlat_fb", "radial_distance", "latitude", "tan_latitude", "cmax_horizontal",
+                "temp_source")
+
+
+class HorizFields(ctypes.Structure):
+    _fields_ = [(n, Field) for n in HORIZ_FIELDS]
+
+
+def latitude_field(ptr: int, n_lats: int) -> Field:
+    """Descriptor of a Latitude-like array (-1:nLats+2, nBlocks): ptr at (1,1)."""
+    return Field(ptr + 8 * 2, 0, 1, n_lats + 4, 0, 0, 0, 0)

@@ -24,9 +24,9 @@ HIPFLAGS = [
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
 ]
 # translation units, compiled in parallel: the single-grid step and C ABI,
-# the ensemble batches, the adjoint, the sources step
+# the ensemble batches, the adjoint, the sources step, the horizontal step
 SOURCES = ("gitm_vert.hip", "gitm_vert_batch.hip", "gitm_vert_adjoint.hip",
-           "gitm_vert_sources.hip")
+           "gitm_vert_sources.hip", "gitm_horiz.hip")
 
 
 def build(verbose: bool = False, extra: list[str] | None = None, out: str | None = None) -> str:

@@ -1,0 +1,367 @@
+// gitm_horiz.hip -- the horizontal step (gitm_horiz_advance of
+// include/gitm_vert.h): advance_horizontal(iBlock)
+// (advance_horizontal.f90:58-924) for every block, each altitude 1..nAlts
+// an independent 2-D problem on the block's cells with the halo cells
+// (-1:0, n+1:n+2) held at their input values (the caller's message
+// exchange fills them, :20).
+//
+// Mapping: a workgroup takes APW altitudes of one block, a lane one cell of
+// one of them.  The workgroup stages the APW slices of every advected
+// variable (halo cells included) in LDS; each stage of the RK4 (:154-345)
+// then runs horizontal_solver (:535-818) lane by lane: the lane forms the
+// MC-limited face values of its own two faces in each direction
+// (calc_facevalues_lons/lats, :928-1034) from the slopes at its cell and
+// its two neighbours, the gradient and the Rusanov flux difference
+// (calc_rusanov_lons/lats, :821-922), and the updates, holding the new
+// values in registers until every lane has read the stage's state.  Every
+// value is formed by the reference's operations in the reference's order,
+// so the results equal the reference's bit for bit (sin(latitude) aside,
+// which is the device's).
+#include "vert_gpu.h"
+
+namespace {
+
+constexpr int NIA = 2;  // nIonsAdvect (ModEarth.f90:39)
+constexpr int V_RHO = 0, V_TEMP = 1, V_VEL = 2, V_NUM = 5, V_VV = 11, V_INUM = 17;
+constexpr int NV_NEUTRAL = V_INUM, NV_IONS = V_INUM + NIA;
+
+struct HzParams {
+  double dt, omega, beta;
+  int use_coriolis;
+  int n_lons, n_lats, n_alts, n_blocks;
+  int apw;         // altitudes per workgroup
+  int alt_groups;  // workgroups per block
+};
+
+struct HzFields {
+  gitm_vert_field inv_dlon_gb, inv_dlon_fb, dlon_fb, inv_dlat_gb, inv_dlat_fb, dlat_fb,
+      raddist, latitude, tan_latitude, cmax, temp_source;
+};
+
+// Element (iLon, iLat, level, comp) of block b: the descriptor's ptr is at
+// (1, 1, first_level) of block 0.
+__device__ __forceinline__ double *hf(const gitm_vert_field &f, int lon, int lat, int lev,
+                                      int comp, int b) {
+  return f.ptr + (long long)(lon - 1) * f.s_lon + (long long)(lat - 1) * f.s_lat +
+         (long long)(lev - f.first_level) * f.s_lev + (long long)comp * f.s_comp +
+         (long long)b * f.s_blk;
+}
+
+__device__ __forceinline__ double limiter(double beta, double up, double dn) {
+  // Limiter_mc (ModLimiter.f90:16-36)
+  if (up > 0.0) {
+    if (dn > 0.0) return fmin(fmin(beta * up, beta * dn), (up + dn) * 0.5);
+    return 0.0;
+  }
+  if (dn < 0.0) return fmax(fmax(beta * up, beta * dn), (up + dn) * 0.5);
+  return 0.0;
+}
+
+// The geometry of one cell i (1..n) along one line: InvD*Dist_FB at faces
+// i-1..i+2, d*Dist_FB at faces i and i+1, InvD*Dist_GB(i) and cMax at cells
+// i-1..i+1.
+struct LineGeo {
+  double ifb[4], dfb0, dfb1, igb, cm[3];
+};
+
+// One cell i (1..n) of one line: v(m) = Var(i + m), m = -3..3 (the line's
+// cells -1..n+2 only).  Returns the gradient and the flux difference.
+template <class VF>
+__device__ __forceinline__ void rusanov_cell(int i, int n, VF v, const LineGeo &g,
+                                             double beta, double &grad, double &diff) {
+  auto ifb = [&](int m) { return g.ifb[m + 1]; };
+  const double dfb0 = g.dfb0, dfb1 = g.dfb1, igb = g.igb, cmm = g.cm[0], cm0 = g.cm[1],
+               cmp = g.cm[2];
+  const double F1 = 0.6250000, F2 = 0.0416667;
+  double dl[3];
+#pragma unroll
+  for (int t = 0; t < 3; t++) {  // slopes at cells i-1, i, i+1
+    const int m = t - 1, c = i + m;
+    double up, dn;
+    if (c <= n) {
+      const double h = ifb(m + 1) * 2;
+      up = h * (F1 * (v(m + 1) - v(m)) - F2 * (v(m + 2) - v(m - 1)));
+    } else {
+      up = (v(m + 1) - v(m)) * ifb(m + 1);
+    }
+    if (c >= 1) {
+      const double h = ifb(m) * 2;
+      dn = h * (F1 * (v(m) - v(m - 1)) - F2 * (v(m + 1) - v(m - 2)));
+    } else {
+      dn = (v(m) - v(m - 1)) * ifb(m);
+    }
+    dl[t] = limiter(beta, up, dn);
+  }
+  const double l0 = v(-1) + 0.5 * dl[0] * dfb0, r0 = v(0) - 0.5 * dl[1] * dfb0;
+  const double l1 = v(0) + 0.5 * dl[1] * dfb1, r1 = v(1) - 0.5 * dl[2] * dfb1;
+  const double f0 = 0.5 * fmax(cmm, cm0) * (r0 - l0);
+  const double f1 = 0.5 * fmax(cm0, cmp) * (r1 - l1);
+  grad = 0.5 * (l1 + r1 - l0 - r0) * igb;
+  diff = (f1 - f0) * igb;
+}
+
+// (256 lanes at most; two wavefronts per SIMD: the LDS of three workgroups
+// per CU holds about that many anyway)
+template <int NV>
+__global__ void __launch_bounds__(256, 2)
+    horiz_kernel(HzParams p, gitm_vert_columns io, HzFields f, unsigned long long *err) {
+  extern __shared__ double lds[];
+  const int nlo = p.n_lons, nla = p.n_lats, NLo = nlo + 4, NLg = nla + 4;
+  const int cells = nlo * nla, slab = NV * NLg * NLo;
+  const int b = (int)blockIdx.x / p.alt_groups;
+  const int k0 = 1 + ((int)blockIdx.x % p.alt_groups) * p.apw;
+  const int napw = min(p.apw, p.n_alts - k0 + 1);
+  const int tid = (int)threadIdx.x;
+  const int a = tid / cells, cell = tid % cells;
+  const bool active = a < napw;
+  const int i = cell % nlo + 1, jl = cell / nlo + 1, k = k0 + a;
+  // the advected fields (advance_horizontal.f90:154-167)
+  auto field = [&](int v, int &comp) -> const gitm_vert_field & {
+    if (v == V_RHO) { comp = 0; return io.rho; }
+    if (v == V_TEMP) { comp = 0; return io.temperature; }
+    if (v < V_NUM) { comp = v - V_VEL; return io.velocity; }
+    if (v < V_VV) { comp = v - V_NUM; return io.ndensity_s; }
+    if (v < V_INUM) { comp = v - V_VV; return io.vertical_velocity; }
+    comp = v - V_INUM;
+    return io.idensity_s;
+  };
+  // stage the slices: x[a][v][lat][lon], halo cells included; after the
+  // slices, per altitude, the stage's new values and the RK4 sums, [v][cell]
+  for (int e = tid; e < napw * slab; e += (int)blockDim.x) {
+    const int aa = e / slab, r = e % slab;
+    const int v = r / (NLg * NLo), r2 = r % (NLg * NLo);
+    const int lat = r2 / NLo - 1, lon = r2 % NLo - 1;
+    int comp;
+    const gitm_vert_field &g = field(v, comp);
+    lds[e] = *hf(g, lon, lat, k0 + aa, comp, b);
+  }
+  const int aw = active ? a : 0;
+  double *x = lds + (long long)aw * slab;
+  double *nwb = lds + (long long)p.apw * slab + (long long)aw * 2 * NV * cells;
+  double *accb = nwb + NV * cells;
+  auto X = [&](int v, int lat, int lon) -> double & {
+    return x[(v * NLg + lat + 1) * NLo + lon + 1];
+  };
+  auto NW = [&](int v) -> double & { return nwb[v * cells + cell]; };
+  auto ACC = [&](int v) -> double & { return accb[v * cells + cell]; };
+  __syncthreads();
+  // per-cell inputs of the updates
+  double invr = 0, rd = 0, gam = 0, tl = 0, cosl = 0, sinl = 0, ive = 0, ivn = 0;
+  if (active) {
+    invr = *hf(io.inv_radial_distance, i, jl, k, 0, b);
+    rd = *hf(f.raddist, i, jl, k, 0, b);
+    gam = *hf(io.gamma, i, jl, k, 0, b);
+    tl = *hf(f.tan_latitude, 1, jl, 0, 0, b);
+    cosl = *hf(io.cos_latitude, 1, jl, 0, 0, b);
+    sinl = sin(*hf(f.latitude, 1, jl, 0, 0, b));
+    if (NV == NV_IONS) {
+      ive = *hf(io.ivelocity, i, jl, k, 0, b);
+      ivn = *hf(io.ivelocity, i, jl, k, 1, b);
+    }
+  }
+  LineGeo glo{}, gla{};
+  if (active) {
+#pragma unroll
+    for (int m = -1; m <= 2; m++) {
+      glo.ifb[m + 1] = *hf(f.inv_dlon_fb, i + m, jl, k, 0, b);
+      gla.ifb[m + 1] = *hf(f.inv_dlat_fb, i, jl + m, k, 0, b);
+    }
+    glo.dfb0 = *hf(f.dlon_fb, i, jl, k, 0, b);
+    glo.dfb1 = *hf(f.dlon_fb, i + 1, jl, k, 0, b);
+    gla.dfb0 = *hf(f.dlat_fb, i, jl, k, 0, b);
+    gla.dfb1 = *hf(f.dlat_fb, i, jl + 1, k, 0, b);
+    glo.igb = *hf(f.inv_dlon_gb, i, jl, k, 0, b);
+    gla.igb = *hf(f.inv_dlat_gb, i, jl, k, 0, b);
+#pragma unroll
+    for (int m = -1; m <= 1; m++) {
+      glo.cm[m + 1] = *hf(f.cmax, i + m, jl, k, 0, b);
+      gla.cm[m + 1] = *hf(f.cmax, i, jl + m, k, 1, b);
+    }
+  }
+  const double dt = p.dt, om = p.omega;
+  const double csin = sinl * 2 * om, ccos = cosl * 2 * om;
+  const double cent = om * om * cosl * sinl;
+
+#pragma unroll 1
+  for (int stage = 0; stage < 4; stage++) {
+    if (active) {
+      // Grad and Diff of variable v in both directions (:588-662)
+      auto gd = [&](int v, double &glon, double &dlon, double &glat, double &dlat) {
+        rusanov_cell(i, nlo, [&](int m) { return X(v, jl, i + m); }, glo, p.beta, glon, dlon);
+        rusanov_cell(jl, nla, [&](int m) { return X(v, jl + m, i); }, gla, p.beta, glat, dlat);
+      };
+      const double rho = X(V_RHO, jl, i), tmp = X(V_TEMP, jl, i);
+      const double ve = X(V_VEL + 0, jl, i), vn = X(V_VEL + 1, jl, i), vu = X(V_VEL + 2, jl, i);
+      double gE[4], gN[4], gU[4], gR[4], gT[4];  // glon, dlon, glat, dlat
+      gd(V_VEL + 0, gE[0], gE[1], gE[2], gE[3]);
+      gd(V_VEL + 1, gN[0], gN[1], gN[2], gN[3]);
+      const double div = gN[2] + gE[0] - tl * vn * invr;
+      gd(V_RHO, gR[0], gR[1], gR[2], gR[3]);
+      gd(V_TEMP, gT[0], gT[1], gT[2], gT[3]);
+      // the updates (:667-816), New starting from the stage's state
+      NW(V_RHO) = rho - dt * (rho * div + gR[2] * vn + gR[0] * ve) + dt * (gR[1] + gR[3]);
+      double we = ve -
+                  dt * (vn * gE[2] + ve * gE[0] + ve * (vu - tl * vn) * invr + gT[0] +
+                        gR[0] * tmp / rho) +
+                  dt * (gE[1] + gE[3]);
+      double wn = vn -
+                  dt * (vn * gN[2] + ve * gN[0] + (vn * vu + tl * (ve * ve)) * invr + gT[2] +
+                        gR[2] * tmp / rho) +
+                  dt * (gN[1] + gN[3]);
+      if (p.use_coriolis) {
+        we = we + dt * (+csin * vn - ccos * vu);
+        wn = wn - dt * (cent * rd + csin * ve);
+      }
+      NW(V_VEL + 0) = we;
+      NW(V_VEL + 1) = wn;
+      NW(V_TEMP) = tmp - dt * ((gam - 1) * tmp * div + gT[2] * vn + gT[0] * ve) +
+                   dt * (gT[1] + gT[3]);
+      gd(V_VEL + 2, gU[0], gU[1], gU[2], gU[3]);
+      NW(V_VEL + 2) = vu - dt * (vn * gU[2] + ve * gU[0]) + dt * (gU[1] + gU[3]);
+#pragma unroll 1
+      for (int v = V_NUM; v < NV; v++) {
+        double g0, g1, g2, g3;
+        gd(v, g0, g1, g2, g3);
+        const double q = X(v, jl, i);
+        double w;
+        if (v < V_VV)  // NDensityS
+          w = q - dt * (q * div + g2 * vn + g0 * ve) + dt * (g1 + g3);
+        else if (v < V_INUM)  // VerticalVelocity
+          w = q - dt * (vn * g2 + ve * g0) + dt * (g3 + g1);
+        else  // IDensityS(1:nIonsAdvect), carried by the ion velocity
+          w = q - dt * (+g2 * ivn + g0 * ive) + dt * (g1 + g3);
+        NW(v) = w;
+      }
+    }
+    __syncthreads();  // every lane has read the stage's state
+    if (active) {
+      // the RK4 bookkeeping (:184-345)
+#pragma unroll 1
+      for (int v = 0; v < NV; v++) {
+        int comp;
+        const double orig = *hf(field(v, comp), i, jl, k, comp, b);
+        const double K = NW(v) - X(v, jl, i);
+        double u;
+        if (stage == 0) {
+          ACC(v) = K;
+          u = 0.5 * K + orig;
+        } else if (stage == 1) {
+          ACC(v) = ACC(v) + 2.0 * K;
+          u = 0.5 * K + orig;
+        } else if (stage == 2) {
+          ACC(v) = ACC(v) + 2.0 * K;
+          u = K + orig;
+        } else {
+          const double sum = ACC(v) + K;
+          u = orig + (1.0 / 6.0) * sum;
+        }
+        X(v, jl, i) = u;
+      }
+    }
+    __syncthreads();
+  }
+  if (!active) return;
+  // write back (:349-429)
+  bool negative = false;
+#pragma unroll
+  for (int s = 0; s < gv::NSP; s++) negative = negative || X(V_NUM + s, jl, i) < 0.0;
+  if (negative) {
+    const int column = (b * nla + (jl - 1)) * nlo + (i - 1);
+    atomicMin(err, gv::error_key(column, GITM_VERT_ENONPOSITIVE, k));
+  }
+  const double told = *hf(io.temperature, i, jl, k, 0, b);
+  *hf(io.rho, i, jl, k, 0, b) = X(V_RHO, jl, i);
+#pragma unroll
+  for (int c = 0; c < 3; c++) *hf(io.velocity, i, jl, k, c, b) = X(V_VEL + c, jl, i);
+  if (f.temp_source.ptr) *hf(f.temp_source, i, jl, k, 0, b) = X(V_TEMP, jl, i) - told;
+  *hf(io.temperature, i, jl, k, 0, b) = X(V_TEMP, jl, i);
+#pragma unroll
+  for (int s = 0; s < gv::NSP; s++) {
+    *hf(io.vertical_velocity, i, jl, k, s, b) = X(V_VV + s, jl, i);
+    *hf(io.ndensity_s, i, jl, k, s, b) = X(V_NUM + s, jl, i);
+  }
+  if (NV == NV_IONS) {
+#pragma unroll
+    for (int s = 0; s < NIA; s++) {
+      double v = X(V_INUM + s, jl, i);
+      if (v < 1.0e2) v = 1.0e2;  // :405-417
+      *hf(io.idensity_s, i, jl, k, s, b) = v;
+    }
+    double e = 0.0;  // electrons, :420-425
+    for (int s = 0; s < gv::NIONS - 1; s++) e = e + *hf(io.idensity_s, i, jl, k, s, b);
+    *hf(io.idensity_s, i, jl, k, gv::NIONS - 1, b) = e;
+  }
+}
+
+}  // namespace
+
+extern "C" int gitm_horiz_advance(gitm_vert_handle *h, const gitm_horiz_params *hp,
+                                  const gitm_vert_columns *c, const gitm_horiz_fields *hf_) {
+  if (!h || !hp || !c || !hf_) return GITM_VERT_EINVAL;
+  if (hp->use_topography || (hp->use_ion_advection && hp->use_improved_ion_advection))
+    return GITM_VERT_EUNSUPPORTED;
+  const int nlo = c->n_lons, nla = c->n_lats;
+  if (nlo < 1 || nla < 1 || c->n_blocks < 1 || c->n_alts < 1) return GITM_VERT_EINVAL;
+  const int cells = nlo * nla;
+  if (cells > 256) return GITM_VERT_EUNSUPPORTED;  // one lane per cell
+  if (!(hp->dt > 0.0)) return GITM_VERT_EINVAL;
+  const gitm_vert_field *req[] = {&c->rho, &c->temperature, &c->velocity,
+                                  &c->vertical_velocity, &c->ndensity_s, &c->gamma,
+                                  &c->inv_radial_distance, &c->cos_latitude,
+                                  &hf_->inv_dlon_gb, &hf_->inv_dlon_fb, &hf_->dlon_fb,
+                                  &hf_->inv_dlat_gb, &hf_->inv_dlat_fb, &hf_->dlat_fb,
+                                  &hf_->radial_distance, &hf_->latitude, &hf_->tan_latitude,
+                                  &hf_->cmax_horizontal};
+  for (const gitm_vert_field *g : req)
+    if (!g->ptr) return GITM_VERT_EINVAL;
+  const bool ions = hp->use_ion_advection != 0;
+  if (ions && (!c->idensity_s.ptr || !c->ivelocity.ptr)) return GITM_VERT_EINVAL;
+  const int nv = ions ? NV_IONS : NV_NEUTRAL;
+  // per altitude: the slab of every variable with halo cells, then the
+  // stage's new values and the RK4 sums of the block's cells
+  const size_t slab = (size_t)nv * ((nla + 4) * (nlo + 4) + 2 * cells) * sizeof(double);
+  // altitudes per workgroup: as many as fit 256 lanes and 80 KB of LDS
+  int apw = hp->altitudes_per_workgroup > 0 ? hp->altitudes_per_workgroup : 0;
+  if (apw == 0) {
+    apw = 1;
+    while ((apw + 1) * cells <= 256 && (apw + 1) * slab <= 80 * 1024 && apw < 4) apw++;
+  }
+  if (apw * cells > 256 || apw * slab > 160 * 1024) return GITM_VERT_EINVAL;
+  apw = std::min(apw, c->n_alts);
+  HzParams p;
+  p.dt = hp->dt;
+  p.omega = hp->omega_body;
+  p.beta = hp->beta_limiter;
+  p.use_coriolis = hp->use_coriolis != 0;
+  p.n_lons = nlo;
+  p.n_lats = nla;
+  p.n_alts = c->n_alts;
+  p.n_blocks = c->n_blocks;
+  p.apw = apw;
+  p.alt_groups = (c->n_alts + apw - 1) / apw;
+  const long long nwg = (long long)p.alt_groups * c->n_blocks;
+  if (nwg > 0x7fffffffLL) return GITM_VERT_EINVAL;
+  HzFields f;
+  f.inv_dlon_gb = hf_->inv_dlon_gb;
+  f.inv_dlon_fb = hf_->inv_dlon_fb;
+  f.dlon_fb = hf_->dlon_fb;
+  f.inv_dlat_gb = hf_->inv_dlat_gb;
+  f.inv_dlat_fb = hf_->inv_dlat_fb;
+  f.dlat_fb = hf_->dlat_fb;
+  f.raddist = hf_->radial_distance;
+  f.latitude = hf_->latitude;
+  f.tan_latitude = hf_->tan_latitude;
+  f.cmax = hf_->cmax_horizontal;
+  f.temp_source = hf_->temp_source;
+  GV_HIP(hipSetDevice(h->device));
+  const int threads = ((apw * cells + 63) / 64) * 64;
+  const size_t lds = (size_t)apw * slab;
+  if (ions)
+    hipLaunchKernelGGL(horiz_kernel<NV_IONS>, dim3((unsigned)nwg), dim3(threads), lds,
+                       h->stream, p, *c, f, h->d_err);
+  else
+    hipLaunchKernelGGL(horiz_kernel<NV_NEUTRAL>, dim3((unsigned)nwg), dim3(threads), lds,
+                       h->stream, p, *c, f, h->d_err);
+  GV_HIP(hipGetLastError());
+  return GITM_VERT_OK;
+}

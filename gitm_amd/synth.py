@@ -420,7 +420,7 @@ BLOCK_CONFIGS = {
 }
 
 
-def block_grid(spec: BlockGrid, seed: int = 1, ions: bool = False) -> dict:
+def block_grid(spec: BlockGrid, seed: int = 1, ions: bool = False, blocks=None) -> dict:
     """Block-layout state, geometry and cMax of a smooth synthetic
     atmosphere.  Every per-cell field is [block][component][level][lat][lon]
     with levels -1..nAlts+2 and lat/lon -1..n+2 (halo cells hold the
@@ -430,13 +430,16 @@ def block_grid(spec: BlockGrid, seed: int = 1, ions: bool = False) -> dict:
     distances between centres, cos(latitude) floored at 0.01, tangent
     capped at 100); cMax(east, north) = |v| + sqrt(gamma T) on cells 0..n+1
     as calc_timestep_horizontal (calc_timestep.f90:34-41), up as the
-    vertical CFL speed."""
-    nb, nlo, nla, nA = spec.nblocks, spec.nlons, spec.nlats, spec.nalts
+    vertical CFL speed.  ``blocks``: the block numbers to generate (all by
+    default; block b is at longitude b % nblocks_lon, latitude
+    b // nblocks_lon)."""
+    ids = np.arange(spec.nblocks) if blocks is None else np.asarray(blocks, dtype=np.int64)
+    nb, nlo, nla, nA = len(ids), spec.nlons, spec.nlats, spec.nalts
     L, NLo, NLg = nA + 4, nlo + 4, nla + 4
     dlon = 2.0 * np.pi / (spec.nblocks_lon * nlo)
     dlat = np.pi / (spec.nblocks_lat * nla)
-    bi = np.arange(nb) % spec.nblocks_lon
-    bj = np.arange(nb) // spec.nblocks_lon
+    bi = ids % spec.nblocks_lon
+    bj = ids // spec.nblocks_lon
     ii = np.arange(-1, nlo + 3)   # lon index -1..nLons+2
     jj = np.arange(-1, nla + 3)
     lon = (bi[:, None] * nlo + ii[None, :] - 0.5) * dlon           # [nb][NLo]

@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import ctypes
 import dataclasses
+import math
 import os
 import weakref
 
@@ -89,6 +90,10 @@ def lib() -> ctypes.CDLL:
         _lib.gitm_vert_sources.argtypes = [vp, ctypes.POINTER(_abi.SourcesParams),
                                            ctypes.POINTER(_abi.Columns),
                                            ctypes.POINTER(_abi.SourcesFields)]
+        _lib.gitm_horiz_advance.restype = ctypes.c_int
+        _lib.gitm_horiz_advance.argtypes = [vp, ctypes.POINTER(_abi.HorizParams),
+                                            ctypes.POINTER(_abi.Columns),
+                                            ctypes.POINTER(_abi.HorizFields)]
         _lib.gitm_vert_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]
         _lib.gitm_vert_destroy.argtypes = [ctypes.c_void_p]
         _lib.gitm_vert_set_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
@@ -285,6 +290,60 @@ class VerticalSolver:
         self.set_stream(torch.cuda.current_stream().cuda_stream)
         rc = lib().gitm_vert_sources(self._h, ctypes.byref(p), ctypes.byref(c),
                                      ctypes.byref(f))
+        if rc:
+            raise VerticalStepError(rc)
+
+    def horizontal(self, grid: dict, dt: float, omega_body: float = 2.0 * math.pi / 86400.0,
+                   beta_limiter: float = 2.0, use_coriolis: bool = True,
+                   use_ion_advection: bool = False, altitudes_per_workgroup: int = 0) -> None:
+        """The horizontal step (``gitm_horiz_advance``: advance_horizontal on
+        every block) on device tensors in GITM's block layout
+        ([block][component][level][lat][lon], halo cells included;
+        gitm_amd.synth.block_grid names and shapes).  The state ("rho",
+        "temp", "vel", "vvel", "nds", with ions "ions") is updated in place;
+        "temp_source", when present, receives HorizontalTempSource.  Queued
+        on the current stream; errors surface in ``check()``."""
+        import torch
+
+        nb, L, NLg, NLo = (int(v) for v in grid["rho"].shape)
+        nlo, nla, na = NLo - 4, NLg - 4, L - 4
+        for k, t in grid.items():
+            if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+                raise ValueError(f"{k}: need a contiguous float64 device tensor")
+
+        def blk(key, ncomp=1):
+            return _abi.gitm_block_field(grid[key].data_ptr(), nlo, nla, na, ncomp)
+
+        c = _abi.Columns()
+        c.n_lons, c.n_lats, c.n_blocks, c.n_alts = nlo, nla, nb, na
+        c.rho, c.temperature, c.gamma = blk("rho"), blk("temp"), blk("gamma")
+        c.velocity = blk("vel", 3)
+        c.vertical_velocity = blk("vvel", _abi.NSPECIES)
+        c.ndensity_s = blk("nds", _abi.NSPECIES)
+        c.inv_radial_distance = blk("invr")
+        c.cos_latitude = _abi.latitude_field(grid["cos_lat"].data_ptr(), nla)
+        if use_ion_advection:
+            c.idensity_s = blk("ions", _abi.NIONS)
+            c.ivelocity = blk("ivel", 3)
+        f = _abi.HorizFields()
+        for key in ("inv_dlon_gb", "inv_dlon_fb", "dlon_fb", "inv_dlat_gb", "inv_dlat_fb",
+                    "dlat_fb"):
+            setattr(f, key, blk(key))
+        f.radial_distance = blk("raddist")
+        f.latitude = _abi.latitude_field(grid["latitude"].data_ptr(), nla)
+        f.tan_latitude = _abi.latitude_field(grid["tan_lat"].data_ptr(), nla)
+        f.cmax_horizontal = _abi.gitm_block_field(grid["cmax"].data_ptr(), nlo, nla, na, 3,
+                                                  lon_lo=0, lat_lo=0, alt_lo=0,
+                                                  alt_hi=na + 1)
+        if "temp_source" in grid:
+            f.temp_source = blk("temp_source")
+        p = _abi.HorizParams()
+        p.dt, p.omega_body, p.beta_limiter = dt, omega_body, beta_limiter
+        p.use_coriolis, p.use_ion_advection = int(use_coriolis), int(use_ion_advection)
+        p.altitudes_per_workgroup = altitudes_per_workgroup
+        self.set_stream(torch.cuda.current_stream().cuda_stream)
+        rc = lib().gitm_horiz_advance(self._h, ctypes.byref(p), ctypes.byref(c),
+                                      ctypes.byref(f))
         if rc:
             raise VerticalStepError(rc)
 

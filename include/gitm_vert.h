@@ -333,10 +333,65 @@ typedef struct gitm_vert_sources_fields {
 
 /* From c's Temperature, TempUnit, Rho, KappaEddyDiffusion, NDensityS,
  * Velocity, VerticalVelocity and Altitude (device pointers) and the fields
- * of `f`.  nAlts <= 231 (a column's record fits the CU's LDS).
+ * of `f`.
  * Asynchronous on the handle's stream. */
 int gitm_vert_sources(gitm_vert_handle *h, const gitm_vert_sources_params *p,
                       const gitm_vert_columns *c, const gitm_vert_sources_fields *f);
+
+/* ------------------------------------------------------------------------
+ * The horizontal step: advance_horizontal(iBlock) (advance_horizontal.f90:
+ * 58-924) for every block, each altitude 1..nAlts: the RK4 (:154-345) of
+ * horizontal_solver (:535-818), MC-limited face values
+ * (calc_facevalues_lons/lats, :928-1034) and Rusanov fluxes in longitude
+ * and latitude, for Rho, Temperature, Velocity, NDensityS(1:6),
+ * VerticalVelocity and, with UseIonAdvection, IDensityS(1:2) (floored at
+ * 1e2, electrons re-summed).  The halo cells (-1:0, n+1:n+2) are read,
+ * never written: the caller fills them (exchange_messages_sphere,
+ * advance_horizontal.f90:20) and, off the full sphere, applies
+ * set_horizontal_bcs.  A negative neutral density after the step (the
+ * reference stops, :358-383) is reported as GITM_VERT_ENONPOSITIVE through
+ * gitm_vert_get_error.
+ * ------------------------------------------------------------------------ */
+typedef struct gitm_horiz_params {
+  double dt;                          /* Dt                                  */
+  double omega_body;                  /* OmegaBodyInput                      */
+  double beta_limiter;                /* BetaLimiter                         */
+  int32_t use_coriolis;               /* UseCoriolis                         */
+  int32_t use_ion_advection;          /* UseIonAdvection                     */
+  /* Must be 0 (otherwise GITM_VERT_EUNSUPPORTED): the divergence of the
+   * ion velocity (:432-518, UAM_Gradient) and the topography terms */
+  int32_t use_improved_ion_advection; /* UseImprovedIonAdvection           */
+  int32_t use_topography;             /* UseTopography                       */
+  int32_t altitudes_per_workgroup;    /* 0: chosen by the library            */
+  int32_t reserved;
+} gitm_horiz_params;
+
+typedef struct gitm_horiz_fields {
+  /* (-1:nLons+2,-1:nLats+2,-1:nAlts+2,nBlocks) geometry (initialize.f90:
+   * 198-267), descriptors as the state's */
+  gitm_vert_field inv_dlon_gb;     /* InvDLonDist_GB                         */
+  gitm_vert_field inv_dlon_fb;     /* InvDLonDist_FB                         */
+  gitm_vert_field dlon_fb;         /* dLonDist_FB                            */
+  gitm_vert_field inv_dlat_gb;     /* InvDLatDist_GB                         */
+  gitm_vert_field inv_dlat_fb;     /* InvDLatDist_FB                         */
+  gitm_vert_field dlat_fb;         /* dLatDist_FB                            */
+  gitm_vert_field radial_distance; /* RadialDistance_GB                      */
+  /* (-1:nLats+2,nBlocksMax), descriptors as cos_latitude's */
+  gitm_vert_field latitude;        /* Latitude                               */
+  gitm_vert_field tan_latitude;    /* TanLatitude                            */
+  /* cMax_GDB(0:nLons+1,0:nLats+1,0:nAlts+1,3,nBlocks): ptr at
+   * (1,1,0,iEast_,1), s_comp to iNorth_, first_level 0 */
+  gitm_vert_field cmax_horizontal;
+  /* output, interior cells of levels 1..nAlts; ptr may be NULL */
+  gitm_vert_field temp_source;     /* HorizontalTempSource                   */
+} gitm_horiz_fields;
+
+/* Rho, Temperature, Velocity, NDensityS, VerticalVelocity, Gamma,
+ * InvRadialDistance_GB, CosLatitude and (with ions) IDensityS, IVelocity
+ * come from c (device pointers; c->n_lons * c->n_lats <= 256 cells per
+ * block).  Asynchronous on the handle's stream. */
+int gitm_horiz_advance(gitm_vert_handle *h, const gitm_horiz_params *p,
+                       const gitm_vert_columns *c, const gitm_horiz_fields *f);
 
 /* Member and member column of a batch-wide column number. */
 int gitm_vert_batch_locate(const gitm_vert_batch *b, int column, int *member,

@@ -1,0 +1,124 @@
+"""GPU horizontal step (gitm_horiz_advance) against the oracle's restatement
+(oh_advance_horizontal, itself bit for bit with the reference Fortran
+build): the same IEEE operations in the same order, so bit for bit on every
+field the step writes (sin(latitude) is the device's; the Coriolis cases
+have passed bit for bit too, and are held to 1e-13 relative otherwise);
+halo cells and ghost levels are left alone; blocks of other shapes, several
+altitudes per workgroup, ion advection, the golden fixture, the
+negative-density report and the unsupported flags."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from gitm_amd import synth
+from oracle import cpu
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OMEGA = 2.0 * np.pi / 86400.0
+STATE = ("rho", "temp", "vel", "vvel", "nds")
+
+
+def run_gpu(solver, g, dt, coriolis=True, ions=False, apw=0, temp_source=True):
+    import torch
+    d = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in g.items()}
+    if temp_source:
+        d["temp_source"] = torch.zeros_like(d["rho"])
+    solver.horizontal(d, dt, OMEGA, 2.0, use_coriolis=coriolis, use_ion_advection=ions,
+                      altitudes_per_workgroup=apw)
+    torch.cuda.synchronize()
+    solver.check()
+    return {k: v.cpu().numpy() for k, v in d.items()}
+
+
+def compare(out, ref, keys, coriolis):
+    for k in keys:
+        a, b = out[k], ref[k]
+        if k == "temp_source":
+            a, b = a[:, 2:-2, 2:-2, 2:-2], b[:, 2:-2, 2:-2, 2:-2]
+        if np.array_equal(a, b):
+            continue
+        assert coriolis, f"{k} differs without Coriolis"
+        np.testing.assert_allclose(a, b, rtol=1e-13, atol=0, err_msg=k)
+
+
+@pytest.mark.parametrize("spec,coriolis,ions,apw", [
+    (synth.BlockGrid(4, 2, 50), True, False, 0),
+    (synth.BlockGrid(4, 2, 50), False, True, 0),
+    (synth.BlockGrid(3, 2, 30, nlons=6, nlats=5), True, True, 3),
+    (synth.BlockGrid(2, 2, 20, nlons=16, nlats=16), True, False, 1),
+    (synth.BlockGrid(2, 1, 7, nlons=4, nlats=9), False, False, 4),
+])
+def test_matches_oracle(solver, spec, coriolis, ions, apw):
+    g = synth.block_grid(spec, seed=21, ions=ions)
+    dt = synth.horizontal_dt(g, cfl=0.4)
+    print(f"horizontal {spec} coriolis={coriolis} ions={ions} apw={apw}", flush=True)
+    out = run_gpu(solver, g, dt, coriolis, ions, apw)
+    ref, rc, _ = cpu.horizontal(g, dt, OMEGA, 2.0, coriolis, ions)
+    assert rc == 0
+    keys = STATE + ("temp_source",) + (("ions",) if ions else ())
+    compare(out, ref, keys, coriolis)
+    for k in STATE:  # halo cells and ghost levels untouched
+        mask = np.ones(g[k].shape[-3:], bool)
+        mask[2:-2, 2:-2, 2:-2] = False
+        assert np.array_equal(out[k][..., mask], g[k][..., mask]), k
+
+
+def test_golden_fixture(solver):
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "mkh", os.path.join(HERE, "golden", "make_golden_horizontal.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    fx = np.load(os.path.join(HERE, "golden", mk.NAME))
+    g, dt = mk.inputs()
+    assert str(fx["inputs_sha256"]) == mk.digest(g)
+    out = run_gpu(solver, g, dt, True, True)
+    lv = fx["levels"]
+    for k in STATE + ("ions",):
+        got = out[k][..., lv + 1, 2:-2, 2:-2]
+        if not np.array_equal(got, fx["out_" + k]):
+            np.testing.assert_allclose(got, fx["out_" + k], rtol=1e-13, atol=0, err_msg=k)
+
+
+def test_one_degree_sample(solver):
+    """Six blocks of the 1-degree grid (100 levels; polar, mid-latitude and
+    equatorial), against the oracle."""
+    full = synth.BLOCK_CONFIGS["1deg_blocks_800x100"]
+    g = synth.block_grid(full, seed=3, blocks=[0, 1, 39, 200, 410, 799])
+    dt = synth.horizontal_dt(g, cfl=0.4)
+    out = run_gpu(solver, g, dt)
+    ref, rc, _ = cpu.horizontal(g, dt)
+    assert rc == 0 and full.ncells == 64800
+    compare(out, ref, STATE + ("temp_source",), True)
+
+
+def test_negative_density_is_reported(solver):
+    import torch
+    from gitm_amd.vertical import VerticalStepError
+    g = synth.block_grid(synth.BlockGrid(1, 1, 10), seed=2)
+    g["nds"][0, 3, 5, 6, 4] = -1.0e12
+    d = {k: torch.from_numpy(v).cuda() for k, v in g.items()}
+    solver.horizontal(d, 1.0)
+    torch.cuda.synchronize()
+    with pytest.raises(VerticalStepError) as e:
+        solver.check()
+    assert e.value.level == 4 and e.value.column == (6 - 2) * 9 + (4 - 2)
+
+
+def test_unsupported_flags(solver):
+    import torch
+    from gitm_amd import _abi, vertical
+    g = synth.block_grid(synth.BlockGrid(1, 1, 10), seed=2, ions=True)
+    d = {k: torch.from_numpy(v).cuda() for k, v in g.items()}
+    with pytest.raises(vertical.VerticalStepError) as e:
+        solver.horizontal(d, 1.0, use_ion_advection=True, altitudes_per_workgroup=9)
+    assert e.value.code == 4  # 9 x 81 lanes do not fit a workgroup
+    p = _abi.HorizParams()
+    p.dt, p.use_topography = 1.0, 1
+    c, f = _abi.Columns(), _abi.HorizFields()
+    assert vertical.lib().gitm_horiz_advance(solver._h, p, c, f) == 5
