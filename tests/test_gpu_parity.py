@@ -38,7 +38,8 @@ pytestmark = pytest.mark.gpu
 RTOL = 1e-11
 STATE = ("nds", "vvel", "vel", "temp", "rho", "ions")
 THREADS = min(16, os.cpu_count() or 1)
-GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+GOLDEN = sorted(g for g in glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz"))
+                if not os.path.basename(g).startswith("horizontal_"))  # the vertical step's
 
 
 def gpu_run(solver, cols, params, which="ausm", n_steps=1):
