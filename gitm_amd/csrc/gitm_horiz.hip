@@ -16,7 +16,8 @@
 // values in registers until every lane has read the stage's state.  Every
 // value is formed by the reference's operations in the reference's order,
 // so the results equal the reference's bit for bit (sin(latitude) aside,
-// which is the device's).
+// which is the device's).  With UseImprovedIonAdvection a second kernel
+// applies the divergence of the ion velocity afterwards.
 #include "vert_gpu.h"
 
 namespace {
@@ -293,13 +294,87 @@ __global__ void __launch_bounds__(256, 2)
   }
 }
 
+// UseImprovedIonAdvection (advance_horizontal.f90:432-518), after the
+// step: the divergence of the field-aligned ion velocity from UAM_Gradient
+// (gradient.f90:1-86) divides the advected ions by 1 + Dt div on the
+// interior cells of levels 1..nAlts; then the electrons are re-summed on
+// every cell and level of the block.  One thread per element of the
+// block's (-1:nLons+2,-1:nLats+2,-1:nAlts+2) array; each touches its own
+// cell's ions only.
+struct DivFields {
+  gitm_vert_field ivel_par, glm, gl0, glp, gam, ga0, gap;
+};
+
+__global__ void __launch_bounds__(256)
+    ion_divergence_kernel(HzParams p, gitm_vert_columns io, HzFields f, DivFields g) {
+  const int nlo = p.n_lons, nla = p.n_lats, nA = p.n_alts;
+  const long long NLo = nlo + 4, NLg = nla + 4, per_block = NLo * NLg * (nA + 4);
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= per_block * p.n_blocks) return;
+  const int b = (int)(e / per_block);
+  long long r = e % per_block;
+  const int k = (int)(r / (NLo * NLg)) - 1;
+  r %= NLo * NLg;
+  const int lat = (int)(r / NLo) - 1, lon = (int)(r % NLo) - 1;
+  if (lon >= 1 && lon <= nlo && lat >= 1 && lat <= nla && k >= 1 && k <= nA) {
+    const double ir = *hf(io.inv_radial_distance, lon, lat, k, 0, b);
+    auto v = [&](int comp, int lo, int la, int kk) {
+      return *hf(g.ivel_par, lo, la, kk, comp, b);
+    };
+    double div = 0.0;
+    if (nlo > 1) {  // east (gradient.f90:27-41)
+      const double icl = 1. / fmax(*hf(io.cos_latitude, 1, lat, 0, 0, b), 0.17);
+      const double gr = (*hf(g.glm, lon, 1, 0, 0, b) * v(0, lon - 1, lat, k) +
+                         *hf(g.gl0, lon, 1, 0, 0, b) * v(0, lon, lat, k) +
+                         *hf(g.glp, lon, 1, 0, 0, b) * v(0, lon + 1, lat, k)) *
+                        ir * icl;
+      div = div + gr;
+    }
+    {  // north (:44-56), then the tan(latitude) term (:462-477)
+      const double gr = nla > 1 ? (*hf(g.gam, 1, lat, 0, 0, b) * v(1, lon, lat - 1, k) +
+                                   *hf(g.ga0, 1, lat, 0, 0, b) * v(1, lon, lat, k) +
+                                   *hf(g.gap, 1, lat, 0, 0, b) * v(1, lon, lat + 1, k)) *
+                                      ir
+                                : 0.0;
+      div = div + gr;
+      double tl = *hf(f.tan_latitude, 1, lat, 0, 0, b);
+      if (tl > 5.0) tl = 5.0;
+      if (tl < -5.0) tl = -5.0;
+      div = div - tl * v(1, lon, lat, k) * ir;
+    }
+    {  // radial (:59-78), then 2 w / r (:480-493)
+      const double r0 = *hf(f.raddist, lon, lat, k, 0, b);
+      const double drp = *hf(f.raddist, lon, lat, k + 1, 0, b) - r0;
+      const double drm = r0 - *hf(f.raddist, lon, lat, k - 1, 0, b);
+      const double q = drm / drp;
+      const double q2 = q * q;
+      const double bottom = drm * (1 + q);
+      const double gr =
+          nA > 1 ? (q2 * v(2, lon, lat, k + 1) - v(2, lon, lat, k - 1) - (q2 - 1) * v(2, lon, lat, k)) /
+                       bottom
+                 : 0.0;
+      div = div + gr;
+      div = div + 2 * v(2, lon, lat, k) * ir;
+    }
+    div = div * *hf(io.cos_latitude, 1, lat, 0, 0, b);  // :498-502
+    const double change = p.dt * div;
+#pragma unroll
+    for (int s = 0; s < NIA; s++) {  // :505-510
+      double *n = hf(io.idensity_s, lon, lat, k, s, b);
+      *n = *n / (1 + change);
+    }
+  }
+  double el = 0.0;  // :512-516, every cell and level
+  for (int s = 0; s < gv::NIONS - 1; s++) el = el + *hf(io.idensity_s, lon, lat, k, s, b);
+  *hf(io.idensity_s, lon, lat, k, gv::NIONS - 1, b) = el;
+}
+
 }  // namespace
 
 extern "C" int gitm_horiz_advance(gitm_vert_handle *h, const gitm_horiz_params *hp,
                                   const gitm_vert_columns *c, const gitm_horiz_fields *hf_) {
   if (!h || !hp || !c || !hf_) return GITM_VERT_EINVAL;
-  if (hp->use_topography || (hp->use_ion_advection && hp->use_improved_ion_advection))
-    return GITM_VERT_EUNSUPPORTED;
+  if (hp->use_topography) return GITM_VERT_EUNSUPPORTED;
   const int nlo = c->n_lons, nla = c->n_lats;
   if (nlo < 1 || nla < 1 || c->n_blocks < 1 || c->n_alts < 1) return GITM_VERT_EINVAL;
   const int cells = nlo * nla;
@@ -316,6 +391,19 @@ extern "C" int gitm_horiz_advance(gitm_vert_handle *h, const gitm_horiz_params *
     if (!g->ptr) return GITM_VERT_EINVAL;
   const bool ions = hp->use_ion_advection != 0;
   if (ions && (!c->idensity_s.ptr || !c->ivelocity.ptr)) return GITM_VERT_EINVAL;
+  const bool improved = ions && hp->use_improved_ion_advection != 0;
+  DivFields dv;
+  dv.ivel_par = hf_->ivelocity_par;
+  dv.glm = hf_->grad_lon_m;
+  dv.gl0 = hf_->grad_lon_0;
+  dv.glp = hf_->grad_lon_p;
+  dv.gam = hf_->grad_lat_m;
+  dv.ga0 = hf_->grad_lat_0;
+  dv.gap = hf_->grad_lat_p;
+  if (improved)
+    for (const gitm_vert_field *g : {&dv.ivel_par, &dv.glm, &dv.gl0, &dv.glp, &dv.gam, &dv.ga0,
+                                     &dv.gap})
+      if (!g->ptr) return GITM_VERT_EINVAL;
   const int nv = ions ? NV_IONS : NV_NEUTRAL;
   // per altitude: the slab of every variable with halo cells, then the
   // stage's new values and the RK4 sums of the block's cells
@@ -363,5 +451,14 @@ extern "C" int gitm_horiz_advance(gitm_vert_handle *h, const gitm_horiz_params *
     hipLaunchKernelGGL(horiz_kernel<NV_NEUTRAL>, dim3((unsigned)nwg), dim3(threads), lds,
                        h->stream, p, *c, f, h->d_err);
   GV_HIP(hipGetLastError());
+  if (improved) {
+    const long long n =
+        (long long)(nlo + 4) * (nla + 4) * (c->n_alts + 4) * (long long)c->n_blocks;
+    const long long nb = (n + 255) / 256;
+    if (nb > 0x7fffffffLL) return GITM_VERT_EINVAL;
+    hipLaunchKernelGGL(ion_divergence_kernel, dim3((unsigned)nb), dim3(256), 0, h->stream, p,
+                       *c, f, dv);
+    GV_HIP(hipGetLastError());
+  }
   return GITM_VERT_OK;
 }

@@ -484,7 +484,18 @@ def block_grid(spec: BlockGrid, seed: int = 1, ions: bool = False, blocks=None) 
         ivel[:, 0] = 150.0 * np.sin(LON + ph) * np.cos(LAT)
         ivel[:, 1] = 100.0 * np.cos(2 * LON) * np.sin(LAT)
         ivel[:, 2] = 10.0 * np.sin(LAT + ph)
-        out["ions"], out["ivel"] = idens, ivel
+        ipar = np.empty((nb, 3) + shp[1:])  # the field-aligned part
+        ipar[:, 0] = 40.0 * np.cos(LON + ph) * np.sin(2 * LAT)
+        ipar[:, 1] = 60.0 * np.sin(LAT) * np.cos(LON)
+        ipar[:, 2] = 30.0 * np.sin(LAT + ph) * (Z / 400.0e3)
+        out["ions"], out["ivel"], out["ivel_par"] = idens, ivel, ipar
+        # UAM_Gradient's coefficients (initialize.f90:268-292), [block][M, 0, P][cell]
+        for key, ang, n in (("grad_lon", lon, nlo), ("grad_lat", lat, nla)):
+            dm = ang[:, 2:n + 2] - ang[:, 1:n + 1]   # cells 1..n minus cells 0..n-1
+            dp = ang[:, 3:n + 3] - ang[:, 2:n + 2]
+            r2 = (dm / dp) ** 2
+            inv = 1.0 / (r2 * dp + dm)
+            out[key] = np.stack([-inv, inv * (1 - r2), inv * r2], axis=1)
     # geometry (initialize.f90:185-267)
     cosl = np.cos(lat)
     fl = np.maximum(np.abs(cosl), 0.01)[:, None, :, None]

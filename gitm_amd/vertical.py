@@ -295,14 +295,18 @@ class VerticalSolver:
 
     def horizontal(self, grid: dict, dt: float, omega_body: float = 2.0 * math.pi / 86400.0,
                    beta_limiter: float = 2.0, use_coriolis: bool = True,
-                   use_ion_advection: bool = False, altitudes_per_workgroup: int = 0) -> None:
+                   use_ion_advection: bool = False, altitudes_per_workgroup: int = 0,
+                   use_improved_ion_advection: bool = False) -> None:
         """The horizontal step (``gitm_horiz_advance``: advance_horizontal on
         every block) on device tensors in GITM's block layout
         ([block][component][level][lat][lon], halo cells included;
         gitm_amd.synth.block_grid names and shapes).  The state ("rho",
         "temp", "vel", "vvel", "nds", with ions "ions") is updated in place;
         "temp_source", when present, receives HorizontalTempSource.  Queued
-        on the current stream; errors surface in ``check()``."""
+        on the current stream; errors surface in ``check()``.  With ions and
+        ``use_improved_ion_advection`` the divergence of "ivel_par" follows,
+        with UAM_Gradient's coefficients "grad_lon" and "grad_lat"
+        ([block][M, 0, P][cell])."""
         import torch
 
         nb, L, NLg, NLo = (int(v) for v in grid["rho"].shape)
@@ -337,9 +341,17 @@ class VerticalSolver:
                                                   alt_hi=na + 1)
         if "temp_source" in grid:
             f.temp_source = blk("temp_source")
+        if use_ion_advection and use_improved_ion_advection:
+            f.ivelocity_par = blk("ivel_par", 3)
+            lo, la = grid["grad_lon"].data_ptr(), grid["grad_lat"].data_ptr()
+            f.grad_lon_m, f.grad_lon_0, f.grad_lon_p = (
+                _abi.Field(lo + 8 * j * nlo, 1, 0, 3 * nlo, 0, 0, 0, 0) for j in range(3))
+            f.grad_lat_m, f.grad_lat_0, f.grad_lat_p = (
+                _abi.Field(la + 8 * j * nla, 0, 1, 3 * nla, 0, 0, 0, 0) for j in range(3))
         p = _abi.HorizParams()
         p.dt, p.omega_body, p.beta_limiter = dt, omega_body, beta_limiter
         p.use_coriolis, p.use_ion_advection = int(use_coriolis), int(use_ion_advection)
+        p.use_improved_ion_advection = int(use_improved_ion_advection)
         p.altitudes_per_workgroup = altitudes_per_workgroup
         self.set_stream(torch.cuda.current_stream().cuda_stream)
         rc = lib().gitm_horiz_advance(self._h, ctypes.byref(p), ctypes.byref(c),

@@ -345,8 +345,10 @@ int gitm_vert_sources(gitm_vert_handle *h, const gitm_vert_sources_params *p,
  * (calc_facevalues_lons/lats, :928-1034) and Rusanov fluxes in longitude
  * and latitude, for Rho, Temperature, Velocity, NDensityS(1:6),
  * VerticalVelocity and, with UseIonAdvection, IDensityS(1:2) (floored at
- * 1e2, electrons re-summed).  The halo cells (-1:0, n+1:n+2) are read,
- * never written: the caller fills them (exchange_messages_sphere,
+ * 1e2, electrons re-summed; with UseImprovedIonAdvection the divergence
+ * of the ion velocity follows, :432-518).  The halo cells (-1:0, n+1:n+2)
+ * are read, never written (the electron re-sum of UseImprovedIonAdvection
+ * aside, which covers every cell, :512-516): the caller fills them (exchange_messages_sphere,
  * advance_horizontal.f90:20) and, off the full sphere, applies
  * set_horizontal_bcs.  A negative neutral density after the step (the
  * reference stops, :358-383) is reported as GITM_VERT_ENONPOSITIVE through
@@ -358,10 +360,11 @@ typedef struct gitm_horiz_params {
   double beta_limiter;                /* BetaLimiter                         */
   int32_t use_coriolis;               /* UseCoriolis                         */
   int32_t use_ion_advection;          /* UseIonAdvection                     */
-  /* Must be 0 (otherwise GITM_VERT_EUNSUPPORTED): the divergence of the
-   * ion velocity (:432-518, UAM_Gradient) and the topography terms */
-  int32_t use_improved_ion_advection; /* UseImprovedIonAdvection           */
-  int32_t use_topography;             /* UseTopography                       */
+  int32_t use_improved_ion_advection; /* UseImprovedIonAdvection: with ions,
+                                         the divergence of IVelocityPar
+                                         (:432-518, UAM_Gradient) after the
+                                         step; needs the grad_* fields     */
+  int32_t use_topography;             /* must be 0 (GITM_VERT_EUNSUPPORTED) */
   int32_t altitudes_per_workgroup;    /* 0: chosen by the library            */
   int32_t reserved;
 } gitm_horiz_params;
@@ -384,6 +387,15 @@ typedef struct gitm_horiz_fields {
   gitm_vert_field cmax_horizontal;
   /* output, interior cells of levels 1..nAlts; ptr may be NULL */
   gitm_vert_field temp_source;     /* HorizontalTempSource                   */
+  /* UseImprovedIonAdvection only (else ignored): the ion velocity of the
+   * divergence, IVelocityPar(..,3,nBlocks) on Earth (IVelocity where
+   * NonMagnetic), described as ivelocity; UAM_Gradient's coefficients
+   * GradLon{M,0,P}_CB(nLons,nBlocksMax), ptr at (1,1), s_lon 1,
+   * s_blk nLons, and GradLat{M,0,P}_CB(nLats,nBlocksMax), ptr at (1,1),
+   * s_lat 1, s_blk nLats (initialize.f90:268-292) */
+  gitm_vert_field ivelocity_par;
+  gitm_vert_field grad_lon_m, grad_lon_0, grad_lon_p;
+  gitm_vert_field grad_lat_m, grad_lat_0, grad_lat_p;
 } gitm_horiz_fields;
 
 /* Rho, Temperature, Velocity, NDensityS, VerticalVelocity, Gamma,

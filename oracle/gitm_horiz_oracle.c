@@ -294,11 +294,83 @@ static int slice(const oh_grid *p, const geo *g, int b, int k, ov_status *st) {
   return rc;
 }
 
+/* UseImprovedIonAdvection, after every altitude of block b
+ * (advance_horizontal.f90:432-518): the divergence of the field-aligned ion
+ * velocity from UAM_Gradient (gradient.f90:1-86), the ions divided by
+ * 1 + Dt div, then the electrons re-summed on every cell and level. */
+static void ion_divergence(const oh_grid *p, const geo *g, int b) {
+  const int nlo = g->nlo, nla = g->nla, nA = g->nA, NLg = nla + 4;
+  const long long fsz = (long long)(nA + 4) * g->slev;
+  const double *vi = p->ivel_par + (long long)b * 3 * fsz;
+  const double *inv = p->invr + (long long)b * fsz, *rd = p->raddist + (long long)b * fsz;
+  const double *gl = p->grad_lon + (long long)b * 3 * nlo, *ga = p->grad_lat + (long long)b * 3 * nla;
+  const double *cosl = p->cos_lat + (long long)b * NLg, *tanl = p->tan_lat + (long long)b * NLg;
+  double *ions = p->ions + (long long)b * OV_NIONS * fsz;
+  for (int k = 1; k <= nA; k++)
+    for (int la = 1; la <= nla; la++)
+      for (int lo = 1; lo <= nlo; lo++) {
+        const long long c = at3(g, lo, la, k);
+        const double ir = inv[c];
+        double div = 0.0;
+        /* UAM_Gradient leaves a direction of one cell at 0 (:24-26) */
+        if (nlo > 1) { /* iDir = 1: east (gradient.f90:27-41) */
+          const double *v = vi;
+          const double icl = 1. / fmax(cosl[la + 1], 0.17);
+          const double gr = (gl[lo - 1] * v[at3(g, lo - 1, la, k)] +
+                             gl[nlo + lo - 1] * v[c] +
+                             gl[2 * nlo + lo - 1] * v[at3(g, lo + 1, la, k)]) *
+                            ir * icl;
+          div = div + gr;
+        }
+        { /* iDir = 2: north (:44-56), then the tan(latitude) term (:462-477) */
+          const double *v = vi + fsz;
+          const double gr = nla > 1 ? (ga[la - 1] * v[at3(g, lo, la - 1, k)] +
+                                       ga[nla + la - 1] * v[c] +
+                                       ga[2 * nla + la - 1] * v[at3(g, lo, la + 1, k)]) *
+                                          ir
+                                    : 0.0;
+          div = div + gr;
+          double tl = tanl[la + 1];
+          if (tl > 5.0) tl = 5.0;
+          if (tl < -5.0) tl = -5.0;
+          div = div - tl * v[c] * ir;
+        }
+        { /* iDir = 3: radial (:59-78), then 2 w / r (:480-493) */
+          const double *v = vi + 2 * fsz;
+          const double drp = rd[at3(g, lo, la, k + 1)] - rd[c];
+          const double drm = rd[c] - rd[at3(g, lo, la, k - 1)];
+          const double q = drm / drp;
+          const double q2 = q * q;
+          const double bottom = drm * (1 + q);
+          const double gr = nA > 1 ? (q2 * v[at3(g, lo, la, k + 1)] -
+                                      v[at3(g, lo, la, k - 1)] - (q2 - 1) * v[c]) /
+                                         bottom
+                                   : 0.0;
+          div = div + gr;
+          div = div + 2 * v[c] * ir;
+        }
+        div = div * cosl[la + 1]; /* :498-502 */
+        const double change = p->dt * div;
+        for (int s = 0; s < OV_NIONS_ADVECT; s++) { /* :505-510 */
+          double *n = ions + (long long)s * fsz + c;
+          *n = *n / (1 + change);
+        }
+      }
+  /* electrons on every cell and level (:512-516) */
+  for (long long c = 0; c < fsz; c++) {
+    double e = 0.0;
+    for (int s = 0; s < OV_NIONS - 1; s++) e = e + ions[(long long)s * fsz + c];
+    ions[(long long)(OV_NIONS - 1) * fsz + c] = e;
+  }
+}
+
 int oh_advance_horizontal(const oh_grid *p, ov_status *st) {
   if (!p || p->n_lons < 1 || p->n_lats < 1 || p->n_lons > 60 || p->n_lats > 60 ||
       p->n_alts < 1 || p->n_blocks < 1)
     return OV_ERR_BAD_ARGS;
   if (p->use_ion_advection && (!p->ions || !p->ivel)) return OV_ERR_BAD_ARGS;
+  const int improved = p->use_ion_advection && p->use_improved_ion_advection;
+  if (improved && (!p->ivel_par || !p->grad_lon || !p->grad_lat)) return OV_ERR_BAD_ARGS;
   geo g;
   g.nlo = p->n_lons;
   g.nla = p->n_lats;
@@ -312,7 +384,7 @@ int oh_advance_horizontal(const oh_grid *p, ov_status *st) {
     st->level = -99;
   }
   int first = OV_OK;
-  for (int b = 0; b < p->n_blocks; b++)
+  for (int b = 0; b < p->n_blocks; b++) {
     for (int k = 1; k <= p->n_alts; k++) {
       ov_status s1;
       const int rc = slice(p, &g, b, k, &s1);
@@ -321,5 +393,7 @@ int oh_advance_horizontal(const oh_grid *p, ov_status *st) {
         if (st) *st = s1;
       }
     }
+    if (improved) ion_divergence(p, &g, b);
+  }
   return first;
 }

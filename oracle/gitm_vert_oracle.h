@@ -192,6 +192,13 @@ typedef struct {
   const double *latitude, *tan_lat, *cos_lat;
   const double *cmax;
   double *temp_source; /* HorizontalTempSource (interior), or NULL        */
+  /* UseImprovedIonAdvection (with ions): the divergence of the ion
+   * velocity (advance_horizontal.f90:432-518) through UAM_Gradient
+   * (gradient.f90:1-86) */
+  int use_improved_ion_advection;
+  const double *ivel_par;            /* IVelocityPar [3] (Earth: magnetic)  */
+  const double *grad_lon;            /* GradLonM/0/P_CB [block][3][nLons]  */
+  const double *grad_lat;            /* GradLatM/0/P_CB [block][3][nLats]  */
 } oh_grid;
 
 int oh_advance_horizontal(const oh_grid *p, ov_status *status);

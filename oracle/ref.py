@@ -82,7 +82,8 @@ def horizontal_available() -> bool:
 
 
 def horizontal(grid: dict, dt: float, omega_body: float, beta_limiter: float,
-               use_coriolis: bool = True, use_ion_advection: bool = False) -> dict:
+               use_coriolis: bool = True, use_ion_advection: bool = False,
+               use_improved_ion_advection: bool = False) -> dict:
     """The reference's advance_horizontal on every block of the block-layout
     ``grid`` (gitm_amd.synth.block_grid; [block][comp][level][lat][lon] is
     the reference's own (lon, lat, level, comp, block) order)."""
@@ -91,7 +92,9 @@ def horizontal(grid: dict, dt: float, omega_body: float, beta_limiter: float,
     if (nlo, nla, na) != (REF_BLOCK[0], REF_BLOCK[1], REF_NALTS) or nb > REF_BLOCK[2]:
         raise ValueError("reference build is compiled for 9x9 blocks, nAlts=50, <= 4 blocks")
     hdr = np.zeros(16, dtype=np.int32)
-    hdr[:7] = [HMAGIC, nb, nlo, nla, na, int(use_coriolis), int(use_ion_advection)]
+    improved = use_ion_advection and use_improved_ion_advection
+    hdr[:8] = [HMAGIC, nb, nlo, nla, na, int(use_coriolis), int(use_ion_advection),
+               int(improved)]
     par = np.zeros(8)
     par[:3] = [dt, omega_body, beta_limiter]
     order = ["rho", "temp", "vel", "vvel", "nds", "gamma", "invr", "raddist", "inv_dlon_gb",
@@ -107,6 +110,11 @@ def horizontal(grid: dict, dt: float, omega_body: float, beta_limiter: float,
             f.write(par.tobytes())
             for k in order:
                 f.write(_fortran_order(grid[k]).tobytes())
+            if improved:
+                f.write(_fortran_order(grid["ivel_par"]).tobytes())
+                for key in ("grad_lon", "grad_lat"):  # M, 0, P: (n, nBlocks) each
+                    for j in range(3):
+                        f.write(_fortran_order(grid[key][:, j, :]).tobytes())
         subprocess.run([HEXE, fin, fout], check=True, capture_output=True)
         raw = np.fromfile(fout, dtype=np.uint8)
     status = raw[:16].view(np.int32)

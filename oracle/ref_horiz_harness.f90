@@ -12,13 +12,16 @@
 ! File formats (stream, native little-endian; every array in the reference's
 ! own declared shape and order, blocks 1..nB):
 !   input : int32 hdr(16) = magic, nB, nLons, nLats, nAlts, UseCoriolis,
-!           UseIonAdvection, 0...; real64 par(8) = Dt, OmegaBodyInput,
-!           BetaLimiter, 0...; then Rho, Temperature, Velocity(3),
+!           UseIonAdvection, UseImprovedIonAdvection, 0...; real64 par(8) =
+!           Dt, OmegaBodyInput, BetaLimiter, 0...; then Rho, Temperature,
+!           Velocity(3),
 !           VerticalVelocity(6), NDensityS(1:6), Gamma, InvRadialDistance_GB,
 !           RadialDistance_GB, InvDLonDist_GB, InvDLonDist_FB, dLonDist_FB,
 !           InvDLatDist_GB, InvDLatDist_FB, dLatDist_FB, Latitude,
 !           TanLatitude, CosLatitude, cMax_GDB(3), and with ions
-!           IDensityS(1:nIons), IVelocity(3)
+!           IDensityS(1:nIons), IVelocity(3); with UseImprovedIonAdvection
+!           (hdr(8)) also IVelocityPar(3), GradLonM_CB, GradLon0_CB,
+!           GradLonP_CB, GradLatM_CB, GradLat0_CB, GradLatP_CB
 !   output: int32 status(4), then Rho, Temperature, Velocity,
 !           VerticalVelocity, NDensityS(1:6), HorizontalTempSource per block
 !           (nLons, nLats, nAlts), and with ions IDensityS(1:nIons)
@@ -60,7 +63,7 @@ program gitm_horiz_ref
 
   UseCoriolis             = (hdr(6) /= 0)
   UseIonAdvection         = doIons
-  UseImprovedIonAdvection = .false.
+  UseImprovedIonAdvection = doIons .and. hdr(8) /= 0
   UseTopography           = .false.
   DoCheckForNans          = .false.
   Dt                      = par(1)
@@ -88,6 +91,15 @@ program gitm_horiz_ref
   if (doIons) then
      read(10) IDensityS(:,:,:,1:nIons,1:nB)
      read(10) IVelocity(:,:,:,:,1:nB)
+     if (UseImprovedIonAdvection) then
+        read(10) IVelocityPar(:,:,:,:,1:nB)
+        read(10) GradLonM_CB(:,1:nB)
+        read(10) GradLon0_CB(:,1:nB)
+        read(10) GradLonP_CB(:,1:nB)
+        read(10) GradLatM_CB(:,1:nB)
+        read(10) GradLat0_CB(:,1:nB)
+        read(10) GradLatP_CB(:,1:nB)
+     endif
   else
      IDensityS = 1.0e6
      IVelocity = 0.0

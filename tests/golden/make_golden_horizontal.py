@@ -1,7 +1,9 @@
 """Writes the golden fixture of the horizontal step from the reference build.
 
 The fixture holds, for one seeded synthetic block grid (two 9x9 blocks of
-the 4x2-block 90-degree grid, 50 levels, ion advection on, Coriolis on),
+the 4x2-block 90-degree grid, 50 levels, Coriolis on, ion advection on with
+GITM's default UseImprovedIonAdvection, so the ion-velocity divergence of
+UAM_Gradient follows the step),
 the interior cells the reference writes after advance_horizontal
 (src/advance_horizontal.f90, compiled from the reference sources by ``make
 -C oracle ref``) on six of the levels (each level is an independent 2-D
@@ -50,7 +52,8 @@ def inputs() -> tuple[dict, float]:
 
 def main() -> None:
     g, dt = inputs()
-    out = ref.horizontal(g, dt, OMEGA, BETA, use_coriolis=True, use_ion_advection=True)
+    out = ref.horizontal(g, dt, OMEGA, BETA, use_coriolis=True, use_ion_advection=True,
+                         use_improved_ion_advection=True)
     lv = np.array(LEVELS)
     keep = {"inputs_sha256": np.array(digest(g)), "dt": np.array(dt), "levels": lv}
     for k, v in out.items():  # interior cells of the chosen levels

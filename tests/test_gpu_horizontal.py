@@ -5,7 +5,8 @@ field the step writes (sin(latitude) is the device's; the Coriolis cases
 have passed bit for bit too, and are held to 1e-13 relative otherwise);
 halo cells and ghost levels are left alone; blocks of other shapes, several
 altitudes per workgroup, ion advection, the golden fixture, the
-negative-density report and the unsupported flags."""
+negative-density report and the unsupported flags; with UseImprovedIonAdvection
+the ion-velocity divergence of the second kernel (GITM's default for ions)."""
 from __future__ import annotations
 
 import os
@@ -23,13 +24,14 @@ OMEGA = 2.0 * np.pi / 86400.0
 STATE = ("rho", "temp", "vel", "vvel", "nds")
 
 
-def run_gpu(solver, g, dt, coriolis=True, ions=False, apw=0, temp_source=True):
+def run_gpu(solver, g, dt, coriolis=True, ions=False, apw=0, temp_source=True,
+            improved=False):
     import torch
     d = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in g.items()}
     if temp_source:
         d["temp_source"] = torch.zeros_like(d["rho"])
     solver.horizontal(d, dt, OMEGA, 2.0, use_coriolis=coriolis, use_ion_advection=ions,
-                      altitudes_per_workgroup=apw)
+                      altitudes_per_workgroup=apw, use_improved_ion_advection=improved)
     torch.cuda.synchronize()
     solver.check()
     return {k: v.cpu().numpy() for k, v in d.items()}
@@ -46,19 +48,21 @@ def compare(out, ref, keys, coriolis):
         np.testing.assert_allclose(a, b, rtol=1e-13, atol=0, err_msg=k)
 
 
-@pytest.mark.parametrize("spec,coriolis,ions,apw", [
-    (synth.BlockGrid(4, 2, 50), True, False, 0),
-    (synth.BlockGrid(4, 2, 50), False, True, 0),
-    (synth.BlockGrid(3, 2, 30, nlons=6, nlats=5), True, True, 3),
-    (synth.BlockGrid(2, 2, 20, nlons=16, nlats=16), True, False, 1),
-    (synth.BlockGrid(2, 1, 7, nlons=4, nlats=9), False, False, 4),
+@pytest.mark.parametrize("spec,coriolis,ions,apw,improved", [
+    (synth.BlockGrid(4, 2, 50), True, False, 0, False),
+    (synth.BlockGrid(4, 2, 50), False, True, 0, False),
+    (synth.BlockGrid(4, 2, 50), True, True, 0, True),
+    (synth.BlockGrid(3, 2, 30, nlons=6, nlats=5), True, True, 3, True),
+    (synth.BlockGrid(2, 2, 20, nlons=16, nlats=16), True, False, 1, False),
+    (synth.BlockGrid(2, 1, 7, nlons=4, nlats=9), False, False, 4, False),
 ])
-def test_matches_oracle(solver, spec, coriolis, ions, apw):
+def test_matches_oracle(solver, spec, coriolis, ions, apw, improved):
     g = synth.block_grid(spec, seed=21, ions=ions)
     dt = synth.horizontal_dt(g, cfl=0.4)
-    print(f"horizontal {spec} coriolis={coriolis} ions={ions} apw={apw}", flush=True)
-    out = run_gpu(solver, g, dt, coriolis, ions, apw)
-    ref, rc, _ = cpu.horizontal(g, dt, OMEGA, 2.0, coriolis, ions)
+    print(f"horizontal {spec} coriolis={coriolis} ions={ions} apw={apw} improved={improved}",
+          flush=True)
+    out = run_gpu(solver, g, dt, coriolis, ions, apw, improved=improved)
+    ref, rc, _ = cpu.horizontal(g, dt, OMEGA, 2.0, coriolis, ions, improved)
     assert rc == 0
     keys = STATE + ("temp_source",) + (("ions",) if ions else ())
     compare(out, ref, keys, coriolis)
@@ -77,7 +81,7 @@ def test_golden_fixture(solver):
     fx = np.load(os.path.join(HERE, "golden", mk.NAME))
     g, dt = mk.inputs()
     assert str(fx["inputs_sha256"]) == mk.digest(g)
-    out = run_gpu(solver, g, dt, True, True)
+    out = run_gpu(solver, g, dt, True, True, improved=True)
     lv = fx["levels"]
     for k in STATE + ("ions",):
         got = out[k][..., lv + 1, 2:-2, 2:-2]
@@ -118,6 +122,9 @@ def test_unsupported_flags(solver):
     with pytest.raises(vertical.VerticalStepError) as e:
         solver.horizontal(d, 1.0, use_ion_advection=True, altitudes_per_workgroup=9)
     assert e.value.code == 4  # 9 x 81 lanes do not fit a workgroup
+    del d["grad_lat"]
+    with pytest.raises(KeyError):  # the divergence needs UAM_Gradient's coefficients
+        solver.horizontal(d, 1.0, use_ion_advection=True, use_improved_ion_advection=True)
     p = _abi.HorizParams()
     p.dt, p.use_topography = 1.0, 1
     c, f = _abi.Columns(), _abi.HorizFields()

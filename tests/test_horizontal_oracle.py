@@ -3,8 +3,13 @@ reference build, its golden fixture and what the mathematics fixes.
 
 * Against the reference Fortran advance_horizontal (compiled from its own
   sources by ``make -C oracle ref``, where they are present): bit for bit on
-  seeded block grids with and without Coriolis and ion advection, and
+  seeded block grids with and without Coriolis, ion advection and the
+  divergence of UseImprovedIonAdvection (UAM_Gradient, gradient.f90), and
   against the committed fixture written by that build.
+* UAM_Gradient's stencils are exact for quadratics in each direction, the
+  property its coefficients are built for (initialize.f90:268-292): the
+  divergence of a field-aligned velocity quadratic in longitude, latitude
+  and radius matches the analytic one.
 * Linear density profiles under a uniform wind (zero temperature, so no
   pressure force changes the wind) are advected exactly: the averaged face
   values of linear data give the exact gradient, the Rusanov flux
@@ -36,12 +41,14 @@ STATE = ("rho", "temp", "vel", "vvel", "nds")
 
 
 @pytest.mark.skipif(not ref.horizontal_available(), reason="reference build absent")
-@pytest.mark.parametrize("coriolis,ions", [(True, True), (False, False), (True, False)])
-def test_matches_reference_build(coriolis, ions):
+@pytest.mark.parametrize("coriolis,ions,improved", [
+    (True, True, False), (False, False, False), (True, False, False), (True, True, True),
+    (False, True, True)])
+def test_matches_reference_build(coriolis, ions, improved):
     g = synth.block_grid(synth.BlockGrid(2, 2, 50), seed=11 + int(ions), ions=ions)
     dt = synth.horizontal_dt(g, cfl=0.4)
-    out, rc, _ = cpu.horizontal(g, dt, OMEGA, 2.0, coriolis, ions)
-    want = ref.horizontal(g, dt, OMEGA, 2.0, coriolis, ions)
+    out, rc, _ = cpu.horizontal(g, dt, OMEGA, 2.0, coriolis, ions, improved)
+    want = ref.horizontal(g, dt, OMEGA, 2.0, coriolis, ions, improved)
     assert rc == 0
     for k, v in want.items():
         got = out[k][:, 2:-2, 2:-2, 2:-2] if k == "temp_source" else out[k]
@@ -58,7 +65,7 @@ def test_golden_fixture():
     g, dt = mk.inputs()
     assert str(fx["inputs_sha256"]) == mk.digest(g), "generator changed since the fixture"
     assert float(fx["dt"]) == dt
-    out, rc, _ = cpu.horizontal(g, dt, mk.OMEGA, mk.BETA, True, True)
+    out, rc, _ = cpu.horizontal(g, dt, mk.OMEGA, mk.BETA, True, True, True)
     assert rc == 0
     lv = fx["levels"]
     for k in STATE + ("ions",):
@@ -187,3 +194,54 @@ def test_negative_density_is_reported():
     g["nds"][0, 3, 5, 6, 4] = -1.0e12  # level 4, lat 4, lon 2 (halo offset 2)
     _, rc, (col, lev) = cpu.horizontal(g, 1.0)
     assert rc == 1 and lev == 4 and col == (6 - 2) * 9 + (4 - 2)
+
+
+def test_ion_divergence_of_quadratic_velocities():
+    """With zero neutral and plain ion velocities the step leaves the ions
+    alone except for the divergence term: n / (1 + Dt div), div from
+    UAM_Gradient of a velocity quadratic in the cell indices, which its
+    three-point stencils differentiate exactly."""
+    g = synth.block_grid(synth.BlockGrid(2, 2, 20, nlons=6, nlats=5), seed=4, ions=True)
+    for k in ("vel", "vvel", "ivel"):
+        g[k][:] = 0.0
+    g["cmax"][:] = 0.0  # no Rusanov term: the ions are uniform per level below
+    g["ions"][:, :2] = 1.0e10
+    nb, L, NLg, NLo = g["rho"].shape
+    lon = np.arange(NLo) - 1.0
+    lat = np.arange(NLg) - 1.0
+    r = g["raddist"]
+    LA, LO = np.meshgrid(lat, lon, indexing="ij")
+    g["ivel_par"][:, 0] = (3.0 + 0.5 * LO + 0.02 * LO ** 2)[None, None]
+    g["ivel_par"][:, 1] = (1.0 - 0.3 * LA + 0.04 * LA ** 2)[None, None]
+    g["ivel_par"][:, 2] = 2.0 + 1e-5 * (r - 6.4e6) + 3e-12 * (r - 6.4e6) ** 2
+    dt = 5.0
+    out, rc, _ = cpu.horizontal(g, dt, use_coriolis=False, use_ion_advection=True,
+                                use_improved_ion_advection=True)
+    assert rc == 0
+    # the analytic divergence in index space (the grid's longitudes and
+    # latitudes are uniform, so d/dlon = (d/di) / dlon)
+    lonr, latr = g["latitude"], g["cos_lat"]
+    dlon = 2.0 * np.pi / (2 * 6)
+    dlatr = np.pi / (2 * 5)
+    i0 = np.s_[2:-2]
+    for b in range(nb):
+        for k in range(2, L - 2):
+            rr = r[b, k, 2:-2, 2:-2]
+            ir = 1.0 / rr
+            LOi, LAi = LO[2:-2, 2:-2], LA[2:-2, 2:-2]
+            cl = latr[b, 2:-2][:, None]
+            tl = np.clip(g["tan_lat"][b, 2:-2][:, None], -5.0, 5.0)
+            de = (0.5 + 0.04 * LOi) / dlon * ir / np.maximum(cl, 0.17)
+            dn = (-0.3 + 0.08 * LAi) / dlatr * ir
+            vn = 1.0 - 0.3 * LAi + 0.04 * LAi ** 2
+            x = rr - 6.4e6
+            du = 1e-5 + 6e-12 * x
+            vu = 2.0 + 1e-5 * x + 3e-12 * x ** 2
+            div = (de + dn - tl * vn * ir + du + 2 * vu * ir) * cl
+            want = 1.0e10 / (1 + dt * div)
+            np.testing.assert_allclose(out["ions"][b, 0, k, 2:-2, 2:-2], want, rtol=1e-12)
+    # electrons re-summed on every cell, halo cells and ghost levels included
+    e = np.zeros_like(out["ions"][:, 9])
+    for s in range(9):
+        e = e + out["ions"][:, s]
+    assert np.array_equal(out["ions"][:, 9], e)
