@@ -25,6 +25,7 @@ ERRORS = {
     6: "HIP runtime error",
     7: "out of device memory",
     8: "adjoint tape too small for the column",
+    9: "temperature below 100 after the message exchange",
 }
 
 
@@ -188,3 +189,13 @@ class HorizFields(ctypes.Structure):
 def latitude_field(ptr: int, n_lats: int) -> Field:
     """Descriptor of a Latitude-like array (-1:nLats+2, nBlocks): ptr at (1,1)."""
     return Field(ptr + 8 * 2, 0, 1, n_lats + 4, 0, 0, 0, 0)
+
+
+class ExchangeParams(ctypes.Structure):
+    _fields_ = [("n_blocks_lon", ctypes.c_int32), ("n_blocks_lat", ctypes.c_int32),
+                ("use_ion_advection", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class ExchangeFields(ctypes.Structure):
+    _fields_ = [(n, Field) for n in ("i_temperature", "e_temperature", "ion_velocity",
+                                     "latitude")]

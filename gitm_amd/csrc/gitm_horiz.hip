@@ -105,7 +105,8 @@ __device__ __forceinline__ void rusanov_cell(int i, int n, VF v, const LineGeo &
 // per CU holds about that many anyway)
 template <int NV>
 __global__ void __launch_bounds__(256, 2)
-    horiz_kernel(HzParams p, gitm_vert_columns io, HzFields f, unsigned long long *err) {
+    horiz_kernel(HzParams p, gitm_vert_columns io, HzFields f, unsigned long long *err,
+                 double *accg) {
   extern __shared__ double lds[];
   const int nlo = p.n_lons, nla = p.n_lats, NLo = nlo + 4, NLg = nla + 4;
   const int cells = nlo * nla, slab = NV * NLg * NLo;
@@ -127,7 +128,10 @@ __global__ void __launch_bounds__(256, 2)
     return io.idensity_s;
   };
   // stage the slices: x[a][v][lat][lon], halo cells included; after the
-  // slices, per altitude, the stage's new values and the RK4 sums, [v][cell]
+  // slices, per altitude, the stage's new values, [v][cell].  The RK4 sums
+  // live in device memory (accg, [workgroup][altitude][v][cell]): they are
+  // touched once per stage, and the LDS they would take costs a workgroup
+  // per CU.
   for (int e = tid; e < napw * slab; e += (int)blockDim.x) {
     const int aa = e / slab, r = e % slab;
     const int v = r / (NLg * NLo), r2 = r % (NLg * NLo);
@@ -138,8 +142,8 @@ __global__ void __launch_bounds__(256, 2)
   }
   const int aw = active ? a : 0;
   double *x = lds + (long long)aw * slab;
-  double *nwb = lds + (long long)p.apw * slab + (long long)aw * 2 * NV * cells;
-  double *accb = nwb + NV * cells;
+  double *nwb = lds + (long long)p.apw * slab + (long long)aw * NV * cells;
+  double *accb = accg + ((long long)blockIdx.x * p.apw + aw) * NV * cells;
   auto X = [&](int v, int lat, int lon) -> double & {
     return x[(v * NLg + lat + 1) * NLo + lon + 1];
   };
@@ -369,7 +373,152 @@ __global__ void __launch_bounds__(256)
   *hf(io.idensity_s, lon, lat, k, gv::NIONS - 1, b) = el;
 }
 
+// The message exchange on one device (exchange_messages_sphere.f90:4-95).
+// One thread per (block, halo cell, level -1..nAlts+2): on levels 1..nAlts
+// it copies every carried field from the global-grid cell the halo cell
+// overlaps (past a pole: the mirrored row, the longitude turned by 180
+// degrees); in the halo rows past a pole it then turns the east and north
+// winds around on every level (:30-72).  Sources are blocks' own cells,
+// which no thread writes.
+struct XParams {
+  int n_lons, n_lats, n_alts, nbl, nbt, ions;
+};
+
+struct XFields {
+  gitm_vert_field itemp, etemp, ivel, latitude;
+};
+
+__global__ void __launch_bounds__(256)
+    exchange_kernel(XParams p, gitm_vert_columns io, XFields f) {
+  const int nlo = p.n_lons, nla = p.n_lats, nA = p.n_alts;
+  const int NLo = nlo + 4, NLg = nla + 4, halo = NLo * NLg - nlo * nla;
+  const long long per_block = (long long)halo * (nA + 4);
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= per_block * p.nbl * p.nbt) return;
+  const int b = (int)(e / per_block);
+  const int rem = (int)(e % per_block);
+  const int k = rem / halo - 1, hc = rem % halo;
+  // halo cell hc: the two bottom rows, the side columns of the block's
+  // rows, the two top rows
+  int lo, la;
+  if (hc < 2 * NLo) {
+    la = hc / NLo - 1;
+    lo = hc % NLo - 1;
+  } else if (hc < 2 * NLo + 4 * nla) {
+    const int q = hc - 2 * NLo;
+    la = q / 4 + 1;
+    const int c = q % 4;
+    lo = c < 2 ? c - 1 : nlo + c - 1;
+  } else {
+    const int q = hc - 2 * NLo - 4 * nla;
+    la = nla + 1 + q / NLo;
+    lo = q % NLo - 1;
+  }
+  const long long NLON = (long long)p.nbl * nlo, NLAT = (long long)p.nbt * nla;
+  if (k >= 1 && k <= nA) {
+    long long J = (long long)(b % p.nbl) * nlo + lo - 1, I = (long long)(b / p.nbl) * nla + la - 1;
+    if (I < 0) {
+      I = -1 - I;
+      J += NLON / 2;
+    } else if (I >= NLAT) {
+      I = 2 * NLAT - 1 - I;
+      J += NLON / 2;
+    }
+    J = ((J % NLON) + NLON) % NLON;
+    const int sb = (int)(I / nla) * p.nbl + (int)(J / nlo);
+    const int slo = (int)(J % nlo) + 1, sla = (int)(I % nla) + 1;
+    auto cp = [&](const gitm_vert_field &g, int nc) {
+      for (int c = 0; c < nc; c++) *hf(g, lo, la, k, c, b) = *hf(g, slo, sla, k, c, sb);
+    };
+    cp(io.rho, 1);
+    cp(io.temperature, 1);
+    cp(f.itemp, 1);
+    cp(f.etemp, 1);
+    cp(io.velocity, 3);
+    cp(io.vertical_velocity, gv::NSP);
+    cp(io.ndensity_s, gv::NSP);
+    if (p.ions) {
+      cp(io.idensity_s, gv::NIONS);
+      cp(f.ivel, 3);
+    }
+  }
+  const double half_pi = 3.14159265358979323846 / 2.0;
+  const bool south = la <= 0 && *hf(f.latitude, 1, 0, 0, 0, b) < -half_pi;
+  const bool north = la >= nla + 1 && *hf(f.latitude, 1, nla + 1, 0, 0, b) > half_pi;
+  if (south || north) {
+    for (int c = 0; c < 2; c++) {
+      double *v = hf(io.velocity, lo, la, k, c, b);
+      *v = -*v;
+      if (f.ivel.ptr) {
+        double *w = hf(f.ivel, lo, la, k, c, b);
+        *w = -*w;
+      }
+    }
+  }
+}
+
+// The checks after the exchange (:74-84): one thread per cell and level
+// 1..nAlts of the blocks.
+__global__ void __launch_bounds__(256)
+    exchange_check_kernel(XParams p, gitm_vert_columns io, unsigned long long *err) {
+  const int nlo = p.n_lons, nla = p.n_lats, nA = p.n_alts;
+  const long long per_block = (long long)nlo * nla * nA;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= per_block * p.nbl * p.nbt) return;
+  const int b = (int)(e / per_block);
+  const int rem = (int)(e % per_block);
+  const int k = rem / (nlo * nla) + 1, cell = rem % (nlo * nla);
+  const int lo = cell % nlo + 1, la = cell / nlo + 1;
+  const int column = (b * nla + la - 1) * nlo + lo - 1;
+  if (*hf(io.temperature, lo, la, k, 0, b) < 100.0)
+    atomicMin(err, gv::error_key(column, GITM_VERT_ELOWTEMP, k));
+  if (*hf(io.rho, lo, la, k, 0, b) < 0.0)
+    atomicMin(err, gv::error_key(column, GITM_VERT_ENONPOSITIVE, k));
+}
+
 }  // namespace
+
+extern "C" int gitm_horiz_exchange(gitm_vert_handle *h, const gitm_horiz_exchange_params *xp,
+                                   const gitm_vert_columns *c,
+                                   const gitm_horiz_exchange_fields *xf) {
+  if (!h || !xp || !c || !xf) return GITM_VERT_EINVAL;
+  const int nlo = c->n_lons, nla = c->n_lats;
+  if (nlo < 1 || nla < 2 || c->n_alts < 1 || xp->n_blocks_lon < 1 || xp->n_blocks_lat < 1)
+    return GITM_VERT_EINVAL;
+  if ((long long)xp->n_blocks_lon * xp->n_blocks_lat != c->n_blocks) return GITM_VERT_EINVAL;
+  if (((long long)xp->n_blocks_lon * nlo) % 2) return GITM_VERT_EINVAL;
+  const gitm_vert_field *req[] = {&c->rho, &c->temperature, &c->velocity,
+                                  &c->vertical_velocity, &c->ndensity_s, &xf->i_temperature,
+                                  &xf->e_temperature, &xf->latitude};
+  for (const gitm_vert_field *g : req)
+    if (!g->ptr) return GITM_VERT_EINVAL;
+  const bool ions = xp->use_ion_advection != 0;
+  if (ions && (!c->idensity_s.ptr || !xf->ion_velocity.ptr)) return GITM_VERT_EINVAL;
+  XParams p;
+  p.n_lons = nlo;
+  p.n_lats = nla;
+  p.n_alts = c->n_alts;
+  p.nbl = xp->n_blocks_lon;
+  p.nbt = xp->n_blocks_lat;
+  p.ions = ions;
+  XFields f;
+  f.itemp = xf->i_temperature;
+  f.etemp = xf->e_temperature;
+  f.ivel = xf->ion_velocity;
+  f.latitude = xf->latitude;
+  const long long halo = (long long)(nlo + 4) * (nla + 4) - (long long)nlo * nla;
+  const long long n1 = halo * (c->n_alts + 4) * c->n_blocks;
+  const long long n2 = (long long)nlo * nla * c->n_alts * c->n_blocks;
+  if ((n1 + 255) / 256 > 0x7fffffffLL || (n2 + 255) / 256 > 0x7fffffffLL) return GITM_VERT_EINVAL;
+  GV_HIP(hipSetDevice(h->device));
+  hipLaunchKernelGGL(exchange_kernel, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0,
+                     h->stream, p, *c, f);
+  GV_HIP(hipGetLastError());
+  hipLaunchKernelGGL(exchange_check_kernel, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0,
+                     h->stream, p, *c, h->d_err);
+  GV_HIP(hipGetLastError());
+  return GITM_VERT_OK;
+}
 
 extern "C" int gitm_horiz_advance(gitm_vert_handle *h, const gitm_horiz_params *hp,
                                   const gitm_vert_columns *c, const gitm_horiz_fields *hf_) {
@@ -406,13 +555,15 @@ extern "C" int gitm_horiz_advance(gitm_vert_handle *h, const gitm_horiz_params *
       if (!g->ptr) return GITM_VERT_EINVAL;
   const int nv = ions ? NV_IONS : NV_NEUTRAL;
   // per altitude: the slab of every variable with halo cells, then the
-  // stage's new values and the RK4 sums of the block's cells
-  const size_t slab = (size_t)nv * ((nla + 4) * (nlo + 4) + 2 * cells) * sizeof(double);
-  // altitudes per workgroup: as many as fit 256 lanes and 80 KB of LDS
+  // stage's new values of the block's cells
+  const size_t slab = (size_t)nv * ((nla + 4) * (nlo + 4) + cells) * sizeof(double);
+  // altitudes per workgroup: as many as fit 256 lanes and 40 KB of LDS, so
+  // that four workgroups share a CU (two wavefronts per SIMD, the register
+  // limit); 9x9 blocks take one
   int apw = hp->altitudes_per_workgroup > 0 ? hp->altitudes_per_workgroup : 0;
   if (apw == 0) {
     apw = 1;
-    while ((apw + 1) * cells <= 256 && (apw + 1) * slab <= 80 * 1024 && apw < 4) apw++;
+    while ((apw + 1) * cells <= 256 && (apw + 1) * slab <= 40 * 1024 && apw < 4) apw++;
   }
   if (apw * cells > 256 || apw * slab > 160 * 1024) return GITM_VERT_EINVAL;
   apw = std::min(apw, c->n_alts);
@@ -444,12 +595,25 @@ extern "C" int gitm_horiz_advance(gitm_vert_handle *h, const gitm_horiz_params *
   GV_HIP(hipSetDevice(h->device));
   const int threads = ((apw * cells + 63) / 64) * 64;
   const size_t lds = (size_t)apw * slab;
+  // the RK4 sums, in the handle's scratch (shared with the vertical step's
+  // carries: both only use it within a launch on the handle's stream)
+  const size_t need = (size_t)nwg * apw * nv * cells * sizeof(double);
+  if (h->scratch_bytes < need) {
+    if (h->scratch) {
+      GV_HIP(hipStreamSynchronize(h->stream));
+      (void)hipFree(h->scratch);
+      h->scratch = nullptr;
+      h->scratch_bytes = 0;
+    }
+    if (hipMalloc(&h->scratch, need) != hipSuccess) return GITM_VERT_ENOMEM;
+    h->scratch_bytes = need;
+  }
   if (ions)
     hipLaunchKernelGGL(horiz_kernel<NV_IONS>, dim3((unsigned)nwg), dim3(threads), lds,
-                       h->stream, p, *c, f, h->d_err);
+                       h->stream, p, *c, f, h->d_err, h->scratch);
   else
     hipLaunchKernelGGL(horiz_kernel<NV_NEUTRAL>, dim3((unsigned)nwg), dim3(threads), lds,
-                       h->stream, p, *c, f, h->d_err);
+                       h->stream, p, *c, f, h->d_err, h->scratch);
   GV_HIP(hipGetLastError());
   if (improved) {
     const long long n =

@@ -65,6 +65,7 @@ const char *gitm_vert_strerror(int code) {
     case GITM_VERT_EHIP: return "HIP runtime error";
     case GITM_VERT_ENOMEM: return "out of device memory";
     case GITM_VERT_ETAPE: return "adjoint tape too small for the column";
+    case GITM_VERT_ELOWTEMP: return "temperature below 100 after the message exchange";
     default: return "unknown error";
   }
 }

@@ -90,6 +90,10 @@ def lib() -> ctypes.CDLL:
         _lib.gitm_vert_sources.argtypes = [vp, ctypes.POINTER(_abi.SourcesParams),
                                            ctypes.POINTER(_abi.Columns),
                                            ctypes.POINTER(_abi.SourcesFields)]
+        _lib.gitm_horiz_exchange.restype = ctypes.c_int
+        _lib.gitm_horiz_exchange.argtypes = [vp, ctypes.POINTER(_abi.ExchangeParams),
+                                             ctypes.POINTER(_abi.Columns),
+                                             ctypes.POINTER(_abi.ExchangeFields)]
         _lib.gitm_horiz_advance.restype = ctypes.c_int
         _lib.gitm_horiz_advance.argtypes = [vp, ctypes.POINTER(_abi.HorizParams),
                                             ctypes.POINTER(_abi.Columns),
@@ -356,6 +360,48 @@ class VerticalSolver:
         self.set_stream(torch.cuda.current_stream().cuda_stream)
         rc = lib().gitm_horiz_advance(self._h, ctypes.byref(p), ctypes.byref(c),
                                       ctypes.byref(f))
+        if rc:
+            raise VerticalStepError(rc)
+
+    def exchange(self, grid: dict, n_blocks_lon: int, use_ion_advection: bool = False,
+                 ion_velocity: str = "ivel_par") -> None:
+        """The message exchange around the horizontal step
+        (``gitm_horiz_exchange``) on device tensors in the block layout of
+        ``horizontal``: the blocks form a regular grid ``n_blocks_lon``
+        wide; "rho", "temp", "itemp", "etemp", "vel", "vvel", "nds" and,
+        with ions, "ions" and the ion velocity ``grid[ion_velocity]`` get
+        their halo cells; "latitude" marks the blocks at the poles.  Queued
+        on the current stream; the checks surface in ``check()``."""
+        import torch
+
+        nb, L, NLg, NLo = (int(v) for v in grid["rho"].shape)
+        nlo, nla, na = NLo - 4, NLg - 4, L - 4
+        for k, t in grid.items():
+            if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+                raise ValueError(f"{k}: need a contiguous float64 device tensor")
+
+        def blk(key, ncomp=1):
+            return _abi.gitm_block_field(grid[key].data_ptr(), nlo, nla, na, ncomp)
+
+        c = _abi.Columns()
+        c.n_lons, c.n_lats, c.n_blocks, c.n_alts = nlo, nla, nb, na
+        c.rho, c.temperature = blk("rho"), blk("temp")
+        c.velocity = blk("vel", 3)
+        c.vertical_velocity = blk("vvel", _abi.NSPECIES)
+        c.ndensity_s = blk("nds", _abi.NSPECIES)
+        f = _abi.ExchangeFields()
+        f.i_temperature, f.e_temperature = blk("itemp"), blk("etemp")
+        f.latitude = _abi.latitude_field(grid["latitude"].data_ptr(), nla)
+        if ion_velocity in grid:
+            f.ion_velocity = blk(ion_velocity, 3)
+        if use_ion_advection:
+            c.idensity_s = blk("ions", _abi.NIONS)
+        p = _abi.ExchangeParams()
+        p.n_blocks_lon, p.n_blocks_lat = n_blocks_lon, nb // max(n_blocks_lon, 1)
+        p.use_ion_advection = int(use_ion_advection)
+        self.set_stream(torch.cuda.current_stream().cuda_stream)
+        rc = lib().gitm_horiz_exchange(self._h, ctypes.byref(p), ctypes.byref(c),
+                                       ctypes.byref(f))
         if rc:
             raise VerticalStepError(rc)
 

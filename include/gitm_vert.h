@@ -52,7 +52,9 @@ enum {
                                  run the Fortran path                      */
   GITM_VERT_EHIP = 6,         /* HIP runtime error                         */
   GITM_VERT_ENOMEM = 7,
-  GITM_VERT_ETAPE = 8         /* adjoint tape too small for the column     */
+  GITM_VERT_ETAPE = 8,        /* adjoint tape too small for the column     */
+  GITM_VERT_ELOWTEMP = 9      /* Temperature < 100 after the message
+                                 exchange (exchange_messages_sphere.f90:74-78) */
 };
 
 /* One array of the column working set.  Element (column c, level k,
@@ -404,6 +406,45 @@ typedef struct gitm_horiz_fields {
  * block).  Asynchronous on the handle's stream. */
 int gitm_horiz_advance(gitm_vert_handle *h, const gitm_horiz_params *p,
                        const gitm_vert_columns *c, const gitm_horiz_fields *f);
+
+/* ------------------------------------------------------------------------
+ * The message exchange around the horizontal step
+ * (exchange_messages_sphere.f90:4-95, the block-to-block transfer of
+ * ModSphereInterface.f90:245-513) for blocks that share one device.  The
+ * blocks form a regular grid, n_blocks_lon wide and n_blocks_lat high, block
+ * b at longitude index b % n_blocks_lon and latitude index b / n_blocks_lon
+ * (both increasing eastward and northward; n_lons * n_blocks_lon even).
+ * Every halo cell (-1:0, n+1:n+2, corners included) of levels 1..nAlts
+ * takes the value of the global-grid cell it overlaps: longitudes wrap;
+ * past a pole the row is mirrored and the longitude turned by 180 degrees,
+ * as the pole packing of srcSphereAB/AB_XFER_util.f90:40-80 does.  Carried:
+ * Rho, Temperature, iTemperature, eTemperature, Velocity, VerticalVelocity,
+ * NDensityS(1:6) and, with use_ion_advection, IDensityS(1:10) and the ion
+ * velocity.  Then the east and north Velocity and ion velocity change sign
+ * in the halo rows past a pole, on every level (:30-72), and Temperature <
+ * 100 (GITM_VERT_ELOWTEMP) or Rho < 0 (GITM_VERT_ENONPOSITIVE) on the
+ * blocks' cells is reported through gitm_vert_get_error (:74-84).
+ * ------------------------------------------------------------------------ */
+typedef struct gitm_horiz_exchange_params {
+  int32_t n_blocks_lon, n_blocks_lat;
+  int32_t use_ion_advection;          /* UseIonAdvection                     */
+  int32_t reserved;
+} gitm_horiz_exchange_params;
+
+typedef struct gitm_horiz_exchange_fields {
+  gitm_vert_field i_temperature;   /* iTemperature, described as rho         */
+  gitm_vert_field e_temperature;   /* eTemperature                           */
+  gitm_vert_field ion_velocity;    /* IVelocityPar (IVelocity where
+                                      NonMagnetic), as ivelocity; may be NULL
+                                      without ion advection                 */
+  gitm_vert_field latitude;        /* Latitude, as cos_latitude             */
+} gitm_horiz_exchange_fields;
+
+/* Rho, Temperature, Velocity, VerticalVelocity, NDensityS and (with ions)
+ * IDensityS come from c (device pointers).  Asynchronous on the handle's
+ * stream. */
+int gitm_horiz_exchange(gitm_vert_handle *h, const gitm_horiz_exchange_params *p,
+                        const gitm_vert_columns *c, const gitm_horiz_exchange_fields *f);
 
 /* Member and member column of a batch-wide column number. */
 int gitm_vert_batch_locate(const gitm_vert_batch *b, int column, int *member,

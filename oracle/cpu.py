@@ -261,3 +261,38 @@ def horizontal(grid: dict, dt: float, omega_body: float = 2.0 * np.pi / 86400.0,
     st = _Status()
     rc = lib().oh_advance_horizontal(ctypes.byref(h), ctypes.byref(st))
     return out, rc, (st.col, st.level)
+
+
+# --- the message exchange (oh_exchange) -------------------------------------
+class _Exchange(ctypes.Structure):
+    _fields_ = [("n_lons", ctypes.c_int), ("n_lats", ctypes.c_int), ("n_alts", ctypes.c_int),
+                ("n_blocks_lon", ctypes.c_int), ("n_blocks_lat", ctypes.c_int),
+                ("use_ion_advection", ctypes.c_int)] + [
+        (n, _P) for n in ("rho", "temp", "itemp", "etemp", "vel", "vvel", "nds", "ions",
+                          "ivel", "latitude")]
+
+
+EXCHANGED = ("rho", "temp", "itemp", "etemp", "vel", "vvel", "nds")
+
+
+def exchange(grid: dict, n_blocks_lon: int, use_ion_advection: bool = False):
+    """The oracle's message exchange on a copy of the block-layout ``grid``
+    (blocks in a regular n_blocks_lon-wide grid); returns (updated arrays,
+    status code, (column, level)).  "ivel" is the ion velocity the exchange
+    carries and flips (IVelocityPar on Earth)."""
+    nb, L, NLg, NLo = grid["rho"].shape
+    keys = EXCHANGED + (("ions", "ivel") if use_ion_advection else ())
+    out = {k: np.array(grid[k], dtype=np.float64, order="C") for k in keys}
+    if not use_ion_advection and "ivel" in grid:
+        out["ivel"] = np.array(grid["ivel"], dtype=np.float64, order="C")  # flipped only
+    lat = np.ascontiguousarray(grid["latitude"], dtype=np.float64)
+    x = _Exchange()
+    x.n_lons, x.n_lats, x.n_alts = NLo - 4, NLg - 4, L - 4
+    x.n_blocks_lon, x.n_blocks_lat = n_blocks_lon, nb // n_blocks_lon
+    x.use_ion_advection = int(use_ion_advection)
+    for k, v in out.items():
+        setattr(x, k, _ptr(v))
+    x.latitude = _ptr(lat)
+    st = _Status()
+    rc = lib().oh_exchange(ctypes.byref(x), ctypes.byref(st))
+    return out, rc, (st.col, st.level)

@@ -397,3 +397,98 @@ int oh_advance_horizontal(const oh_grid *p, ov_status *st) {
   }
   return first;
 }
+
+/* ---------------------------------------------------------------------------
+ * The message exchange on one node (exchange_messages_sphere.f90:4-95).
+ * ------------------------------------------------------------------------ */
+int oh_exchange(const oh_exchange_args *a, ov_status *st) {
+  const int nlo = a->n_lons, nla = a->n_lats, nA = a->n_alts;
+  const int nbl = a->n_blocks_lon, nbt = a->n_blocks_lat, nb = nbl * nbt;
+  const long long NLON = (long long)nbl * nlo, NLAT = (long long)nbt * nla;
+  if (nlo < 1 || nla < 2 || nA < 1 || nbl < 1 || nbt < 1 || NLON % 2) return OV_ERR_BAD_ARGS;
+  if (a->use_ion_advection && (!a->ions || !a->ivel)) return OV_ERR_BAD_ARGS;
+  geo g;
+  g.nlo = nlo;
+  g.nla = nla;
+  g.nA = nA;
+  g.slo = 1;
+  g.sla = nlo + 4;
+  g.slev = (long long)(nlo + 4) * (nla + 4);
+  const long long fsz = (long long)(nA + 4) * g.slev;
+  struct { double *p; int nc; } f[9] = {{a->rho, 1}, {a->temp, 1}, {a->itemp, 1}, {a->etemp, 1},
+                                        {a->vel, 3}, {a->vvel, OV_NSPECIES},
+                                        {a->nds, OV_NSPECIES}, {a->ions, OV_NIONS},
+                                        {a->ivel, 3}};
+  const int nf = a->use_ion_advection ? 9 : 7;
+  for (int b = 0; b < nb; b++) {
+    const int bi = b % nbl, bj = b / nbl;
+    for (int la = -1; la <= nla + 2; la++)
+      for (int lo = -1; lo <= nlo + 2; lo++) {
+        if (la >= 1 && la <= nla && lo >= 1 && lo <= nlo) continue; /* interior */
+        long long J = (long long)bi * nlo + lo - 1, I = (long long)bj * nla + la - 1;
+        if (I < 0) { /* past the south pole */
+          I = -1 - I;
+          J += NLON / 2;
+        } else if (I >= NLAT) { /* past the north pole */
+          I = 2 * NLAT - 1 - I;
+          J += NLON / 2;
+        }
+        J = ((J % NLON) + NLON) % NLON;
+        const int sb = (int)(I / nla) * nbl + (int)(J / nlo);
+        const int slo = (int)(J % nlo) + 1, sla = (int)(I % nla) + 1;
+        for (int k = 1; k <= nA; k++)
+          for (int q = 0; q < nf; q++)
+            for (int c = 0; c < f[q].nc; c++) {
+              double *base = f[q].p + (long long)c * fsz;
+              base[((long long)b * f[q].nc) * fsz + at3(&g, lo, la, k)] =
+                  base[((long long)sb * f[q].nc) * fsz + at3(&g, slo, sla, k)];
+            }
+      }
+  }
+  /* the pole halo rows: east and north winds change sign (:30-72) */
+  for (int b = 0; b < nb; b++) {
+    const double *lat = a->latitude + (long long)b * (nla + 4);
+    const double half_pi = 3.14159265358979323846 / 2.0;
+    for (int side = 0; side < 2; side++) {
+      const int flip = side == 0 ? lat[0 + 1] < -half_pi : lat[nla + 1 + 1] > half_pi;
+      if (!flip) continue;
+      const int r0 = side == 0 ? -1 : nla + 1;
+      double *vv[2] = {a->vel, a->ivel};
+      for (int w = 0; w < 2; w++) {
+        if (!vv[w]) continue;
+        for (int c = 0; c < 2; c++)
+          for (int k = -1; k <= nA + 2; k++)
+            for (int la = r0; la <= r0 + 1; la++)
+              for (int lo = -1; lo <= nlo + 2; lo++) {
+                double *x = vv[w] + ((long long)b * 3 + c) * fsz + at3(&g, lo, la, k);
+                *x = -*x;
+              }
+      }
+    }
+  }
+  /* checks on the blocks' cells (:74-84) */
+  if (st) {
+    st->code = OV_OK;
+    st->col = -1;
+    st->level = -99;
+  }
+  for (int b = 0; b < nb; b++)
+    for (int pass = 0; pass < 2; pass++) { /* per block: Temperature, then Rho */
+      const int code = pass == 0 ? OV_ERR_LOW_TEMPERATURE : OV_ERR_NONPOSITIVE_DENSITY;
+      const double *x = (pass == 0 ? a->temp : a->rho) + (long long)b * fsz;
+      for (int k = 1; k <= nA; k++)
+        for (int la = 1; la <= nla; la++)
+          for (int lo = 1; lo <= nlo; lo++) {
+            const double v = x[at3(&g, lo, la, k)];
+            if (pass == 0 ? v < 100.0 : v < 0.0) {
+              if (st) {
+                st->code = code;
+                st->col = (b * nla + la - 1) * nlo + lo - 1;
+                st->level = k;
+              }
+              return code;
+            }
+          }
+    }
+  return OV_OK;
+}

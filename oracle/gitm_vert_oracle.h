@@ -203,6 +203,31 @@ typedef struct {
 
 int oh_advance_horizontal(const oh_grid *p, ov_status *status);
 
+/* The message exchange before and after the horizontal step
+ * (exchange_messages_sphere.f90:4-95) on one node: every block of a regular
+ * grid of n_blocks_lon x n_blocks_lat blocks (block b at longitude
+ * b % n_blocks_lon, latitude b / n_blocks_lon, both increasing) gets its
+ * halo cells (-1:0, n+1:n+2, corners included) on levels 1..nAlts from the
+ * cells of the global grid they overlap: longitudes wrap; past a pole the
+ * row is mirrored and the longitude turned by 180 degrees (the pole packing
+ * of srcSphereAB/AB_XFER_util.f90:40-80).  Rho, Temperature,
+ * iTemperature, eTemperature, Velocity(3), VerticalVelocity(6),
+ * NDensityS(1:6) and, with ions, IDensityS(1:10) and the ion velocity
+ * (IVelocityPar on Earth).  Then, in the halo rows past a pole, the east and
+ * north components of Velocity and of the ion velocity change sign on every
+ * level (exchange_messages_sphere.f90:30-72), and Temperature < 100 or
+ * Rho < 0 on the blocks' cells is an error (:74-84). */
+typedef struct {
+  int n_lons, n_lats, n_alts, n_blocks_lon, n_blocks_lat;
+  int use_ion_advection;
+  double *rho, *temp, *itemp, *etemp, *vel, *vvel, *nds, *ions, *ivel;
+  const double *latitude; /* [block][-1..nLats+2] */
+} oh_exchange_args;
+
+enum { OV_ERR_LOW_TEMPERATURE = 5 };
+
+int oh_exchange(const oh_exchange_args *a, ov_status *status);
+
 #ifdef __cplusplus
 }
 #endif
