@@ -289,8 +289,21 @@ def bench_horizontal(torch, solver, reps: int = 10) -> dict:
     torch.cuda.synchronize()
     solver.check()
     ms = e0.elapsed_time(e1) / reps
+    # the message exchange on the device (gitm_horiz_exchange), same grid
+    g["itemp"], g["etemp"] = g["temp"].clone(), g["temp"].clone()
+    solver.exchange(g, spec.nblocks_lon)
+    e0.record(stream)
+    for _ in range(reps):
+        solver.exchange(g, spec.nblocks_lon)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    solver.check()
+    xms = e0.elapsed_time(e1) / reps
     ok = bool(torch.isfinite(g["temp"]).all()) and bool((g["nds"] > 0).all())
     slabs = spec.nblocks * spec.nalts
+    halo = (spec.nlons + 4) * (spec.nlats + 4) - spec.nlons * spec.nlats
+    # 19 carried fields, one read and one write per halo cell and level
+    xbytes = 2 * 8 * 19 * halo * spec.nalts * spec.nblocks
     nbytes = horizontal_bytes_per_slab(spec.nlons, spec.nlats) * slabs
     del g
     return {"kernel": "horiz_kernel<17> (gitm_horiz_advance: advance_horizontal, RK4 of "
@@ -301,6 +314,10 @@ def bench_horizontal(torch, solver, reps: int = 10) -> dict:
             "achieved": nbytes / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "cell_updates_per_s": spec.ncells * spec.nalts / (ms * 1e-3),
+            "exchange": {"kernel": "exchange_kernel + exchange_check_kernel "
+                                   "(gitm_horiz_exchange, neutrals)",
+                         "launch_ms": xms, "halo_bytes": xbytes,
+                         "achieved_gbs": xbytes / (xms * 1e-3) / 1e9},
             "state_finite_and_positive": ok}
 
 

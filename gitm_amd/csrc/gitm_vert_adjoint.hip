@@ -94,17 +94,57 @@ __global__ void __launch_bounds__(NT)
   const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int nph = t.nphase[g];  // the same on every lane of the column
   double *adj = t.adj;
+  // Nodes go in blocks of SB, last to first: the block's tape entries and
+  // adjoints are loaded at once, a contribution to a parent inside the
+  // block (the lane's own node of this phase, which no other lane touches
+  // until the barrier) is added in registers, the others atomically.  A
+  // node's adjoint is complete when it is reached: its children come after
+  // it, inside the block (already in registers) or above it (their
+  // atomics were issued before this block's loads).
+  constexpr int SB = 16;
   for (int ph = nph - 1; ph >= 0; ph--) {
     const int hi = t.phase[(long long)ph * t.stride + g];
     const int lo = ph ? t.phase[(long long)(ph - 1) * t.stride + g] : 0;
-    for (int n = hi - 1; n >= lo; n--) {
-      const long long id = (long long)n * t.stride + g;
-      const int p1 = t.p1[id], p2 = t.p2[id];
-      const double d1 = t.d1[id], d2 = t.d2[id];
-      const double a = __hip_atomic_load(adj + id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (a == 0.0) continue;
-      if (p1 >= 0) unsafeAtomicAdd(adj + p1, a * d1);
-      if (p2 >= 0) unsafeAtomicAdd(adj + p2, a * d2);
+    for (int top = hi - 1; top >= lo; top -= SB) {
+      const int cnt = min(SB, top - lo + 1);  // nodes top, top-1, .., top-cnt+1
+      double a[SB], d1[SB], d2[SB];
+      int p1[SB], p2[SB];
+#pragma unroll
+      for (int j = 0; j < SB; j++) {
+        a[j] = 0.0;
+        p1[j] = p2[j] = -1;
+        d1[j] = d2[j] = 0.0;
+        if (j < cnt) {
+          const long long id = (long long)(top - j) * t.stride + g;
+          p1[j] = t.p1[id];
+          p2[j] = t.p2[id];
+          d1[j] = t.d1[id];
+          d2[j] = t.d2[id];
+          a[j] = __hip_atomic_load(adj + id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      const long long base = (long long)(top - cnt + 1) * t.stride + g;  // lowest node's id
+      const long long last = (long long)top * t.stride + g;
+#pragma unroll
+      for (int j = 0; j < SB; j++) {
+        const double aj = a[j];
+        if (aj == 0.0) continue;
+        auto push = [&](int p, double d) {
+          if (p < 0) return;
+          const double v = aj * d;
+          // inside the block: node top - k for id = last - k * stride
+          if (p >= base && p < last && (p - base) % t.stride == 0) {
+            const int k = (int)((last - p) / t.stride);
+#pragma unroll
+            for (int q = 0; q < SB; q++)
+              if (q == k) a[q] = a[q] + v;
+          } else {
+            unsafeAtomicAdd(adj + p, v);
+          }
+        };
+        push(p1[j], d1[j]);
+        push(p2[j], d2[j]);
+      }
     }
     __threadfence();
     __syncthreads();
