@@ -94,6 +94,10 @@ def lib() -> ctypes.CDLL:
         _lib.gitm_horiz_exchange.argtypes = [vp, ctypes.POINTER(_abi.ExchangeParams),
                                              ctypes.POINTER(_abi.Columns),
                                              ctypes.POINTER(_abi.ExchangeFields)]
+        _lib.gitm_horiz_advance_host.restype = ctypes.c_int
+        _lib.gitm_horiz_advance_host.argtypes = [vp, ctypes.POINTER(_abi.HorizParams),
+                                                 ctypes.POINTER(_abi.Columns),
+                                                 ctypes.POINTER(_abi.HorizFields)]
         _lib.gitm_horiz_advance.restype = ctypes.c_int
         _lib.gitm_horiz_advance.argtypes = [vp, ctypes.POINTER(_abi.HorizParams),
                                             ctypes.POINTER(_abi.Columns),
@@ -172,6 +176,53 @@ def _device_columns(cols: dict, extra: tuple = ()):
     nc, L = cols["temp"].shape
     more = {k: ptrs.pop(k) for k in extra if k in ptrs}
     return _abi.columns_struct(ptrs, nc, L - 4), more
+
+
+def _horiz_structs(ptr: dict, shape, dt, omega_body, beta_limiter, use_coriolis,
+                   use_ion_advection, altitudes_per_workgroup, use_improved_ion_advection):
+    """Columns, HorizFields and HorizParams for block-layout arrays at the
+    addresses ``ptr`` (gitm_amd.synth.block_grid names)."""
+    nb, L, NLg, NLo = (int(v) for v in shape)
+    nlo, nla, na = NLo - 4, NLg - 4, L - 4
+
+    def blk(key, ncomp=1):
+        return _abi.gitm_block_field(ptr[key], nlo, nla, na, ncomp)
+
+    c = _abi.Columns()
+    c.n_lons, c.n_lats, c.n_blocks, c.n_alts = nlo, nla, nb, na
+    c.rho, c.temperature, c.gamma = blk("rho"), blk("temp"), blk("gamma")
+    c.velocity = blk("vel", 3)
+    c.vertical_velocity = blk("vvel", _abi.NSPECIES)
+    c.ndensity_s = blk("nds", _abi.NSPECIES)
+    c.inv_radial_distance = blk("invr")
+    c.cos_latitude = _abi.latitude_field(ptr["cos_lat"], nla)
+    if use_ion_advection:
+        c.idensity_s = blk("ions", _abi.NIONS)
+        c.ivelocity = blk("ivel", 3)
+    f = _abi.HorizFields()
+    for key in ("inv_dlon_gb", "inv_dlon_fb", "dlon_fb", "inv_dlat_gb", "inv_dlat_fb",
+                "dlat_fb"):
+        setattr(f, key, blk(key))
+    f.radial_distance = blk("raddist")
+    f.latitude = _abi.latitude_field(ptr["latitude"], nla)
+    f.tan_latitude = _abi.latitude_field(ptr["tan_lat"], nla)
+    f.cmax_horizontal = _abi.gitm_block_field(ptr["cmax"], nlo, nla, na, 3, lon_lo=0,
+                                              lat_lo=0, alt_lo=0, alt_hi=na + 1)
+    if "temp_source" in ptr:
+        f.temp_source = blk("temp_source")
+    if use_ion_advection and use_improved_ion_advection:
+        f.ivelocity_par = blk("ivel_par", 3)
+        lo, la = ptr["grad_lon"], ptr["grad_lat"]
+        f.grad_lon_m, f.grad_lon_0, f.grad_lon_p = (
+            _abi.Field(lo + 8 * j * nlo, 1, 0, 3 * nlo, 0, 0, 0, 0) for j in range(3))
+        f.grad_lat_m, f.grad_lat_0, f.grad_lat_p = (
+            _abi.Field(la + 8 * j * nla, 0, 1, 3 * nla, 0, 0, 0, 0) for j in range(3))
+    p = _abi.HorizParams()
+    p.dt, p.omega_body, p.beta_limiter = dt, omega_body, beta_limiter
+    p.use_coriolis, p.use_ion_advection = int(use_coriolis), int(use_ion_advection)
+    p.use_improved_ion_advection = int(use_improved_ion_advection)
+    p.altitudes_per_workgroup = altitudes_per_workgroup
+    return c, f, p
 
 
 class VerticalSolver:
@@ -313,53 +364,35 @@ class VerticalSolver:
         ([block][M, 0, P][cell])."""
         import torch
 
-        nb, L, NLg, NLo = (int(v) for v in grid["rho"].shape)
-        nlo, nla, na = NLo - 4, NLg - 4, L - 4
         for k, t in grid.items():
             if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
                 raise ValueError(f"{k}: need a contiguous float64 device tensor")
-
-        def blk(key, ncomp=1):
-            return _abi.gitm_block_field(grid[key].data_ptr(), nlo, nla, na, ncomp)
-
-        c = _abi.Columns()
-        c.n_lons, c.n_lats, c.n_blocks, c.n_alts = nlo, nla, nb, na
-        c.rho, c.temperature, c.gamma = blk("rho"), blk("temp"), blk("gamma")
-        c.velocity = blk("vel", 3)
-        c.vertical_velocity = blk("vvel", _abi.NSPECIES)
-        c.ndensity_s = blk("nds", _abi.NSPECIES)
-        c.inv_radial_distance = blk("invr")
-        c.cos_latitude = _abi.latitude_field(grid["cos_lat"].data_ptr(), nla)
-        if use_ion_advection:
-            c.idensity_s = blk("ions", _abi.NIONS)
-            c.ivelocity = blk("ivel", 3)
-        f = _abi.HorizFields()
-        for key in ("inv_dlon_gb", "inv_dlon_fb", "dlon_fb", "inv_dlat_gb", "inv_dlat_fb",
-                    "dlat_fb"):
-            setattr(f, key, blk(key))
-        f.radial_distance = blk("raddist")
-        f.latitude = _abi.latitude_field(grid["latitude"].data_ptr(), nla)
-        f.tan_latitude = _abi.latitude_field(grid["tan_lat"].data_ptr(), nla)
-        f.cmax_horizontal = _abi.gitm_block_field(grid["cmax"].data_ptr(), nlo, nla, na, 3,
-                                                  lon_lo=0, lat_lo=0, alt_lo=0,
-                                                  alt_hi=na + 1)
-        if "temp_source" in grid:
-            f.temp_source = blk("temp_source")
-        if use_ion_advection and use_improved_ion_advection:
-            f.ivelocity_par = blk("ivel_par", 3)
-            lo, la = grid["grad_lon"].data_ptr(), grid["grad_lat"].data_ptr()
-            f.grad_lon_m, f.grad_lon_0, f.grad_lon_p = (
-                _abi.Field(lo + 8 * j * nlo, 1, 0, 3 * nlo, 0, 0, 0, 0) for j in range(3))
-            f.grad_lat_m, f.grad_lat_0, f.grad_lat_p = (
-                _abi.Field(la + 8 * j * nla, 0, 1, 3 * nla, 0, 0, 0, 0) for j in range(3))
-        p = _abi.HorizParams()
-        p.dt, p.omega_body, p.beta_limiter = dt, omega_body, beta_limiter
-        p.use_coriolis, p.use_ion_advection = int(use_coriolis), int(use_ion_advection)
-        p.use_improved_ion_advection = int(use_improved_ion_advection)
-        p.altitudes_per_workgroup = altitudes_per_workgroup
+        c, f, p = _horiz_structs({k: t.data_ptr() for k, t in grid.items()},
+                                 tuple(int(v) for v in grid["rho"].shape), dt, omega_body,
+                                 beta_limiter, use_coriolis, use_ion_advection,
+                                 altitudes_per_workgroup, use_improved_ion_advection)
         self.set_stream(torch.cuda.current_stream().cuda_stream)
         rc = lib().gitm_horiz_advance(self._h, ctypes.byref(p), ctypes.byref(c),
                                       ctypes.byref(f))
+        if rc:
+            raise VerticalStepError(rc)
+
+    def horizontal_host(self, grid: dict, dt: float, omega_body: float = 2.0 * math.pi / 86400.0,
+                        beta_limiter: float = 2.0, use_coriolis: bool = True,
+                        use_ion_advection: bool = False,
+                        use_improved_ion_advection: bool = False) -> None:
+        """``horizontal`` on host numpy arrays (FP64, C-contiguous), through
+        ``gitm_horiz_advance_host``: staged on the device and back, in place,
+        synchronous; raises VerticalStepError on a failed step."""
+        for k, a in grid.items():
+            if a.dtype != np.float64 or not a.flags.c_contiguous:
+                raise ValueError(f"{k}: need a C-contiguous float64 array")
+        c, f, p = _horiz_structs({k: a.ctypes.data for k, a in grid.items()},
+                                 grid["rho"].shape, dt, omega_body, beta_limiter,
+                                 use_coriolis, use_ion_advection, 0,
+                                 use_improved_ion_advection)
+        rc = lib().gitm_horiz_advance_host(self._h, ctypes.byref(p), ctypes.byref(c),
+                                           ctypes.byref(f))
         if rc:
             raise VerticalStepError(rc)
 

@@ -407,6 +407,14 @@ typedef struct gitm_horiz_fields {
 int gitm_horiz_advance(gitm_vert_handle *h, const gitm_horiz_params *p,
                        const gitm_vert_columns *c, const gitm_horiz_fields *f);
 
+/* The same step on host arrays (GITM's own module arrays): every field is
+ * staged on the device over the index ranges the step touches, halo cells
+ * and ghost levels included, and the written fields (the state, IDensityS,
+ * temp_source) are copied back.  Synchronous; returns the step's status as
+ * gitm_vert_get_error does. */
+int gitm_horiz_advance_host(gitm_vert_handle *h, const gitm_horiz_params *p,
+                            const gitm_vert_columns *c, const gitm_horiz_fields *f);
+
 /* ------------------------------------------------------------------------
  * The message exchange around the horizontal step
  * (exchange_messages_sphere.f90:4-95, the block-to-block transfer of

@@ -129,3 +129,21 @@ def test_unsupported_flags(solver):
     p.dt, p.use_topography = 1.0, 1
     c, f = _abi.Columns(), _abi.HorizFields()
     assert vertical.lib().gitm_horiz_advance(solver._h, p, c, f) == 5
+
+
+def test_host_entry_matches_device_entry(solver):
+    """gitm_horiz_advance_host (host arrays, staged over the ranges the step
+    touches) leaves the same arrays as the device entry, ions and the
+    improved ion advection included; the inputs are left alone."""
+    g = synth.block_grid(synth.BlockGrid(3, 2, 20), seed=31, ions=True)
+    dt = synth.horizontal_dt(g, cfl=0.4)
+    dev = run_gpu(solver, g, dt, True, True, improved=True)  # adds a zero temp_source
+    host = {k: np.array(v) for k, v in g.items()}
+    host["temp_source"] = np.zeros_like(g["rho"])
+    solver.horizontal_host(host, dt, use_coriolis=True, use_ion_advection=True,
+                           use_improved_ion_advection=True)
+    for k in ("rho", "temp", "vel", "vvel", "nds", "ions"):
+        assert np.array_equal(host[k], dev[k]), k
+    assert np.array_equal(host["temp_source"], dev["temp_source"])
+    for k in ("gamma", "cmax", "raddist", "ivel_par"):
+        assert np.array_equal(host[k], g[k]), k
