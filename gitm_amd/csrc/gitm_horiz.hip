@@ -633,28 +633,7 @@ extern "C" int gitm_horiz_advance(gitm_vert_handle *h, const gitm_horiz_params *
 // cells included; the written ones come back.
 // ---------------------------------------------------------------------------
 namespace {
-
-struct HSpan {
-  gitm_vert_field *f;  // the descriptor to stage (redirected to the device copy)
-  int lon_lo, lon_hi, lat_lo, lat_hi, lev_lo, lev_hi, ncomp;
-  bool written;
-};
-
-// Element offsets (from the descriptor's ptr) of the first and last element
-// the span covers; strides must be non-negative.
-bool span_offsets(const HSpan &s, int nblocks, long long &lo, long long &hi) {
-  const gitm_vert_field &f = *s.f;
-  if (f.s_lon < 0 || f.s_lat < 0 || f.s_lev < 0 || f.s_comp < 0 || f.s_blk < 0) return false;
-  lo = (long long)(s.lon_lo - 1) * f.s_lon + (long long)(s.lat_lo - 1) * f.s_lat +
-       (long long)(s.lev_lo - f.first_level) * f.s_lev;
-  hi = (long long)(s.lon_hi - 1) * f.s_lon + (long long)(s.lat_hi - 1) * f.s_lat +
-       (long long)(s.lev_hi - f.first_level) * f.s_lev + (long long)(s.ncomp - 1) * f.s_comp +
-       (long long)(nblocks - 1) * f.s_blk;
-  return hi >= lo;
-}
-
 constexpr size_t kHostStageBase = 64;  // staging slots after the vertical entry's
-
 }  // namespace
 
 extern "C" int gitm_horiz_advance_host(gitm_vert_handle *h, const gitm_horiz_params *hp,
@@ -667,7 +646,7 @@ extern "C" int gitm_horiz_advance_host(gitm_vert_handle *h, const gitm_horiz_par
   if (n < 1 || m < 1 || nA < 1 || nb < 1) return GITM_VERT_EINVAL;
   const bool ions = hp->use_ion_advection != 0;
   const bool improved = ions && hp->use_improved_ion_advection != 0;
-  std::vector<HSpan> sp;
+  std::vector<HostSpan> sp;
   auto full = [&](gitm_vert_field *f, int nc, bool w) {
     sp.push_back({f, -1, n + 2, -1, m + 2, -1, nA + 2, nc, w});
   };
@@ -701,39 +680,12 @@ extern "C" int gitm_horiz_advance_host(gitm_vert_handle *h, const gitm_horiz_par
       sp.push_back({f, 1, 1, 1, m, f->first_level, f->first_level, 1, false});
   }
   GV_HIP(hipSetDevice(h->device));
-  if (h->stage.size() < kHostStageBase + sp.size()) {
-    h->stage.resize(kHostStageBase + sp.size(), nullptr);
-    h->stage_bytes.resize(kHostStageBase + sp.size(), 0);
-  }
-  std::vector<long long> los(sp.size()), counts(sp.size());
-  std::vector<double *> host(sp.size());
-  for (size_t i = 0; i < sp.size(); i++) {
-    gitm_vert_field &f = *sp[i].f;
-    host[i] = f.ptr;
-    if (!f.ptr) continue;  // gitm_horiz_advance reports what is missing
-    long long lo, hi;
-    if (!span_offsets(sp[i], nb, lo, hi)) return GITM_VERT_EINVAL;
-    const size_t bytes = (size_t)(hi - lo + 1) * sizeof(double);
-    const size_t k = kHostStageBase + i;
-    if (h->stage_bytes[k] < bytes) {
-      if (h->stage[k]) (void)hipFree(h->stage[k]);
-      h->stage[k] = nullptr;
-      h->stage_bytes[k] = 0;
-      if (hipMalloc(&h->stage[k], bytes) != hipSuccess) return GITM_VERT_ENOMEM;
-      h->stage_bytes[k] = bytes;
-    }
-    GV_HIP(hipMemcpyAsync(h->stage[k], f.ptr + lo, bytes, hipMemcpyHostToDevice, h->stream));
-    f.ptr = (double *)h->stage[k] - lo;
-    los[i] = lo;
-    counts[i] = hi - lo + 1;
-  }
-  int rc = gitm_horiz_advance(h, hp, &d, &g);
+  HostStaging st;
+  int rc = stage_in(h, sp, nb, kHostStageBase, st);
   if (rc) return rc;
-  for (size_t i = 0; i < sp.size(); i++) {
-    if (!host[i] || !sp[i].written) continue;
-    GV_HIP(hipMemcpyAsync(host[i] + los[i], h->stage[kHostStageBase + i],
-                          (size_t)counts[i] * sizeof(double), hipMemcpyDeviceToHost,
-                          h->stream));
-  }
+  rc = gitm_horiz_advance(h, hp, &d, &g);
+  if (rc) return rc;
+  rc = stage_out(h, sp, kHostStageBase, st);
+  if (rc) return rc;
   return gitm_vert_get_error(h, nullptr);
 }

@@ -332,3 +332,50 @@ extern "C" int gitm_vert_sources(gitm_vert_handle *h, const gitm_vert_sources_pa
   GV_HIP(hipGetLastError());
   return GITM_VERT_OK;
 }
+
+// Host arrays (GITM's own): every field staged on the device over the levels
+// the step touches (the altitude on -1..nAlts+2, the heating terms on
+// 1..nAlts, the rest on 0..nAlts+1), the written ones copied back.
+extern "C" int gitm_vert_sources_host(gitm_vert_handle *h, const gitm_vert_sources_params *sp,
+                                      const gitm_vert_columns *c,
+                                      const gitm_vert_sources_fields *sf) {
+  if (!h || !sp || !c || !sf) return GITM_VERT_EINVAL;
+  const int n = c->n_lons, m = c->n_lats, nA = c->n_alts, nb = c->n_blocks;
+  if (n < 1 || m < 1 || nA < 1 || nb < 1) return GITM_VERT_EINVAL;
+  gitm_vert_columns d = *c;
+  gitm_vert_sources_fields g = *sf;
+  const bool apply = sp->apply != 0;
+  std::vector<HostSpan> spans;
+  auto add = [&](gitm_vert_field *f, int nc, int lo, int hi, bool written) {
+    spans.push_back({f, 1, n, 1, m, lo, hi, nc, written});
+  };
+  add(&d.altitude, 1, -1, nA + 2, false);
+  add(&d.temperature, 1, 0, nA + 1, apply);
+  add(&d.velocity, 3, 0, nA + 1, apply);
+  add(&d.vertical_velocity, gv::NSP, 0, nA + 1, apply);
+  add(&d.temp_unit, 1, 0, nA + 1, false);
+  add(&d.rho, 1, 0, nA + 1, false);
+  add(&d.eddy, 1, 0, nA + 1, false);
+  add(&d.ndensity_s, gv::NSP, 0, nA + 1, false);
+  add(&g.kappa_temp, 1, 0, nA + 1, false);
+  add(&g.visc_coef, 1, 0, nA + 1, false);
+  add(&g.visc_coef_s, gv::NSP, 0, nA + 1, false);
+  add(&g.cp, 1, 0, nA + 1, false);
+  add(&g.radial_distance, 1, 0, nA + 1, false);
+  add(&g.heating, 1, 1, nA, false);
+  add(&g.chem_heating, 1, 1, nA, false);
+  add(&g.user_heating, 1, 1, nA, false);
+  add(&g.conduction, 1, 0, nA + 1, true);
+  add(&g.viscosity, 3, 0, nA + 1, true);
+  add(&g.vertical_viscosity, gv::NSP, 0, nA + 1, true);
+  GV_HIP(hipSetDevice(h->device));
+  HostStaging st;
+  int rc = stage_in(h, spans, nb, 96, st);  // slots after the other host entries'
+  if (rc) return rc;
+  rc = gitm_vert_sources(h, sp, &d, &g);
+  if (rc) return rc;
+  rc = stage_out(h, spans, 96, st);
+  if (rc) return rc;
+  GV_HIP(hipStreamSynchronize(h->stream));
+  return GITM_VERT_OK;
+}

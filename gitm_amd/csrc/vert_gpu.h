@@ -182,3 +182,75 @@ struct gitm_vert_handle {
     if (e_ != hipSuccess) return GITM_VERT_EHIP; \
   } while (0)
 
+
+// Host-array entry points: a field staged on the device over the index
+// ranges a step touches (iLon, iLat in the descriptor's 1-based numbering,
+// levels as first_level does), then redirected to the device copy.
+struct HostSpan {
+  gitm_vert_field *f;
+  int lon_lo, lon_hi, lat_lo, lat_hi, lev_lo, lev_hi, ncomp;
+  bool written;
+};
+
+struct HostStaging {
+  std::vector<double *> host;
+  std::vector<long long> lo, count;
+};
+
+// Copies every span in (into the handle's staging slots slot0..) and points
+// its descriptor at the device copy; NULL fields are skipped.
+static inline int stage_in(gitm_vert_handle *h, std::vector<HostSpan> &sp, int nblocks,
+                           size_t slot0, HostStaging &st) {
+  if (h->stage.size() < slot0 + sp.size()) {
+    h->stage.resize(slot0 + sp.size(), nullptr);
+    h->stage_bytes.resize(slot0 + sp.size(), 0);
+  }
+  st.host.assign(sp.size(), nullptr);
+  st.lo.assign(sp.size(), 0);
+  st.count.assign(sp.size(), 0);
+  for (size_t i = 0; i < sp.size(); i++) {
+    gitm_vert_field &f = *sp[i].f;
+    st.host[i] = f.ptr;
+    if (!f.ptr) continue;
+    if (f.s_lon < 0 || f.s_lat < 0 || f.s_lev < 0 || f.s_comp < 0 || f.s_blk < 0)
+      return GITM_VERT_EINVAL;
+    const long long lo = (long long)(sp[i].lon_lo - 1) * f.s_lon +
+                         (long long)(sp[i].lat_lo - 1) * f.s_lat +
+                         (long long)(sp[i].lev_lo - f.first_level) * f.s_lev;
+    const long long hi = (long long)(sp[i].lon_hi - 1) * f.s_lon +
+                         (long long)(sp[i].lat_hi - 1) * f.s_lat +
+                         (long long)(sp[i].lev_hi - f.first_level) * f.s_lev +
+                         (long long)(sp[i].ncomp - 1) * f.s_comp +
+                         (long long)(nblocks - 1) * f.s_blk;
+    if (hi < lo) return GITM_VERT_EINVAL;
+    const size_t bytes = (size_t)(hi - lo + 1) * sizeof(double), k = slot0 + i;
+    if (h->stage_bytes[k] < bytes) {
+      if (h->stage[k]) (void)hipFree(h->stage[k]);
+      h->stage[k] = nullptr;
+      h->stage_bytes[k] = 0;
+      if (hipMalloc(&h->stage[k], bytes) != hipSuccess) return GITM_VERT_ENOMEM;
+      h->stage_bytes[k] = bytes;
+    }
+    if (hipMemcpyAsync(h->stage[k], f.ptr + lo, bytes, hipMemcpyHostToDevice, h->stream) !=
+        hipSuccess)
+      return GITM_VERT_EHIP;
+    f.ptr = (double *)h->stage[k] - lo;
+    st.lo[i] = lo;
+    st.count[i] = hi - lo + 1;
+  }
+  return GITM_VERT_OK;
+}
+
+// Copies the written spans back to the host arrays.
+static inline int stage_out(gitm_vert_handle *h, const std::vector<HostSpan> &sp, size_t slot0,
+                            const HostStaging &st) {
+  for (size_t i = 0; i < sp.size(); i++) {
+    if (!st.host[i] || !sp[i].written) continue;
+    if (hipMemcpyAsync(st.host[i] + st.lo[i], h->stage[slot0 + i],
+                       (size_t)st.count[i] * sizeof(double), hipMemcpyDeviceToHost,
+                       h->stream) != hipSuccess)
+      return GITM_VERT_EHIP;
+  }
+  return GITM_VERT_OK;
+}
+

@@ -86,6 +86,10 @@ def lib() -> ctypes.CDLL:
                      "gitm_vert_batch_set_tile", "gitm_vert_batch_advance",
                      "gitm_vert_batch_prepare", "gitm_vert_batch_locate"):
             getattr(_lib, name).restype = ctypes.c_int
+        _lib.gitm_vert_sources_host.restype = ctypes.c_int
+        _lib.gitm_vert_sources_host.argtypes = [vp, ctypes.POINTER(_abi.SourcesParams),
+                                                ctypes.POINTER(_abi.Columns),
+                                                ctypes.POINTER(_abi.SourcesFields)]
         _lib.gitm_vert_sources.restype = ctypes.c_int
         _lib.gitm_vert_sources.argtypes = [vp, ctypes.POINTER(_abi.SourcesParams),
                                            ctypes.POINTER(_abi.Columns),
@@ -176,6 +180,27 @@ def _device_columns(cols: dict, extra: tuple = ()):
     nc, L = cols["temp"].shape
     more = {k: ptrs.pop(k) for k in extra if k in ptrs}
     return _abi.columns_struct(ptrs, nc, L - 4), more
+
+
+def _sources_structs(ptr: dict, L: int, dt, apply, use_turbulent_cond, mass):
+    """SourcesFields and SourcesParams for column-contiguous arrays at the
+    addresses ``ptr`` (keys as VerticalSolver.sources takes them)."""
+    from .synth import species_masses
+
+    names = {"raddist": "radial_distance", "vert_viscosity": "vertical_viscosity"}
+    f = _abi.SourcesFields()
+    for key, addr in ptr.items():
+        name = names.get(key, key)
+        ncomp = dict(_abi.SOURCES_FIELDS).get(name)
+        if ncomp is not None:
+            setattr(f, name, _abi.column_field(addr, ncomp, L))
+    p = _abi.SourcesParams()
+    p.dt = dt
+    for s, m in enumerate(species_masses() if mass is None else mass):
+        p.mass[s] = m
+    p.use_turbulent_cond = int(use_turbulent_cond)
+    p.apply = int(apply)
+    return f, p
 
 
 def _horiz_structs(ptr: dict, shape, dt, omega_body, beta_limiter, use_coriolis,
@@ -322,29 +347,36 @@ class VerticalSolver:
         src["vert_viscosity"] when those tensors are present.  Queued on the
         current stream."""
         import torch
-        from .synth import species_masses
 
         c, _ = _device_columns(cols)
-        L = int(cols["temp"].shape[1])
-        names = {"raddist": "radial_distance", "vert_viscosity": "vertical_viscosity"}
-        f = _abi.SourcesFields()
         for key, t in src.items():
-            name = names.get(key, key)
-            ncomp = dict(_abi.SOURCES_FIELDS).get(name)
-            if ncomp is None:
-                continue
             if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
                 raise ValueError(f"{key}: need a contiguous float64 device tensor")
-            setattr(f, name, _abi.column_field(t.data_ptr(), ncomp, L))
-        p = _abi.SourcesParams()
-        p.dt = dt
-        for s, m in enumerate(species_masses() if mass is None else mass):
-            p.mass[s] = m
-        p.use_turbulent_cond = int(use_turbulent_cond)
-        p.apply = int(apply)
+        f, p = _sources_structs({k: t.data_ptr() for k, t in src.items()},
+                                int(cols["temp"].shape[1]), dt, apply, use_turbulent_cond,
+                                mass)
         self.set_stream(torch.cuda.current_stream().cuda_stream)
         rc = lib().gitm_vert_sources(self._h, ctypes.byref(p), ctypes.byref(c),
                                      ctypes.byref(f))
+        if rc:
+            raise VerticalStepError(rc)
+
+    def sources_host(self, cols: dict, src: dict, dt: float, apply: bool = True,
+                     use_turbulent_cond: bool = False, mass=None) -> None:
+        """``sources`` on host numpy arrays (FP64, C-contiguous) through
+        ``gitm_vert_sources_host``: staged on the device and back, in place,
+        synchronous."""
+        for k, a in list(cols.items()) + list(src.items()):
+            if a.dtype != np.float64 or not a.flags.c_contiguous:
+                raise ValueError(f"{k}: need a C-contiguous float64 array")
+        ptrs = {k: a.ctypes.data for k, a in cols.items()
+                if k in _abi.CANON or k in _abi.PER_COLUMN}
+        nc, L = cols["temp"].shape
+        c = _abi.columns_struct(ptrs, nc, L - 4)
+        f, p = _sources_structs({k: a.ctypes.data for k, a in src.items()}, L, dt, apply,
+                                use_turbulent_cond, mass)
+        rc = lib().gitm_vert_sources_host(self._h, ctypes.byref(p), ctypes.byref(c),
+                                          ctypes.byref(f))
         if rc:
             raise VerticalStepError(rc)
 

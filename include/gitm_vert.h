@@ -340,6 +340,11 @@ typedef struct gitm_vert_sources_fields {
 int gitm_vert_sources(gitm_vert_handle *h, const gitm_vert_sources_params *p,
                       const gitm_vert_columns *c, const gitm_vert_sources_fields *f);
 
+/* The same on host arrays (GITM's own): staged on the device over the
+ * levels the step touches and copied back.  Synchronous. */
+int gitm_vert_sources_host(gitm_vert_handle *h, const gitm_vert_sources_params *p,
+                           const gitm_vert_columns *c, const gitm_vert_sources_fields *f);
+
 /* ------------------------------------------------------------------------
  * The horizontal step: advance_horizontal(iBlock) (advance_horizontal.f90:
  * 58-924) for every block, each altitude 1..nAlts: the RK4 (:154-345) of

@@ -69,7 +69,8 @@ def test_adjoint_matches_oracle_differences(case, field):
     g = gin[field]
     sens = np.abs(g * cols[field])
     cand = np.argwhere(sens > 1e-6 * sens.max())
-    rng = np.random.default_rng(hash(field) % 1000)
+    # a fixed seed per field (str hashes change from process to process)
+    rng = np.random.default_rng(sum(map(ord, field)))
     picks = cand[rng.choice(len(cand), size=min(4, len(cand)), replace=False)]
     for idx in map(tuple, picks):
         f1 = central(cols, p, w, field, idx, 1e-6)

@@ -56,3 +56,23 @@ def test_sources_match_oracle(solver, config, apply, turbulent):
     for k in INC:  # levels 0..nAlts+1
         assert np.array_equal(out[k][..., 1:L - 1], ref[k][..., 1:L - 1]), k
     assert np.max(np.abs(out["conduction"])) > 0 and np.max(np.abs(out["vert_viscosity"])) > 0
+
+
+def test_host_entry_matches_device_entry(solver):
+    """gitm_vert_sources_host (host arrays, staged over the levels the step
+    touches) leaves the same state and increments as the device entry."""
+    cols = synth.make_columns(synth.CONFIGS["cpu_324x50"], seed=52)
+    src = synth.source_fields(cols, seed=52)
+    dev = gpu_sources(solver, cols, src, 2.0, True, True)
+    host = {k: np.array(cols[k]) for k in ("temp", "tempunit", "rho", "eddy", "nds", "vel",
+                                           "vvel", "alt")}
+    hs = {k: np.array(v) for k, v in src.items()}
+    hs["conduction"] = np.zeros_like(host["temp"])
+    hs["viscosity"] = np.zeros_like(host["vel"])
+    hs["vert_viscosity"] = np.zeros_like(host["vvel"])
+    solver.sources_host(host, hs, 2.0, apply=True, use_turbulent_cond=True)
+    for k in STATE:
+        assert np.array_equal(host[k], dev[k]), k
+    L = host["temp"].shape[1]
+    for k in INC:
+        assert np.array_equal(hs[k][..., 1:L - 1], dev[k][..., 1:L - 1]), k
