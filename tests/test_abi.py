@@ -20,14 +20,16 @@ LIB = os.path.join(ROOT, "gitm_amd", "libgitm_vert.so")
 
 def declared_functions():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(gitm_vert_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(gitm_(?:vert|horiz)_[a-z_0-9]+)\s*\(", text)))
 
 
 def test_header_declares_the_entry_points():
     names = declared_functions()
     for n in ("gitm_vert_create", "gitm_vert_destroy", "gitm_vert_advance",
               "gitm_vert_advance_host", "gitm_vert_get_error", "gitm_vert_set_stream",
-              "gitm_vert_set_tile", "gitm_vert_default_params", "gitm_vert_strerror"):
+              "gitm_vert_set_tile", "gitm_vert_default_params", "gitm_vert_strerror",
+              "gitm_vert_sources", "gitm_vert_sources_host", "gitm_horiz_advance",
+              "gitm_horiz_advance_host", "gitm_horiz_exchange"):
         assert n in names
 
 
@@ -35,7 +37,7 @@ def test_header_declares_the_entry_points():
 def test_library_exports_every_declared_symbol():
     out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True,
                          text=True, check=True).stdout
-    exported = set(re.findall(r" T (gitm_vert_\w+)", out))
+    exported = set(re.findall(r" T (gitm_(?:vert|horiz)_\w+)", out))
     missing = [n for n in declared_functions() if n not in exported]
     assert not missing, missing
 
