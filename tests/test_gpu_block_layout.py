@@ -115,3 +115,42 @@ def test_block_layout_matches_column_layout(solver, which):
     for key in ("nds", "vvel", "vel", "temp", "rho", "logns", "nden"):
         assert np.all(np.isnan(out[key][..., hm])), key
     assert np.all(np.isnan(out["nds"][:, 6:]))
+
+
+def test_host_entry_on_block_layout(solver):
+    """gitm_vert_advance_host on the same host arrays (what the Fortran shim
+    calls): the bits of the device entry on the column layout."""
+    from gitm_amd import vertical
+
+    nb, nalts = 4, 30
+    L = nalts + 4
+    cols = synth.make_columns(synth.GridSpec(2, 2, nalts), seed=14)
+    p = synth.StepParams()
+    ref = vertical.to_device({k: np.array(v) for k, v in cols.items()}, "cuda")
+    solver.advance(ref, p, "ausm", 1)
+    ref = vertical.to_host(ref)
+    blk, idx = to_blocks(cols, nb, L)
+    c = _abi.Columns()
+    c.n_lons, c.n_lats, c.n_blocks, c.n_alts = NLON, NLAT, nb, nalts
+
+    def f(key, nc, **kw):
+        return _abi.gitm_block_field(blk[key].ctypes.data, NLON, NLAT, nalts, nc, **kw)
+
+    c.ndensity_s = f("nds", NSP_TOTAL)
+    c.vertical_velocity, c.velocity, c.log_ns = f("vvel", 6), f("vel", 3), f("logns", 6)
+    for key, name in (("temp", "temperature"), ("rho", "rho"), ("nden", "ndensity"),
+                      ("tempunit", "temp_unit"), ("mmm", "mean_major_mass"),
+                      ("gamma", "gamma"), ("alt", "altitude"), ("grav", "gravity"),
+                      ("invr", "inv_radial_distance"), ("dalt", "dalt")):
+        setattr(c, name, f(key, 1))
+    c.cmax = f("cmax", 3, comp0=2, lon_lo=0, lat_lo=0, alt_lo=0, alt_hi=nalts + 1)
+    c.eddy = f("eddy", 1, lon_lo=1, lat_lo=1)
+    c.cos_latitude = _abi.latitude_field(blk["coslat"].ctypes.data, NLAT)
+    params = vertical.make_params(p)
+    rc = vertical.lib().gitm_vert_advance_host(solver._h, ctypes.byref(params),
+                                               ctypes.byref(c), 1, 1)
+    assert rc == 0
+    for key, nc in (("nds", 6), ("vvel", 6), ("vel", 3), ("temp", 1), ("rho", 1),
+                    ("logns", 6), ("nden", 1)):
+        want = ref[key] if ref[key].ndim == 3 else ref[key][:, None, :]
+        assert np.array_equal(from_blocks(blk[key], idx, nc), want), key
