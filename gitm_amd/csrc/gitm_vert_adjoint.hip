@@ -56,9 +56,10 @@ struct GpuAdCtx {
         const long long id = (long long)n * t.stride + g;
         const double a = adjoint((int)id);
         if (a == 0.0) continue;
-        const int p1 = t.p1[id], p2 = t.p2[id];
-        if (p1 >= 0) add(p1, a * t.d1[id]);
-        if (p2 >= 0) add(p2, a * t.d2[id]);
+        const gv::AdPar pp = t.p[id];
+        const gv::AdDer dd = t.d[id];
+        if (pp.a >= 0) add(pp.a, a * dd.a);
+        if (pp.b >= 0) add(pp.b, a * dd.b);
       }
       __threadfence();
       __syncthreads();
@@ -116,10 +117,12 @@ __global__ void __launch_bounds__(NT)
         d1[j] = d2[j] = 0.0;
         if (j < cnt) {
           const long long id = (long long)(top - j) * t.stride + g;
-          p1[j] = t.p1[id];
-          p2[j] = t.p2[id];
-          d1[j] = t.d1[id];
-          d2[j] = t.d2[id];
+          const gv::AdPar pp = t.p[id];
+          const gv::AdDer dd = t.d[id];
+          p1[j] = pp.a;
+          p2[j] = pp.b;
+          d1[j] = dd.a;
+          d2[j] = dd.b;
           a[j] = __hip_atomic_load(adj + id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
       }
@@ -178,7 +181,7 @@ size_t default_adjoint_budget(int device) {
 // serial loops of the boundary lanes, which grow with the column.
 int adjoint_nmax(int n_alts) { return 12288 + 8 * n_alts; }
 constexpr int kAdjMaxPhases = 128;
-constexpr size_t kNodeBytes = 2 * sizeof(double) + sizeof(double) + 2 * sizeof(int);
+constexpr size_t kNodeBytes = sizeof(gv::AdDer) + sizeof(double) + sizeof(gv::AdPar);
 
 template <int NT>
 void launch_adjoint(int ncols, int col0, hipStream_t stream, const gv::KParams &kp,
@@ -245,12 +248,10 @@ int gitm_vert_adjoint(gitm_vert_handle *h, const gitm_vert_params *p,
   char *base = static_cast<char *>(h->tape);
   const size_t nn = (size_t)nmax * stride;
   gv::AdTape tape;
-  tape.d1 = reinterpret_cast<double *>(base);
-  tape.d2 = tape.d1 + nn;
-  tape.adj = tape.d2 + nn;
-  tape.p1 = reinterpret_cast<int *>(tape.adj + nn);
-  tape.p2 = tape.p1 + nn;
-  tape.phase = tape.p2 + nn;
+  tape.d = reinterpret_cast<gv::AdDer *>(base);
+  tape.adj = reinterpret_cast<double *>(tape.d + nn);
+  tape.p = reinterpret_cast<gv::AdPar *>(tape.adj + nn);
+  tape.phase = reinterpret_cast<int *>(tape.p + nn);
   tape.nphase = tape.phase + (size_t)kAdjMaxPhases * stride;
   tape.stride = (int)stride;
   tape.nmax = nmax;

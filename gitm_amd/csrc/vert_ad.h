@@ -39,9 +39,18 @@ namespace gv {
 
 // One launch's tape: node arrays indexed by node number, plus the phase
 // table (node count of every lane at every barrier).
+// One node's parents (-1: none) and the partial derivatives with respect to
+// them, each pair stored and loaded as one access.
+struct alignas(8) AdPar {
+  int a, b;
+};
+struct alignas(16) AdDer {
+  double a, b;
+};
+
 struct AdTape {
-  double *d1, *d2;  // partial derivatives w.r.t. the parents
-  int *p1, *p2;     // parent nodes (-1: none)
+  AdDer *d;         // partial derivatives w.r.t. the parents
+  AdPar *p;         // parent nodes
   double *adj;      // adjoints, zeroed when the node is made
   int *phase;       // [phase * stride + lane] node count at the barrier
   int *nphase;      // [lane] barriers of the lane's forward pass
@@ -83,10 +92,8 @@ GV_AD int ad_record(int a, double da, int b, double db, bool leaf = false) {
   }
   const long long id = (long long)n * t.stride + ad_gid();
   n = n + 1;
-  t.p1[id] = a;
-  t.d1[id] = da;
-  t.p2[id] = b;
-  t.d2[id] = db;
+  t.p[id] = AdPar{a, b};
+  t.d[id] = AdDer{da, db};
   t.adj[id] = 0.0;
   return (int)id;
 }

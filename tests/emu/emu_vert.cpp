@@ -152,8 +152,10 @@ struct EmuAdCtx {
         const long long id = (long long)n * t.stride + lane_;
         const double a = std::atomic_ref<double>(t.adj[id]).load();
         if (a == 0.0) continue;
-        if (t.p1[id] >= 0) add(t.p1[id], a * t.d1[id]);
-        if (t.p2[id] >= 0) add(t.p2[id], a * t.d2[id]);
+        const gv::AdPar pp = t.p[id];
+        const gv::AdDer dd = t.d[id];
+        if (pp.a >= 0) add(pp.a, a * dd.a);
+        if (pp.b >= 0) add(pp.b, a * dd.b);
       }
       sh->bar.arrive_and_wait();
     }
@@ -179,11 +181,12 @@ extern "C" int gv_emu_adjoint(const gitm_vert_params *p, const gitm_vert_columns
   const int L = c->n_alts + 4;
   std::vector<gv::adm::ad> scratch((size_t)(c1 > 0 ? c1 : 1) * kp.scr_fields * L + 1);
   AdShared sh(L);
-  std::vector<double> d1((size_t)nmax * L), d2((size_t)nmax * L), adj((size_t)nmax * L);
-  std::vector<int> p1((size_t)nmax * L), p2((size_t)nmax * L), phase((size_t)maxph * L),
-      nphase(L);
+  std::vector<gv::AdDer> der((size_t)nmax * L);
+  std::vector<gv::AdPar> par((size_t)nmax * L);
+  std::vector<double> adj((size_t)nmax * L);
+  std::vector<int> phase((size_t)maxph * L), nphase(L);
   int overflow = 0;
-  sh.tape = gv::AdTape{d1.data(), d2.data(), p1.data(), p2.data(), adj.data(), phase.data(),
+  sh.tape = gv::AdTape{der.data(), par.data(), adj.data(), phase.data(),
                        nphase.data(), L, nmax, maxph, &overflow};
   std::vector<gv::adm::ad> lds(gv::lds_doubles(c->n_alts, false, gv::carry_in_lds(waves)));
   std::atomic<int> max_nodes{0}, max_phases{0};
