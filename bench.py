@@ -272,7 +272,7 @@ def bench_horizontal(torch, solver, reps: int = 10) -> dict:
     from gitm_amd import synth
 
     spec = synth.BLOCK_CONFIGS["1deg_blocks_800x100"]
-    host = synth.block_grid(spec, seed=1)
+    host = synth.block_grid(spec, seed=1, topography=True)
     dt = synth.horizontal_dt(host, cfl=0.4)
     g = {k: torch.from_numpy(v).cuda() for k, v in host.items()}
     del host
@@ -289,6 +289,16 @@ def bench_horizontal(torch, solver, reps: int = 10) -> dict:
     torch.cuda.synchronize()
     solver.check()
     ms = e0.elapsed_time(e1) / reps
+    # UseTopography: the same step with the levels advanced in order, one
+    # launch of 800 workgroups per level
+    solver.horizontal(g, dt, use_topography=True)
+    e0.record(stream)
+    for _ in range(reps):
+        solver.horizontal(g, dt, use_topography=True)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    solver.check()
+    tms = e0.elapsed_time(e1) / reps
     # the message exchange on the device (gitm_horiz_exchange), same grid
     g["itemp"], g["etemp"] = g["temp"].clone(), g["temp"].clone()
     solver.exchange(g, spec.nblocks_lon)
@@ -306,7 +316,7 @@ def bench_horizontal(torch, solver, reps: int = 10) -> dict:
     xbytes = 2 * 8 * 19 * halo * spec.nalts * spec.nblocks
     nbytes = horizontal_bytes_per_slab(spec.nlons, spec.nlats) * slabs
     del g
-    return {"kernel": "horiz_kernel<17> (gitm_horiz_advance: advance_horizontal, RK4 of "
+    return {"kernel": "horiz_kernel<17, false> (gitm_horiz_advance: advance_horizontal, RK4 of "
                       "the Rusanov/MC lat-lon solver)",
             "workload": "1deg_blocks_800x100: 360x180 cells in 800 blocks of 9x9, "
                         "100 levels, Coriolis on, ions off",
@@ -314,6 +324,9 @@ def bench_horizontal(torch, solver, reps: int = 10) -> dict:
             "achieved": nbytes / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "cell_updates_per_s": spec.ncells * spec.nalts / (ms * 1e-3),
+            "topography": {"kernel": "horiz_kernel<17, true>, one launch per level",
+                           "ms_per_step": tms,
+                           "cell_updates_per_s": spec.ncells * spec.nalts / (tms * 1e-3)},
             "exchange": {"kernel": "exchange_kernel + exchange_check_kernel "
                                    "(gitm_horiz_exchange, neutrals)",
                          "launch_ms": xms, "halo_bytes": xbytes,

@@ -179,7 +179,7 @@ class HorizParams(ctypes.Structure):
 HORIZ_FIELDS = ("inv_dlon_gb", "inv_dlon_fb", "dlon_fb", "inv_dlat_gb", "inv_dlat_fb",
                 "dlat_fb", "radial_distance", "latitude", "tan_latitude", "cmax_horizontal",
                 "temp_source", "ivelocity_par", "grad_lon_m", "grad_lon_0", "grad_lon_p",
-                "grad_lat_m", "grad_lat_0", "grad_lat_p")
+                "grad_lat_m", "grad_lat_0", "grad_lat_p", "d_alt", "dalt_dlon", "dalt_dlat")
 
 
 class HorizFields(ctypes.Structure):

@@ -32,11 +32,13 @@ struct HzParams {
   int n_lons, n_lats, n_alts, n_blocks;
   int apw;         // altitudes per workgroup
   int alt_groups;  // workgroups per block
+  int k_first;     // the launch's first altitude
 };
 
 struct HzFields {
   gitm_vert_field inv_dlon_gb, inv_dlon_fb, dlon_fb, inv_dlat_gb, inv_dlat_fb, dlat_fb,
       raddist, latitude, tan_latitude, cmax, temp_source;
+  gitm_vert_field dalt, dalt_dlon, dalt_dlat;  // UseTopography
 };
 
 // Element (iLon, iLat, level, comp) of block b: the descriptor's ptr is at
@@ -102,8 +104,10 @@ __device__ __forceinline__ void rusanov_cell(int i, int n, VF v, const LineGeo &
 }
 
 // (256 lanes at most; two wavefronts per SIMD: the LDS of three workgroups
-// per CU holds about that many anyway)
-template <int NV>
+// per CU holds about that many anyway).  TOPO: UseTopography, launched for
+// one altitude at a time (the host walks k = 1..nAlts), so that the level
+// below already holds the step's result when dVarDAlt reads it.
+template <int NV, bool TOPO>
 __global__ void __launch_bounds__(256, 2)
     horiz_kernel(HzParams p, gitm_vert_columns io, HzFields f, unsigned long long *err,
                  double *accg) {
@@ -111,7 +115,7 @@ __global__ void __launch_bounds__(256, 2)
   const int nlo = p.n_lons, nla = p.n_lats, NLo = nlo + 4, NLg = nla + 4;
   const int cells = nlo * nla, slab = NV * NLg * NLo;
   const int b = (int)blockIdx.x / p.alt_groups;
-  const int k0 = 1 + ((int)blockIdx.x % p.alt_groups) * p.apw;
+  const int k0 = p.k_first + ((int)blockIdx.x % p.alt_groups) * p.apw;
   const int napw = min(p.apw, p.n_alts - k0 + 1);
   const int tid = (int)threadIdx.x;
   const int a = tid / cells, cell = tid % cells;
@@ -183,6 +187,14 @@ __global__ void __launch_bounds__(256, 2)
       gla.cm[m + 1] = *hf(f.cmax, i, jl + m, k, 1, b);
     }
   }
+  // UseTopography: 0.5 / dAlt and the slopes of the level surface
+  // (advance_horizontal.f90:581-582, :870-871, :919-920)
+  double hz = 0, slon = 0, slat = 0;
+  if (TOPO && active) {
+    hz = 0.5 / *hf(f.dalt, i, jl, k, 0, b);
+    slon = *hf(f.dalt_dlon, i, jl, k, 0, b);
+    slat = *hf(f.dalt_dlat, i, jl, k, 0, b);
+  }
   const double dt = p.dt, om = p.omega;
   const double csin = sinl * 2 * om, ccos = cosl * 2 * om;
   const double cent = om * om * cosl * sinl;
@@ -194,6 +206,13 @@ __global__ void __launch_bounds__(256, 2)
       auto gd = [&](int v, double &glon, double &dlon, double &glat, double &dlat) {
         rusanov_cell(i, nlo, [&](int m) { return X(v, jl, i + m); }, glo, p.beta, glon, dlon);
         rusanov_cell(jl, nla, [&](int m) { return X(v, jl + m, i); }, gla, p.beta, glat, dlat);
+        if (TOPO) {  // dVarDAlt_C from the state arrays (:583-586, :591-662)
+          int comp;
+          const gitm_vert_field &g = field(v, comp);
+          const double dz = hz * (*hf(g, i, jl, k + 1, comp, b) - *hf(g, i, jl, k - 1, comp, b));
+          glon = glon - dz * slon;
+          glat = glat - dz * slat;
+        }
       };
       const double rho = X(V_RHO, jl, i), tmp = X(V_TEMP, jl, i);
       const double ve = X(V_VEL + 0, jl, i), vn = X(V_VEL + 1, jl, i), vu = X(V_VEL + 2, jl, i);
@@ -523,7 +542,6 @@ extern "C" int gitm_horiz_exchange(gitm_vert_handle *h, const gitm_horiz_exchang
 extern "C" int gitm_horiz_advance(gitm_vert_handle *h, const gitm_horiz_params *hp,
                                   const gitm_vert_columns *c, const gitm_horiz_fields *hf_) {
   if (!h || !hp || !c || !hf_) return GITM_VERT_EINVAL;
-  if (hp->use_topography) return GITM_VERT_EUNSUPPORTED;
   const int nlo = c->n_lons, nla = c->n_lats;
   if (nlo < 1 || nla < 1 || c->n_blocks < 1 || c->n_alts < 1) return GITM_VERT_EINVAL;
   const int cells = nlo * nla;
@@ -541,6 +559,9 @@ extern "C" int gitm_horiz_advance(gitm_vert_handle *h, const gitm_horiz_params *
   const bool ions = hp->use_ion_advection != 0;
   if (ions && (!c->idensity_s.ptr || !c->ivelocity.ptr)) return GITM_VERT_EINVAL;
   const bool improved = ions && hp->use_improved_ion_advection != 0;
+  const bool topo = hp->use_topography != 0;
+  if (topo && (!hf_->d_alt.ptr || !hf_->dalt_dlon.ptr || !hf_->dalt_dlat.ptr))
+    return GITM_VERT_EINVAL;
   DivFields dv;
   dv.ivel_par = hf_->ivelocity_par;
   dv.glm = hf_->grad_lon_m;
@@ -566,7 +587,7 @@ extern "C" int gitm_horiz_advance(gitm_vert_handle *h, const gitm_horiz_params *
     while ((apw + 1) * cells <= 256 && (apw + 1) * slab <= 40 * 1024 && apw < 4) apw++;
   }
   if (apw * cells > 256 || apw * slab > 160 * 1024) return GITM_VERT_EINVAL;
-  apw = std::min(apw, c->n_alts);
+  apw = topo ? 1 : std::min(apw, c->n_alts);
   HzParams p;
   p.dt = hp->dt;
   p.omega = hp->omega_body;
@@ -577,7 +598,8 @@ extern "C" int gitm_horiz_advance(gitm_vert_handle *h, const gitm_horiz_params *
   p.n_alts = c->n_alts;
   p.n_blocks = c->n_blocks;
   p.apw = apw;
-  p.alt_groups = (c->n_alts + apw - 1) / apw;
+  p.alt_groups = topo ? 1 : (c->n_alts + apw - 1) / apw;
+  p.k_first = 1;
   const long long nwg = (long long)p.alt_groups * c->n_blocks;
   if (nwg > 0x7fffffffLL) return GITM_VERT_EINVAL;
   HzFields f;
@@ -592,6 +614,9 @@ extern "C" int gitm_horiz_advance(gitm_vert_handle *h, const gitm_horiz_params *
   f.tan_latitude = hf_->tan_latitude;
   f.cmax = hf_->cmax_horizontal;
   f.temp_source = hf_->temp_source;
+  f.dalt = hf_->d_alt;
+  f.dalt_dlon = hf_->dalt_dlon;
+  f.dalt_dlat = hf_->dalt_dlat;
   GV_HIP(hipSetDevice(h->device));
   const int threads = ((apw * cells + 63) / 64) * 64;
   const size_t lds = (size_t)apw * slab;
@@ -608,13 +633,34 @@ extern "C" int gitm_horiz_advance(gitm_vert_handle *h, const gitm_horiz_params *
     if (hipMalloc(&h->scratch, need) != hipSuccess) return GITM_VERT_ENOMEM;
     h->scratch_bytes = need;
   }
-  if (ions)
-    hipLaunchKernelGGL(horiz_kernel<NV_IONS>, dim3((unsigned)nwg), dim3(threads), lds,
-                       h->stream, p, *c, f, h->d_err, h->scratch);
-  else
-    hipLaunchKernelGGL(horiz_kernel<NV_NEUTRAL>, dim3((unsigned)nwg), dim3(threads), lds,
-                       h->stream, p, *c, f, h->d_err, h->scratch);
-  GV_HIP(hipGetLastError());
+  auto launch = [&](const HzParams &q) -> int {
+    const dim3 grid((unsigned)nwg), block(threads);
+    if (ions && topo)
+      hipLaunchKernelGGL((horiz_kernel<NV_IONS, true>), grid, block, lds, h->stream, q, *c, f,
+                         h->d_err, h->scratch);
+    else if (ions)
+      hipLaunchKernelGGL((horiz_kernel<NV_IONS, false>), grid, block, lds, h->stream, q, *c,
+                         f, h->d_err, h->scratch);
+    else if (topo)
+      hipLaunchKernelGGL((horiz_kernel<NV_NEUTRAL, true>), grid, block, lds, h->stream, q, *c,
+                         f, h->d_err, h->scratch);
+    else
+      hipLaunchKernelGGL((horiz_kernel<NV_NEUTRAL, false>), grid, block, lds, h->stream, q,
+                         *c, f, h->d_err, h->scratch);
+    GV_HIP(hipGetLastError());
+    return GITM_VERT_OK;
+  };
+  if (topo) {
+    // one launch per altitude, in order on the stream (:150-430)
+    for (int k = 1; k <= c->n_alts; k++) {
+      p.k_first = k;
+      const int rc = launch(p);
+      if (rc) return rc;
+    }
+  } else {
+    const int rc = launch(p);
+    if (rc) return rc;
+  }
   if (improved) {
     const long long n =
         (long long)(nlo + 4) * (nla + 4) * (c->n_alts + 4) * (long long)c->n_blocks;
@@ -672,6 +718,9 @@ extern "C" int gitm_horiz_advance_host(gitm_vert_handle *h, const gitm_horiz_par
   lat1(&g.tan_latitude);
   sp.push_back({&g.cmax_horizontal, 0, n + 1, 0, m + 1, 0, nA + 1, 2, false});
   if (g.temp_source.ptr) sp.push_back({&g.temp_source, 1, n, 1, m, 1, nA, 1, true});
+  if (hp->use_topography)
+    for (gitm_vert_field *f : {&g.d_alt, &g.dalt_dlon, &g.dalt_dlat})
+      sp.push_back({f, 1, n, 1, m, 1, nA, 1, false});
   if (improved) {
     full(&g.ivelocity_par, 3, false);
     for (gitm_vert_field *f : {&g.grad_lon_m, &g.grad_lon_0, &g.grad_lon_p})

@@ -420,7 +420,8 @@ BLOCK_CONFIGS = {
 }
 
 
-def block_grid(spec: BlockGrid, seed: int = 1, ions: bool = False, blocks=None) -> dict:
+def block_grid(spec: BlockGrid, seed: int = 1, ions: bool = False, blocks=None,
+               topography: bool = False) -> dict:
     """Block-layout state, geometry and cMax of a smooth synthetic
     atmosphere.  Every per-cell field is [block][component][level][lat][lon]
     with levels -1..nAlts+2 and lat/lon -1..n+2 (halo cells hold the
@@ -432,7 +433,12 @@ def block_grid(spec: BlockGrid, seed: int = 1, ions: bool = False, blocks=None) 
     as calc_timestep_horizontal (calc_timestep.f90:34-41), up as the
     vertical CFL speed.  ``blocks``: the block numbers to generate (all by
     default; block b is at longitude b % nblocks_lon, latitude
-    b // nblocks_lon)."""
+    b // nblocks_lon).  ``topography``: the levels follow a terrain of a few
+    km that fades with height, Altitude_GB = alt + h(lon, lat) exp(-(alt -
+    alt_min) / 200 km); adds "dalt" (dAlt_GB, half the distance between the
+    neighbouring levels) and "dalt_dlon", "dalt_dlat" (dAltDLon_CB,
+    dAltDLat_CB: the slopes of the level surfaces per metre east and north,
+    initialize.f90:294-300), all shaped as a state field."""
     ids = np.arange(spec.nblocks) if blocks is None else np.asarray(blocks, dtype=np.int64)
     nb, nlo, nla, nA = len(ids), spec.nlons, spec.nlats, spec.nalts
     L, NLo, NLg = nA + 4, nlo + 4, nla + 4
@@ -515,6 +521,18 @@ def block_grid(spec: BlockGrid, seed: int = 1, ions: bool = False, blocks=None) 
                 "latitude": np.ascontiguousarray(lat),
                 "tan_lat": np.minimum(np.abs(np.tan(lat)), 100.0) * np.sign(lat),
                 "cos_lat": np.ascontiguousarray(cosl)})
+    if topography:
+        fade = np.exp(-(Z - spec.alt_min) / 200.0e3)
+        h = 3000.0 * (1.0 + np.sin(2 * LON + ph) * np.cos(LAT)) + 800.0 * np.cos(3 * LAT)
+        dh_dlon = 6000.0 * np.cos(2 * LON + ph) * np.cos(LAT)
+        dh_dlat = -3000.0 * np.sin(2 * LON + ph) * np.sin(LAT) - 2400.0 * np.sin(3 * LAT)
+        A = Z + h * fade
+        dalt = np.empty(shp)
+        dalt[:, 1:-1] = 0.5 * (A[:, 2:] - A[:, :-2])
+        dalt[:, 0], dalt[:, -1] = dalt[:, 1], dalt[:, -2]
+        out["dalt"] = dalt
+        out["dalt_dlon"] = dh_dlon * fade / (R * np.maximum(np.abs(np.cos(LAT)), 0.01))
+        out["dalt_dlat"] = dh_dlat * fade / R
     # cMax_GDB [nb][3][0..nAlts+1][0..nLats+1][0..nLons+1] (iEast_, iNorth_, iUp_)
     cs = np.sqrt(gamma * temp)[:, 1:nA + 3, 1:nla + 3, 1:nlo + 3]
     cmax = np.empty((nb, 3, nA + 2, nla + 2, nlo + 2))

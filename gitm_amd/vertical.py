@@ -204,7 +204,8 @@ def _sources_structs(ptr: dict, L: int, dt, apply, use_turbulent_cond, mass):
 
 
 def _horiz_structs(ptr: dict, shape, dt, omega_body, beta_limiter, use_coriolis,
-                   use_ion_advection, altitudes_per_workgroup, use_improved_ion_advection):
+                   use_ion_advection, altitudes_per_workgroup, use_improved_ion_advection,
+                   use_topography=False):
     """Columns, HorizFields and HorizParams for block-layout arrays at the
     addresses ``ptr`` (gitm_amd.synth.block_grid names)."""
     nb, L, NLg, NLo = (int(v) for v in shape)
@@ -242,8 +243,11 @@ def _horiz_structs(ptr: dict, shape, dt, omega_body, beta_limiter, use_coriolis,
             _abi.Field(lo + 8 * j * nlo, 1, 0, 3 * nlo, 0, 0, 0, 0) for j in range(3))
         f.grad_lat_m, f.grad_lat_0, f.grad_lat_p = (
             _abi.Field(la + 8 * j * nla, 0, 1, 3 * nla, 0, 0, 0, 0) for j in range(3))
+    if use_topography:
+        f.d_alt, f.dalt_dlon, f.dalt_dlat = blk("dalt"), blk("dalt_dlon"), blk("dalt_dlat")
     p = _abi.HorizParams()
     p.dt, p.omega_body, p.beta_limiter = dt, omega_body, beta_limiter
+    p.use_topography = int(use_topography)
     p.use_coriolis, p.use_ion_advection = int(use_coriolis), int(use_ion_advection)
     p.use_improved_ion_advection = int(use_improved_ion_advection)
     p.altitudes_per_workgroup = altitudes_per_workgroup
@@ -383,7 +387,8 @@ class VerticalSolver:
     def horizontal(self, grid: dict, dt: float, omega_body: float = 2.0 * math.pi / 86400.0,
                    beta_limiter: float = 2.0, use_coriolis: bool = True,
                    use_ion_advection: bool = False, altitudes_per_workgroup: int = 0,
-                   use_improved_ion_advection: bool = False) -> None:
+                   use_improved_ion_advection: bool = False,
+                   use_topography: bool = False) -> None:
         """The horizontal step (``gitm_horiz_advance``: advance_horizontal on
         every block) on device tensors in GITM's block layout
         ([block][component][level][lat][lon], halo cells included;
@@ -393,7 +398,9 @@ class VerticalSolver:
         on the current stream; errors surface in ``check()``.  With ions and
         ``use_improved_ion_advection`` the divergence of "ivel_par" follows,
         with UAM_Gradient's coefficients "grad_lon" and "grad_lat"
-        ([block][M, 0, P][cell])."""
+        ([block][M, 0, P][cell]).  ``use_topography`` reads "dalt",
+        "dalt_dlon" and "dalt_dlat" (state-shaped) and advances the levels
+        in order, one launch each."""
         import torch
 
         for k, t in grid.items():
@@ -402,7 +409,8 @@ class VerticalSolver:
         c, f, p = _horiz_structs({k: t.data_ptr() for k, t in grid.items()},
                                  tuple(int(v) for v in grid["rho"].shape), dt, omega_body,
                                  beta_limiter, use_coriolis, use_ion_advection,
-                                 altitudes_per_workgroup, use_improved_ion_advection)
+                                 altitudes_per_workgroup, use_improved_ion_advection,
+                                 use_topography)
         self.set_stream(torch.cuda.current_stream().cuda_stream)
         rc = lib().gitm_horiz_advance(self._h, ctypes.byref(p), ctypes.byref(c),
                                       ctypes.byref(f))
@@ -412,7 +420,8 @@ class VerticalSolver:
     def horizontal_host(self, grid: dict, dt: float, omega_body: float = 2.0 * math.pi / 86400.0,
                         beta_limiter: float = 2.0, use_coriolis: bool = True,
                         use_ion_advection: bool = False,
-                        use_improved_ion_advection: bool = False) -> None:
+                        use_improved_ion_advection: bool = False,
+                        use_topography: bool = False) -> None:
         """``horizontal`` on host numpy arrays (FP64, C-contiguous), through
         ``gitm_horiz_advance_host``: staged on the device and back, in place,
         synchronous; raises VerticalStepError on a failed step."""
@@ -422,7 +431,7 @@ class VerticalSolver:
         c, f, p = _horiz_structs({k: a.ctypes.data for k, a in grid.items()},
                                  grid["rho"].shape, dt, omega_body, beta_limiter,
                                  use_coriolis, use_ion_advection, 0,
-                                 use_improved_ion_advection)
+                                 use_improved_ion_advection, use_topography)
         rc = lib().gitm_horiz_advance_host(self._h, ctypes.byref(p), ctypes.byref(c),
                                            ctypes.byref(f))
         if rc:

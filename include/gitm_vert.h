@@ -371,8 +371,17 @@ typedef struct gitm_horiz_params {
                                          the divergence of IVelocityPar
                                          (:432-518, UAM_Gradient) after the
                                          step; needs the grad_* fields     */
-  int32_t use_topography;             /* must be 0 (GITM_VERT_EUNSUPPORTED) */
-  int32_t altitudes_per_workgroup;    /* 0: chosen by the library            */
+  int32_t use_topography;             /* UseTopography: every horizontal
+                                         gradient less dVarDAlt times the
+                                         level surface's slope (:581-662,
+                                         :870-871, :919-920); needs d_alt,
+                                         dalt_dlon, dalt_dlat.  dVarDAlt
+                                         reads level k-1 after the step has
+                                         written it (:349-429), so the
+                                         levels advance in order, one launch
+                                         each                              */
+  int32_t altitudes_per_workgroup;    /* 0: chosen by the library; 1 with
+                                         use_topography                    */
   int32_t reserved;
 } gitm_horiz_params;
 
@@ -403,6 +412,12 @@ typedef struct gitm_horiz_fields {
   gitm_vert_field ivelocity_par;
   gitm_vert_field grad_lon_m, grad_lon_0, grad_lon_p;
   gitm_vert_field grad_lat_m, grad_lat_0, grad_lat_p;
+  /* use_topography only (else ignored), interior cells of levels 1..nAlts
+   * read: dAlt_GB (described as the state), dAltDLon_CB and dAltDLat_CB
+   * ((nLons,nLats,nAlts,nBlocks): ptr at (1,1,1,1), first_level 1,
+   * initialize.f90:294-300) */
+  gitm_vert_field d_alt;
+  gitm_vert_field dalt_dlon, dalt_dlat;
 } gitm_horiz_fields;
 
 /* Rho, Temperature, Velocity, NDensityS, VerticalVelocity, Gamma,

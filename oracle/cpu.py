@@ -230,15 +230,19 @@ class _Horiz(ctypes.Structure):
         (n, _P) for n in ("rho", "temp", "vel", "vvel", "nds", "ions") + HORIZ_INPUTS
         + ("temp_source",)] + [
         ("use_improved_ion_advection", ctypes.c_int),
-        ("ivel_par", _P), ("grad_lon", _P), ("grad_lat", _P)]
+        ("ivel_par", _P), ("grad_lon", _P), ("grad_lat", _P),
+        ("use_topography", ctypes.c_int),
+        ("dalt", _P), ("dalt_dlon", _P), ("dalt_dlat", _P)]
 
 
 def horizontal(grid: dict, dt: float, omega_body: float = 2.0 * np.pi / 86400.0,
                beta_limiter: float = 2.0, use_coriolis: bool = True,
                use_ion_advection: bool = False,
-               use_improved_ion_advection: bool = False) -> tuple[dict, int, tuple]:
+               use_improved_ion_advection: bool = False,
+               use_topography: bool = False) -> tuple[dict, int, tuple]:
     """The oracle's horizontal step on a copy of the block-layout ``grid``
-    (gitm_amd.synth.block_grid); returns (updated arrays incl. temp_source,
+    (gitm_amd.synth.block_grid; with ``use_topography`` it holds "dalt",
+    "dalt_dlon" and "dalt_dlat"); returns (updated arrays incl. temp_source,
     status code, (column, level))."""
     nb, L, NLg, NLo = grid["rho"].shape
     out = {k: np.array(grid[k], dtype=np.float64, order="C")
@@ -251,11 +255,15 @@ def horizontal(grid: dict, dt: float, omega_body: float = 2.0 * np.pi / 86400.0,
     if use_ion_advection and use_improved_ion_advection:
         for k in ("ivel_par", "grad_lon", "grad_lat"):
             keep[k] = np.ascontiguousarray(grid[k], dtype=np.float64)
+    if use_topography:
+        for k in ("dalt", "dalt_dlon", "dalt_dlat"):
+            keep[k] = np.ascontiguousarray(grid[k], dtype=np.float64)
     h = _Horiz()
     h.n_lons, h.n_lats, h.n_alts, h.n_blocks = NLo - 4, NLg - 4, L - 4, nb
     h.dt, h.omega_body, h.beta_limiter = dt, omega_body, beta_limiter
     h.use_coriolis, h.use_ion_advection = int(use_coriolis), int(use_ion_advection)
     h.use_improved_ion_advection = int(use_improved_ion_advection)
+    h.use_topography = int(use_topography)
     for k, v in list(out.items()) + list(keep.items()):
         setattr(h, k, _ptr(v))
     st = _Status()

@@ -6,7 +6,8 @@
  * with calc_facevalues_lats/lons, :928-1034).  Nothing in the product library
  * links, loads or calls it.
  *
- * Each altitude 1..nAlts of each block is an independent 2-D problem: the
+ * Each altitude 1..nAlts of each block is a 2-D problem (independent of the
+ * others unless UseTopography, which reads the levels above and below): the
  * state of the slice (halo cells -1:0 and n+1:n+2 included) is advanced by
  * the reference's RK4 (:154-345) of horizontal_solver (:535-818), the halo
  * cells keeping their input values through the four stages.  The arithmetic
@@ -132,6 +133,21 @@ static int slice(const oh_grid *p, const geo *g, int b, int k, ov_status *st) {
   const double *cmn = p->cmax + ((long long)b * 3 + 1) * cmo; /* iNorth_ */
 #define CM(a, lo, la) a[((long long)k * (nla + 2) + (la)) * (nlo + 2) + (lo)]
 #define GF(f, lo, la) p->f[bo + at3(g, lo, la, k)]
+  /* UseTopography: each variable's altitude derivative dVarDAlt_C (:581-586
+   * and before each variable's gradients, :591-662), fixed through the RK4
+   * and taken from the state arrays, where level k+1 still holds its input
+   * and level k-1 already the step's result */
+  double *dz = NULL;
+  if (p->use_topography) {
+    dz = malloc(sizeof(double) * (size_t)V_MAX * nlo * nla);
+    for (int vv = 0; vv < nv; vv++)
+      for (int la = 1; la <= nla; la++)
+        for (int lo = 1; lo <= nlo; lo++) {
+          const double h = 0.5 / GF(dalt, lo, la);
+          dz[IN(vv, lo, la)] =
+              h * (src[vv][at3(g, lo, la, k + 1)] - src[vv][at3(g, lo, la, k - 1)]);
+        }
+  }
 
   for (int stage = 0; stage < 4; stage++) {
     /* horizontal_solver: Grad and Diff of every variable (:588-662) */
@@ -146,6 +162,7 @@ static int slice(const oh_grid *p, const geo *g, int b, int k, ov_status *st) {
         for (int lo = 0; lo <= nlo + 1; lo++) cm[lo] = CM(cme, lo, la);
         rusanov(nlo, line, ifb, dfb, igb, cm, p->beta_limiter, gr, df);
         for (int lo = 1; lo <= nlo; lo++) {
+          if (dz) gr[lo] = gr[lo] - dz[IN(vv, lo, la)] * GF(dalt_dlon, lo, la); /* :919-920 */
           d[IN(vv, lo, la)].glon = gr[lo];
           d[IN(vv, lo, la)].dlon = df[lo];
         }
@@ -158,6 +175,7 @@ static int slice(const oh_grid *p, const geo *g, int b, int k, ov_status *st) {
         for (int la = 0; la <= nla + 1; la++) cm[la] = CM(cmn, lo, la);
         rusanov(nla, line, ifb, dfb, igb, cm, p->beta_limiter, gr, df);
         for (int la = 1; la <= nla; la++) {
+          if (dz) gr[la] = gr[la] - dz[IN(vv, lo, la)] * GF(dalt_dlat, lo, la); /* :870-871 */
           d[IN(vv, lo, la)].glat = gr[la];
           d[IN(vv, lo, la)].dlat = df[la];
         }
@@ -291,6 +309,7 @@ static int slice(const oh_grid *p, const geo *g, int b, int k, ov_status *st) {
   free(acc);
   free(nw);
   free(d);
+  free(dz);
   return rc;
 }
 
@@ -371,6 +390,7 @@ int oh_advance_horizontal(const oh_grid *p, ov_status *st) {
   if (p->use_ion_advection && (!p->ions || !p->ivel)) return OV_ERR_BAD_ARGS;
   const int improved = p->use_ion_advection && p->use_improved_ion_advection;
   if (improved && (!p->ivel_par || !p->grad_lon || !p->grad_lat)) return OV_ERR_BAD_ARGS;
+  if (p->use_topography && (!p->dalt || !p->dalt_dlon || !p->dalt_dlat)) return OV_ERR_BAD_ARGS;
   geo g;
   g.nlo = p->n_lons;
   g.nla = p->n_lats;

@@ -199,6 +199,15 @@ typedef struct {
   const double *ivel_par;            /* IVelocityPar [3] (Earth: magnetic)  */
   const double *grad_lon;            /* GradLonM/0/P_CB [block][3][nLons]  */
   const double *grad_lat;            /* GradLatM/0/P_CB [block][3][nLats]  */
+  /* UseTopography: every horizontal gradient less the altitude derivative
+   * (0.5/dAlt)(Var(k+1) - Var(k-1)) times dAltDLon or dAltDLat
+   * (advance_horizontal.f90:581-662, 870-871, 919-920).  Var(k-1) is the
+   * value the step already gave level k-1 (the reference writes each level
+   * back before the next, :349-429), so the levels advance in order. */
+  int use_topography;
+  const double *dalt;      /* dAlt_GB, as a state field                     */
+  const double *dalt_dlon; /* dAltDLon_CB, as a state field (interior read) */
+  const double *dalt_dlat; /* dAltDLat_CB, likewise                         */
 } oh_grid;
 
 int oh_advance_horizontal(const oh_grid *p, ov_status *status);

@@ -83,7 +83,8 @@ def horizontal_available() -> bool:
 
 def horizontal(grid: dict, dt: float, omega_body: float, beta_limiter: float,
                use_coriolis: bool = True, use_ion_advection: bool = False,
-               use_improved_ion_advection: bool = False) -> dict:
+               use_improved_ion_advection: bool = False,
+               use_topography: bool = False) -> dict:
     """The reference's advance_horizontal on every block of the block-layout
     ``grid`` (gitm_amd.synth.block_grid; [block][comp][level][lat][lon] is
     the reference's own (lon, lat, level, comp, block) order)."""
@@ -93,8 +94,8 @@ def horizontal(grid: dict, dt: float, omega_body: float, beta_limiter: float,
         raise ValueError("reference build is compiled for 9x9 blocks, nAlts=50, <= 4 blocks")
     hdr = np.zeros(16, dtype=np.int32)
     improved = use_ion_advection and use_improved_ion_advection
-    hdr[:8] = [HMAGIC, nb, nlo, nla, na, int(use_coriolis), int(use_ion_advection),
-               int(improved)]
+    hdr[:9] = [HMAGIC, nb, nlo, nla, na, int(use_coriolis), int(use_ion_advection),
+               int(improved), int(use_topography)]
     par = np.zeros(8)
     par[:3] = [dt, omega_body, beta_limiter]
     order = ["rho", "temp", "vel", "vvel", "nds", "gamma", "invr", "raddist", "inv_dlon_gb",
@@ -115,6 +116,10 @@ def horizontal(grid: dict, dt: float, omega_body: float, beta_limiter: float,
                 for key in ("grad_lon", "grad_lat"):  # M, 0, P: (n, nBlocks) each
                     for j in range(3):
                         f.write(_fortran_order(grid[key][:, j, :]).tobytes())
+            if use_topography:  # dAlt_GB, then dAltDLon/DLat_CB on the interior
+                f.write(_fortran_order(grid["dalt"]).tobytes())
+                for key in ("dalt_dlon", "dalt_dlat"):
+                    f.write(_fortran_order(grid[key][:, 2:-2, 2:-2, 2:-2]).tobytes())
         subprocess.run([HEXE, fin, fout], check=True, capture_output=True)
         raw = np.fromfile(fout, dtype=np.uint8)
     status = raw[:16].view(np.int32)

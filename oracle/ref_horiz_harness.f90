@@ -12,7 +12,8 @@
 ! File formats (stream, native little-endian; every array in the reference's
 ! own declared shape and order, blocks 1..nB):
 !   input : int32 hdr(16) = magic, nB, nLons, nLats, nAlts, UseCoriolis,
-!           UseIonAdvection, UseImprovedIonAdvection, 0...; real64 par(8) =
+!           UseIonAdvection, UseImprovedIonAdvection, UseTopography, 0...;
+!           real64 par(8) =
 !           Dt, OmegaBodyInput, BetaLimiter, 0...; then Rho, Temperature,
 !           Velocity(3),
 !           VerticalVelocity(6), NDensityS(1:6), Gamma, InvRadialDistance_GB,
@@ -21,7 +22,8 @@
 !           TanLatitude, CosLatitude, cMax_GDB(3), and with ions
 !           IDensityS(1:nIons), IVelocity(3); with UseImprovedIonAdvection
 !           (hdr(8)) also IVelocityPar(3), GradLonM_CB, GradLon0_CB,
-!           GradLonP_CB, GradLatM_CB, GradLat0_CB, GradLatP_CB
+!           GradLonP_CB, GradLatM_CB, GradLat0_CB, GradLatP_CB; with
+!           UseTopography (hdr(9)) also dAlt_GB, dAltDLon_CB, dAltDLat_CB
 !   output: int32 status(4), then Rho, Temperature, Velocity,
 !           VerticalVelocity, NDensityS(1:6), HorizontalTempSource per block
 !           (nLons, nLats, nAlts), and with ions IDensityS(1:nIons)
@@ -64,7 +66,7 @@ program gitm_horiz_ref
   UseCoriolis             = (hdr(6) /= 0)
   UseIonAdvection         = doIons
   UseImprovedIonAdvection = doIons .and. hdr(8) /= 0
-  UseTopography           = .false.
+  UseTopography           = (hdr(9) /= 0)
   DoCheckForNans          = .false.
   Dt                      = par(1)
   OmegaBodyInput          = par(2)
@@ -103,6 +105,11 @@ program gitm_horiz_ref
   else
      IDensityS = 1.0e6
      IVelocity = 0.0
+  endif
+  if (UseTopography) then
+     read(10) dAlt_GB(:,:,:,1:nB)
+     read(10) dAltDLon_CB(:,:,:,1:nB)
+     read(10) dAltDLat_CB(:,:,:,1:nB)
   endif
   close(10)
 

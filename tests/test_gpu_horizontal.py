@@ -6,7 +6,9 @@ have passed bit for bit too, and are held to 1e-13 relative otherwise);
 halo cells and ghost levels are left alone; blocks of other shapes, several
 altitudes per workgroup, ion advection, the golden fixture, the
 negative-density report and the unsupported flags; with UseImprovedIonAdvection
-the ion-velocity divergence of the second kernel (GITM's default for ions)."""
+the ion-velocity divergence of the second kernel (GITM's default for ions);
+with UseTopography the altitude-derivative term, the levels advanced in
+order (one launch each), through the device and the host entries."""
 from __future__ import annotations
 
 import os
@@ -25,13 +27,14 @@ STATE = ("rho", "temp", "vel", "vvel", "nds")
 
 
 def run_gpu(solver, g, dt, coriolis=True, ions=False, apw=0, temp_source=True,
-            improved=False):
+            improved=False, topography=False):
     import torch
     d = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in g.items()}
     if temp_source:
         d["temp_source"] = torch.zeros_like(d["rho"])
     solver.horizontal(d, dt, OMEGA, 2.0, use_coriolis=coriolis, use_ion_advection=ions,
-                      altitudes_per_workgroup=apw, use_improved_ion_advection=improved)
+                      altitudes_per_workgroup=apw, use_improved_ion_advection=improved,
+                      use_topography=topography)
     torch.cuda.synchronize()
     solver.check()
     return {k: v.cpu().numpy() for k, v in d.items()}
@@ -67,6 +70,26 @@ def test_matches_oracle(solver, spec, coriolis, ions, apw, improved):
     keys = STATE + ("temp_source",) + (("ions",) if ions else ())
     compare(out, ref, keys, coriolis)
     for k in STATE:  # halo cells and ghost levels untouched
+        mask = np.ones(g[k].shape[-3:], bool)
+        mask[2:-2, 2:-2, 2:-2] = False
+        assert np.array_equal(out[k][..., mask], g[k][..., mask]), k
+
+
+@pytest.mark.parametrize("spec,ions,apw", [
+    (synth.BlockGrid(4, 2, 30), False, 0),
+    (synth.BlockGrid(4, 2, 30), True, 0),
+    (synth.BlockGrid(3, 2, 12, nlons=6, nlats=5), True, 3),  # apw is 1 with topography
+])
+def test_topography_matches_oracle(solver, spec, ions, apw):
+    g = synth.block_grid(spec, seed=23 + int(ions), ions=ions, topography=True)
+    dt = synth.horizontal_dt(g, cfl=0.4)
+    print(f"horizontal topography {spec} ions={ions} apw={apw}", flush=True)
+    out = run_gpu(solver, g, dt, True, ions, apw, improved=ions, topography=True)
+    ref, rc, _ = cpu.horizontal(g, dt, OMEGA, 2.0, True, ions, ions, use_topography=True)
+    assert rc == 0
+    keys = STATE + ("temp_source",) + (("ions",) if ions else ())
+    compare(out, ref, keys, True)
+    for k in STATE:
         mask = np.ones(g[k].shape[-3:], bool)
         mask[2:-2, 2:-2, 2:-2] = False
         assert np.array_equal(out[k][..., mask], g[k][..., mask]), k
@@ -116,7 +139,7 @@ def test_negative_density_is_reported(solver):
 
 def test_unsupported_flags(solver):
     import torch
-    from gitm_amd import _abi, vertical
+    from gitm_amd import vertical
     g = synth.block_grid(synth.BlockGrid(1, 1, 10), seed=2, ions=True)
     d = {k: torch.from_numpy(v).cuda() for k, v in g.items()}
     with pytest.raises(vertical.VerticalStepError) as e:
@@ -125,10 +148,11 @@ def test_unsupported_flags(solver):
     del d["grad_lat"]
     with pytest.raises(KeyError):  # the divergence needs UAM_Gradient's coefficients
         solver.horizontal(d, 1.0, use_ion_advection=True, use_improved_ion_advection=True)
-    p = _abi.HorizParams()
-    p.dt, p.use_topography = 1.0, 1
-    c, f = _abi.Columns(), _abi.HorizFields()
-    assert vertical.lib().gitm_horiz_advance(solver._h, p, c, f) == 5
+    c, f, p = vertical._horiz_structs({k: t.data_ptr() for k, t in d.items()},
+                                      tuple(d["rho"].shape), 1.0, OMEGA, 2.0, True, False, 0,
+                                      False)
+    p.use_topography = 1  # without dAlt_GB and the slopes
+    assert vertical.lib().gitm_horiz_advance(solver._h, p, c, f) == 4
 
 
 def test_host_entry_matches_device_entry(solver):
@@ -146,4 +170,20 @@ def test_host_entry_matches_device_entry(solver):
         assert np.array_equal(host[k], dev[k]), k
     assert np.array_equal(host["temp_source"], dev["temp_source"])
     for k in ("gamma", "cmax", "raddist", "ivel_par"):
+        assert np.array_equal(host[k], g[k]), k
+
+
+def test_host_entry_topography(solver):
+    """The host entry with UseTopography stages dAlt_GB and the slopes and
+    gives the device entry's arrays."""
+    g = synth.block_grid(synth.BlockGrid(2, 2, 16), seed=37, ions=True, topography=True)
+    dt = synth.horizontal_dt(g, cfl=0.4)
+    dev = run_gpu(solver, g, dt, True, True, improved=True, topography=True)
+    host = {k: np.array(v) for k, v in g.items()}
+    host["temp_source"] = np.zeros_like(g["rho"])
+    solver.horizontal_host(host, dt, use_coriolis=True, use_ion_advection=True,
+                           use_improved_ion_advection=True, use_topography=True)
+    for k in ("rho", "temp", "vel", "vvel", "nds", "ions", "temp_source"):
+        assert np.array_equal(host[k], dev[k]), k
+    for k in ("dalt", "dalt_dlon", "dalt_dlat"):
         assert np.array_equal(host[k], g[k]), k

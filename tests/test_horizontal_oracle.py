@@ -24,6 +24,13 @@ reference build, its golden fixture and what the mathematics fixes.
   Coriolis and centrifugal terms and the tan(latitude) part of the
   divergence: the RK4 step matches a tight ODE integration of those terms.
 * Halo cells and ghost levels are never written.
+* UseTopography: bit for bit against the reference build; with level
+  surfaces of zero slope it is the flat step bit for bit; and a profile
+  linear in height, horizontally uniform, under a uniform east wind over
+  sloping levels follows the closed-form recurrence in which each level
+  reads the step's new value of the level below (the reference writes each
+  level back before the next, advance_horizontal.f90:349-429), which
+  updating every level from its input would miss by far.
 """
 from __future__ import annotations
 
@@ -53,6 +60,72 @@ def test_matches_reference_build(coriolis, ions, improved):
     for k, v in want.items():
         got = out[k][:, 2:-2, 2:-2, 2:-2] if k == "temp_source" else out[k]
         assert np.array_equal(got, v), k
+
+
+@pytest.mark.skipif(not ref.horizontal_available(), reason="reference build absent")
+@pytest.mark.parametrize("ions", [False, True])
+def test_topography_matches_reference_build(ions):
+    g = synth.block_grid(synth.BlockGrid(2, 2, 50), seed=5 + int(ions), ions=ions,
+                         topography=True)
+    dt = synth.horizontal_dt(g, cfl=0.4)
+    out, rc, _ = cpu.horizontal(g, dt, OMEGA, 2.0, True, ions, ions, use_topography=True)
+    want = ref.horizontal(g, dt, OMEGA, 2.0, True, ions, ions, use_topography=True)
+    flat, _, _ = cpu.horizontal(g, dt, OMEGA, 2.0, True, ions, ions)
+    assert rc == 0
+    assert not np.array_equal(out["nds"], flat["nds"])  # the term is not negligible
+    for k, v in want.items():
+        got = out[k][:, 2:-2, 2:-2, 2:-2] if k == "temp_source" else out[k]
+        assert np.array_equal(got, v), k
+
+
+def test_topography_with_level_surfaces_flat_is_flat_step():
+    g = synth.block_grid(synth.BlockGrid(2, 1, 12, nlons=6, nlats=5), seed=3, ions=True,
+                         topography=True)
+    g["dalt_dlon"][:] = 0.0
+    g["dalt_dlat"][:] = 0.0
+    dt = synth.horizontal_dt(g, cfl=0.4)
+    topo, rc1, _ = cpu.horizontal(g, dt, OMEGA, 2.0, True, True, True, use_topography=True)
+    flat, rc2, _ = cpu.horizontal(g, dt, OMEGA, 2.0, True, True, True)
+    assert rc1 == rc2 == 0
+    for k in flat:
+        assert np.array_equal(topo[k], flat[k]), k
+
+
+def test_topography_levels_advance_in_order():
+    nA = 10
+    g = synth.block_grid(synth.BlockGrid(1, 1, nA), seed=2, topography=True)
+    ve, slope, dalt, dt = 50.0, 1.0, 500.0, 10.0
+    g["rho"][:] = 1.0
+    g["temp"][:] = 0.0
+    g["vel"][:] = 0.0
+    g["vel"][:, 0] = ve
+    g["nds"][:] = 1.0e15
+    g["tan_lat"][:] = 0.0
+    g["dalt"][:] = dalt
+    g["dalt_dlon"][:] = slope
+    g["dalt_dlat"][:] = -0.3
+    lev = np.arange(nA + 4, dtype=np.float64)  # levels -1..nA+2
+    for s in range(6):
+        g["vvel"][:, s] = (10.0 + s + (3.0 - 0.4 * s) * lev)[None, :, None, None]
+    out, rc, _ = cpu.horizontal(g, dt, OMEGA, 2.0, False, use_topography=True)
+    assert rc == 0
+    c = dt * ve * slope * (0.5 / dalt)
+    for s in range(6):
+        v = g["vvel"][0, s, :, 2, 2].copy()
+        new = v.copy()
+        for k in range(1, nA + 1):  # index k + 1
+            new[k + 1] = v[k + 1] + c * (v[k + 2] - new[k])
+        # cells 3 or more from the halo: the halo cells keep their input
+        # values through the stages, so the cells next to them see a step,
+        # which the level-to-level coupling carries inwards at about 2e-9;
+        # reading the level below's input instead is off by about 0.1
+        got = out["vvel"][0, s, 2:nA + 2, 4:-4, 4:-4]
+        np.testing.assert_allclose(got, np.broadcast_to(new[2:nA + 2, None, None], got.shape),
+                                   rtol=1e-7, atol=0)
+        jacobi = v[2:nA + 2] + c * (v[3:nA + 3] - v[1:nA + 1])
+        assert np.min(np.abs(new[3:nA + 2] - jacobi[1:])) > 0.1
+    for k in ("rho", "vel", "nds"):  # nothing else moves
+        assert np.array_equal(out[k], g[k]), k
 
 
 def test_golden_fixture():
