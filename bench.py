@@ -24,7 +24,9 @@ Further figures ride on the same line: ``prep`` (the column prep),
 batch, a hipGraph of 20 prep+step iterations; ``--no-ensemble`` skips it)
 and ``adjoint`` with
 ``adjoint_forward_ratio`` (a 10-step window on the 5 deg grid, backward pass
-with recompute over forward pass; ``--no-adjoint`` skips it).
+with recompute over forward pass; ``--no-adjoint`` skips it), and
+``cpu_reference``: the reference Fortran build on the host cores on the 5 deg
+grid (its compiled nAlts), with the GPU's rate on the same columns.
 """
 from __future__ import annotations
 
@@ -137,6 +139,51 @@ def cpu_baseline(spec, solver: str, seconds: float, threads: int) -> dict:
             "sample": f"first {done} of {spec.ncols} columns of the {spec.nalts}-level "
                       f"grid, one {solver.upper()} RK4 step each, {el:.1f} s wall on "
                       f"{threads} threads (C restatement, gcc -O2)"}
+
+
+def cpu_reference(torch, solver, params, solver_name: str, threads: int,
+                  n_steps: int = 200) -> dict | None:
+    """The reference Fortran itself (oracle/_ref/gitm_vert_ref, compiled from
+    its own sources, whose grid is nAlts = 50 as shipped) on the 5 deg grid
+    (2592 columns x 50 levels): ``threads`` processes at once, each
+    advancing its share of the columns ``n_steps`` steps, as the reference's
+    MPI run spreads blocks over cores.  The wall time includes each process's
+    file staging.  The GPU step is timed on the same columns beside it.
+    None where the reference build is absent."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from gitm_amd import synth, vertical
+    from oracle import cpu, ref
+
+    if not ref.available():
+        return None
+    spec = synth.CONFIGS["5deg_2592x50"]
+    cols = synth.make_columns(spec, seed=1)
+    sid = cpu.AUSM if solver_name == "ausm" else cpu.RUSANOV
+    parts = [{k: np.ascontiguousarray(v[i]) for k, v in cols.items()}
+             for i in np.array_split(np.arange(spec.ncols), threads)]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda c: ref.advance(c, params, sid, n_steps), parts))
+    el = time.perf_counter() - t0
+    dev = vertical.to_device(cols, "cuda")
+    for _ in range(3):
+        solver.advance(dev, params, solver_name, 1, check=False)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(n_steps):
+        solver.advance(dev, params, solver_name, 1, check=False)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    solver.check()
+    gpu = spec.ncols * n_steps / (e0.elapsed_time(e1) * 1e-3)
+    return {"value": spec.ncols * n_steps / el, "unit": "column-updates/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"all {spec.ncols} columns of the 5 deg grid (50 levels), {n_steps} "
+                      f"{solver_name.upper()} RK4 steps, {threads} processes of the reference "
+                      f"Fortran build, {el:.1f} s wall",
+            "gpu_value_same_workload": gpu}
 
 
 def copy_probe(torch, mib: int = 1024) -> None:
@@ -479,11 +526,13 @@ def main() -> int:
         adj = bench_adjoint(torch, solver)
         torch.cuda.empty_cache()
 
-    cpu = None
+    cpu = cpu_ref = None
     if args.cpu_seconds > 0:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         print(f"bench: CPU baseline on {threads} threads", file=sys.stderr, flush=True)
         cpu = cpu_baseline(spec, args.solver, args.cpu_seconds, threads)
+        print(f"bench: reference Fortran on {threads} processes", file=sys.stderr, flush=True)
+        cpu_ref = cpu_reference(torch, solver, params, args.solver, threads)
 
     line = {
         "metric": "FP64 AUSM RK4 column-updates/s @1°×1°×100 alts; adjoint/forward "
@@ -512,6 +561,7 @@ def main() -> int:
                      "kernel": f"vert_column_kernel<{64 * vertical.auto_waves(spec.nalts)},"
                                f"{1 if args.solver == 'ausm' else 0}>"},
         "cpu_baseline": cpu,
+        "cpu_reference": cpu_ref,
         "prep": {"kernel": "prep_kernel (gitm_vert_prepare: calc_pressure, calc_rates "
                            "mean mass, eddy coefficient, calc_timestep_vertical)",
                  "launch_ms": prep["launch_ms"], "bound": "hbm",
