@@ -581,13 +581,14 @@ extern "C" int gitm_horiz_advance(gitm_vert_handle *h, const gitm_horiz_params *
   // altitudes per workgroup: as many as fit 256 lanes and 40 KB of LDS, so
   // that four workgroups share a CU (two wavefronts per SIMD, the register
   // limit); 9x9 blocks take one
-  int apw = hp->altitudes_per_workgroup > 0 ? hp->altitudes_per_workgroup : 0;
+  // (with topography one altitude per workgroup, whatever is asked)
+  int apw = topo ? 1 : hp->altitudes_per_workgroup > 0 ? hp->altitudes_per_workgroup : 0;
   if (apw == 0) {
     apw = 1;
     while ((apw + 1) * cells <= 256 && (apw + 1) * slab <= 40 * 1024 && apw < 4) apw++;
   }
   if (apw * cells > 256 || apw * slab > 160 * 1024) return GITM_VERT_EINVAL;
-  apw = topo ? 1 : std::min(apw, c->n_alts);
+  apw = std::min(apw, c->n_alts);
   HzParams p;
   p.dt = hp->dt;
   p.omega = hp->omega_body;

@@ -183,6 +183,14 @@ int adjoint_nmax(int n_alts) { return 12288 + 8 * n_alts; }
 constexpr int kAdjMaxPhases = 128;
 constexpr size_t kNodeBytes = sizeof(gv::AdDer) + sizeof(double) + sizeof(gv::AdPar);
 
+// The tape's adjoints start at zero: one coalesced fill per chunk, rather
+// than a store per node while taping.
+__global__ void __launch_bounds__(256) clear_adjoints(double *adj, long long n) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (long long)gridDim.x * 256)
+    adj[i] = 0.0;
+}
+
 template <int NT>
 void launch_adjoint(int ncols, int col0, hipStream_t stream, const gv::KParams &kp,
                     const gitm_vert_columns &c, const gv::AdjFields &go,
@@ -273,6 +281,10 @@ int gitm_vert_adjoint(gitm_vert_handle *h, const gitm_vert_params *p,
   gi.rho = gin->rho;
   for (long long c0 = 0; c0 < ncols; c0 += chunk) {
     const int n = (int)std::min(chunk, ncols - c0);
+    const long long nclear = (long long)nn;  // node n of lane g is n * stride + g
+    hipLaunchKernelGGL(clear_adjoints, dim3((unsigned)std::min<long long>(
+                                           (nclear + 255) / 256, 8192)),
+                       dim3(256), 0, h->stream, tape.adj, nclear);
     if (waves == 1)
       launch_adjoint<64>(n, (int)c0, h->stream, kp, *c, go, gi, tape);
     else

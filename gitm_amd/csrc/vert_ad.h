@@ -51,7 +51,7 @@ struct alignas(16) AdDer {
 struct AdTape {
   AdDer *d;         // partial derivatives w.r.t. the parents
   AdPar *p;         // parent nodes
-  double *adj;      // adjoints, zeroed when the node is made
+  double *adj;      // adjoints, all zero before the forward part (cleared by the caller)
   int *phase;       // [phase * stride + lane] node count at the barrier
   int *nphase;      // [lane] barriers of the lane's forward pass
   int stride;       // lanes in the launch
@@ -94,7 +94,6 @@ GV_AD int ad_record(int a, double da, int b, double db, bool leaf = false) {
   n = n + 1;
   t.p[id] = AdPar{a, b};
   t.d[id] = AdDer{da, db};
-  t.adj[id] = 0.0;
   return (int)id;
 }
 

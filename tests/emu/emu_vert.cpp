@@ -198,6 +198,9 @@ extern "C" int gv_emu_adjoint(const gitm_vert_params *p, const gitm_vert_columns
       for (int col = c0; col < c1; col++) {
         EmuAdCtx cx{t, &sh};
         gv::ad_lane.n = 0;
+        // the lane's adjoints start at zero (before its first barrier; the
+        // previous column's adds all ended at the barrier closing it)
+        for (int n = 0; n < nmax; n++) adj[(size_t)n * L + t] = 0.0;
         auto run = [&](auto nt) {
           gv::adjoint_column<decltype(nt)::value>(cx, kp, *c, *gout, *gin, col, lds.data(),
                                                    reinterpret_cast<double *>(scratch.data()),

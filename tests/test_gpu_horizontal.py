@@ -79,6 +79,7 @@ def test_matches_oracle(solver, spec, coriolis, ions, apw, improved):
     (synth.BlockGrid(4, 2, 30), False, 0),
     (synth.BlockGrid(4, 2, 30), True, 0),
     (synth.BlockGrid(3, 2, 12, nlons=6, nlats=5), True, 3),  # apw is 1 with topography
+    (synth.BlockGrid(2, 1, 8), False, 9),  # 9 x 81 lanes would not fit; ignored too
 ])
 def test_topography_matches_oracle(solver, spec, ions, apw):
     g = synth.block_grid(spec, seed=23 + int(ions), ions=ions, topography=True)
