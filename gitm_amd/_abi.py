@@ -26,6 +26,7 @@ ERRORS = {
     7: "out of device memory",
     8: "adjoint tape too small for the column",
     9: "temperature below 100 after the message exchange",
+    10: "columns too large for the fixed scratch of gitm_vert_advance_noalloc",
 }
 
 

@@ -66,6 +66,8 @@ const char *gitm_vert_strerror(int code) {
     case GITM_VERT_ENOMEM: return "out of device memory";
     case GITM_VERT_ETAPE: return "adjoint tape too small for the column";
     case GITM_VERT_ELOWTEMP: return "temperature below 100 after the message exchange";
+    case GITM_VERT_ESCRATCH:
+      return "columns too large for the fixed scratch of gitm_vert_advance_noalloc";
     default: return "unknown error";
   }
 }
@@ -309,4 +311,84 @@ extern "C" int gitm_vert_advance_host(gitm_vert_handle *h, const gitm_vert_param
     GV_HIP(hipMemcpyAsync(f.ptr, h->stage[i], bytes, hipMemcpyDeviceToHost, h->stream));
   }
   return gitm_vert_get_error(h, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// gitm_vert_advance_noalloc: gitm_vert_advance without a handle and without
+// any allocation, for callers that cannot allocate device memory (code under
+// hipGraph capture, serving loops).  Its scratch memory is the file-scope
+// array below, which every device the library runs on holds until the
+// process ends: the first 256 bytes hold the error word, the rest the RK4
+// carries that gitm_vert_advance keeps in its handle's scratch (ion carries
+// and the carries of columns taller than 188 levels).
+// ---------------------------------------------------------------------------
+#include <mutex>
+
+namespace {
+
+constexpr size_t kNoallocScratchBytes = (size_t)64 << 20;
+constexpr size_t kNoallocCarryOffset = 256 / sizeof(double);  // doubles before the carries
+static __device__ __align__(256) double scratch[kNoallocScratchBytes / sizeof(double)];
+
+// The array's address on each device, from hipGetSymbolAddress with that
+// device current; the name `scratch` is a host-side stand-in in host code and
+// is never passed to a kernel.
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double *g_noalloc_ptr[kMaxDevices] = {};
+
+int noalloc_scratch(int dev, double **out) {
+  if (dev < 0 || dev >= kMaxDevices) return GITM_VERT_EINVAL;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr)
+      return GITM_VERT_EHIP;
+    g_noalloc_ptr[dev] = static_cast<double *>(ptr);
+  }
+  *out = g_noalloc_ptr[dev];
+  return GITM_VERT_OK;
+}
+
+__global__ void copy_word(unsigned long long *dst, const unsigned long long *src) { *dst = *src; }
+
+}  // namespace
+
+extern "C" int gitm_vert_advance_noalloc(const gitm_vert_params *p, const gitm_vert_columns *c,
+                                         int solver, int n_steps, void *hip_stream,
+                                         unsigned long long *err_word) {
+  int rc = gv::check_args(p, c, solver, n_steps);
+  if (rc) return rc;
+  const int waves = gv::auto_waves(c->n_alts);
+  if (!gv::valid_waves(waves)) return GITM_VERT_EINVAL;
+  const size_t ncols = (size_t)c->n_lons * c->n_lats * c->n_blocks;
+  const int fields = gv::scratch_fields(gv::carry_in_lds(waves), p->use_ion_advection != 0);
+  const size_t need = ncols * (size_t)fields * (size_t)(c->n_alts + 4);
+  if (need > kNoallocScratchBytes / sizeof(double) - kNoallocCarryOffset)
+    return GITM_VERT_ESCRATCH;
+  if (n_steps == 0) return GITM_VERT_OK;
+  int dev = 0;
+  GV_HIP(hipGetDevice(&dev));
+  double *base = nullptr;
+  rc = noalloc_scratch(dev, &base);
+  if (rc) return rc;
+  hipStream_t stream = (hipStream_t)hip_stream;
+  unsigned long long *err = reinterpret_cast<unsigned long long *>(base);
+  // every call starts with a clear error word (a fill kernel, so that the
+  // call can be captured in a hipGraph)
+  hipLaunchKernelGGL(set_word, dim3(1), dim3(1), 0, stream, err, ~0ULL);
+  gv::KParams kp = gv::make_kparams(p, c, solver, n_steps, err);
+  kp.scr_fields = fields;
+  kp.scratch = base + kNoallocCarryOffset;
+  const dim3 grid((unsigned)ncols);
+  switch (waves) {
+    case 1: launch<64>(solver, grid, stream, kp, *c); break;
+    case 2: launch<128>(solver, grid, stream, kp, *c); break;
+    case 3: launch<192>(solver, grid, stream, kp, *c); break;
+    case 4: launch<256>(solver, grid, stream, kp, *c); break;
+    default: return GITM_VERT_EINVAL;
+  }
+  if (err_word) hipLaunchKernelGGL(copy_word, dim3(1), dim3(1), 0, stream, err_word, err);
+  GV_HIP(hipGetLastError());
+  return GITM_VERT_OK;
 }

@@ -53,8 +53,11 @@ enum {
   GITM_VERT_EHIP = 6,         /* HIP runtime error                         */
   GITM_VERT_ENOMEM = 7,
   GITM_VERT_ETAPE = 8,        /* adjoint tape too small for the column     */
-  GITM_VERT_ELOWTEMP = 9      /* Temperature < 100 after the message
+  GITM_VERT_ELOWTEMP = 9,     /* Temperature < 100 after the message
                                  exchange (exchange_messages_sphere.f90:74-78) */
+  GITM_VERT_ESCRATCH = 10     /* the columns need more carries than the
+                                 fixed scratch of gitm_vert_advance_noalloc
+                                 holds                                     */
 };
 
 /* One array of the column working set.  Element (column c, level k,
@@ -166,6 +169,25 @@ int gitm_vert_advance(gitm_vert_handle *h, const gitm_vert_params *p,
  * stepping status (as gitm_vert_get_error would). */
 int gitm_vert_advance_host(gitm_vert_handle *h, const gitm_vert_params *p,
                            const gitm_vert_columns *c, int solver, int n_steps);
+
+/* gitm_vert_advance for callers that cannot allocate device memory (code
+ * under hipGraph capture, serving loops): no handle, no hipMalloc/hipFree,
+ * no library allocation.  It has no counterpart in the reference.  Its
+ * scratch (the error word and the RK4 carries that gitm_vert_advance keeps in
+ * its handle) is a fixed 64 MiB device array that every device the library
+ * runs on holds until the process ends.  Runs that need more carries
+ * return GITM_VERT_ESCRATCH before touching the device: with ion advection
+ * more than 8,630 columns at nAlts = 50 or 4,481 at nAlts = 100; without,
+ * only columns of nAlts > 188 need carries, at most 963 of them at nAlts =
+ * 252.  Runs on the current device, asynchronously on
+ * `hip_stream` (NULL: the null stream); the kernels are those of
+ * gitm_vert_advance, so results are identical.  Calls that overlap on one
+ * device share the scratch and race: serialise them (one stream).
+ * err_word: NULL, or one device word that receives ~0 when no column failed,
+ * else (column << 32 | code << 16 | (level + 1024)) of the first failure. */
+int gitm_vert_advance_noalloc(const gitm_vert_params *p, const gitm_vert_columns *c,
+                              int solver, int n_steps, void *hip_stream,
+                              unsigned long long *err_word);
 
 /* Synchronise the handle's stream and return (and clear) the first error
  * recorded by the kernels since the last call. */
