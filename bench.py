@@ -49,6 +49,25 @@ FP64_PEAK_TFLOPS = 78.6
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "vert_column_ausm_1deg_64800x100_traffic.txt")
 
 
+# FP64 operations of one AUSM column-update as the product column code does
+# them (device forms; faces once; every lane of the column, idle tile lanes
+# excluded), counted by class by running that code over a counting number
+# type on the host (tests/emu/count.h; tests/test_flop_count.py recomputes
+# them): column 0 of the seeded 1-degree workload (nAlts 100) and of the
+# 5-degree one (nAlts 50).  Columns differ by about 0.1% (branches).
+OPS_AUSM = {100: dict(add=351994, mul=540220, div=29130, sqrt=3232, trans=31057, cmp=137398),
+            50: dict(add=178720, mul=273185, div=14880, sqrt=1632, trans=16157, cmp=68890)}
+# weights: +, -, *, min, max, compare 1; divide and sqrt 10; exp, log, pow,
+# tanh 20
+OP_WEIGHTS = dict(add=1, mul=1, cmp=1, div=10, sqrt=10, trans=20)
+
+
+def ops_per_column(n_alts: int, weighted: bool = True) -> float:
+    """FP64 operations of one AUSM column-update (OPS_AUSM), weighted or raw."""
+    c = OPS_AUSM[n_alts]
+    return float(sum(c[k] * (OP_WEIGHTS[k] if weighted else 1) for k in c))
+
+
 def bytes_per_column(n_alts: int) -> int:
     """Algorithmic HBM bytes of one column-update: 42 FP64 values per level.
 
@@ -383,11 +402,13 @@ def bench_horizontal(torch, solver, reps: int = 10) -> dict:
 
 def bench_adjoint(torch, solver, steps: int = 10, seed: int = 5) -> dict:
     """Differentiation workload: the 5 deg x 5 deg grid (2592 columns x 50
-    levels), a window of ``steps`` x (AUSM step, Gaussian heating source
-    peaked at 150 km, 1e-5 W/kg), loss = mean mass density at 400 km.  The
-    forward pass (plain HIP step, one checkpoint per step) and the backward
-    pass (per step: the taped recompute and the sweep of gitm_vert_adjoint)
-    are timed after one untimed pass.  The ratio is backward / forward."""
+    levels), a window of ``steps`` x (column prep, AUSM step, Gaussian
+    heating source peaked at 150 km, 1e-5 W/kg), loss = mean mass density at
+    400 km.  The forward pass (the plain HIP prep and step, one checkpoint per
+    step) and the backward pass (per step: the prep recomputed from the
+    checkpoint, the hand-written adjoint of the step with the prep inputs'
+    adjoints, the prep's adjoint) are timed after one untimed pass.  The
+    ratio is backward / forward."""
     from gitm_amd import adjoint, synth, vertical
 
     spec = synth.CONFIGS["5deg_2592x50"]
@@ -417,11 +438,28 @@ def bench_adjoint(torch, solver, steps: int = 10, seed: int = 5) -> dict:
         t2 = time.perf_counter()
         fwd_ms, bwd_ms, loss = (t1 - t0) * 1e3, (t2 - t1) * 1e3, float(J.detach())
     ok = bool(torch.isfinite(state["temp"].grad).all()) and bool(torch.isfinite(q.grad).all())
-    return {"workload": f"5deg_2592x50: 2592 columns x 50 levels, {steps} x (AUSM step + "
-                        "heating source), loss = mean mass density at 400 km",
+    return {"workload": f"5deg_2592x50: 2592 columns x 50 levels, {steps} x (column prep + "
+                        "AUSM step + heating source), loss = mean mass density at 400 km",
             "forward_ms": fwd_ms, "backward_ms": bwd_ms, "ratio": bwd_ms / fwd_ms,
             "forward_ms_per_step": fwd_ms / steps, "backward_ms_per_step": bwd_ms / steps,
             "loss": loss, "gradients_finite": ok}
+
+
+def roofline_fp64(spec, launch_ms: float) -> dict:
+    """The FP64 roofline of the same launch: the weighted operation count of
+    the column code (OPS_AUSM) over the launch duration, against MI355X's
+    78.6 TFLOP/s FP64 vector peak.  This is the binding bound: the step's
+    intensity (about 56 weighted operations per algorithmic byte) is far
+    above the FP64 ridge (78.6e12 / 8e12 = 9.8 per byte)."""
+    ops = ops_per_column(spec.nalts) * spec.ncols
+    raw = ops_per_column(spec.nalts, weighted=False) * spec.ncols
+    achieved = ops / (launch_ms * 1e-3) / 1e12
+    return {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+            "ops_per_launch": ops, "raw_ops_per_launch": raw,
+            "weights": "+-*,min,max,cmp 1; div,sqrt 10; exp,log,pow,tanh 20",
+            "intensity_ops_per_byte": ops / (bytes_per_column(spec.nalts) * spec.ncols),
+            "binding": True}
 
 
 def main() -> int:
@@ -560,6 +598,7 @@ def main() -> int:
                      "launch_ms": launch_ms,
                      "kernel": f"vert_column_kernel<{64 * vertical.auto_waves(spec.nalts)},"
                                f"{1 if args.solver == 'ausm' else 0}>"},
+        "roofline_fp64": roofline_fp64(spec, launch_ms) if args.solver == "ausm" else None,
         "cpu_baseline": cpu,
         "cpu_reference": cpu_ref,
         "prep": {"kernel": "prep_kernel (gitm_vert_prepare: calc_pressure, calc_rates "

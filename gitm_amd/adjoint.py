@@ -1,18 +1,28 @@
 """Reverse-mode sensitivities of the vertical step on the GPU.
 
-``adjoint_step`` wraps ``gitm_vert_adjoint``: the adjoint of one AUSM step
-(src/vertical_solver_ausm.f90:26-433 with the gather/scatter of
-src/advance_vertical.f90:45-219), recomputed from the step's input state with
-taped arithmetic and swept backwards on the device.  ``VerticalWindow`` is a
-``torch.autograd.Function`` over a window of steps: the forward pass runs the
-plain HIP step and keeps one checkpoint per step; the backward pass recomputes
-each step from its checkpoint inside the adjoint kernel, last step first.
+``adjoint_step`` wraps ``gitm_vert_adjoint`` / ``gitm_vert_adjoint_all``: the
+hand-written adjoint of one AUSM step (src/vertical_solver_ausm.f90:26-433
+with the gather/scatter of src/advance_vertical.f90:45-219).  Each column
+recomputes its step from the step's input state, recording every stage's
+intermediates in device memory, and reverses the stages' phases on the device.
+``taped=True`` takes the same adjoint through ``gitm_vert_adjoint_taped``
+(every operation on a tape, swept back with atomics), the slow independent
+check.  ``prepare_adjoint`` wraps ``gitm_vert_prepare_adjoint``, the adjoint
+of the column prep.
 
-Between vertical steps the window applies a heating source the way
-add_sources applies GITM's heating terms (src/add_sources.f90:53-66):
-Temperature(1:nAlts) += Dt * Q / (cp * TempUnit), with Q in W/kg, cp from
-calc_pressure and TempUnit from calc_rates (both held at their values at the
-start of the window, as are the other column-prep inputs).
+``VerticalWindow`` is a ``torch.autograd.Function`` over a window of steps
+that follows GITM's order within a time step: the column prep from the
+state (calc_pressure, calc_rates' mean mass and TempUnit, the eddy
+coefficient and cMax; ``gitm_vert_prepare``), the vertical step, then a
+heating source applied the way add_sources applies GITM's heating terms
+(src/add_sources.f90:53-66): Temperature(1:nAlts) += Dt * Q / (cp *
+TempUnit), with Q in W/kg.  The forward pass keeps one checkpoint of the
+state per step; the backward pass recomputes each step's prep from its
+checkpoint, runs the step's adjoint with the prep inputs' adjoints, and
+carries those back through the prep's adjoint, last step first.  The time
+step is held at ``params.dt`` (GITM's FixedDt; the CFL limit of the prep is
+not applied inside the window).  ``with_prep=False`` holds the prep outputs
+at their values at the start of the window instead.
 
 The HIP library must be built; there is no fallback.
 """
@@ -24,16 +34,23 @@ import numpy as np
 import torch
 
 from . import _abi, vertical
-from .synth import StepParams
+from .synth import PrepParams, StepParams
 
 ADJ = (("nds", "ndensity_s", 6), ("vvel", "vertical_velocity", 6), ("vel", "velocity", 3),
        ("temp", "temperature", 1), ("rho", "rho", 1))
 STATE = tuple(a for a, _, _ in ADJ)
-FIXED = ("tempunit", "mmm", "gamma", "cmax", "eddy", "alt", "grav", "invr", "dalt", "coslat")
+PREP = ("tempunit", "gamma", "mmm", "cmax", "eddy")  # the step's prep inputs
+GEOMETRY = ("alt", "grav", "invr", "dalt", "coslat")
+FIXED = ("tempunit", "mmm", "gamma", "cmax", "eddy") + GEOMETRY
 
 
 class AdjointFields(ctypes.Structure):
     _fields_ = [(c, _abi.Field) for _, c, _ in ADJ]
+
+
+class AdjointPrepFields(ctypes.Structure):
+    _fields_ = [(n, _abi.Field) for n in ("temp_unit", "gamma", "mean_major_mass", "cmax",
+                                          "eddy")]
 
 
 _bound = False
@@ -43,35 +60,64 @@ def _lib():
     global _bound
     lib = vertical.lib()
     if not _bound:
-        lib.gitm_vert_adjoint.restype = ctypes.c_int
-        lib.gitm_vert_adjoint.argtypes = [ctypes.c_void_p, ctypes.POINTER(_abi.Params),
-                                          ctypes.POINTER(_abi.Columns),
-                                          ctypes.POINTER(AdjointFields),
-                                          ctypes.POINTER(AdjointFields)]
+        vp = ctypes.c_void_p
+        for name in ("gitm_vert_adjoint", "gitm_vert_adjoint_taped"):
+            f = getattr(lib, name)
+            f.restype = ctypes.c_int
+            f.argtypes = [vp, ctypes.POINTER(_abi.Params), ctypes.POINTER(_abi.Columns),
+                          ctypes.POINTER(AdjointFields), ctypes.POINTER(AdjointFields)]
+        lib.gitm_vert_adjoint_all.restype = ctypes.c_int
+        lib.gitm_vert_adjoint_all.argtypes = [vp, ctypes.POINTER(_abi.Params),
+                                              ctypes.POINTER(_abi.Columns),
+                                              ctypes.POINTER(AdjointFields),
+                                              ctypes.POINTER(AdjointFields),
+                                              ctypes.POINTER(AdjointPrepFields)]
+        lib.gitm_vert_prepare_adjoint.restype = ctypes.c_int
+        lib.gitm_vert_prepare_adjoint.argtypes = [vp, ctypes.POINTER(_abi.PrepParams),
+                                                  ctypes.POINTER(_abi.Columns),
+                                                  ctypes.POINTER(AdjointPrepFields),
+                                                  ctypes.POINTER(_abi.Field),
+                                                  ctypes.POINTER(AdjointFields)]
         lib.gitm_vert_set_adjoint_memory.restype = ctypes.c_int
-        lib.gitm_vert_set_adjoint_memory.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+        lib.gitm_vert_set_adjoint_memory.argtypes = [vp, ctypes.c_size_t]
+        lib.gitm_vert_release_adjoint_memory.restype = ctypes.c_int
+        lib.gitm_vert_release_adjoint_memory.argtypes = [vp]
         _bound = True
     return lib
+
+
+def _check(t, name):
+    if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+        raise ValueError(f"{name}: need a contiguous float64 device tensor")
 
 
 def _fields(t: dict, L: int) -> AdjointFields:
     f = AdjointFields()
     for name, cname, ncomp in ADJ:
-        x = t[name]
-        if x.dtype != torch.float64 or not x.is_cuda or not x.is_contiguous():
-            raise ValueError(f"{name}: need a contiguous float64 device tensor")
-        setattr(f, cname, _abi.column_field(x.data_ptr(), ncomp, L))
+        _check(t[name], name)
+        setattr(f, cname, _abi.column_field(t[name].data_ptr(), ncomp, L))
+    return f
+
+
+def _prep_fields(t: dict, L: int) -> AdjointPrepFields:
+    f = AdjointPrepFields()
+    for key, cname in zip(PREP, ("temp_unit", "gamma", "mean_major_mass", "cmax", "eddy")):
+        _check(t[key], key)
+        setattr(f, cname, _abi.column_field(t[key].data_ptr(), 1, L))
     return f
 
 
 def adjoint_step(solver: "vertical.VerticalSolver", cols: dict, gout: dict,
-                 params: StepParams | None = None, gin: dict | None = None) -> dict:
+                 params: StepParams | None = None, gin: dict | None = None,
+                 taped: bool = False, gprep: dict | None = None) -> dict:
     """Adjoints of one AUSM step's inputs from those of its outputs.
 
     ``cols``: the step's input state and column inputs (device tensors, not
     modified); ``gout``: d loss / d (nds, vvel, vel, temp, rho) after the
     step.  Returns d loss / d the same fields before the step (``gin`` if
-    given; it may be ``gout``).
+    given; it may be ``gout``).  With ``gprep`` (a dict of tensors shaped like
+    "temp" under the keys of PREP) those receive d loss / d the step's prep
+    inputs.  Queued on the current stream (the taped form synchronises).
     """
     params = params or StepParams()
     if params.use_ion_advection:
@@ -82,15 +128,53 @@ def adjoint_step(solver: "vertical.VerticalSolver", cols: dict, gout: dict,
     gi = gin if gin is not None else {k: torch.empty_like(go[k]) for k in STATE}
     p = vertical.make_params(params)
     solver.set_stream(torch.cuda.current_stream().cuda_stream)
-    rc = _lib().gitm_vert_adjoint(solver._h, ctypes.byref(p), ctypes.byref(c),
-                                  ctypes.byref(_fields(go, L)), ctypes.byref(_fields(gi, L)))
+    lib = _lib()
+    args = [solver._h, ctypes.byref(p), ctypes.byref(c), ctypes.byref(_fields(go, L)),
+            ctypes.byref(_fields(gi, L))]
+    if taped:
+        if gprep is not None:
+            raise ValueError("the taped adjoint holds the prep inputs fixed")
+        rc = lib.gitm_vert_adjoint_taped(*args)
+    elif gprep is not None:
+        rc = lib.gitm_vert_adjoint_all(*args, ctypes.byref(_prep_fields(gprep, L)))
+    else:
+        rc = lib.gitm_vert_adjoint(*args)
     if rc:
         raise vertical.VerticalStepError(rc)
     return gi
 
 
+def prepare_adjoint(solver: "vertical.VerticalSolver", cols: dict, gprep: dict, gin: dict,
+                    gcp=None, pp: PrepParams | None = None) -> dict:
+    """Adds to ``gin`` (nds, vvel, temp, rho) the adjoints that the column
+    prep's outputs (``gprep``: PREP keys; ``gcp``: cp, optional) carry back to
+    the state in ``cols`` (the prep's input).  Queued on the current stream."""
+    c, _ = vertical._device_columns({k: cols[k] for k in STATE + ("dalt", "gamma", "mmm",
+                                                                  "tempunit", "eddy", "cmax")})
+    L = int(cols["temp"].shape[1])
+    p = vertical.make_prep_params(pp)
+    cpf = None
+    if gcp is not None:
+        _check(gcp, "gcp")
+        cpf = _abi.column_field(gcp.data_ptr(), 1, L)
+    solver.set_stream(torch.cuda.current_stream().cuda_stream)
+    rc = _lib().gitm_vert_prepare_adjoint(solver._h, ctypes.byref(p), ctypes.byref(c),
+                                          ctypes.byref(_prep_fields(gprep, L)),
+                                          ctypes.byref(cpf) if cpf is not None else None,
+                                          ctypes.byref(_fields(gin, L)))
+    if rc:
+        raise vertical.VerticalStepError(rc)
+    return gin
+
+
 def set_adjoint_memory(solver: "vertical.VerticalSolver", nbytes: int) -> None:
     rc = _lib().gitm_vert_set_adjoint_memory(solver._h, nbytes)
+    if rc:
+        raise vertical.VerticalStepError(rc)
+
+
+def release_adjoint_memory(solver: "vertical.VerticalSolver") -> None:
+    rc = _lib().gitm_vert_release_adjoint_memory(solver._h)
     if rc:
         raise vertical.VerticalStepError(rc)
 
@@ -123,26 +207,48 @@ def _source(temp, heat, cp_tu, dt: float):
     return out
 
 
+def _prep(solver, state: dict, geometry: dict, pp: PrepParams) -> dict:
+    """The column prep of ``state``: a dict with the prep outputs (PREP keys,
+    "cp") in fresh tensors."""
+    d = dict(state)
+    d.update({k: geometry[k] for k in GEOMETRY})
+    for k in PREP + ("cp", "nden"):
+        d[k] = torch.zeros_like(state["temp"])
+    solver.prepare(d, pp)
+    return {k: d[k] for k in PREP + ("cp",)}
+
+
 class VerticalWindow(torch.autograd.Function):
-    """n_steps x (AUSM vertical step, heating source) with one checkpoint
-    per step; differentiable in the initial state and the heating rate."""
+    """n_steps x (column prep, AUSM vertical step, heating source) with one
+    checkpoint per step; differentiable in the initial state and the heating
+    rate.  ``fixed`` holds the geometry (GEOMETRY keys) and, with
+    ``with_prep=False``, the prep outputs (PREP keys and "cp") to hold."""
 
     @staticmethod
-    def forward(ctx, nds, vvel, vel, temp, rho, heat, solver, fixed, params, n_steps):
+    def forward(ctx, nds, vvel, vel, temp, rho, heat, solver, fixed, params, n_steps,
+                with_prep, prep_params):
         params = params or StepParams()
-        cols = {"nds": nds.detach().clone(), "vvel": vvel.detach().clone(),
-                "vel": vel.detach().clone(), "temp": temp.detach().clone(),
-                "rho": rho.detach().clone()}
-        cols.update({k: fixed[k] for k in FIXED})
-        cp_tu = fixed["cp"] * fixed["tempunit"]
+        pp = prep_params or PrepParams()
+        state = {"nds": nds.detach().clone(), "vvel": vvel.detach().clone(),
+                 "vel": vel.detach().clone(), "temp": temp.detach().clone(),
+                 "rho": rho.detach().clone()}
+        geometry = {k: fixed[k] for k in GEOMETRY}
+        held = None if with_prep else {k: fixed[k] for k in PREP + ("cp",)}
         checkpoints = []
         for _ in range(n_steps):
-            checkpoints.append({k: cols[k].clone() for k in STATE})
-            solver.advance(cols, params, "ausm", 1)
-            cols["temp"] = _source(cols["temp"], heat.detach(), cp_tu, params.dt)
-        ctx.solver, ctx.fixed, ctx.params = solver, fixed, params
-        ctx.checkpoints, ctx.cp_tu = checkpoints, cp_tu
-        return tuple(cols[k] for k in STATE)
+            checkpoints.append({k: state[k].clone() for k in STATE})
+            prep = _prep(solver, state, geometry, pp) if with_prep else held
+            cols = dict(state)
+            cols.update(geometry)
+            cols.update({k: prep[k] for k in PREP})
+            solver.advance(cols, params, "ausm", 1, check=False)
+            state = {k: cols[k] for k in STATE}
+            state["temp"] = _source(state["temp"], heat.detach(), prep["cp"] * prep["tempunit"],
+                                    params.dt)
+        solver.check()
+        ctx.solver, ctx.geometry, ctx.held, ctx.params, ctx.pp = solver, geometry, held, params, pp
+        ctx.checkpoints, ctx.heat = checkpoints, heat.detach()
+        return tuple(state[k] for k in STATE)
 
     @staticmethod
     def backward(ctx, *grads):
@@ -151,22 +257,41 @@ class VerticalWindow(torch.autograd.Function):
             ref = ctx.checkpoints[0][k]
             lam[k] = torch.zeros_like(ref) if g is None else g.contiguous().clone()
         dt = ctx.params.dt
-        g_heat = torch.zeros_like(ctx.cp_tu)
+        heat = ctx.heat
+        g_heat = torch.zeros_like(heat)
+        gprep = {k: torch.empty_like(heat) for k in PREP}
         for ck in reversed(ctx.checkpoints):
-            # the source step: d temp_out / d heat = Dt / (cp TempUnit)
-            g_heat[:, 2:-2] += lam["temp"][:, 2:-2] * dt / ctx.cp_tu[:, 2:-2]
+            prep = _prep(ctx.solver, ck, ctx.geometry, ctx.pp) if ctx.held is None else ctx.held
+            cp, tu = prep["cp"], prep["tempunit"]
+            # the source: temp += Dt Q / (cp TempUnit) on 1..nAlts
+            lt = torch.zeros_like(heat)
+            lt[:, 2:-2] = lam["temp"][:, 2:-2]
+            q = dt / (cp * tu)
+            g_heat += lt * q
             cols = dict(ck)
-            cols.update({k: ctx.fixed[k] for k in FIXED})
-            lam = adjoint_step(ctx.solver, cols, lam, ctx.params, gin=lam)
+            cols.update(ctx.geometry)
+            cols.update({k: prep[k] for k in PREP})
+            if ctx.held is None:
+                lam = adjoint_step(ctx.solver, cols, lam, ctx.params, gin=lam, gprep=gprep)
+                # the source's cp and TempUnit come from the same prep
+                gprep["tempunit"] -= lt * heat * q / tu
+                gcp = -(lt * heat * q / cp).contiguous()
+                lam = prepare_adjoint(ctx.solver, cols, gprep, lam, gcp=gcp, pp=ctx.pp)
+            else:
+                lam = adjoint_step(ctx.solver, cols, lam, ctx.params, gin=lam)
+        ctx.solver.check()
         return (lam["nds"], lam["vvel"], lam["vel"], lam["temp"], lam["rho"], g_heat,
-                None, None, None, None)
+                None, None, None, None, None, None)
 
 
 def window(state: dict, heat, solver, fixed: dict, params: StepParams | None = None,
-           n_steps: int = 10) -> dict:
+           n_steps: int = 10, with_prep: bool = True,
+           prep_params: PrepParams | None = None) -> dict:
     """Differentiable window: the state (nds, vvel, vel, temp, rho) after
-    ``n_steps`` x (AUSM step, heating source).  ``fixed`` holds the column
-    inputs (FIXED) and "cp"."""
+    ``n_steps`` x (column prep, AUSM step, heating source).  ``fixed`` holds
+    the geometry (GEOMETRY) and, with ``with_prep=False``, the prep outputs
+    to hold (PREP and "cp")."""
     out = VerticalWindow.apply(state["nds"], state["vvel"], state["vel"], state["temp"],
-                               state["rho"], heat, solver, fixed, params, n_steps)
+                               state["rho"], heat, solver, fixed, params, n_steps, with_prep,
+                               prep_params)
     return dict(zip(STATE, out))

@@ -102,6 +102,7 @@ int gitm_vert_destroy(gitm_vert_handle *h) {
   for (void *p : h->stage) (void)hipFree(p);
   if (h->scratch) (void)hipFree(h->scratch);
   if (h->tape) (void)hipFree(h->tape);
+  if (h->rec) (void)hipFree(h->rec);
   if (h->d_overflow) (void)hipFree(h->d_overflow);
   (void)hipFree(h->d_err);
   delete h;

@@ -1,6 +1,9 @@
-// gitm_vert_adjoint.hip -- the adjoint of the AUSM step (gitm_vert_adjoint
-// of include/gitm_vert.h; the taped column code is in vert_ad.h).
+// gitm_vert_adjoint.hip -- the adjoint of the AUSM step: the hand-written
+// adjoint (gitm_vert_adjoint, gitm_vert_adjoint_all; vert_adj.h), the
+// adjoint of the column prep (gitm_vert_prepare_adjoint) and the taped
+// adjoint (gitm_vert_adjoint_taped; vert_ad.h), which checks the former.
 #include "vert_ad.h"
+#include "vert_adj.h"
 #include "vert_gpu.h"
 
 namespace {
@@ -210,12 +213,40 @@ extern "C" {
 int gitm_vert_set_adjoint_memory(gitm_vert_handle *h, size_t bytes) {
   if (!h || (bytes != 0 && bytes < ((size_t)64 << 20))) return GITM_VERT_EINVAL;
   h->adj_budget = bytes;
+  // a buffer larger than the new budget is given back at once
+  const size_t cap = bytes ? bytes : (size_t)-1;
+  if ((h->tape && h->tape_bytes > cap) || (h->rec && h->rec_bytes > cap)) {
+    GV_HIP(hipSetDevice(h->device));
+    GV_HIP(hipStreamSynchronize(h->stream));
+    if (h->tape && h->tape_bytes > cap) {
+      (void)hipFree(h->tape);
+      h->tape = nullptr;
+      h->tape_bytes = 0;
+    }
+    if (h->rec && h->rec_bytes > cap) {
+      (void)hipFree(h->rec);
+      h->rec = nullptr;
+      h->rec_bytes = 0;
+    }
+  }
   return GITM_VERT_OK;
 }
 
-int gitm_vert_adjoint(gitm_vert_handle *h, const gitm_vert_params *p,
-                      const gitm_vert_columns *c, const gitm_vert_adjoint_fields *gout,
-                      const gitm_vert_adjoint_fields *gin) {
+int gitm_vert_release_adjoint_memory(gitm_vert_handle *h) {
+  if (!h) return GITM_VERT_EINVAL;
+  GV_HIP(hipSetDevice(h->device));
+  GV_HIP(hipStreamSynchronize(h->stream));
+  if (h->tape) (void)hipFree(h->tape);
+  if (h->rec) (void)hipFree(h->rec);
+  h->tape = nullptr;
+  h->rec = nullptr;
+  h->tape_bytes = h->rec_bytes = 0;
+  return GITM_VERT_OK;
+}
+
+int gitm_vert_adjoint_taped(gitm_vert_handle *h, const gitm_vert_params *p,
+                            const gitm_vert_columns *c, const gitm_vert_adjoint_fields *gout,
+                            const gitm_vert_adjoint_fields *gin) {
   if (!h || !gout || !gin) return GITM_VERT_EINVAL;
   int rc = gv::check_args(p, c, GITM_VERT_AUSM, 1);
   if (rc) return rc;
@@ -297,6 +328,175 @@ int gitm_vert_adjoint(gitm_vert_handle *h, const gitm_vert_params *p,
   GV_HIP(hipStreamSynchronize(h->stream));
   if (overflow) return GITM_VERT_ETAPE;
   return gitm_vert_get_error(h, nullptr);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The hand-written adjoint (vert_adj.h): one workgroup per column, one lane
+// per level; the record of the forward recompute in device memory.
+// ---------------------------------------------------------------------------
+namespace {
+
+template <int NT>
+__global__ void __launch_bounds__(NT)
+    vert_adj_kernel(gv::KParams kp, gitm_vert_columns io, gv::AdjFields gout,
+                    gv::AdjFields gin, gv::AdjFixFields gfix, int with_fix, double *rec,
+                    int col0) {
+  extern __shared__ double lds[];
+  gv::TapCtx<GpuCtx> cx;
+  cx.err = kp.err;
+  const int col = col0 + (int)blockIdx.x;
+  double *r = rec + (size_t)blockIdx.x * (size_t)gv::adj_column_doubles(NT);
+  gv::adj_column<NT>(cx, kp, io, gout, gin, with_fix ? &gfix : nullptr, col, lds, r, col);
+}
+
+// The record and work area of a column: ADJ_STAGES x NREC + NWORK doubles
+// per lane.
+size_t adj_record_bytes(int nt) { return (size_t)gv::adj_column_doubles(nt) * sizeof(double); }
+
+// Default record budget: 10% of the device's memory, at most 4 GiB (the
+// 1-degree grid at nAlts = 100 then runs in chunks of about 7,900 columns).
+size_t default_record_budget(int device) {
+  hipDeviceProp_t prop;
+  size_t total = (size_t)16 << 30;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) total = prop.totalGlobalMem;
+  return std::min((size_t)(0.1 * (double)total), (size_t)4 << 30);
+}
+
+template <int NT>
+void launch_adj(int n, int col0, hipStream_t stream, const gv::KParams &kp,
+                const gitm_vert_columns &c, const gv::AdjFields &go, const gv::AdjFields &gi,
+                const gv::AdjFixFields &gf, int with_fix, double *rec) {
+  const size_t lds = gv::adj_lds_bytes(kp.n_alts, true);
+  hipLaunchKernelGGL(vert_adj_kernel<NT>, dim3(n), dim3(kp.n_alts + 4), lds, stream, kp, c,
+                     go, gi, gf, with_fix, rec, col0);
+}
+
+gv::AdjFields adj_fields(const gitm_vert_adjoint_fields *g) {
+  gv::AdjFields a;
+  a.nds = g->ndensity_s;
+  a.vvel = g->vertical_velocity;
+  a.vel = g->velocity;
+  a.temp = g->temperature;
+  a.rho = g->rho;
+  return a;
+}
+
+int adjoint_hand(gitm_vert_handle *h, const gitm_vert_params *p, const gitm_vert_columns *c,
+                 const gitm_vert_adjoint_fields *gout, const gitm_vert_adjoint_fields *gin,
+                 const gitm_vert_adjoint_prep_fields *gfix) {
+  if (!h || !gout || !gin) return GITM_VERT_EINVAL;
+  int rc = gv::check_args(p, c, GITM_VERT_AUSM, 1);
+  if (rc) return rc;
+  if (p->use_ion_advection || p->use_msis) return GITM_VERT_EUNSUPPORTED;
+  const gitm_vert_field *req[] = {&gout->ndensity_s, &gout->vertical_velocity, &gout->velocity,
+                                  &gout->temperature, &gout->rho, &gin->ndensity_s,
+                                  &gin->vertical_velocity, &gin->velocity, &gin->temperature,
+                                  &gin->rho};
+  for (const gitm_vert_field *f : req)
+    if (!f->ptr) return GITM_VERT_EINVAL;
+  gv::AdjFixFields gf = {};
+  if (gfix) {
+    const gitm_vert_field *fr[] = {&gfix->temp_unit, &gfix->gamma, &gfix->mean_major_mass,
+                                   &gfix->cmax, &gfix->eddy};
+    for (const gitm_vert_field *f : fr)
+      if (!f->ptr) return GITM_VERT_EINVAL;
+    if (gfix->cmax.first_level < -1 || gfix->cmax.first_level > 0) return GITM_VERT_EINVAL;
+    gf.temp_unit = gfix->temp_unit;
+    gf.gamma = gfix->gamma;
+    gf.mean_major_mass = gfix->mean_major_mass;
+    gf.cmax = gfix->cmax;
+    gf.eddy = gfix->eddy;
+  }
+  const int waves = gv::auto_waves(c->n_alts);
+  if (!gv::carry_in_lds(waves)) return GITM_VERT_EUNSUPPORTED;  // nAlts <= 188
+  GV_HIP(hipSetDevice(h->device));
+  const long long ncols = (long long)c->n_lons * c->n_lats * c->n_blocks;
+  const int nt = 64 * waves;
+  const size_t per_col = adj_record_bytes(nt);
+  const size_t budget = h->adj_budget ? h->adj_budget : default_record_budget(h->device);
+  long long chunk = std::min((long long)(budget / per_col), ncols);
+  if (chunk < 1) return GITM_VERT_ENOMEM;
+  const size_t need = (size_t)chunk * per_col;
+  if (h->rec_bytes < need) {
+    GV_HIP(hipStreamSynchronize(h->stream));
+    if (h->rec) (void)hipFree(h->rec);
+    h->rec = nullptr;
+    h->rec_bytes = 0;
+    if (hipMalloc(&h->rec, need) != hipSuccess) return GITM_VERT_ENOMEM;
+    h->rec_bytes = need;
+  }
+  gv::KParams kp = gv::make_kparams(p, c, GITM_VERT_AUSM, 1, h->d_err);
+  kp.scr_fields = 0;
+  kp.scratch = nullptr;
+  const gv::AdjFields go = adj_fields(gout), gi = adj_fields(gin);
+  double *rec = static_cast<double *>(h->rec);
+  for (long long c0 = 0; c0 < ncols; c0 += chunk) {
+    const int n = (int)std::min(chunk, ncols - c0);
+    switch (waves) {
+      case 1: launch_adj<64>(n, (int)c0, h->stream, kp, *c, go, gi, gf, gfix != nullptr, rec); break;
+      case 2: launch_adj<128>(n, (int)c0, h->stream, kp, *c, go, gi, gf, gfix != nullptr, rec); break;
+      default: launch_adj<192>(n, (int)c0, h->stream, kp, *c, go, gi, gf, gfix != nullptr, rec); break;
+    }
+    GV_HIP(hipGetLastError());
+  }
+  return GITM_VERT_OK;
+}
+
+// The column prep's adjoint, one thread per (column, level).
+__global__ void __launch_bounds__(256)
+    prep_adjoint_kernel(gv::PrepParams pp, gitm_vert_columns io,
+                        gitm_vert_adjoint_prep_fields gf, gitm_vert_field gcp,
+                        gitm_vert_adjoint_fields gin, long long n) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int L = pp.n_alts + 4;
+  const int col = (int)(i / L), k = (int)(i % L) - 1;
+  const int nlb = io.n_lons * io.n_lats;
+  const int ci = col % io.n_lons, cj = (col / io.n_lons) % io.n_lats, cb = col / nlb;
+  gv::prep_level_adjoint(pp, io, gf, gcp, gin, ci, cj, cb, k);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gitm_vert_adjoint(gitm_vert_handle *h, const gitm_vert_params *p,
+                      const gitm_vert_columns *c, const gitm_vert_adjoint_fields *gout,
+                      const gitm_vert_adjoint_fields *gin) {
+  return adjoint_hand(h, p, c, gout, gin, nullptr);
+}
+
+int gitm_vert_adjoint_all(gitm_vert_handle *h, const gitm_vert_params *p,
+                          const gitm_vert_columns *c, const gitm_vert_adjoint_fields *gout,
+                          const gitm_vert_adjoint_fields *gin,
+                          const gitm_vert_adjoint_prep_fields *gprep) {
+  if (!gprep) return GITM_VERT_EINVAL;
+  return adjoint_hand(h, p, c, gout, gin, gprep);
+}
+
+int gitm_vert_prepare_adjoint(gitm_vert_handle *h, const gitm_vert_prep_params *p,
+                              const gitm_vert_columns *c,
+                              const gitm_vert_adjoint_prep_fields *gprep,
+                              const gitm_vert_field *gcp, const gitm_vert_adjoint_fields *gin) {
+  if (!h || !gprep || !gin) return GITM_VERT_EINVAL;
+  int rc = check_prep(p, c);
+  if (rc) return rc;
+  const gitm_vert_field *req[] = {&gprep->temp_unit, &gprep->gamma, &gprep->mean_major_mass,
+                                  &gprep->cmax, &gprep->eddy, &gin->ndensity_s,
+                                  &gin->vertical_velocity, &gin->temperature, &gin->rho};
+  for (const gitm_vert_field *f : req)
+    if (!f->ptr) return GITM_VERT_EINVAL;
+  if (gprep->cmax.first_level < -1 || gprep->cmax.first_level > 0) return GITM_VERT_EINVAL;
+  GV_HIP(hipSetDevice(h->device));
+  const gv::PrepParams pp = make_prep(p, c->n_alts);
+  const gitm_vert_field none = {};
+  const long long n = (long long)c->n_lons * c->n_lats * c->n_blocks * (c->n_alts + 4);
+  hipLaunchKernelGGL(prep_adjoint_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     h->stream, pp, *c, *gprep, gcp ? *gcp : none, *gin, n);
+  GV_HIP(hipGetLastError());
+  return GITM_VERT_OK;
 }
 
 }  // extern "C"

@@ -169,12 +169,6 @@ GV_AD ad fabs(ad a) { return mk(::fabs(a.v), a.i, a.v < 0.0 ? -1.0 : 1.0); }
 GV_FN double val(double x) { return x; }
 GV_FN double val(adm::ad x) { return x.v; }
 
-// Field descriptors of the state adjoints: the same layout as the state
-// fields of gitm_vert_columns.
-struct AdjFields {
-  gitm_vert_field nds, vvel, vel, temp, rho;
-};
-
 // Adjoint of one AUSM step of column `col` of io (no ions), forward part:
 // the step is recomputed from io's state with taping and the adjoints
 // `gout` of the step's outputs (NDensityS, VerticalVelocity, Velocity,

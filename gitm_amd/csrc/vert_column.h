@@ -40,6 +40,18 @@
 
 namespace gv {
 
+// Forward taps: a context whose kTaps is true receives a stage's
+// intermediates (the adjoint kernel records them for its reverse sweep);
+// for every other context the calls compile away.
+template <class C, class = void>
+struct ctx_taps {
+  static constexpr bool value = false;
+};
+template <class C>
+struct ctx_taps<C, decltype((void)C::kTaps)> {
+  static constexpr bool value = C::kTaps;
+};
+
 constexpr int NSP = 6;
 constexpr int NIONS = 10;
 constexpr double KB = 1.3807e-23;         // ModConstants.f90:15
@@ -995,6 +1007,7 @@ struct Column {
       const R gv3[3] = {T, U[F_VE], U[F_VN]};
       rusanov_grad_diff<3>(gv3, gr3, df3);
     }
+    if constexpr (ctx_taps<Ctx>::value) cx.tap_grads(glc, gr3[0], df3[1], df3[2]);
     if (IONS) ion_stage(dtin, stg, true);  // :817-829
 
     // --- hydrostatic background (:694-751): per-level factors in parallel,
@@ -1074,16 +1087,18 @@ struct Column {
     R dev[NSP];
     int sub = 0;  // bit s: SubtractHydrostatic(k, s)
     {
-      R lhps[NSP];
+      R lhps[NSP], hnsv[NSP];
 #pragma unroll
       for (int s = 0; s < NSP; s++) {
         const R hns = xb(8 + s, j);
+        hnsv[s] = hns;
         lhps[s] = log(hns * KB * T);
         const R hrhos = m[s] * hns;
         const R ratio = fabs(rhos[s] - hrhos) / rhos[s];
         dev[s] = rhos[s] - hrhos;
         if (!(fabs(ratio) > 1.0)) sub |= 1 << s;
       }
+      if constexpr (ctx_taps<Ctx>::value) cx.tap_hydro(hnsv, sub);
       cx.sync();
       // batch 2 of the face states goes first: ln p_hydro,s and w_s
 #pragma unroll
@@ -1151,6 +1166,8 @@ struct Column {
       GV_KT(13, VNL, VNR);
 #undef GV_KT
     }
+    if constexpr (ctx_taps<Ctx>::value)
+      cx.tap_faces(HL, HR, VL, VR, RSL, RSR, PSL, PSR, VEL, VER, VNL, VNR);
     cx.sync();
 
     // --- AUSM+-up fluxes of face k|k+1 (:1459-1970).  The P12 side belongs
@@ -1294,6 +1311,7 @@ struct Column {
       mfE = dq(ap * MOEp - am * xb(19, j - 1), vol, ivol);
       mfN = dq(ap * MONp - am * xb(20, j - 1), vol, ivol);
     }
+    if constexpr (ctx_taps<Ctx>::value) cx.tap_div(rsf, msf, ef, mfE, mfN);
     cx.sync();
 
     // --- update of cell k (:802-997); each new value is folded into the RK4
@@ -1345,6 +1363,7 @@ struct Column {
       ausm_commit(stg, F_VN, nvN);
       if (kp.friction && !friction(dtin, w, nnt, nns, glc, T))
         fail(GITM_VERT_ESINGULAR, k);
+      if constexpr (ctx_taps<Ctx>::value) cx.tap_friction(w);
       R nvu = 0.0;
 #pragma unroll
       for (int s = 0; s < NSP; s++) {
@@ -1520,6 +1539,12 @@ struct Column {
       bcs();
     }
   }
+};
+
+// Field descriptors of the state adjoints (the adjoint builds): the same
+// layout as the state fields of gitm_vert_columns.
+struct AdjFields {
+  gitm_vert_field nds, vvel, vel, temp, rho;
 };
 
 // ---------------------------------------------------------------------------

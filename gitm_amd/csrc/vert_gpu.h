@@ -172,6 +172,9 @@ struct gitm_vert_handle {
   // adjoint tape (grown on demand up to adj_budget bytes, reused)
   void *tape = nullptr;
   size_t tape_bytes = 0;
+  // the hand-written adjoint's record (likewise)
+  void *rec = nullptr;
+  size_t rec_bytes = 0;
   size_t adj_budget = 0;  // 0: default_adjoint_budget()
   int *d_overflow = nullptr;
 };

@@ -296,24 +296,71 @@ typedef struct gitm_vert_adjoint_fields {
   gitm_vert_field rho;               /* d/d Rho                              */
 } gitm_vert_adjoint_fields;
 
-/* One AUSM vertical step of every column of `c` (the step's input state,
- * device pointers, not modified) is recomputed with a taped number type, the
- * adjoints `gout` of the step's outputs (NDensityS, VerticalVelocity,
- * Velocity, Temperature and Rho as advance_vertical scatters them, levels
- * -1..nAlts+2) are propagated back, and `gin` receives the adjoints of the
- * step's inputs (the same fields before the step).  gin may be gout (in
- * place).  The column prep inputs (TempUnit, Gamma, MeanMajorMass, cMax,
- * eddy coefficient, geometry) are held fixed.  Neutrals only
- * (use_ion_advection must be 0), nAlts <= 124.  Synchronous: returns the
- * status (GITM_VERT_ETAPE if a column outgrew its tape). */
+/* Adjoints of the column-prep outputs that the vertical step reads
+ * (gitm_vert_prepare writes the values): TempUnit, Gamma, MeanMajorMass,
+ * cMax (levels 0..nAlts+1, described as c->cmax) and KappaEddyDiffusion. */
+typedef struct gitm_vert_adjoint_prep_fields {
+  gitm_vert_field temp_unit, gamma, mean_major_mass, cmax, eddy;
+} gitm_vert_adjoint_prep_fields;
+
+/* The adjoint of one AUSM vertical step of every column of `c` (the step's
+ * input state, device pointers, not modified): the adjoints `gout` of the
+ * step's outputs (NDensityS, VerticalVelocity, Velocity, Temperature and Rho
+ * as advance_vertical scatters them, levels -1..nAlts+2) are carried back,
+ * and `gin` receives the adjoints of the step's inputs (the same fields
+ * before the step).  gin may be gout (in place).  Hand-written: each column
+ * recomputes its step, recording every stage's intermediates in the
+ * handle's device memory, then reverses the stages' phases on the device
+ * (vertical_solver_ausm.f90:26-999,1093-2117, set_vertical_bcs.Earth.f90,
+ * calc_neutral_friction.f90:47-106 and the gather/scatter of
+ * advance_vertical.f90:84-185).  The column-prep inputs (TempUnit, Gamma,
+ * MeanMajorMass, cMax, eddy coefficient) and the geometry are held fixed;
+ * gitm_vert_adjoint_all also returns their adjoints.  Neutrals only
+ * (use_ion_advection and use_msis must be 0), nAlts <= 188.  Asynchronous
+ * on the handle's stream; errors as for gitm_vert_advance. */
 int gitm_vert_adjoint(gitm_vert_handle *h, const gitm_vert_params *p,
                       const gitm_vert_columns *c, const gitm_vert_adjoint_fields *gout,
                       const gitm_vert_adjoint_fields *gin);
 
-/* Device memory the adjoint's tape may use (0: the default, 40% of the
- * device's memory up to 96 GiB).  A column takes about 22 MB at nAlts = 50;
- * larger grids run in chunks of columns. */
+/* gitm_vert_adjoint, and `gprep` receives the adjoints of the step's
+ * column-prep inputs (overwritten, every level). */
+int gitm_vert_adjoint_all(gitm_vert_handle *h, const gitm_vert_params *p,
+                          const gitm_vert_columns *c, const gitm_vert_adjoint_fields *gout,
+                          const gitm_vert_adjoint_fields *gin,
+                          const gitm_vert_adjoint_prep_fields *gprep);
+
+/* The adjoint of gitm_vert_prepare (calc_pressure.f90:56-86,
+ * calc_rates.Earth.f90:34-50, Earth.f90:376-411, calc_timestep.f90:113-118):
+ * the adjoints of its outputs (`gprep`, and of cp in `gcp`, which may be
+ * NULL) are carried to NDensityS, Temperature, Rho and VerticalVelocity of
+ * `c`'s state and ADDED to `gin` (its velocity field is not used).  The time
+ * step (a MIN over columns) is held fixed.  Asynchronous. */
+int gitm_vert_prepare_adjoint(gitm_vert_handle *h, const gitm_vert_prep_params *p,
+                              const gitm_vert_columns *c,
+                              const gitm_vert_adjoint_prep_fields *gprep,
+                              const gitm_vert_field *gcp, const gitm_vert_adjoint_fields *gin);
+
+/* The same adjoint as gitm_vert_adjoint, taken by recomputing the step with
+ * a taped number type (every FP64 operation a node of a tape in device
+ * memory) and sweeping the tape back with atomic adds: slow (about 170x the
+ * forward step) and memory-hungry (about 22 MB per 54-level column), kept
+ * as an independent check of the hand-written adjoint.  nAlts <= 124.
+ * Synchronous: returns the status (GITM_VERT_ETAPE if a column outgrew its
+ * tape). */
+int gitm_vert_adjoint_taped(gitm_vert_handle *h, const gitm_vert_params *p,
+                            const gitm_vert_columns *c, const gitm_vert_adjoint_fields *gout,
+                            const gitm_vert_adjoint_fields *gin);
+
+/* Device memory the adjoints may use (0: the defaults; the hand-written
+ * adjoint's record: 10% of the device's memory up to 4 GiB, 271 KB per
+ * column at nAlts <= 60; the taped adjoint's tape: 40% up to 96 GiB).
+ * Larger grids run in chunks of columns.  A buffer already larger than the
+ * new budget is freed. */
 int gitm_vert_set_adjoint_memory(gitm_vert_handle *h, size_t bytes);
+
+/* Frees the adjoints' device buffers (they are otherwise kept, and reused,
+ * until gitm_vert_destroy). */
+int gitm_vert_release_adjoint_memory(gitm_vert_handle *h);
 
 /* ------------------------------------------------------------------------
  * The operator-split sources step's neutral terms (add_sources.f90:53-70,

@@ -14,7 +14,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 LIB = os.path.join(HERE, "libgv_emu.so")
 SRC = os.path.join(HERE, "emu_vert.cpp")
 DEPS = [SRC] + [os.path.join(ROOT, "gitm_amd", "csrc", f)
-                for f in ("vert_column.h", "vert_params.h", "vert_ad.h")] + \
+                for f in ("vert_column.h", "vert_params.h", "vert_ad.h", "vert_adj.h")] + \
+    [os.path.join(HERE, "count.h")] + \
     [os.path.join(ROOT, "include", "gitm_vert.h")]
 
 _lib = None
@@ -24,10 +25,14 @@ def build(force: bool = False) -> None:
     if (not force and os.path.exists(LIB) and
             all(os.path.getmtime(LIB) >= os.path.getmtime(d) for d in DEPS)):
         return
+    # built under a private name and renamed, so that concurrent test
+    # processes never load a half-written library
+    tmp = f"{LIB}.{os.getpid()}.tmp"
     subprocess.run(["g++", "-std=c++20", "-O2", "-fPIC", "-shared", "-ffp-contract=off",
                     "-pthread", "-I" + os.path.join(ROOT, "include"),
-                    "-I" + os.path.join(ROOT, "gitm_amd", "csrc"), SRC, "-o", LIB],
+                    "-I" + os.path.join(ROOT, "gitm_amd", "csrc"), SRC, "-o", tmp],
                    check=True)
+    os.replace(tmp, LIB)
 
 
 def lib():
@@ -37,6 +42,7 @@ def lib():
         _lib = ctypes.CDLL(LIB)
         _lib.gv_emu_advance.restype = ctypes.c_int
         _lib.gv_emu_adjoint.restype = ctypes.c_int
+        _lib.gv_emu_adjoint_hand.restype = ctypes.c_int
     return _lib
 
 
@@ -133,3 +139,57 @@ def adjoint(cols: dict, sp, gout: dict, cols_range=None, nmax: int = 1 << 15,
         raise RuntimeError(f"emulated adjoint: {_abi.ERRORS.get(rc, rc)} "
                            f"(column {e.column}, level {e.level})")
     return gi, nodes.value, phases.value
+
+
+class AdjFixFields(ctypes.Structure):
+    _fields_ = [(n, _abi.Field) for n in ("tempunit", "gamma", "mmm", "cmax", "eddy")]
+
+
+def adjoint_hand(cols: dict, sp, gout: dict, cols_range=None, device_math: bool = False,
+                 fixed: bool = False):
+    """The hand-written adjoint of one AUSM step (host build of vert_adj.h):
+    the adjoints of the step's inputs from those of its outputs (nds, vvel,
+    vel, temp, rho); with ``fixed`` also those of the column-prep inputs
+    (tempunit, gamma, mmm, cmax, eddy)."""
+    nc, L = cols["temp"].shape
+    keep = {k: np.ascontiguousarray(cols[k], dtype=np.float64)
+            for k in ("nds", "vvel", "vel", "temp", "rho", "tempunit", "mmm", "gamma",
+                      "cmax", "eddy", "alt", "grav", "invr", "dalt", "coslat")}
+    c = _abi.columns_struct({k: v.ctypes.data for k, v in keep.items()}, nc, L - 4)
+    go = {k: np.ascontiguousarray(gout[k], dtype=np.float64) for k, _ in ADJ}
+    gi = {k: np.zeros_like(go[k]) for k, _ in ADJ}
+    gf = {k: np.zeros((nc, L)) for k in ("tempunit", "gamma", "mmm", "cmax", "eddy")}
+    fx = AdjFixFields()
+    for k in gf:
+        setattr(fx, k, _abi.column_field(gf[k].ctypes.data, 1, L))
+    p = params_from(sp)
+    e = _abi.Error()
+    c0, c1 = cols_range if cols_range else (0, nc)
+    rc = lib().gv_emu_adjoint_hand(ctypes.byref(p), ctypes.byref(c),
+                                   ctypes.byref(adj_fields(go, L)), ctypes.byref(adj_fields(gi, L)),
+                                   ctypes.byref(fx) if fixed else None, c0, c1,
+                                   int(device_math), ctypes.byref(e))
+    if rc:
+        raise RuntimeError(f"emulated adjoint: {_abi.ERRORS.get(rc, rc)} "
+                           f"(column {e.column}, level {e.level})")
+    return (gi, gf) if fixed else gi
+
+
+OP_CLASSES = ("add", "mul", "div", "sqrt", "trans", "cmp")
+
+
+def count_ops(cols: dict, sp, solver: int = 1, col: int = 0, device_math: bool = True) -> dict:
+    """FP64 operations of one step of column ``col`` (the column code over a
+    counting number type): counts per class, summed over the lanes."""
+    nc, L = cols["temp"].shape
+    keep = {k: np.ascontiguousarray(cols[k], dtype=np.float64)
+            for k in ("nds", "vvel", "vel", "temp", "rho", "tempunit", "mmm", "gamma",
+                      "cmax", "eddy", "alt", "grav", "invr", "dalt", "coslat")}
+    c = _abi.columns_struct({k: v.ctypes.data for k, v in keep.items()}, nc, L - 4)
+    p = params_from(sp)
+    out = (ctypes.c_longlong * 6)()
+    rc = lib().gv_emu_count_ops(ctypes.byref(p), ctypes.byref(c), solver, col,
+                                int(device_math), out)
+    if rc:
+        raise RuntimeError(f"operation count: {_abi.ERRORS.get(rc, rc)}")
+    return dict(zip(OP_CLASSES, list(out)))

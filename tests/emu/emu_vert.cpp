@@ -12,7 +12,9 @@
 #include <thread>
 #include <vector>
 
+#include "count.h"
 #include "vert_ad.h"
+#include "vert_adj.h"
 #include "vert_params.h"
 
 namespace {
@@ -229,6 +231,161 @@ extern "C" int gv_emu_adjoint(const gitm_vert_params *p, const gitm_vert_columns
   gv::decode_error(sh.err.load(), e);
   if (overflow && e->code == GITM_VERT_OK) e->code = GITM_VERT_ENOMEM;
   return e->code;
+}
+
+// The hand-written adjoint (vert_adj.h) of one AUSM step of columns c0..c1-1,
+// lanes as CPU threads: gout holds the adjoints of the step's outputs, gin
+// receives those of its inputs, gfix (may be NULL) those of the column-prep
+// inputs.  device_math as for gv_emu_advance.
+extern "C" int gv_emu_adjoint_hand(const gitm_vert_params *p, const gitm_vert_columns *c,
+                                   const gv::AdjFields *gout, const gv::AdjFields *gin,
+                                   const gv::AdjFixFields *gfix, int c0, int c1,
+                                   int device_math, gitm_vert_error *e) {
+  int rc = gv::check_args(p, c, GITM_VERT_AUSM, 1);
+  if (rc) return rc;
+  if (p->use_ion_advection || p->use_msis) return GITM_VERT_EUNSUPPORTED;
+  gv::KParams kp = gv::make_kparams(p, c, GITM_VERT_AUSM, 1, nullptr);
+  const int waves = gv::auto_waves(c->n_alts);
+  if (waves > 3) return GITM_VERT_EUNSUPPORTED;
+  kp.scr_fields = 0;
+  const int L = c->n_alts + 4;
+  Shared sh(L, device_math != 0);
+  std::vector<double> lds(gv::adj_lds_bytes(c->n_alts, true) / sizeof(double), 0.0);
+  std::vector<double> rec((size_t)gv::adj_column_doubles(64 * waves), 0.0);
+  std::vector<std::thread> th;
+  for (int t = 0; t < L; t++)
+    th.emplace_back([&, t] {
+      for (int col = c0; col < c1; col++) {
+        gv::TapCtx<EmuCtx> cx;
+        cx.lane_ = t;
+        cx.sh = &sh;
+        switch (waves) {
+          case 1:
+            gv::adj_column<64>(cx, kp, *c, *gout, *gin, gfix, col, lds.data(), rec.data(), col);
+            break;
+          case 2:
+            gv::adj_column<128>(cx, kp, *c, *gout, *gin, gfix, col, lds.data(), rec.data(), col);
+            break;
+          default:
+            gv::adj_column<192>(cx, kp, *c, *gout, *gin, gfix, col, lds.data(), rec.data(), col);
+            break;
+        }
+        sh.bar.arrive_and_wait();
+      }
+    });
+  for (auto &x : th) x.join();
+  gitm_vert_error tmp;
+  if (!e) e = &tmp;
+  gv::decode_error(sh.err.load(), e);
+  return e->code;
+}
+
+// The operation count of the column step: the column code run over the
+// counting number type (count.h), lanes as CPU threads.
+struct EmuCntCtx {
+  using real = cnt::fd;
+  int lane_;
+  Shared *sh;
+  int lane() const { return lane_; }
+  void sync() { sh->bar.arrive_and_wait(); }
+  bool sync_or(bool p) {
+    if (p) sh->orflag.store(1);
+    sync();
+    bool r = sh->orflag.load() != 0;
+    sync();
+    if (lane_ == 0) sh->orflag.store(0);
+    sync();
+    return r;
+  }
+  void report(unsigned long long) {}
+  bool device_math() const { return sh->device_math; }
+  void diff_powers(real temp, double sign, real (&out)[6]) const {
+    if (sh->device_math) {
+      const real lt = log(temp);
+      for (int u = 0; u < 6; u++) out[u] = exp((sign * gv::DEXU[u]) * lt);
+    } else {
+      for (int u = 0; u < 6; u++) out[u] = pow(temp, sign * gv::DEXU[u]);
+    }
+  }
+};
+
+// One column's AUSM (solver 1) or Rusanov (0) step with gather and scatter
+// as run_column does them; out[6] = the summed counts (add, mul, div, sqrt,
+// trans, cmp) of all lanes, lanes_active[0] = the number of lanes.
+extern "C" int gv_emu_count_ops(const gitm_vert_params *p, const gitm_vert_columns *c,
+                                int solver, int col, int device_math, long long *out) {
+  int rc = gv::check_args(p, c, solver, 1);
+  if (rc) return rc;
+  if (p->use_ion_advection || p->use_msis) return GITM_VERT_EUNSUPPORTED;
+  gv::KParams kp = gv::make_kparams(p, c, solver, 1, nullptr);
+  const int waves = gv::auto_waves(c->n_alts);
+  if (waves > 3) return GITM_VERT_EUNSUPPORTED;
+  kp.scr_fields = 0;
+  const int L = c->n_alts + 4;
+  Shared sh(L, device_math != 0);
+  std::vector<cnt::fd> lds(gv::lds_doubles(c->n_alts, false, true));
+  std::vector<cnt::Counts> per(L);
+  std::vector<std::thread> th;
+  for (int t = 0; t < L; t++)
+    th.emplace_back([&, t] {
+      cnt::tally = cnt::Counts();
+      EmuCntCtx cx{t, &sh};
+      auto run = [&](auto nt) {
+        constexpr int NT = decltype(nt)::value;
+        using cnt::fd;
+        gv::Column<NT, EmuCntCtx, false> cl(cx, kp, lds.data(), col, fd(kp.dt));
+        cl.scr = nullptr;
+        const int nlb = c->n_lons * c->n_lats;
+        const int ci = col % c->n_lons, cj = (col / c->n_lons) % c->n_lats, cb = col / nlb;
+        const int k = cl.k, nA = cl.nA;
+        auto ld = [&](const gitm_vert_field &f, int comp) {
+          return *gv::fld(f, gv::col_base(f, ci, cj, cb), k, comp);
+        };
+        const double coslat = c->cos_latitude.ptr[gv::col_base(c->cos_latitude, ci, cj, cb)];
+        const fd tu = ld(c->temp_unit, 0);
+        cl.setup(ld(c->altitude, 0), ld(c->gravity, 0), ld(c->inv_radial_distance, 0),
+                 ld(c->dalt, 0), ld(c->mean_major_mass, 0), ld(c->gamma, 0),
+                 (k >= 1 && k <= nA) ? ld(c->eddy, 0) : 0.0,
+                 (k >= 0 && k <= nA + 1) ? ld(c->cmax, 0) : 0.0, tu, coslat);
+        // gather (advance_vertical.f90:105-118)
+        for (int s = 0; s < gv::NSP; s++) {
+          cl.U[gv::F_LN + s] = log(fd(ld(c->ndensity_s, s)));
+          cl.U[gv::F_W + s] = ld(c->vertical_velocity, s);
+        }
+        for (int d = 0; d < 3; d++) cl.U[gv::F_VE + d] = ld(c->velocity, d);
+        cl.U[gv::F_T] = fd(ld(c->temperature, 0)) * tu;
+        cl.U[gv::F_LR] = log(fd(ld(c->rho, 0)));
+        if (solver == GITM_VERT_AUSM)
+          cl.ausm_step();
+        else
+          cl.rusanov_step();
+        // scatter (:163-185)
+        fd nsum = 0.0;
+        for (int s = 0; s < gv::NSP; s++) nsum = nsum + exp(cl.U[gv::F_LN + s]);
+        fd r = exp(cl.U[gv::F_LR]);
+        fd t2 = cl.U[gv::F_T] / tu;
+        (void)r;
+        (void)t2;
+      };
+      switch (waves) {
+        case 1: run(std::integral_constant<int, 64>()); break;
+        case 2: run(std::integral_constant<int, 128>()); break;
+        default: run(std::integral_constant<int, 192>()); break;
+      }
+      per[t] = cnt::tally;
+    });
+  for (auto &x : th) x.join();
+  long long sum[6] = {0, 0, 0, 0, 0, 0};
+  for (const cnt::Counts &q : per) {
+    sum[0] += q.add;
+    sum[1] += q.mul;
+    sum[2] += q.div;
+    sum[3] += q.sqrt;
+    sum[4] += q.trans;
+    sum[5] += q.cmp;
+  }
+  for (int i = 0; i < 6; i++) out[i] = sum[i];
+  return GITM_VERT_OK;
 }
 
 // device_math != 0 evaluates as the GPU does (friction powers as exp(e ln T),

@@ -1,7 +1,9 @@
 """The adjoint of the AUSM step on the CPU: the host build of the taped
 column code (gitm_amd/csrc/vert_ad.h through tests/emu) against central
 finite differences of the oracle, which is an independent restatement of the
-reference step.
+reference step; and the host build of the hand-written adjoint
+(gitm_amd/csrc/vert_adj.h) against the taped one, its prep inputs'
+adjoints against central differences of the oracle.
 
 A loss J = sum of w . (state after one step) is differentiated with respect
 to single input cells.  Central differences of the oracle at two step sizes
@@ -101,3 +103,81 @@ def test_level_weights_and_heating_profile():
     assert w.sum() == pytest.approx(1.0)
     q = heating_profile(np.array([150e3, 170e3]))
     assert q[0] == pytest.approx(1e-5) and q[1] == pytest.approx(1e-5 * np.exp(-1.0))
+
+
+# ---------------------------------------------------------------------------
+# The hand-written adjoint (gitm_amd/csrc/vert_adj.h), host build.
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("device_math", [False, True])
+@pytest.mark.parametrize("n_alts,seed", [(50, 3), (20, 4), (100, 5)])
+def test_hand_adjoint_matches_taped_build(n_alts, seed, device_math):
+    """The hand-written adjoint against the taped one (both host builds) for
+    a random seed on every output: 1e-12 normwise per field.  100 levels use
+    two wavefronts; the device-math forms (log-space sums, friction powers as
+    exp(e ln T)) give the same derivatives to rounding."""
+    from tests.emu import emu
+    cols = synth.make_columns(synth.GridSpec(1, 1, n_alts), seed=seed)
+    cols = subset(cols, [0, 40] if n_alts != 100 else [7])
+    p = synth.StepParams()
+    rng = np.random.default_rng(seed)
+    gout = {k: rng.standard_normal(cols[k].shape) / np.max(np.abs(cols[k])) for k in STATE}
+    ref, _, _ = emu.adjoint(cols, p, gout)
+    hand = emu.adjoint_hand(cols, p, gout, device_math=device_math)
+    for k in STATE:
+        err = np.max(np.abs(hand[k] - ref[k])) / np.max(np.abs(ref[k]))
+        assert err < 1e-12, (k, err)
+
+
+@pytest.mark.parametrize("flags", [dict(use_neutral_friction=False),
+                                   dict(use_coriolis=False),
+                                   dict(dt=5.0)])
+def test_hand_adjoint_options(flags):
+    from tests.emu import emu
+    cols = subset(synth.make_columns(synth.GridSpec(1, 1, 30), seed=9), [3])
+    p = synth.StepParams(**flags)
+    rng = np.random.default_rng(2)
+    gout = {k: rng.standard_normal(cols[k].shape) / np.max(np.abs(cols[k])) for k in STATE}
+    ref, _, _ = emu.adjoint(cols, p, gout)
+    hand = emu.adjoint_hand(cols, p, gout)
+    for k in STATE:
+        err = np.max(np.abs(hand[k] - ref[k])) / np.max(np.abs(ref[k]))
+        assert err < 1e-12, (flags, k, err)
+
+
+@pytest.fixture(scope="module")
+def prep_case():
+    from tests.emu import emu
+    cols = subset(synth.make_columns(synth.CONFIGS["cpu_324x50"], seed=3), [0])
+    p = synth.StepParams()
+    rng = np.random.default_rng(2)
+    w = {k: rng.standard_normal(cols[k].shape) / np.max(np.abs(cols[k])) for k in STATE}
+    _, gf = emu.adjoint_hand(cols, p, w, fixed=True)
+    return cols, p, w, gf
+
+
+# (field, level): every prep input at levels where it enters -- the bottom
+# ghost levels (MeanMajorMass of the level -1 fill and the hydrostatic
+# background; Gamma's face values), interior levels and the top
+@pytest.mark.parametrize("field,lev", [("tempunit", 1), ("tempunit", 30), ("tempunit", 53),
+                                       ("gamma", 0), ("gamma", 2), ("gamma", 25),
+                                       ("gamma", 52), ("mmm", 0), ("mmm", 1), ("cmax", 2),
+                                       ("cmax", 3), ("cmax", 20), ("eddy", 2), ("eddy", 20)])
+def test_prep_input_adjoints_match_oracle_differences(prep_case, field, lev):
+    """The adjoints of the column-prep inputs (TempUnit, Gamma, MeanMajorMass,
+    cMax, the eddy coefficient) against central differences of the oracle
+    step.  Zero inputs (the eddy coefficient above its pressure ramp) are
+    stepped by an absolute h."""
+    cols, p, w, gf = prep_case
+    g = gf[field][0, lev]
+    x0 = cols[field][0, lev]
+    f = {}
+    for rel in (1e-6, 1e-5):
+        h = rel * abs(x0) if x0 != 0 else 1e3 * rel
+        a = {k: np.array(v) for k, v in cols.items()}
+        a[field][0, lev] = x0 + h
+        b = {k: np.array(v) for k, v in cols.items()}
+        b[field][0, lev] = x0 - h
+        f[rel] = (loss(a, p, w) - loss(b, p, w)) / (2 * h)
+    err = abs(g - f[1e-6]) / abs(g)
+    bound = max(1e-6, 2.0 * abs(f[1e-6] - f[1e-5]) / abs(g))
+    assert err <= bound, (field, lev, g, f, err, bound)
