@@ -49,7 +49,10 @@ enum AdjRecField {
   RC_RSF, RC_MSF = RC_RSF + NSP,  // flux divergences, 6 each
   RC_EF = RC_MSF + NSP, RC_MFE, RC_MFN,
   RC_WFR,                    // species winds after the friction solve, 6
-  NREC = RC_WFR + NSP
+  RC_KT = RC_WFR + NSP,      // the limiter choices of the cell's face-state slopes
+  RC_NVP,                    // numerical velocities of face k|k+1, P12 side, 6
+  RC_NVM = RC_NVP + NSP,     // and M12 side, 6
+  NREC = RC_NVM + NSP
 };
 // Records per column: the four stages and the step's final state.
 constexpr int ADJ_STAGES = 5;
@@ -61,9 +64,13 @@ enum AdjWork {
   WK_VEL = WK_PSR + NSP, WK_VER, WK_VNL, WK_VNR, WK_GL, WK_GR,
   NWORK
 };
-// Doubles of one column's record and work area, lanes of NT.
+// The boundary conditions' results of each of the step's five passes (the
+// scalars SC_BOT..SC_TOP+24 of the column's LDS), per column.
+constexpr int ADJ_BC = 60;
+// Doubles of one column's record, work area and boundary-condition record,
+// lanes of NT.
 GV_HD constexpr long adj_column_doubles(int nt) {
-  return ((long)ADJ_STAGES * NREC + NWORK) * nt;
+  return ((long)ADJ_STAGES * NREC + NWORK) * nt + (long)ADJ_STAGES * ADJ_BC;
 }
 
 // Adjoints of the column-prep inputs, per level (gfix of the kernel).
@@ -72,13 +79,13 @@ enum AdjFix { FX_TU = 0, FX_GAMMA, FX_MMM, FX_CMAX, FX_EDDY, NFIX };
 // Extra LDS of the adjoint kernel beyond the forward column's: the boundary
 // conditions' adjoint hand-offs.
 // Layout: [0, 36) bottom Rb adjoints, [36, 60) top ones, [60, 84) scalars,
-// [84, 156) bottom species contributions, [156, 180) top ones, [192, 448)
-// the column's cMax by lane, [448, 704) its Gamma.
-constexpr int ADJ_LDS_EXTRA = 704;
-constexpr int AX_CMAX = 192;
-constexpr int AX_GAM = 448;  // the column's Gamma by lane
+// [84, 156) bottom species contributions, [156, 180) top ones, then the
+// column's cMax and Gamma by lane (nAlts+4 each).  At nAlts = 50 the
+// column's LDS is 39.8 KiB, four columns per CU.
+constexpr int AX_CMAX = 180;
+GV_HD constexpr int adj_lds_extra(int n_alts) { return AX_CMAX + 2 * (n_alts + 4); }
 GV_HD constexpr unsigned long adj_lds_bytes(int n_alts, bool carry_lds) {
-  return (lds_doubles(n_alts, false, carry_lds) + ADJ_LDS_EXTRA) * sizeof(double);
+  return (lds_doubles(n_alts, false, carry_lds) + adj_lds_extra(n_alts)) * sizeof(double);
 }
 
 // The forward context with taps: Base is the plain kernel context.
@@ -88,6 +95,7 @@ struct TapCtx : Base {
   double *rec = nullptr;  // this column's record
   int nt = 64;            // lanes of the record's rows
   int stage = 0;
+  int gamma_code = 0;     // the limiter choice of Gamma's face-value slope
   GV_MEM double &r(int f) { return rec[((long)stage * NREC + f) * nt + this->lane()]; }
   GV_MEM void tap_grads(const double (&glc)[NSP], double gt, double dfe, double dfn) {
 #pragma unroll
@@ -101,11 +109,22 @@ struct TapCtx : Base {
     for (int s = 0; s < NSP; s++) r(RC_HNS + s) = hns[s];
     r(RC_SUB) = (double)sub;
   }
+  GV_MEM void tap_gamma_code(int code) { gamma_code = code; }
+  GV_MEM void tap_nv(const double (&nvp)[NSP], const double (&nvm)[NSP]) {
+#pragma unroll
+    for (int s = 0; s < NSP; s++) {
+      r(RC_NVP + s) = nvp[s];
+      r(RC_NVM + s) = nvm[s];
+    }
+  }
   GV_MEM void tap_faces(const double (&HL)[NSP], const double (&HR)[NSP],
                         const double (&VL)[NSP], const double (&VR)[NSP],
                         const double (&RSL)[NSP], const double (&RSR)[NSP],
                         const double (&PSL)[NSP], const double (&PSR)[NSP], double VEL,
-                        double VER, double VNL, double VNR) {
+                        double VER, double VNL, double VNR, unsigned long long kt) {
+    // 2 bits per field (27 fields: face states 0..25, Gamma 26)
+    kt |= (unsigned long long)gamma_code << 52;
+    r(RC_KT) = __builtin_bit_cast(double, kt);
 #pragma unroll
     for (int s = 0; s < NSP; s++) {
       r(RC_HL + s) = HL[s];
@@ -184,13 +203,15 @@ struct AdjColumn {
   Ctx &cx;
   const KParams &kp;
   double *rec;   // this column's record
-  double *ax;    // the extra LDS (ADJ_LDS_EXTRA doubles)
+  const double *bcrec;  // its boundary-condition record
+  double *ax;    // the extra LDS (adj_lds_extra(nAlts) doubles)
   const int j, k, nA, L;
   double lam[NF];
   double fix[NFIX];  // adjoints of this level's column-prep inputs
 
   GV_MEM AdjColumn(Col &col, Ctx &ctx, double *record, double *extra)
-      : c(col), cx(ctx), kp(col.kp), rec(record), ax(extra), j(col.j), k(col.k),
+      : c(col), cx(ctx), kp(col.kp), rec(record),
+        bcrec(record + ((long)ADJ_STAGES * NREC + NWORK) * NT), ax(extra), j(col.j), k(col.k),
         nA(col.nA), L(col.L) {
 #pragma unroll
     for (int f = 0; f < NF; f++) lam[f] = 0.0;
@@ -201,17 +222,19 @@ struct AdjColumn {
   GV_MEM double R(int s, int f) const { return rec[((long)s * NREC + f) * NT + j]; }
   // the work area of this lane
   GV_MEM double &W(int f) const { return rec[((long)ADJ_STAGES * NREC + f) * NT + j]; }
+  // work slots of face-state field f (R4's numbering): its left state, and
+  // the offset of the right state from it
+  GV_MEM static int wk_left(int f) {
+    return f < 6 ? WK_HL + f : f < 12 ? WK_VL + f - 6 : f < 18 ? WK_RSL + f - 12
+         : f < 24 ? WK_PSL + f - 18 : f == 24 ? WK_VEL : f == 25 ? WK_VNL : WK_GL;
+  }
+  GV_MEM static int wk_right_offset(int f) { return f < 24 ? NSP : 1; }
   GV_MEM double &xb(int b, int jj) const { return c.xb(b, jj); }
   GV_MEM double st(int a, int jj) const { return c.st(a, jj); }
   GV_MEM bool cell() const { return k >= 1 && k <= nA; }
   GV_MEM bool face() const { return k >= 0 && k <= nA; }
   // a lane in 0..L-1 (the workgroup has exactly L lanes)
   GV_MEM bool in(int jj) const { return jj >= 0 && jj < L; }
-
-  GV_MEM void load_state(int s) {
-#pragma unroll
-    for (int f = 0; f < NF; f++) c.U[f] = R(s, RC_U + f);
-  }
 
   // -------------------------------------------------------------------------
   // set_vertical_bcs (set_vertical_bcs.Earth.f90:8-652, UseMSIS=F), reversed.
@@ -224,26 +247,29 @@ struct AdjColumn {
   // conditions; the ghost values are recomputed here).
   // -------------------------------------------------------------------------
   GV_MEM void bcs_rev(int s) {
-    load_state(s);
-    c.bcs();  // recompute the ghost values (sc(SC_BOT..), sc(SC_TOP..)); ends on a barrier
     const double *m = kp.mass;
     const int jn = nA + 1;  // lane of level nAlts
-    // values the species lanes read (staged by bcs in xb 0..15), before
-    // the buffers are reused
+    // the pass's results: Rb of the bottom, 6 per species, then the top's,
+    // 4 per species (the forward recorded them)
+    const double *bc = bcrec + s * ADJ_BC;
+    // the values the species lanes read, from the record of state s: its
+    // interior levels are the state before the boundary conditions, and
+    // Temp(0) and the non-photochemical LogNS(0) are never changed by them
+    auto Us = [&](int f, int lane) { return rec[((long)s * NREC + RC_U + f) * NT + lane]; };
     double lnb[6], wb[6], t0 = 0.0;
     double tn = 0.0, wn = 0.0, wnm1 = 0.0;
     if (j < NSP) {
 #pragma unroll
       for (int q = 0; q < 6; q++) {
-        lnb[q] = xb(j, 1 + q);            // LogNS of levels 0..5
-        wb[q] = xb(F_W - F_LN + j, 1 + q);  // VertVel of levels 0..5
+        lnb[q] = Us(F_LN + j, 1 + q);  // LogNS of levels 0..5
+        wb[q] = Us(F_W + j, 1 + q);    // VertVel of levels 0..5
       }
-      t0 = xb(12, 1);
+      t0 = Us(F_T, 1);
     } else if (j >= 8 && j < 8 + NSP) {
       const int sp = j - 8;
-      tn = xb(12, jn);
-      wn = xb(F_W - F_LN + sp, jn);
-      wnm1 = xb(F_W - F_LN + sp, jn - 1);
+      tn = Us(F_T, jn);
+      wn = Us(F_W + sp, jn);
+      wnm1 = Us(F_W + sp, jn - 1);
     }
     // hand-off area: bottom [sp*6 + 0..5] adjoints of Rb (n0, nm1, w0, wm1,
     // w0r, wm1r); top [36 + sp*4 + 0..3] of (l1, l2, w1, w2); [60..63]
@@ -258,7 +284,7 @@ struct AdjColumn {
       double ns[NSP], sr = 0.0;
 #pragma unroll
       for (int sp = 0; sp < NSP; sp++) {
-        ns[sp] = exp(c.sc(SC_BOT + 6 * sp + o));
+        ns[sp] = exp(bc[6 * sp + o]);
         sr = sr + m[sp] * ns[sp];
       }
       const double lr = log(sr);
@@ -266,13 +292,13 @@ struct AdjColumn {
       double vu = 0.0;
 #pragma unroll
       for (int sp = 0; sp < NSP; sp++)
-        vu = vu + ns[sp] * m[sp] * c.sc(SC_BOT + 6 * sp + 2 + o) / er;
+        vu = vu + ns[sp] * m[sp] * bc[6 * sp + 2 + o] / er;
       const double lvu = lam[F_VU];
       const double ler = -lvu * vu / er;
       const double lsr = (lam[F_LR] + ler * er) / sr;
 #pragma unroll
       for (int sp = 0; sp < NSP; sp++) {
-        const double w = c.sc(SC_BOT + 6 * sp + 2 + o);
+        const double w = bc[6 * sp + 2 + o];
         const double lns = lsr * m[sp] + lvu * m[sp] * w / er;
         AB[sp * 6 + o] = lam[F_LN + sp] + lns * ns[sp];
         AB[sp * 6 + 2 + o] = lvu * ns[sp] * m[sp] / er;
@@ -289,7 +315,7 @@ struct AdjColumn {
       double sr = 0.0, e[NSP];
 #pragma unroll
       for (int sp = 0; sp < NSP; sp++) {
-        e[sp] = exp(c.sc(SC_TOP + 4 * sp + o));
+        e[sp] = exp(bc[36 + 4 * sp + o]);
         sr = sr + m[sp] * e[sp];
       }
 #pragma unroll
@@ -433,7 +459,7 @@ struct AdjColumn {
       lam[F_VE] += AS[6] + AS[7];
       lam[F_VN] += AS[8] + AS[9];
       // Vel(up) of the top ghosts = |Vel(up, nAlts)|
-      const double sgn = (c.U[F_VU] < 0.0) ? -1.0 : 1.0;
+      const double sgn = (Us(F_VU, jn) < 0.0) ? -1.0 : 1.0;
       lam[F_VU] += sgn * (AS[10] + AS[11]);
     }
     if (j == jn - 1) {
@@ -681,107 +707,76 @@ struct AdjColumn {
     cx.sync();
 
     // --- R4: face states (calc_kt_facevalues, :2073-2117) ---------------
-    // 27 fields in batches of 5 (4 exchange slots each): ln p_hydro,s, w_s,
-    // ln rho_s, ln p_s, Vel(E), Vel(N) and Gamma (its face values, :1535,
-    // made once per column by the setup)
+    // 27 fields: ln p_hydro,s, w_s, ln rho_s, ln p_s, Vel(E), Vel(N) and
+    // Gamma (its face values, :1535, made once per column by the setup).
+    // Cell j's limited slope d(j) enters the left state of face j (+d dF/2)
+    // and the right state of face j-1 (-d dF/2); d = MC(up, dn) with up =
+    // (f(j+1)-f(j))/dF(j+1), dn = (f(j)-f(j-1))/dF(j), on the branch the
+    // forward recorded (RC_KT).  Batches of 7 fields, 3 exchange slots each.
     double hns[NSP];
 #pragma unroll
     for (int q = 0; q < NSP; q++) hns[q] = R(s, RC_HNS + q);
-    const double ltk = log(T * KB);
-    for (int b0 = 0; b0 < 27; b0 += 5) {
-      double fv[5], lo[5], lr[5], lf[5];
-#pragma unroll
-      for (int i = 0; i < 5; i++) {
-        const int f = b0 + i;
-        fv[i] = 0.0;
-        lo[i] = 0.0;
-        lr[i] = 0.0;
-        lf[i] = 0.0;
-        if (f >= 27) continue;
-        int wl, wr;
-        if (f < 6) {
-          fv[i] = log(hns[f] * KB * T);
-          wl = WK_HL + f;
-          wr = WK_HR + f;
-        } else if (f < 12) {
-          fv[i] = U[F_W + f - 6];
-          wl = WK_VL + f - 6;
-          wr = WK_VR + f - 6;
-        } else if (f < 18) {
-          const int q = f - 12;
-          fv[i] = cx.device_math() ? kp.lmass[q] + U[F_LN + q] : log(rhos[q]);
-          wl = WK_RSL + q;
-          wr = WK_RSR + q;
-        } else if (f < 24) {
-          const int q = f - 18;
-          fv[i] = cx.device_math() ? U[F_LN + q] + ltk : log(NS[q] * T * KB);
-          wl = WK_PSL + q;
-          wr = WK_PSR + q;
-        } else if (f == 24) {
-          fv[i] = U[F_VE];
-          wl = WK_VEL;
-          wr = WK_VER;
-        } else if (f == 25) {
-          fv[i] = U[F_VN];
-          wl = WK_VNL;
-          wr = WK_VNR;
-        } else {
-          fv[i] = ax[AX_GAM + j];
-          wl = WK_GL;
-          wr = WK_GR;
-        }
-        if (face()) {
-          lo[i] = W(wl);
-          lr[i] = W(wr);
-        }
-        xb(4 * i, j) = fv[i];
-        xb(4 * i + 1, j) = lr[i];
-      }
-      cx.sync();
+    {
+      const unsigned long long kt = __builtin_bit_cast(unsigned long long, R(s, RC_KT));
       const bool act = j >= 1 && j <= nA + 2;
-      const double iF0 = st(A_IF, j), iF1 = st(A_IF, j + 1), dF0 = st(A_DF, j);
+      const double iF0 = st(A_IF, j), iF1 = st(A_IF, j + 1), hdF0 = 0.5 * st(A_DF, j);
+      for (int b0 = 0; b0 < 27; b0 += 7) {
+        double lo[7], lf[7];
 #pragma unroll
-      for (int i = 0; i < 5; i++) {
-        double a = 0.0, b = 0.0;
-        if (act && b0 + i < 27) {
-          const double lrp = xb(4 * i + 1, j - 1);  // adjoint of face j-1's right state
-          const double lOL = lo[i];
-          const double ld = 0.5 * dF0 * (lOL - lrp);
-          lf[i] += lOL + lrp;
-          const double up = (xb(4 * i, j + 1) - fv[i]) * iF1;
-          const double dn = (fv[i] - xb(4 * i, j - 1)) * iF0;
-          double lup, ldn;
-          limiter_mc_rev(kp.beta, up, dn, ld, lup, ldn);
-          a = lup * iF1;
-          b = ldn * iF0;
-          lf[i] += b - a;
+        for (int i = 0; i < 7; i++) {
+          const int f = b0 + i;
+          lo[i] = 0.0;
+          lf[i] = 0.0;
+          double lr = 0.0;
+          if (f < 27 && face()) {
+            lo[i] = W(wk_left(f));
+            lr = W(wk_left(f) + wk_right_offset(f));
+          }
+          xb(3 * i, j) = lr;
         }
-        xb(4 * i + 2, j) = a;
-        xb(4 * i + 3, j) = b;
-      }
-      cx.sync();
+        cx.sync();
 #pragma unroll
-      for (int i = 0; i < 5; i++) {
-        const int f = b0 + i;
-        if (f >= 27) continue;
-        if (j - 1 >= 1 && j - 1 <= nA + 2) lf[i] += xb(4 * i + 2, j - 1);
-        if (j + 1 <= nA + 2) lf[i] -= xb(4 * i + 3, j + 1);
-        if (f < 6) {
-          lhns[f] += lf[i] / hns[f];
-          lam[F_T] += lf[i] / T;
-        } else if (f < 12) {
-          lam[F_W + f - 6] += lf[i];
-        } else if (f < 18) {
-          lam[F_LN + f - 12] += lf[i];
-        } else if (f < 24) {
-          lam[F_LN + f - 18] += lf[i];
-          lam[F_T] += lf[i] / T;
-        } else if (f == 24) {
-          lam[F_VE] += lf[i];
-        } else if (f == 25) {
-          lam[F_VN] += lf[i];
-        } else {
-          fix[FX_GAMMA] += lf[i];
+        for (int i = 0; i < 7; i++) {
+          const int f = b0 + i;
+          double a = 0.0, b = 0.0;
+          if (act && f < 27) {
+            const double lrp = xb(3 * i, j - 1);  // adjoint of face j-1's right state
+            const double ld = hdF0 * (lo[i] - lrp);
+            lf[i] += lo[i] + lrp;
+            const int code = (int)((kt >> (2 * f)) & 3);
+            const double lup = code == 1 ? kp.beta * ld : (code == 3 ? 0.5 * ld : 0.0);
+            const double ldn = code == 2 ? kp.beta * ld : (code == 3 ? 0.5 * ld : 0.0);
+            a = lup * iF1;
+            b = ldn * iF0;
+            lf[i] += b - a;
+          }
+          xb(3 * i + 1, j) = a;
+          xb(3 * i + 2, j) = b;
+        }
+        cx.sync();
+#pragma unroll
+        for (int i = 0; i < 7; i++) {
+          const int f = b0 + i;
+          if (f >= 27) continue;
+          if (j - 1 >= 1 && j - 1 <= nA + 2) lf[i] += xb(3 * i + 1, j - 1);
+          if (j + 1 <= nA + 2) lf[i] -= xb(3 * i + 2, j + 1);
+          if (f < 6) {
+            lhns[f] += lf[i] / hns[f];  // ln(hns KB T)
+            lam[F_T] += lf[i] / T;
+          } else if (f < 12) {
+            lam[F_W + f - 6] += lf[i];
+          } else if (f < 18) {
+            lam[F_LN + f - 12] += lf[i];  // ln(m ns)
+          } else if (f < 24) {
+            lam[F_LN + f - 18] += lf[i];  // ln(ns KB T)
+            lam[F_T] += lf[i] / T;
+          } else if (f == 24) {
+            lam[F_VE] += lf[i];
+          } else if (f == 25) {
+            lam[F_VN] += lf[i];
+          } else {
+            fix[FX_GAMMA] += lf[i];
+          }
         }
       }
       cx.sync();
@@ -882,15 +877,13 @@ struct AdjColumn {
     const double iRLR = 1.0 / RLR, icS = 1.0 / CS;
     const int sub = (int)R(s, RC_SUB);
     const int subUp = sub_of(s, j + 1);
-    // the bulk numerical velocities need every species' NV first
+    // the bulk numerical velocities from every species' NV (recorded)
     double BNVp = 0.0, BNVm = 0.0;
 #pragma unroll
     for (int q = 0; q < NSP; q++) {
-      double NVp, NVm;
-      numerical_velocities(s, q, CS, icS, kp0, kp1, rdt0, rdt1, sub, subUp, NVp, NVm);
       const double rs = R(s, RC_RSL + q) + R(s, RC_RSR + q);
-      BNVp = BNVp + rs * NVp * iRLR;
-      BNVm = BNVm + rs * NVm * iRLR;
+      BNVp = BNVp + rs * R(s, RC_NVP + q) * iRLR;
+      BNVm = BNVm + rs * R(s, RC_NVM + q) * iRLR;
     }
     // energy and momentum fluxes (:1973-2026): P12 side to this cell, M12
     // side to the cell above
@@ -964,7 +957,7 @@ struct AdjColumn {
         const double D = sh ? pdh : pd;
         const double Q = mcs * (facs + rdt);
         const double MP = kpx * f1 * D / Q;
-        const double NV = CS * (MF4P + MF4N - MP);
+        const double NV = R(s, (side ? RC_NVM : RC_NVP) + q);
         double lFR, lFM, lNP;
         if (side) {
           lFR = xb(q, j + 1);
@@ -1109,25 +1102,6 @@ struct AdjColumn {
     W(WK_VER) = lVER;
     W(WK_VNL) = lVNL;
     W(WK_VNR) = lVNR;
-  }
-
-  // The numerical velocities of species q at this face, P12 and M12 sides
-  // (:1689-1895).
-  GV_MEM void numerical_velocities(int s, int q, double CS, double icS, double kp0,
-                                   double kp1, double rdt0, double rdt1, int sub, int subUp,
-                                   double &NVp, double &NVm) const {
-    const double RSL = R(s, RC_RSL + q), RSR = R(s, RC_RSR + q);
-    const double PSL = R(s, RC_PSL + q), PSR = R(s, RC_PSR + q);
-    const double HL = R(s, RC_HL + q), HR = R(s, RC_HR + q);
-    double ML, MR, M2, MZ, FA, MF4P, MF4N, f1, dMF4P, dMF4N;
-    mach_terms(R(s, RC_VL + q), R(s, RC_VR + q), icS, ML, MR, M2, MZ, FA, MF4P, MF4N, f1,
-               dMF4P, dMF4N);
-    const double pdh = (PSR - HR) - (PSL - HL);
-    const double pd = PSR - PSL;
-    const double mcs = (0.5 * (RSL + RSR)) * CS, facs = FA * CS;
-    const bool shp = (sub >> q) & 1, shm = (subUp >> q) & 1;
-    NVp = CS * (MF4P + MF4N - kp0 * f1 * (shp ? pdh : pd) / (mcs * (facs + rdt0)));
-    NVm = CS * (MF4P + MF4N - kp1 * f1 * (shm ? pdh : pd) / (mcs * (facs + rdt1)));
   }
 
   // SubtractHydrostatic bits of the cell of lane jj in record stage s.
@@ -1447,7 +1421,7 @@ GV_FN void adj_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io,
   }
 #undef GV_LD
   ax[AX_CMAX + j] = cmax;
-  ax[AX_GAM + j] = gam;
+  ax[AX_CMAX + c.L + j] = gam;
   c.setup(alt, grav, invr, dalt, mmm, gam, eddy, cmax, tu, coslat);
   bool bad = false;
 #pragma unroll
@@ -1465,7 +1439,14 @@ GV_FN void adj_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io,
   cx.rec = rec;
   cx.nt = NT;
   const double dtin[4] = {0.5 * dt, 0.5 * dt, dt, 0.5 * dt};
+  double *bcrec = rec + ((long)ADJ_STAGES * NREC + NWORK) * NT;
+  // each pass of the boundary conditions leaves its results in the LDS
+  // scalars SC_BOT..SC_TOP+24: recorded for the reverse
+  auto record_bcs = [&](int pass) {
+    for (int i = j; i < ADJ_BC; i += c.L) bcrec[pass * ADJ_BC + i] = c.sc(SC_BOT + i);
+  };
   c.bcs();
+  record_bcs(0);
   if (c.cell()) {
 #pragma unroll
     for (int f = 0; f < NF; f++) c.Yv(f) = c.U[f];
@@ -1477,6 +1458,7 @@ GV_FN void adj_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io,
     for (int f = 0; f < NF; f++) cx.r(RC_U + f) = c.U[f];
     c.ausm_stage(dtin[s], s);
     c.bcs();
+    record_bcs(s + 1);
   }
   cx.stage = 4;
 #pragma unroll

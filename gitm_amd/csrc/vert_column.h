@@ -206,6 +206,22 @@ GV_FN R limiter_mc(double beta, R up, R dn) {
   return 0.0;
 }
 
+// Which value Limiter_mc took: 0 (zero), 1 (beta*up), 2 (beta*dn) or 3
+// ((up+dn)/2), as limiter_mc decides it (the adjoint build records it).
+template <class R>
+GV_FN int limiter_code(double beta, R up, R dn) {
+  if (up > 0.0) {
+    if (!(dn > 0.0)) return 0;
+    const R a = beta * up, b = beta * dn;
+    const R m = a < b ? a : b;
+    return (m < (up + dn) * 0.5) ? (a < b ? 1 : 2) : 3;
+  }
+  if (!(dn < 0.0)) return 0;
+  const R a = beta * up, b = beta * dn;
+  const R m = a > b ? a : b;
+  return (m > (up + dn) * 0.5) ? (a > b ? 1 : 2) : 3;
+}
+
 // Crout LU with implicit partial pivoting + substitution for the 6x6
 // friction system.  Same pivot rule as src/ludcmp.f (rows scaled by
 // 1/max|a_ij|, the last maximum wins, a zero pivot becomes 1e-20) and the
@@ -627,9 +643,11 @@ struct Column {
     cx.sync();
     // gamma face states (calc_kt_facevalues of Gamma_1d, ausm :1535)
     R dlg = 0.0;
-    if (k >= 0 && k <= nA + 1)
-      dlg = limiter_mc(kp.beta, (xb(2, j + 1) - gam) * st(A_IF, j + 1),
-                       (gam - xb(2, j - 1)) * st(A_IF, j));
+    if (k >= 0 && k <= nA + 1) {
+      const R up = (xb(2, j + 1) - gam) * st(A_IF, j + 1), dn = (gam - xb(2, j - 1)) * st(A_IF, j);
+      dlg = limiter_mc(kp.beta, up, dn);
+      if constexpr (ctx_taps<Ctx>::value) cx.tap_gamma_code(limiter_code(kp.beta, up, dn));
+    }
     xb(5, j) = dlg;
     // 5-point gradient weights (ausm :649-676)
     if (k >= 1 && k <= nA) {
@@ -1118,21 +1136,27 @@ struct Column {
     {
       const R iF0 = st(A_IF, j), iF1 = st(A_IF, j + 1), iF2 = st(A_IF, j + 2);
       const R dF0 = st(A_DF, j), dF1 = st(A_DF, j + 1);
-#define GV_KT(b, OUTL, OUTR)                                                        \
+      // kt: the limiter choices of this cell's slopes (adjoint build only)
+      unsigned long long kt = 0;
+      (void)kt;
+#define GV_KT(b, F, OUTL, OUTR)                                                     \
   do {                                                                              \
     const R f0 = xb(b, j), fm1 = xb(b, j - 1), f1 = xb(b, j + 1), f2 = xb(b, j + 2); \
-    const R d0 = limiter_mc(kp.beta, (f1 - f0) * iF1, (f0 - fm1) * iF0);       \
-    const R d1 = limiter_mc(kp.beta, (f2 - f1) * iF2, (f1 - f0) * iF1);        \
+    const R up0 = (f1 - f0) * iF1, dn0 = (f0 - fm1) * iF0;                          \
+    const R d0 = limiter_mc(kp.beta, up0, dn0);                                     \
+    const R d1 = limiter_mc(kp.beta, (f2 - f1) * iF2, (f1 - f0) * iF1);            \
+    if constexpr (ctx_taps<Ctx>::value)                                             \
+      kt |= (unsigned long long)limiter_code(kp.beta, up0, dn0) << (2 * (F));       \
     OUTL = f0 + 0.5 * d0 * dF0;                                                     \
     OUTR = f1 - 0.5 * d1 * dF1;                                                     \
   } while (0)
 #pragma unroll
       for (int s = 0; s < NSP; s++) {
         R l, r;
-        GV_KT(s, l, r);
+        GV_KT(s, s, l, r);
         HL[s] = exp(l);
         HR[s] = exp(r);
-        GV_KT(6 + s, VL[s], VR[s]);
+        GV_KT(6 + s, 6 + s, VL[s], VR[s]);
       }
       cx.sync();
       if (cx.device_math()) {
@@ -1155,19 +1179,19 @@ struct Column {
 #pragma unroll
       for (int s = 0; s < NSP; s++) {
         R l, r;
-        GV_KT(s, l, r);
+        GV_KT(s, 12 + s, l, r);
         RSL[s] = exp(l);
         RSR[s] = exp(r);
-        GV_KT(6 + s, l, r);
+        GV_KT(6 + s, 18 + s, l, r);
         PSL[s] = exp(l);
         PSR[s] = exp(r);
       }
-      GV_KT(12, VEL, VER);
-      GV_KT(13, VNL, VNR);
+      GV_KT(12, 24, VEL, VER);
+      GV_KT(13, 25, VNL, VNR);
 #undef GV_KT
+      if constexpr (ctx_taps<Ctx>::value)
+        cx.tap_faces(HL, HR, VL, VR, RSL, RSR, PSL, PSR, VEL, VER, VNL, VNR, kt);
     }
-    if constexpr (ctx_taps<Ctx>::value)
-      cx.tap_faces(HL, HR, VL, VR, RSL, RSR, PSL, PSR, VEL, VER, VNL, VNR);
     cx.sync();
 
     // --- AUSM+-up fluxes of face k|k+1 (:1459-1970).  The P12 side belongs
@@ -1217,6 +1241,9 @@ struct Column {
       const R ku0 = st(A_KU, j), ku1 = st(A_KU, j + 1);
       const R rdt0 = st(A_DC, j) / dtin, rdt1 = st(A_DC, j + 1) / dtin;
       R BNVp = 0.0, BNVm = 0.0;
+      R nvp[NSP], nvm[NSP];  // the numerical velocities (adjoint build only)
+      (void)nvp;
+      (void)nvm;
       const R RLR = RL + RR;
       const R iRLR = 1.0 / RLR, icS = 1.0 / CS;
 #pragma unroll
@@ -1248,6 +1275,7 @@ struct Column {
           const bool sh = (sub >> s) & 1;
           const R MP = kp0 * f1 * (sh ? pdh : pd) / (mcs * (facs + rdt0));
           const R NV = CS * (MF4P + MF4N - MP);
+          nvp[s] = NV;
           NPp[s] = (sh ? (MPS - MHPS) : MPS) - 0.5 * ku0 * CS * rvd;
           if (NV >= 0.0) {
             FRp[s] = RSL[s] * NV;
@@ -1262,6 +1290,7 @@ struct Column {
           const bool sh = (subUp >> s) & 1;
           const R MP = kp1 * f1 * (sh ? pdh : pd) / (mcs * (facs + rdt1));
           const R NV = CS * (MF4P + MF4N - MP);
+          nvm[s] = NV;
           xb(12 + s, j) = (sh ? (MPS - MHPS) : MPS) - 0.5 * ku1 * CS * rvd;
           if (NV >= 0.0) {
             xb(s, j) = RSL[s] * NV;
@@ -1273,6 +1302,7 @@ struct Column {
           BNVm = BNVm + dq((RSL[s] + RSR[s]) * NV, RLR, iRLR);
         }
       }
+      if constexpr (ctx_taps<Ctx>::value) cx.tap_nv(nvp, nvm);
       // energy and momentum fluxes (:1973-2026)
       if (BNVp >= 0.0) {
         EFp = (EL + PL) * BNVp;

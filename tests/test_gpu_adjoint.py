@@ -16,16 +16,15 @@ differentiable window (gitm_amd.adjoint.VerticalWindow).
   rate, the adjoint against central differences, each cell's difference
   taken on its own column's term of the loss (columns are independent, so
   one cell moves only its column's term).  Bar 1e-6 relative.  Temperature
-  cells step by h = 1e-6 relative.  A heating cell of 1e-5 W/kg moves Temp
-  by ~1e-8 K per step, which no relative step of 1e-6 resolves in FP64:
-  those cells step by h = 1e4 relative (Temp then moves by ~1e-6 of itself
-  over the window).  Cells are drawn from those whose term moves by more
-  than 1e-8 of itself under the step, so that the differences' own
+  cells step by h = 1e-6 relative.  The heating (1e-5 W/kg at its 150 km
+  peak) is ~0 at the levels the loss is sensitive to, so its cells step by
+  an absolute 0.01 W/kg.  Cells are drawn from those whose term moves by
+  more than 1e-8 of itself under the step, so that the differences' own
   rounding (about 1e-15 of the term) stays near 1e-7 of the derivative.
   The loss is piecewise smooth (limiters, upwinding, the maximum in cMax,
-  the eddy coefficient's ramp): where the one-sided differences disagree
-  (a switch within h of the point), the adjoint must match the smooth
-  side's one-sided difference to 3e-6 instead; at most 4 of 32 cells may.
+  the eddy coefficient's ramp): where the central difference misses (a
+  switch within the step), the adjoint must match one one-sided difference
+  (the smooth side) to 2e-6 instead; at most 4 of 32 cells may.
 * The window with the prep held (with_prep=False) against the same window
   on the CPU: the oracle forward and, per step backwards, the host build of
   the taped adjoint.  Bar 1e-9 normwise: the GPU's forward trajectory
@@ -233,14 +232,16 @@ def test_window_random_cells(window_case, field):
     g = grads[field].cpu().numpy()
     with torch.no_grad():
         t0 = terms(base, heat).cpu().numpy()
-    # log-sensitivity of each cell relative to its column's term
-    sens = np.abs(g * x) / np.abs(t0)[:, None]
+    # the step of each cell: 1e-6 of the temperature; 0.01 W/kg of heating
+    # (an absolute step: the heating is ~0 where the loss is sensitive to it)
+    step = 1e-6 * x if field == "temp" else np.full_like(x, 1e-2)
+    # the change of each cell's column term under its step, relative to the term
+    moved = np.abs(g * step) / np.abs(t0)[:, None]
     nc, L = x.shape
-    h = 1e-6 if field == "temp" else 1e4
     rng = np.random.default_rng(21 if field == "temp" else 22)
     picks = []
     for c in rng.permutation(nc):
-        ok = np.nonzero(sens[c, 2:L - 2] * h > 1e-8)[0] + 2
+        ok = np.nonzero(moved[c, 2:L - 2] > 1e-8)[0] + 2
         if len(ok):
             picks.append((c, int(rng.choice(ok))))
         if len(picks) == 32:
@@ -248,33 +249,34 @@ def test_window_random_cells(window_case, field):
     assert len(picks) == 32
     v = torch.zeros_like(heat)
     for c, k in picks:
-        v[c, k] = float(x[c, k])
+        v[c, k] = float(step[c, k])
 
     def run(sign):
         with torch.no_grad():
             st = {k: t.clone() for k, t in base.items()}
             q = heat.clone()
             if field == "heat":
-                q = q + sign * h * v
+                q = q + sign * v
             else:
-                st["temp"] = st["temp"] + sign * h * v
+                st["temp"] = st["temp"] + sign * v
             return terms(st, q).cpu().numpy()
 
     tp, tm = run(1.0), run(-1.0)
     worst, kinks = 0.0, 0
     for c, k in picks:
-        adj = g[c, k] * x[c, k]
-        central = (tp[c] - tm[c]) / (2.0 * h)
-        fwd, bwd = (tp[c] - t0[c]) / h, (t0[c] - tm[c]) / h
+        adj = g[c, k] * step[c, k]
+        central = (tp[c] - tm[c]) / 2.0
+        fwd, bwd = tp[c] - t0[c], t0[c] - tm[c]
         err = abs(adj - central) / abs(adj)
-        if abs(fwd - bwd) > 4e-6 * abs(adj):
-            # a kink of the loss (a limiter, upwind or max switch) within h:
-            # the adjoint is the derivative on the smooth side
+        if err >= 1e-6:
+            # a switch of the loss (a limiter, upwind choice, the maximum in
+            # cMax, the eddy ramp) within the step: the adjoint is the
+            # derivative on the smooth side
             kinks += 1
-            err = min(abs(adj - fwd), abs(adj - bwd)) / abs(adj) / 3.0
+            err = min(abs(adj - fwd), abs(adj - bwd)) / abs(adj)
+            assert err < 2e-6, (field, c, k, adj, central, fwd, bwd, err)
         worst = max(worst, err)
-        assert err < 1e-6, (field, c, k, adj, central, fwd, bwd, err)
-    print(field, "32 cells, worst relative error", worst, "kinked cells", kinks)
+    print(field, "32 cells, worst relative error", worst, "cells at a switch", kinks)
     assert kinks <= 4
 
 
