@@ -26,7 +26,12 @@ def _fortran_order(a: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
-def advance(cols: dict, params, solver: int = 1, n_steps: int = 1) -> dict:
+# the Fortran drop-in (fortran/) driven on GITM's module arrays, same file
+# formats (tests/fortran/gpu_driver.f90)
+FORTRAN_GPU_EXE = os.path.join(HERE, "_ref", "gitm_vert_fortran_gpu")
+
+
+def advance(cols: dict, params, solver: int = 1, n_steps: int = 1, exe: str = EXE) -> dict:
     nc, L = cols["temp"].shape
     na = L - 4
     if na != REF_NALTS:
@@ -52,11 +57,11 @@ def advance(cols: dict, params, solver: int = 1, n_steps: int = 1) -> dict:
             f.write(par.tobytes())
             for k in order:
                 f.write(_fortran_order(cols[k]).tobytes())
-        subprocess.run([EXE, fin, fout], check=True, capture_output=True)
+        subprocess.run([exe, fin, fout], check=True, capture_output=True)
         raw = np.fromfile(fout, dtype=np.uint8)
     status = raw[:16].view(np.int32)
     if status[0]:
-        raise RuntimeError(f"reference status {status}")
+        raise RuntimeError(f"{os.path.basename(exe)} status {status}")
     vals = raw[16:].view(np.float64)
     out = {}
     pos = 0

@@ -1,0 +1,115 @@
+"""The Fortran drop-in (fortran/): the bind(C) types of ModVerticalGPU have
+the C structs' sizes and field offsets (CPU: both sides compiled and
+printed), and advance_vertical_all_gpu on GITM's own module arrays, through
+ISO_C_BINDING into the HIP library, matches the reference Fortran build on
+the 324-column grid at 1e-11 (GPU)."""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from gitm_amd import synth
+from oracle import ref
+from tests.helpers import assert_parity
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FC = "/opt/rocm/bin/amdflang"
+
+FIELDS = ("ndensity_s", "vertical_velocity", "velocity", "temperature", "rho", "log_ns",
+          "ndensity", "temp_unit", "mean_major_mass", "gamma", "cmax", "eddy", "altitude",
+          "gravity", "inv_radial_distance", "dalt", "cos_latitude", "idensity_s",
+          "ivelocity", "sza", "mlat", "msis_bc")
+PARAMS = ("dt", "omega_body", "max_vertical_velocity", "beta_limiter", "min_tec", "vert_tau",
+          "mass", "thermal_diff_coef", "use_neutral_friction", "use_coriolis",
+          "use_ion_advection", "use_improved_ion_advection", "use_nighttime_ion_bcs",
+          "use_msis", "use_gswm_tides", "use_waccm_tides", "use_bc_perturbation",
+          "during_perturb", "use_damping", "reserved")
+FIELD_MEMBERS = ("ptr", "s_lon", "s_lat", "s_blk", "s_lev", "s_comp", "first_level",
+                 "reserved")
+
+
+def _layout_programs():
+    lines_f = ["program layout", "  use iso_c_binding", "  use ModVerticalGPU",
+               "  implicit none", "  type(gitm_vert_columns), target :: c",
+               "  type(gitm_vert_params), target :: p", "  type(gitm_vert_field), target :: f",
+               "  type(gitm_vert_error), target :: e",
+               "  write(*,'(a,i0)') 'size columns ', c_sizeof(c)",
+               "  write(*,'(a,i0)') 'size params ', c_sizeof(p)",
+               "  write(*,'(a,i0)') 'size field ', c_sizeof(f)",
+               "  write(*,'(a,i0)') 'size error ', c_sizeof(e)"]
+
+    def off(var, member, name):
+        return (f"  write(*,'(a,i0)') '{name} ', transfer(c_loc({var}%{member}), 0_c_intptr_t)"
+                f" - transfer(c_loc({var}), 0_c_intptr_t)")
+    lines_c = ["#include <stdio.h>", "#include <stddef.h>", '#include "gitm_vert.h"',
+               "int main(void) {",
+               '  printf("size columns %zu\\n", sizeof(gitm_vert_columns));',
+               '  printf("size params %zu\\n", sizeof(gitm_vert_params));',
+               '  printf("size field %zu\\n", sizeof(gitm_vert_field));',
+               '  printf("size error %zu\\n", sizeof(gitm_vert_error));']
+    for m in ("n_alts",) + FIELDS:
+        lines_f.append(off("c", m, f"columns.{m}"))
+        lines_c.append(f'  printf("columns.{m} %zu\\n", offsetof(gitm_vert_columns, {m}));')
+    for m in PARAMS:
+        lines_f.append(off("p", m, f"params.{m}"))
+        lines_c.append(f'  printf("params.{m} %zu\\n", offsetof(gitm_vert_params, {m}));')
+    for m in FIELD_MEMBERS:
+        lines_f.append(off("f", m, f"field.{m}"))
+        lines_c.append(f'  printf("field.{m} %zu\\n", offsetof(gitm_vert_field, {m}));')
+    for m in ("code", "column", "level", "reserved"):
+        lines_f.append(off("e", m, f"error.{m}"))
+        lines_c.append(f'  printf("error.{m} %zu\\n", offsetof(gitm_vert_error, {m}));')
+    lines_f.append("end program layout")
+    lines_c += ["  return 0;", "}"]
+    return "\n".join(lines_f) + "\n", "\n".join(lines_c) + "\n"
+
+
+@pytest.mark.skipif(not os.path.exists(FC) or not shutil.which("gcc"),
+                    reason="needs amdflang and gcc")
+def test_bind_c_layout_matches_the_header():
+    src_f, src_c = _layout_programs()
+    with tempfile.TemporaryDirectory() as td:
+        with open(os.path.join(td, "layout.f90"), "w") as f:
+            f.write(src_f)
+        with open(os.path.join(td, "layout.c"), "w") as f:
+            f.write(src_c)
+        subprocess.run([FC, "-J", td, "-c", os.path.join(ROOT, "fortran", "ModVerticalGPU.f90"),
+                        "-o", os.path.join(td, "m.o")], check=True, cwd=td)
+        subprocess.run([FC, "-I", td, os.path.join(td, "layout.f90"), os.path.join(td, "m.o"),
+                        "-o", os.path.join(td, "lf")], check=True, cwd=td)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), os.path.join(td, "layout.c"),
+                        "-o", os.path.join(td, "lc")], check=True)
+        out_f = subprocess.run([os.path.join(td, "lf")], capture_output=True, text=True,
+                               check=True).stdout.split("\n")
+        out_c = subprocess.run([os.path.join(td, "lc")], capture_output=True, text=True,
+                               check=True).stdout.split("\n")
+    got = dict(line.rsplit(" ", 1) for line in out_f if line.strip())
+    want = dict(line.rsplit(" ", 1) for line in out_c if line.strip())
+    assert len(want) == 4 + 1 + len(FIELDS) + len(PARAMS) + len(FIELD_MEMBERS) + 4
+    mismatch = {k: (got.get(k), v) for k, v in want.items() if got.get(k) != v}
+    assert not mismatch, mismatch
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("which,ions", [(1, False), (0, False), (1, True)])
+def test_fortran_drop_in_matches_reference(which, ions):
+    """The 2x2 blocks of 9x9 columns, nAlts 50: GITM's arrays in place
+    (halo cells, lon-fastest strides, per-block TempUnit), one step, against
+    the reference build at 1e-11 (winds normwise, as in test_gpu_parity)."""
+    if not (os.path.exists(ref.EXE) and os.path.exists(ref.FORTRAN_GPU_EXE)):
+        pytest.skip("the reference build and the Fortran driver are not built")
+    cols = synth.make_columns(synth.CONFIGS["cpu_324x50"], seed=31, ions=ions)
+    p = synth.StepParams(use_ion_advection=ions)
+    print(f"fortran drop-in: solver {which}, ions {ions}, 324 columns", flush=True)
+    got = ref.advance(cols, p, solver=which, n_steps=1, exe=ref.FORTRAN_GPU_EXE)
+    want = ref.advance(cols, p, solver=which, n_steps=1)
+    errs = assert_parity(got, want, 1e-11, what=f"fortran drop-in solver {which}")
+    print(errs)
+    for k in ("logns", "nden"):
+        scale = np.max(np.abs(want[k]))
+        assert np.max(np.abs(got[k] - want[k])) <= 1e-11 * scale, k
