@@ -52,7 +52,8 @@ enum AdjRecField {
   RC_KT = RC_WFR + NSP,      // the limiter choices of the cell's face-state slopes
   RC_NVP,                    // numerical velocities of face k|k+1, P12 side, 6
   RC_NVM = RC_NVP + NSP,     // and M12 side, 6
-  NREC = RC_NVM + NSP
+  RC_TP = RC_NVM + NSP,      // Temp**-DiffExp of the friction, the 6 exponents
+  NREC = RC_TP + 6
 };
 // Records per column: the four stages and the step's final state.
 constexpr int ADJ_STAGES = 5;
@@ -110,6 +111,12 @@ struct TapCtx : Base {
     r(RC_SUB) = (double)sub;
   }
   GV_MEM void tap_gamma_code(int code) { gamma_code = code; }
+  // the friction's temperature powers, recorded as Temp**-DiffExp
+  // (`inverse`: the reference form's Temp**+DiffExp)
+  GV_MEM void tap_powers(const double (&tp)[6], bool inverse) {
+#pragma unroll
+    for (int u = 0; u < 6; u++) r(RC_TP + u) = inverse ? 1.0 / tp[u] : tp[u];
+  }
   GV_MEM void tap_nv(const double (&nvp)[NSP], const double (&nvm)[NSP]) {
 #pragma unroll
     for (int s = 0; s < NSP; s++) {
@@ -157,6 +164,31 @@ struct TapCtx : Base {
     for (int s = 0; s < NSP; s++) r(RC_WFR + s) = w[s];
   }
 };
+
+// Gaussian elimination without pivoting of a 6x6 system whose matrix is
+// diagonally dominant by columns, for which elimination in the natural
+// order is stable: the transposed friction matrix (the friction matrix
+// I + dt (diag(sum_q C_iq) - C), C >= 0, is diagonally dominant by rows).
+GV_FN void solve6_dd(double (&a)[6][6], double (&b)[6]) {
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const double r = 1.0 / a[k][k];
+#pragma unroll
+    for (int i = k + 1; i < 6; i++) {
+      const double f = a[i][k] * r;
+#pragma unroll
+      for (int q = k + 1; q < 6; q++) a[i][q] = a[i][q] - f * a[k][q];
+      b[i] = b[i] - f * b[k];
+    }
+  }
+#pragma unroll
+  for (int i = 5; i >= 0; i--) {
+    double t = b[i];
+#pragma unroll
+    for (int q = i + 1; q < 6; q++) t = t - a[i][q] * b[q];
+    b[i] = t / a[i][i];
+  }
+}
 
 // Reverse of Limiter_mc (ModLimiter.f90:16-36): the adjoints of up and dn
 // from that of the limited slope, on the branch the forward value took.
@@ -584,11 +616,8 @@ struct AdjColumn {
       if (kp.friction) {
         const double eddy = st(A_EDDY, j);
         double x[NSP], M[NSP][NSP], cc[NSP][NSP], tp[6];
-        {
-          const double lt = log(T);
 #pragma unroll
-          for (int u = 0; u < 6; u++) tp[u] = exp(-DEXU[u] * lt);
-        }
+        for (int u = 0; u < 6; u++) tp[u] = R(s, RC_TP + u);
         const double kT = KB * T;
 #pragma unroll
         for (int i = 0; i < NSP; i++) {
@@ -614,7 +643,7 @@ struct AdjColumn {
         double mu[NSP];
 #pragma unroll
         for (int i = 0; i < NSP; i++) mu[i] = lw[i];
-        if (!lu_solve6(M, mu)) c.fail(GITM_VERT_ESINGULAR, k);
+        solve6_dd(M, mu);
         double lT = 0.0;
 #pragma unroll
         for (int i = 0; i < NSP; i++) {
@@ -931,12 +960,27 @@ struct AdjColumn {
     double lCS = 0.0, licS = 0.0, liRLR = 0.0;
     const bool cl = cell();
     const double ap = st(A_AP, j), ivol = 1.0 / st(A_VOL, j), idC = 1.0 / st(A_DC, j);
+    // the species' face states stream from the record, the next species'
+    // loaded while this one computes
+    double nRSL = R(s, RC_RSL), nRSR = R(s, RC_RSR), nPSL = R(s, RC_PSL),
+           nPSR = R(s, RC_PSR), nHL = R(s, RC_HL), nHR = R(s, RC_HR), nVL = R(s, RC_VL),
+           nVR = R(s, RC_VR), nNVp = R(s, RC_NVP), nNVm = R(s, RC_NVM);
 #pragma unroll 1
     for (int q = 0; q < NSP; q++) {
-      const double RSL = R(s, RC_RSL + q), RSR = R(s, RC_RSR + q);
-      const double PSL = R(s, RC_PSL + q), PSR = R(s, RC_PSR + q);
-      const double HL = R(s, RC_HL + q), HR = R(s, RC_HR + q);
-      const double VL = R(s, RC_VL + q), VR = R(s, RC_VR + q);
+      const double RSL = nRSL, RSR = nRSR, PSL = nPSL, PSR = nPSR, HL = nHL, HR = nHR,
+                   VL = nVL, VR = nVR, NVq[2] = {nNVp, nNVm};
+      if (q + 1 < NSP) {
+        nRSL = R(s, RC_RSL + q + 1);
+        nRSR = R(s, RC_RSR + q + 1);
+        nPSL = R(s, RC_PSL + q + 1);
+        nPSR = R(s, RC_PSR + q + 1);
+        nHL = R(s, RC_HL + q + 1);
+        nHR = R(s, RC_HR + q + 1);
+        nVL = R(s, RC_VL + q + 1);
+        nVR = R(s, RC_VR + q + 1);
+        nNVp = R(s, RC_NVP + q + 1);
+        nNVm = R(s, RC_NVM + q + 1);
+      }
       double ML, MR, M2, MZ, FA, MF4P, MF4N, f1, dMF4P, dMF4N;
       mach_terms(VL, VR, icS, ML, MR, M2, MZ, FA, MF4P, MF4N, f1, dMF4P, dMF4N);
       const double pdh = (PSR - HR) - (PSL - HL);
@@ -957,7 +1001,7 @@ struct AdjColumn {
         const double D = sh ? pdh : pd;
         const double Q = mcs * (facs + rdt);
         const double MP = kpx * f1 * D / Q;
-        const double NV = R(s, (side ? RC_NVM : RC_NVP) + q);
+        const double NV = NVq[side];
         double lFR, lFM, lNP;
         if (side) {
           lFR = xb(q, j + 1);
@@ -1085,18 +1129,23 @@ struct AdjColumn {
     lRR -= liRR * iRR * iRR;
     // complete the species' adjoints; the exponentiated states take theirs
     // in log form (d/d ln x = x d/dx)
-#pragma unroll 1
+#pragma unroll 2
     for (int q = 0; q < NSP; q++) {
       const double RSL = R(s, RC_RSL + q), RSR = R(s, RC_RSR + q);
       const double VL = R(s, RC_VL + q), VR = R(s, RC_VR + q);
-      W(WK_RSL + q) = (W(WK_RSL + q) + lVUL * VL * iRL + lRL) * RSL;
-      W(WK_RSR + q) = (W(WK_RSR + q) + lVUR * VR * iRR + lRR) * RSR;
-      W(WK_VL + q) = W(WK_VL + q) + lVUL * RSL * iRL;
-      W(WK_VR + q) = W(WK_VR + q) + lVUR * RSR * iRR;
-      W(WK_PSL + q) = (W(WK_PSL + q) + lPL) * R(s, RC_PSL + q);
-      W(WK_PSR + q) = (W(WK_PSR + q) + lPR) * R(s, RC_PSR + q);
-      W(WK_HL + q) = W(WK_HL + q) * R(s, RC_HL + q);
-      W(WK_HR + q) = W(WK_HR + q) * R(s, RC_HR + q);
+      const double PSL = R(s, RC_PSL + q), PSR = R(s, RC_PSR + q);
+      const double HL = R(s, RC_HL + q), HR = R(s, RC_HR + q);
+      const double wRSL = W(WK_RSL + q), wRSR = W(WK_RSR + q), wVL = W(WK_VL + q),
+                   wVR = W(WK_VR + q), wPSL = W(WK_PSL + q), wPSR = W(WK_PSR + q),
+                   wHL = W(WK_HL + q), wHR = W(WK_HR + q);
+      W(WK_RSL + q) = (wRSL + lVUL * VL * iRL + lRL) * RSL;
+      W(WK_RSR + q) = (wRSR + lVUR * VR * iRR + lRR) * RSR;
+      W(WK_VL + q) = wVL + lVUL * RSL * iRL;
+      W(WK_VR + q) = wVR + lVUR * RSR * iRR;
+      W(WK_PSL + q) = (wPSL + lPL) * PSL;
+      W(WK_PSR + q) = (wPSR + lPR) * PSR;
+      W(WK_HL + q) = wHL * HL;
+      W(WK_HR + q) = wHR * HR;
     }
     W(WK_VEL) = lVEL;
     W(WK_VER) = lVER;

@@ -568,9 +568,11 @@ struct Column {
     R tp[6];
     if (cx.device_math()) {
       cx.diff_powers(T, -1.0, tp);
+      if constexpr (ctx_taps<Ctx>::value) cx.tap_powers(tp, false);
       return neutral_friction<true>(kp, dtin, w, nt, ns, glc, st(A_EDDY, j), T, tp);
     }
     cx.diff_powers(T, 1.0, tp);
+    if constexpr (ctx_taps<Ctx>::value) cx.tap_powers(tp, true);
     return neutral_friction<false>(kp, dtin, w, nt, ns, glc, st(A_EDDY, j), T, tp);
   }
 
