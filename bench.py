@@ -219,10 +219,13 @@ def bench_ensemble(torch, solver, params, steps: int = 20, seed: int = 4) -> dic
     """The ragged ensemble: 64 members with nAlts drawn from 50..120 and 2592,
     10368 or 25920 columns each (gitm_amd.synth.ensemble), advanced as one
     batch.  Each member's columns tile the first min(n, 2592) columns of its
-    own seeded atmosphere.  Tiles are autotuned per nAlts; then a hipGraph
-    holding ``steps`` x (batch prep with per-member CFL steps + batch step)
-    is replayed once and timed.  Also timed: the same steps at a fixed dt
-    as one launch per distinct nAlts."""
+    own seeded atmosphere.  The step's tile (workgroup size, short columns
+    per workgroup, where the RK4 carries live, persistent grid or not) is
+    autotuned on a probe batch; then a hipGraph holding ``steps`` x (batch
+    prep with per-member CFL steps + batch step) is replayed once and timed.
+    A batch step is one launch; the prep is two (the per-member step words
+    are reset, then the columns' CFL limits folded in).  Also timed: the same
+    steps at a fixed dt, one launch of ``steps`` steps."""
     from gitm_amd import synth, vertical
 
     mem = synth.ensemble(64, seed=seed)
@@ -237,7 +240,7 @@ def bench_ensemble(torch, solver, params, steps: int = 20, seed: int = 4) -> dic
     ncols = sum(int(d["temp"].shape[0]) for d in dev)
     levels = sum(int(d["temp"].shape[0]) * (int(d["temp"].shape[1]) - 4) for d in dev)
     b = vertical.VerticalBatch(solver, dev)
-    tune = b.autotune(params, "ausm", probe_columns=4096)
+    tune = b.autotune(params, "ausm")
     pp = synth.PrepParams()
     dt = torch.empty(len(dev), dtype=torch.float64, device="cuda")
     b.prepare(pp, dt_out=dt)
@@ -264,11 +267,14 @@ def bench_ensemble(torch, solver, params, steps: int = 20, seed: int = 4) -> dic
     fixed_ms = e0.elapsed_time(e1) / steps
     ok = all(bool(torch.isfinite(d["temp"]).all()) and bool((d["nds"] > 0).all())
              for d in dev)
+    tile = dict(zip(vertical.VerticalBatch.TILE_KEYS, b.tile()))
     del g
     b.close()
     return {"members": len(dev), "columns": ncols, "mean_n_alts": levels / ncols,
             "distinct_n_alts": sorted({int(d["temp"].shape[1]) - 4 for d in dev}),
-            "tiles": {str(k): min(v, key=v.get) for k, v in tune.items()},
+            "tile": tile,
+            "tiles_tried": {",".join(map(str, k)): round(v, 4) for k, v in tune.items()},
+            "launches_per_step": {"step": 1, "prep": 2},
             "graph_steps": steps,
             "ms_per_step_graph_prep_and_step": graph_ms,
             "column_updates_per_s_graph": ncols / (graph_ms * 1e-3),

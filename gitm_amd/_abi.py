@@ -83,6 +83,12 @@ class Error(ctypes.Structure):
                 ("level", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+class BatchTile(ctypes.Structure):
+    _fields_ = [("waves_per_workgroup", ctypes.c_int32),
+                ("columns_per_workgroup", ctypes.c_int32),
+                ("carries_in_lds", ctypes.c_int32), ("persistent", ctypes.c_int32)]
+
+
 # canonical field name -> (C-ABI field, components)
 CANON = {
     "nds": ("ndensity_s", NSPECIES), "vvel": ("vertical_velocity", NSPECIES),

@@ -24,22 +24,28 @@ HIPFLAGS = [
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
 ]
 # translation units, compiled in parallel: the single-grid step and C ABI,
-# the ensemble batches, the adjoint, the sources step, the horizontal step
-SOURCES = ("gitm_vert.hip", "gitm_vert_batch.hip", "gitm_vert_adjoint.hip",
-           "gitm_vert_sources.hip", "gitm_horiz.hip")
+# the ensemble batches (host side, and the step kernel once per workgroup
+# size of 1..4 wavefronts), the adjoint, the sources step, the horizontal
+# step; (source, extra flags, object tag)
+SOURCES = (("gitm_vert.hip", (), ""), ("gitm_vert_batch.hip", (), ""),
+           *(("gitm_vert_ensemble.hip", (f"-DGV_ENS_WAVES={w}",), f"_w{w}")
+             for w in (1, 2, 3, 4)),
+           ("gitm_vert_adjoint.hip", (), ""), ("gitm_vert_sources.hip", (), ""),
+           ("gitm_horiz.hip", (), ""))
 
 
 def build(verbose: bool = False, extra: list[str] | None = None, out: str | None = None) -> str:
     out = out or LIB
     objs, procs = [], []
-    for src in SOURCES:
-        obj = out + "." + os.path.splitext(src)[0] + ".o"
-        cmd = [HIPCC] + HIPFLAGS + (extra or []) + ["-c", os.path.join(CSRC, src), "-o", obj]
+    for src, flags, tag in SOURCES:
+        obj = out + "." + os.path.splitext(src)[0] + tag + ".o"
+        cmd = ([HIPCC] + HIPFLAGS + list(flags) + (extra or []) +
+               ["-c", os.path.join(CSRC, src), "-o", obj])
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append(subprocess.Popen(cmd))
         objs.append(obj)
-    failed = [src for src, pr in zip(SOURCES, procs) if pr.wait() != 0]
+    failed = [src for (src, _, _), pr in zip(SOURCES, procs) if pr.wait() != 0]
     if failed:
         raise RuntimeError(f"hipcc failed on {failed}")
     link = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}"] + objs + ["-o", out + ".tmp"]

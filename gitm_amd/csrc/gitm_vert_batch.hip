@@ -1,76 +1,87 @@
 // gitm_vert_batch.hip -- ensemble batches of the vertical step (the
 // gitm_vert_batch_* entry points of include/gitm_vert.h).
+//
+// A batch step is one launch of vert_ensemble_kernel (gitm_vert_ensemble.hip,
+// one object per workgroup size) over a task list: the columns of more than
+// 64 levels one workgroup each, then the shorter columns one wavefront each,
+// several per workgroup.  The batch prep is one launch as well.
 #include "vert_gpu.h"
+
+#define GV_ENS_DECL(W)                                                                     \
+  int ensemble_occupancy_w##W(int solver, bool ions, size_t lds_bytes);                    \
+  hipError_t launch_ensemble_w##W(int solver, bool ions, int grid, size_t lds_bytes,       \
+                                  hipStream_t stream, const gv::KParams &kp,               \
+                                  const BatchMember *mem, const EnsLayout &lay,            \
+                                  const double *dt_dev);
+GV_ENS_DECL(1)
+GV_ENS_DECL(2)
+GV_ENS_DECL(3)
+GV_ENS_DECL(4)
+#undef GV_ENS_DECL
 
 namespace {
 
-template <int NT, int SOLVER, bool IONS>
-__global__ void __launch_bounds__(NT)
-    vert_batch_kernel(gv::KParams kp, const BatchMember *__restrict__ mem, int n_mem,
-                      const double *__restrict__ dt_dev) {
-  extern __shared__ double lds[];
-  const int b = (int)blockIdx.x;
-  const BatchMember &e = mem[find_member(mem, n_mem, b)];
-  const double dt = dt_dev ? dt_dev[e.member] : kp.dt;
-  GpuCtx cx{kp.err};
-  const int col = b - e.first_block;
-  gv::run_column<NT, SOLVER, IONS>(cx, kp, e.io, col, lds, e.scratch, dt, e.col_offset + col);
+int ensemble_occupancy(int waves, int solver, bool ions, size_t lds) {
+  switch (waves) {
+    case 1: return ensemble_occupancy_w1(solver, ions, lds);
+    case 2: return ensemble_occupancy_w2(solver, ions, lds);
+    case 3: return ensemble_occupancy_w3(solver, ions, lds);
+    case 4: return ensemble_occupancy_w4(solver, ions, lds);
+    default: return 0;
+  }
 }
 
-template <int NT, bool IONS>
-void launch_batch_ions(int solver, dim3 grid, hipStream_t stream, const gv::KParams &kp,
-                       const BatchMember *mem, int n_mem, const double *dt_dev) {
-  const dim3 block(kp.n_alts + 4);
-  const size_t lds = gv::lds_bytes(kp.n_alts, IONS, gv::carry_in_lds(NT / 64));
-  if (solver == GITM_VERT_AUSM)
-    hipLaunchKernelGGL((vert_batch_kernel<NT, GITM_VERT_AUSM, IONS>), grid, block, lds,
-                       stream, kp, mem, n_mem, dt_dev);
-  else
-    hipLaunchKernelGGL((vert_batch_kernel<NT, GITM_VERT_RUSANOV, IONS>), grid, block, lds,
-                       stream, kp, mem, n_mem, dt_dev);
+hipError_t launch_ensemble(int waves, int solver, bool ions, int grid, size_t lds,
+                           hipStream_t stream, const gv::KParams &kp, const BatchMember *mem,
+                           const EnsLayout &lay, const double *dt_dev) {
+  switch (waves) {
+    case 1: return launch_ensemble_w1(solver, ions, grid, lds, stream, kp, mem, lay, dt_dev);
+    case 2: return launch_ensemble_w2(solver, ions, grid, lds, stream, kp, mem, lay, dt_dev);
+    case 3: return launch_ensemble_w3(solver, ions, grid, lds, stream, kp, mem, lay, dt_dev);
+    case 4: return launch_ensemble_w4(solver, ions, grid, lds, stream, kp, mem, lay, dt_dev);
+    default: return hipErrorInvalidValue;
+  }
 }
 
-template <int NT>
-void launch_batch(int solver, dim3 grid, hipStream_t stream, const gv::KParams &kp,
-                  const BatchMember *mem, int n_mem, const double *dt_dev) {
-  if (kp.ions)
-    launch_batch_ions<NT, true>(solver, grid, stream, kp, mem, n_mem, dt_dev);
-  else
-    launch_batch_ions<NT, false>(solver, grid, stream, kp, mem, n_mem, dt_dev);
-}
-
-// Batch prep: member e's step goes to dt_bits[e.member].
+// Batch prep: one workgroup per column of any member (tall columns first,
+// as the step numbers them), the lanes above the column's top idle; member
+// e's step goes to dt_bits[e.member].
 template <int NT>
 __global__ void __launch_bounds__(NT) prep_batch_kernel(gv::PrepParams pp,
                                                         const BatchMember *__restrict__ mem,
-                                                        int n_mem, unsigned long long *dt_bits) {
+                                                        EnsLayout lay,
+                                                        unsigned long long *dt_bits) {
   const int b = (int)blockIdx.x;
-  const BatchMember &e = mem[find_member(mem, n_mem, b)];
+  const bool tall = b < lay.n_tall;
+  const int c = tall ? b : b - lay.n_tall;
+  const int m = tall ? find_member(mem, lay.n_tall_members, c)
+                     : lay.n_tall_members +
+                           find_member(mem + lay.n_tall_members, lay.n_short_members, c);
+  const BatchMember &e = mem[m];
+  gv::PrepParams q = pp;
+  q.n_alts = e.io.n_alts;
   const gitm_vert_field none = {};
-  prep_column<NT>(pp, e.io, none, none, b - e.first_block, dt_bits + e.member);
+  prep_column<NT>(q, e.io, none, none, c - e.first_block, dt_bits + e.member);
 }
 
 }  // namespace
 
-// ---------------------------------------------------------------------------
-// Batches of ensemble members with their own grids.  Members with the same
-// nAlts form a group and run as one launch (one workgroup per column, one
-// lane per level, so a launch needs a single nAlts); a step of the whole
-// batch is one launch per distinct nAlts.  The member table and the scratch
-// memory are made when the batch is created (or its tiles change), so the
-// advance and prepare calls allocate and copy nothing and can be captured in
-// a hipGraph.
-// ---------------------------------------------------------------------------
 struct gitm_vert_batch {
   gitm_vert_handle *h = nullptr;
   int ions = 0;
   std::vector<gitm_vert_columns> members;  // the caller's order
   std::vector<long long> col_offset;       // batch-wide column numbering, n+1 entries
-  struct Group {
-    int n_alts, waves, first, count, blocks;
-  };
-  std::vector<Group> groups;
-  std::vector<BatchMember> table;  // group order
+  std::vector<BatchMember> table;          // tall members, then short ones
+  EnsLayout lay{};                         // counts; per_wg, n_tasks with the tile
+  int max_alts = 0, max_alts_tall = 0, max_alts_short = 0;
+  bool uniform_tall = true;  // every tall column needs the same wavefront count
+  int tall_waves = 0;        // ... which is this one
+  gitm_vert_batch_tile tile{};
+  int occupancy[2][2] = {};  // workgroups per CU of the tile's kernel [ausm][ions]
+  int slot[2] = {};          // short columns' LDS slot in doubles [ions]
+  size_t lds[2] = {};        // LDS of a workgroup [ions]
+  int n_cu = 0;
+  size_t lds_max = 0;
   BatchMember *d_table = nullptr;
   double *scratch = nullptr;
   size_t scratch_bytes = 0;
@@ -82,18 +93,96 @@ long long member_columns(const gitm_vert_columns &c) {
   return (long long)c.n_lons * c.n_lats * c.n_blocks;
 }
 
-// Scratch offsets of the members for the current tiles; (re)allocates the
-// scratch when it has to grow and uploads the member table.
-int batch_layout(gitm_vert_batch *b) {
-  size_t total = 0;  // doubles
-  for (const gitm_vert_batch::Group &g : b->groups) {
-    const int fields = gv::scratch_fields(gv::carry_in_lds(g.waves), b->ions != 0);
-    for (int i = g.first; i < g.first + g.count; i++) {
-      BatchMember &m = b->table[i];
-      m.scratch = reinterpret_cast<double *>(total);  // offset for now
-      total += (size_t)member_columns(m.io) * fields * (size_t)(g.n_alts + 4);
-    }
+int waves_of(int n_alts) { return (n_alts + 4 + 63) / 64; }
+
+// LDS doubles of a short column of n_alts levels, rounded up to even so
+// that every slot starts 16-byte aligned
+int short_doubles(int n_alts, bool ions, bool carry) {
+  const int d = (int)gv::lds_doubles(n_alts, ions, carry);
+  return d + (d & 1);
+}
+
+// Where the tile puts each member's RK4 carries (1: LDS, 0: scratch), in
+// table order: carries_in_lds 0 and 1 put all of them in scratch or in LDS;
+// 2 puts a member's in LDS when its column's LDS (a tall column's, or a
+// short column's times the short columns per workgroup) then stays within
+// the share of a CU's LDS that each of the workgroups its registers allow
+// gets.
+std::vector<int> place_carries(const gitm_vert_batch *b, const gitm_vert_batch_tile &t) {
+  std::vector<int> flag(b->table.size(), t.carries_in_lds ? 1 : 0);
+  if (t.carries_in_lds != 2) return flag;
+  const bool ions = b->ions != 0;
+  const int occ = std::max(1, ensemble_occupancy(t.waves_per_workgroup, GITM_VERT_AUSM, ions, 0));
+  const size_t budget = b->lds_max / (size_t)occ;
+  for (size_t i = 0; i < b->table.size(); i++) {
+    const int na = b->table[i].io.n_alts;
+    const size_t need = (int)i < b->lay.n_tall_members
+                            ? (size_t)gv::lds_bytes(na, ions, true)
+                            : (size_t)t.columns_per_workgroup * short_doubles(na, ions, true) *
+                                  sizeof(double);
+    flag[i] = need <= budget;
   }
+  return flag;
+}
+
+// LDS doubles of one short column's slot: the largest short column with its
+// carries where `flag` puts them
+int slot_doubles(const gitm_vert_batch *b, bool ions, const std::vector<int> &flag) {
+  int d = 0;
+  for (int i = b->lay.n_tall_members; i < (int)b->table.size(); i++)
+    d = std::max(d, short_doubles(b->table[i].io.n_alts, ions, flag[i] != 0));
+  return d;
+}
+
+// LDS of one workgroup of the step under tile t
+size_t tile_lds(const gitm_vert_batch *b, const gitm_vert_batch_tile &t, bool ions) {
+  const std::vector<int> flag = place_carries(b, t);
+  size_t need = 0;
+  for (int i = 0; i < b->lay.n_tall_members; i++)
+    need = std::max(need, (size_t)gv::lds_bytes(b->table[i].io.n_alts, ions, flag[i] != 0));
+  return std::max(need, (size_t)t.columns_per_workgroup * slot_doubles(b, ions, flag) *
+                            sizeof(double));
+}
+
+int check_tile(const gitm_vert_batch *b, const gitm_vert_batch_tile &t) {
+  const int w = t.waves_per_workgroup;
+  if (w < 1 || w > 4 || 64 * w < b->max_alts + 4) return GITM_VERT_EINVAL;
+  if (t.columns_per_workgroup < 1 || t.columns_per_workgroup > w) return GITM_VERT_EINVAL;
+  if (t.carries_in_lds < 0 || t.carries_in_lds > 2 || (t.persistent != 0 && t.persistent != 1))
+    return GITM_VERT_EINVAL;
+  if (tile_lds(b, t, b->ions != 0) > b->lds_max) return GITM_VERT_EINVAL;
+  // a looping grid needs every wavefront of a tall task to hold levels
+  if (t.persistent && b->lay.n_tall && !(b->uniform_tall && b->tall_waves == w))
+    return GITM_VERT_EINVAL;
+  return GITM_VERT_OK;
+}
+
+// The smallest workgroup that holds the tallest column, one wavefront per
+// short column, the carries in LDS where they fit.
+gitm_vert_batch_tile auto_tile(const gitm_vert_batch *b) {
+  gitm_vert_batch_tile t;
+  t.waves_per_workgroup = waves_of(b->max_alts);
+  t.columns_per_workgroup = t.waves_per_workgroup;
+  t.carries_in_lds = 2;
+  t.persistent = 0;
+  return t;
+}
+
+// Scratch offsets of the members for the tile; (re)allocates the scratch
+// when it has to grow, uploads the member table and records the persistent
+// grid's occupancy.
+int batch_layout(gitm_vert_batch *b) {
+  const std::vector<int> flag = place_carries(b, b->tile);
+  size_t total = 0;  // doubles
+  for (size_t i = 0; i < b->table.size(); i++) {
+    BatchMember &m = b->table[i];
+    m.carry_lds = flag[i];
+    const int fields = gv::scratch_fields(flag[i] != 0, b->ions != 0);
+    m.scratch = reinterpret_cast<double *>(total);  // offset for now
+    total += (size_t)member_columns(m.io) * fields * (size_t)(m.io.n_alts + 4);
+  }
+  b->lay.per_wg = b->tile.columns_per_workgroup;
+  b->lay.n_tasks = b->lay.n_tall + (b->lay.n_short + b->lay.per_wg - 1) / b->lay.per_wg;
   gitm_vert_handle *h = b->h;
   GV_HIP(hipSetDevice(h->device));
   // the launches queued so far may still read the table and the scratch
@@ -111,6 +200,16 @@ int batch_layout(gitm_vert_batch *b) {
     m.scratch = b->scratch + reinterpret_cast<size_t>(m.scratch);
   GV_HIP(hipMemcpy(b->d_table, dev.data(), dev.size() * sizeof(BatchMember),
                    hipMemcpyHostToDevice));
+  for (int i = 0; i < 2; i++) {
+    b->slot[i] = slot_doubles(b, i != 0, flag);
+    b->lds[i] = tile_lds(b, b->tile, i != 0);
+    for (int s = 0; s < 2; s++)
+      b->occupancy[s][i] =
+          (i && !b->ions) ? 0
+                          : ensemble_occupancy(b->tile.waves_per_workgroup,
+                                               s ? GITM_VERT_AUSM : GITM_VERT_RUSANOV, i != 0,
+                                               b->lds[i]);
+  }
   return GITM_VERT_OK;
 }
 
@@ -138,36 +237,53 @@ int gitm_vert_batch_create(gitm_vert_handle *h, const gitm_vert_columns *members
   b->col_offset.resize(n_members + 1, 0);
   for (int m = 0; m < n_members; m++)
     b->col_offset[m + 1] = b->col_offset[m] + member_columns(members[m]);
-  // groups by nAlts, in increasing order; members keep the caller's order
-  // within a group
-  std::vector<int> order(n_members);
-  for (int m = 0; m < n_members; m++) order[m] = m;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
-    return members[a].n_alts < members[c].n_alts;
-  });
-  for (int i = 0; i < n_members; i++) {
-    const int m = order[i];
-    const gitm_vert_columns &c = members[m];
-    if (b->groups.empty() || b->groups.back().n_alts != c.n_alts)
-      b->groups.push_back({c.n_alts, gv::auto_waves(c.n_alts), i, 0, 0});
-    gitm_vert_batch::Group &g = b->groups.back();
-    BatchMember e;
-    memset(&e, 0, sizeof(e));
-    e.io = c;
-    e.first_block = g.blocks;
-    e.member = m;
-    e.col_offset = (int)b->col_offset[m];
-    b->table.push_back(e);
-    g.count++;
-    g.blocks += (int)member_columns(c);
+  // the table: tall members in the caller's order, then the short ones
+  for (int pass = 0; pass < 2; pass++) {
+    long long first = 0;
+    for (int m = 0; m < n_members; m++) {
+      const gitm_vert_columns &c = members[m];
+      const bool tall = waves_of(c.n_alts) > 1;
+      if (tall != (pass == 0)) continue;
+      BatchMember e;
+      memset(&e, 0, sizeof(e));
+      e.io = c;
+      e.first_block = (int)first;
+      e.member = m;
+      e.col_offset = (int)b->col_offset[m];
+      b->table.push_back(e);
+      first += member_columns(c);
+      b->max_alts = std::max(b->max_alts, c.n_alts);
+      if (tall) {
+        b->max_alts_tall = std::max(b->max_alts_tall, c.n_alts);
+        if (b->tall_waves && b->tall_waves != waves_of(c.n_alts)) b->uniform_tall = false;
+        b->tall_waves = waves_of(c.n_alts);
+      } else {
+        b->max_alts_short = std::max(b->max_alts_short, c.n_alts);
+      }
+    }
+    if (pass == 0) {
+      b->lay.n_tall_members = (int)b->table.size();
+      b->lay.n_tall = (int)first;
+    } else {
+      b->lay.n_short_members = (int)b->table.size() - b->lay.n_tall_members;
+      b->lay.n_short = (int)first;
+    }
   }
   int rc = GITM_VERT_OK;
-  if (hipSetDevice(h->device) != hipSuccess) {
+  int lds_max = 0;
+  if (hipSetDevice(h->device) != hipSuccess ||
+      hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, h->device) !=
+          hipSuccess ||
+      hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device) !=
+          hipSuccess) {
     rc = GITM_VERT_EHIP;
   } else if (hipMalloc(&b->d_table, b->table.size() * sizeof(BatchMember)) != hipSuccess) {
     rc = GITM_VERT_ENOMEM;
   } else {
-    rc = batch_layout(b);
+    b->lds_max = (size_t)lds_max;
+    b->tile = auto_tile(b);
+    rc = check_tile(b, b->tile);
+    if (!rc) rc = batch_layout(b);
   }
   if (rc) {
     gitm_vert_batch_destroy(b);
@@ -187,21 +303,20 @@ int gitm_vert_batch_destroy(gitm_vert_batch *b) {
   return GITM_VERT_OK;
 }
 
-int gitm_vert_batch_set_tile(gitm_vert_batch *b, int n_alts, int waves_per_column) {
-  if (!b || (waves_per_column != 0 && !gv::valid_waves(waves_per_column)))
-    return GITM_VERT_EINVAL;
-  bool found = false;
-  for (gitm_vert_batch::Group &g : b->groups) {
-    if (n_alts != 0 && g.n_alts != n_alts) continue;
-    const int w = waves_per_column ? waves_per_column : gv::auto_waves(g.n_alts);
-    if (!gv::tile_fits(w, g.n_alts)) return GITM_VERT_EINVAL;
-    found = true;
-  }
-  if (!found) return GITM_VERT_EINVAL;
-  for (gitm_vert_batch::Group &g : b->groups)
-    if (n_alts == 0 || g.n_alts == n_alts)
-      g.waves = waves_per_column ? waves_per_column : gv::auto_waves(g.n_alts);
+int gitm_vert_batch_set_tile(gitm_vert_batch *b, const gitm_vert_batch_tile *t) {
+  if (!b) return GITM_VERT_EINVAL;
+  GV_HIP(hipSetDevice(b->h->device));
+  const gitm_vert_batch_tile nt = t ? *t : auto_tile(b);
+  const int rc = check_tile(b, nt);
+  if (rc) return rc;
+  b->tile = nt;
   return batch_layout(b);
+}
+
+int gitm_vert_batch_get_tile(const gitm_vert_batch *b, gitm_vert_batch_tile *t) {
+  if (!b || !t) return GITM_VERT_EINVAL;
+  *t = b->tile;
+  return GITM_VERT_OK;
 }
 
 int gitm_vert_batch_advance(gitm_vert_batch *b, const gitm_vert_params *p, int solver,
@@ -215,21 +330,22 @@ int gitm_vert_batch_advance(gitm_vert_batch *b, const gitm_vert_params *p, int s
   if (n_steps == 0) return GITM_VERT_OK;
   gitm_vert_handle *h = b->h;
   GV_HIP(hipSetDevice(h->device));
-  for (const gitm_vert_batch::Group &g : b->groups) {
-    gv::KParams kp = gv::make_kparams(p, &b->members[b->table[g.first].member], solver,
-                                      n_steps, h->d_err);
-    kp.scr_fields = gv::scratch_fields(gv::carry_in_lds(g.waves), kp.ions != 0);
-    const BatchMember *mem = b->d_table + g.first;
-    const dim3 grid(g.blocks);
-    switch (g.waves) {
-      case 1: launch_batch<64>(solver, grid, h->stream, kp, mem, g.count, dt_device); break;
-      case 2: launch_batch<128>(solver, grid, h->stream, kp, mem, g.count, dt_device); break;
-      case 3: launch_batch<192>(solver, grid, h->stream, kp, mem, g.count, dt_device); break;
-      case 4: launch_batch<256>(solver, grid, h->stream, kp, mem, g.count, dt_device); break;
-      default: return GITM_VERT_EINVAL;
-    }
-    GV_HIP(hipGetLastError());
+  // nAlts and the carries' stride come from each member in the kernel
+  gv::KParams kp = gv::make_kparams(p, &b->members[0], solver, n_steps, h->d_err);
+  const bool ions = kp.ions != 0;
+  EnsLayout lay = b->lay;
+  lay.slot_doubles = b->slot[ions];
+  const size_t lds = b->lds[ions];
+  int grid = lay.n_tasks;
+  if (b->tile.persistent) {
+    const int occ = b->occupancy[solver == GITM_VERT_AUSM][ions];
+    if (occ < 1) return GITM_VERT_EHIP;
+    grid = std::min(grid, occ * b->n_cu);
   }
+  if (grid < 1) return GITM_VERT_OK;
+  GV_HIP(launch_ensemble(b->tile.waves_per_workgroup, solver, ions, grid, lds, h->stream, kp,
+                         b->d_table, lay, dt_device));
+  GV_HIP(hipGetLastError());
   return GITM_VERT_OK;
 }
 
@@ -246,19 +362,16 @@ int gitm_vert_batch_prepare(gitm_vert_batch *b, const gitm_vert_prep_params *p,
   const int n = (int)b->members.size();
   hipLaunchKernelGGL(fill_words, dim3((n + 255) / 256), dim3(256), 0, h->stream, bits, n,
                      prep_dt0_bits(p));
-  for (const gitm_vert_batch::Group &g : b->groups) {
-    const gv::PrepParams pp = make_prep(p, g.n_alts);
-    const BatchMember *mem = b->d_table + g.first;
-    const dim3 grid(g.blocks), block(g.n_alts + 4);
-    switch (gv::auto_waves(g.n_alts)) {
-      case 1: hipLaunchKernelGGL(prep_batch_kernel<64>, grid, block, 0, h->stream, pp, mem, g.count, bits); break;
-      case 2: hipLaunchKernelGGL(prep_batch_kernel<128>, grid, block, 0, h->stream, pp, mem, g.count, bits); break;
-      case 3: hipLaunchKernelGGL(prep_batch_kernel<192>, grid, block, 0, h->stream, pp, mem, g.count, bits); break;
-      case 4: hipLaunchKernelGGL(prep_batch_kernel<256>, grid, block, 0, h->stream, pp, mem, g.count, bits); break;
-      default: return GITM_VERT_EINVAL;
-    }
-    GV_HIP(hipGetLastError());
+  const gv::PrepParams pp = make_prep(p, b->max_alts);
+  const dim3 grid(b->lay.n_tall + b->lay.n_short);
+  switch (waves_of(b->max_alts)) {
+    case 1: hipLaunchKernelGGL(prep_batch_kernel<64>, grid, dim3(64), 0, h->stream, pp, b->d_table, b->lay, bits); break;
+    case 2: hipLaunchKernelGGL(prep_batch_kernel<128>, grid, dim3(128), 0, h->stream, pp, b->d_table, b->lay, bits); break;
+    case 3: hipLaunchKernelGGL(prep_batch_kernel<192>, grid, dim3(192), 0, h->stream, pp, b->d_table, b->lay, bits); break;
+    case 4: hipLaunchKernelGGL(prep_batch_kernel<256>, grid, dim3(256), 0, h->stream, pp, b->d_table, b->lay, bits); break;
+    default: return GITM_VERT_EINVAL;
   }
+  GV_HIP(hipGetLastError());
   return GITM_VERT_OK;
 }
 
@@ -274,4 +387,3 @@ int gitm_vert_batch_locate(const gitm_vert_batch *b, int column, int *member,
 }
 
 }  // extern "C"
-

@@ -433,10 +433,12 @@ struct LevelStatic {
 // exp/log do.  The GPU always uses them; the host build of the kernels runs
 // either form, so that the reference forms can be checked bit for bit.
 // ---------------------------------------------------------------------------
-template <int NT, class Ctx, bool IONS>
+// CARRY: the RK4 carries in LDS (true) or in device memory; by default as
+// carry_in_lds of the tile.
+template <int NT, class Ctx, bool IONS, bool CARRY = carry_in_lds(NT / 64)>
 struct Column {
   using R = typename Ctx::real;  // double, or the adjoint build's taped type
-  static constexpr bool CARRY_LDS = carry_in_lds(NT / 64);
+  static constexpr bool CARRY_LDS = CARRY;
   Ctx &cx;
   const KParams &kp;
   R *lds;
@@ -454,10 +456,12 @@ struct Column {
   // north winds on lane 1 (level 0)
   R msis_ln = 0.0, msis_t = 0.0, msis_ve = 0.0, msis_vn = 0.0;
 
+  // n_alts: the column's nAlts (a batch's members have their own)
+  GV_MEM Column(Ctx &c, const KParams &p, R *l, int column, R step_dt, int n_alts)
+      : cx(c), kp(p), lds(l), j(c.lane()), k(c.lane() - 1), nA(n_alts), L(n_alts + 4),
+        P(n_alts + 8), lv(c.lane() < n_alts + 4), col(column), dt(step_dt) {}
   GV_MEM Column(Ctx &c, const KParams &p, R *l, int column, R step_dt)
-      : cx(c), kp(p), lds(l), j(c.lane()), k(c.lane() - 1), nA(p.n_alts),
-        L(p.n_alts + 4), P(p.n_alts + 8), lv(c.lane() < p.n_alts + 4), col(column),
-        dt(step_dt) {}
+      : Column(c, p, l, column, step_dt, p.n_alts) {}
 
   // record layout: tables | exchange buffers | carries Y, acc | ions
   static constexpr int RS = record_len(IONS, CARRY_LDS);
@@ -1589,14 +1593,16 @@ GV_FN double *fld(const gitm_vert_field &f, long long base, int k, int comp) {
   return f.ptr + base + (long long)(k - f.first_level) * f.s_lev + (long long)comp * f.s_comp;
 }
 
-// One column of `io` (col = its index there): gather, n_steps steps of Dt =
-// dt, scatter.  scratch: the RK4 carries of io's columns in device memory;
-// err_col: the column number that error reports carry.
-template <int NT, int SOLVER, bool IONS, class Ctx>
+// One column of `io` (col = its index there, n_alts its nAlts): gather,
+// n_steps steps of Dt = dt, scatter.  scratch: the RK4 carries of io's
+// columns in device memory, scr_fields per level; err_col: the column number
+// that error reports carry.
+template <int NT, int SOLVER, bool IONS, bool CARRY, class Ctx>
 GV_FN void run_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io, int col,
-                      double *lds, double *scratch, double dt, int err_col) {
-  Column<NT, Ctx, IONS> c(cx, kp, lds, err_col, dt);
-  c.scr = scratch + (long long)col * kp.scr_fields * (kp.n_alts + 4);
+                      double *lds, double *scratch, double dt, int err_col, int n_alts,
+                      int scr_fields) {
+  Column<NT, Ctx, IONS, CARRY> c(cx, kp, lds, err_col, dt, n_alts);
+  c.scr = scratch + (long long)col * scr_fields * (n_alts + 4);
   const int nlb = io.n_lons * io.n_lats;
   const int ci = col % io.n_lons, cj = (col / io.n_lons) % io.n_lats, cb = col / nlb;
   const int k = c.k, nA = c.nA;
@@ -1752,11 +1758,13 @@ GV_FN void run_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io, i
   }
 }
 
-// One column of a single-grid launch: the run's Dt, the handle's scratch.
+// One column of a single-grid launch: the run's Dt, nAlts and tile, the
+// handle's scratch.
 template <int NT, int SOLVER, bool IONS, class Ctx>
 GV_FN void run_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io, int col,
                       double *lds) {
-  run_column<NT, SOLVER, IONS>(cx, kp, io, col, lds, kp.scratch, kp.dt, col);
+  run_column<NT, SOLVER, IONS, carry_in_lds(NT / 64)>(cx, kp, io, col, lds, kp.scratch, kp.dt,
+                                                      col, kp.n_alts, kp.scr_fields);
 }
 
 }  // namespace gv

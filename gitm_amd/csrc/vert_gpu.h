@@ -35,17 +35,66 @@ struct GpuCtx {
   }
 };
 
-// One member of a batch as the kernels see it.  A group of members with the
-// same nAlts runs as one launch; workgroup b of the launch belongs to the
-// member with the largest first_block <= b.
+// One column per wavefront of a larger workgroup (the short columns of a
+// batch step): lanes are numbered from the wavefront's first thread, and a
+// synchronisation is the wavefront's own.  A wavefront's LDS operations are
+// carried out in the order it issues them, so the fences (which wait for
+// them and keep the compiler from moving memory operations across) order
+// its lanes' LDS writes before the reads that follow, and no s_barrier,
+// which would stop the workgroup's other columns, is needed.
+struct WaveCtx : GpuCtx {
+  int base;  // first thread of the wavefront
+  __device__ int lane() const { return (int)threadIdx.x - base; }
+  __device__ void sync() const {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
+  __device__ bool sync_or(bool p) const {
+    sync();
+    return __builtin_amdgcn_ballot_w64(p) != 0;
+  }
+};
+
+}  // namespace
+
+// The batch's member table and step layout are shared by the batch's host
+// code and the ensemble kernel's objects, so they have external linkage.
+namespace gv {
+
+// One member of a batch as the kernels see it.  The batch's columns are
+// numbered per class: the columns of more than 64 levels (tall, one
+// workgroup each) and those of at most 64 (short, one wavefront each); the
+// member table holds the tall members, then the short ones, and column c of
+// a class belongs to the member of that class with the largest
+// first_block <= c.
 struct BatchMember {
   gitm_vert_columns io;
   double *scratch;  // RK4 carries of the member's columns
-  int first_block;  // the member's first workgroup in its group's launch
+  int first_block;  // the member's first column in its class
   int member;       // index in the caller's member list (dt_device)
   int col_offset;   // first column of the member in the batch-wide numbering
+  int carry_lds;    // the RK4 carries of its columns in LDS (1) or in scratch (0)
+};
+
+// Work of one batch step: tasks 0..n_tall-1 are the tall columns, one per
+// workgroup; the tasks after them hold per_wg short columns each, one per
+// wavefront, in LDS slots of slot_doubles doubles.
+struct EnsLayout {
+  int n_tall_members, n_short_members;  // table entries of each class
+  int n_tall, n_short;                  // columns of each class
+  int per_wg;                           // short columns per workgroup
+  int n_tasks;                          // n_tall + ceil(n_short / per_wg)
+  int slot_doubles;
   int reserved;
 };
+
+}  // namespace gv
+
+namespace {
+
+using gv::BatchMember;
+using gv::EnsLayout;
 
 __device__ __forceinline__ int find_member(const BatchMember *__restrict__ m, int n,
                                            int block) {

@@ -77,13 +77,15 @@ def lib() -> ctypes.CDLL:
         _lib.gitm_vert_batch_create.argtypes = [vp, ctypes.POINTER(_abi.Columns), ci, ci,
                                                 ctypes.POINTER(vp)]
         _lib.gitm_vert_batch_destroy.argtypes = [vp]
-        _lib.gitm_vert_batch_set_tile.argtypes = [vp, ci, ci]
+        _lib.gitm_vert_batch_set_tile.argtypes = [vp, ctypes.POINTER(_abi.BatchTile)]
+        _lib.gitm_vert_batch_get_tile.argtypes = [vp, ctypes.POINTER(_abi.BatchTile)]
         _lib.gitm_vert_batch_advance.argtypes = [vp, ctypes.POINTER(_abi.Params), ci, ci, vp]
         _lib.gitm_vert_batch_prepare.argtypes = [vp, ctypes.POINTER(_abi.PrepParams), vp]
         _lib.gitm_vert_batch_locate.argtypes = [vp, ci, ctypes.POINTER(ci),
                                                 ctypes.POINTER(ci)]
         for name in ("gitm_vert_batch_create", "gitm_vert_batch_destroy",
-                     "gitm_vert_batch_set_tile", "gitm_vert_batch_advance",
+                     "gitm_vert_batch_set_tile", "gitm_vert_batch_get_tile",
+                     "gitm_vert_batch_advance",
                      "gitm_vert_batch_prepare", "gitm_vert_batch_locate"):
             getattr(_lib, name).restype = ctypes.c_int
         _lib.gitm_vert_sources_host.restype = ctypes.c_int
@@ -507,11 +509,14 @@ class VerticalBatch:
 
     ``members`` is a list of device column dicts (as for
     :meth:`VerticalSolver.advance`); their tensors must stay alive and in
-    place for the life of the batch (the batch keeps references).  Members
-    with the same nAlts share one launch; a step is one launch per distinct
-    nAlts, and ``advance``/``prepare`` allocate nothing, so a loop of them can
-    be captured in a CUDA (HIP) graph.
+    place for the life of the batch (the batch keeps references).  A step is
+    one launch over every member's columns (its tile: :meth:`set_tile`), and
+    ``advance``/``prepare`` allocate nothing, so a loop of them can be
+    captured in a CUDA (HIP) graph.
     """
+
+    TILE_KEYS = ("waves_per_workgroup", "columns_per_workgroup", "carries_in_lds",
+                 "persistent")
 
     def __init__(self, solver: VerticalSolver, members: list, ions: bool = False):
         self.solver = solver
@@ -539,11 +544,36 @@ class VerticalBatch:
         except Exception:
             pass
 
-    def set_tile(self, n_alts: int, waves_per_column: int) -> None:
-        """Waves per column for the members with ``n_alts`` (0: all)."""
-        rc = lib().gitm_vert_batch_set_tile(self._b, n_alts, waves_per_column)
+    def set_tile(self, tile=None) -> None:
+        """Tile of the step (``gitm_vert_batch_tile``): a tuple or dict of
+        (waves_per_workgroup, columns_per_workgroup, carries_in_lds,
+        persistent); None for the automatic one."""
+        if tile is None:
+            rc = lib().gitm_vert_batch_set_tile(self._b, None)
+        else:
+            vals = [tile[k] for k in self.TILE_KEYS] if isinstance(tile, dict) else list(tile)
+            rc = lib().gitm_vert_batch_set_tile(self._b, ctypes.byref(_abi.BatchTile(*vals)))
         if rc:
             raise VerticalStepError(rc)
+
+    def tile(self) -> tuple:
+        t = _abi.BatchTile()
+        rc = lib().gitm_vert_batch_get_tile(self._b, ctypes.byref(t))
+        if rc:
+            raise VerticalStepError(rc)
+        return tuple(getattr(t, k) for k in self.TILE_KEYS)
+
+    def candidate_tiles(self) -> list:
+        """Every tile the search tries: from the smallest workgroup that holds
+        the tallest member up to four wavefronts, each count of short columns
+        per workgroup, the RK4 carries where they fit, in LDS or in device
+        memory, a grid of one workgroup per task or a persistent one (the
+        library refuses the tiles the batch does not fit)."""
+        w0 = auto_waves(max(self.n_alts))
+        short = any(na + 4 <= 64 for na in self.n_alts)
+        return [(w, c, carry, pers) for w in range(w0, 5)
+                for c in (range(1, w + 1) if short else (w,))
+                for carry in (2, 1, 0) for pers in (0, 1)]
 
     def locate(self, column: int) -> tuple:
         m, c = ctypes.c_int(), ctypes.c_int()
@@ -600,28 +630,31 @@ class VerticalBatch:
         return dt
 
     def autotune(self, params: StepParams | None = None, solver="ausm",
-                 probe_columns: int = 4096, repeats: int = 3) -> dict:
-        """Pick the waves per column of every nAlts group by timing.
+                 probe_columns: int = 16384, repeats: int = 3) -> dict:
+        """Pick the step's tile by timing.
 
-        For each distinct nAlts, a probe batch over a copy of (at most
-        ``probe_columns``) columns of the group's largest member is stepped
-        with every tile that holds the column (the smallest wave count up to
-        four); the fastest is set on this batch.  The members' own state is
-        not touched.  Returns {nAlts: {waves: ms per step}}.
+        A probe batch holds a copy of the leading columns of every member,
+        about ``probe_columns`` in all and in the members' proportions, so it
+        has the batch's mix of column heights.  It is stepped with every
+        candidate tile (:meth:`candidate_tiles`) that the library accepts; the
+        fastest is set on this batch.  The members' own state is not touched.
+        Returns {tile: ms per probe step}.
         """
         import torch
 
-        out = {}
-        for na in self.n_alts:
-            group = [m for m in self.members if int(m["temp"].shape[1]) - 4 == na]
-            big = max(group, key=lambda m: int(m["temp"].shape[0]))
-            n = min(int(big["temp"].shape[0]), probe_columns)
-            times = {}
-            for w in range(auto_waves(na), 5):
-                probe = {k: v[:n].clone() for k, v in big.items()}
-                b = VerticalBatch(self.solver, [probe], ions=bool(params and
-                                                                  params.use_ion_advection))
-                b.set_tile(na, w)
+        total = sum(int(m["temp"].shape[0]) for m in self.members)
+        frac = min(1.0, probe_columns / total)
+        probe = [{k: v[:max(1, int(v.shape[0] * frac))].clone() for k, v in m.items()}
+                 for m in self.members]
+        b = VerticalBatch(self.solver, probe,
+                          ions=bool(params and params.use_ion_advection))
+        times = {}
+        try:
+            for tile in self.candidate_tiles():
+                try:
+                    b.set_tile(tile)
+                except VerticalStepError:
+                    continue  # the members do not fit this tile
                 b.advance(params, solver, 1)
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
@@ -631,12 +664,11 @@ class VerticalBatch:
                 e1.record()
                 torch.cuda.synchronize()
                 b.check()
-                times[w] = e0.elapsed_time(e1) / repeats
-                b.close()
-            best = min(times, key=times.get)
-            self.set_tile(na, best)
-            out[na] = times
-        return out
+                times[tile] = e0.elapsed_time(e1) / repeats
+        finally:
+            b.close()
+        self.set_tile(min(times, key=times.get))
+        return times
 
 
 def to_device(cols: dict, device="cuda") -> dict:

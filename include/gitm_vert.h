@@ -245,14 +245,13 @@ int gitm_vert_prepare(gitm_vert_handle *h, const gitm_vert_prep_params *p,
  * reference runs one model per MPI job and has no counterpart; each member
  * is stepped exactly as gitm_vert_advance would step it alone.
  *
- * Members with the same nAlts share one launch (one workgroup per column,
- * one lane per level), so a batch step is one launch per distinct nAlts.
- * gitm_vert_batch_create builds the device member table and reserves the
- * scratch memory; afterwards gitm_vert_batch_advance and
- * gitm_vert_batch_prepare only queue kernels on the handle's stream (no
- * allocation, no copy, no synchronisation), so a multi-step loop of them can
- * be captured in a hipGraph.  The member arrays must stay valid (and at the
- * same addresses) for the life of the batch.
+ * A batch step is one launch over every member's columns, one lane per
+ * level (see gitm_vert_batch_tile).  gitm_vert_batch_create builds the
+ * device member table and reserves the scratch memory; afterwards
+ * gitm_vert_batch_advance and gitm_vert_batch_prepare only queue kernels on
+ * the handle's stream (no allocation, no copy, no synchronisation), so a
+ * multi-step loop of them can be captured in a hipGraph.  The member arrays
+ * must stay valid (and at the same addresses) for the life of the batch.
  * ------------------------------------------------------------------------ */
 typedef struct gitm_vert_batch gitm_vert_batch;
 
@@ -263,10 +262,30 @@ int gitm_vert_batch_create(gitm_vert_handle *h, const gitm_vert_columns *members
                            int n_members, int use_ion_advection, gitm_vert_batch **out);
 int gitm_vert_batch_destroy(gitm_vert_batch *b);
 
-/* Waves per column (1..4; 0 = automatic) for the members with nAlts =
- * n_alts, or for every member when n_alts = 0.  May grow the scratch memory
- * (synchronises the handle's stream). */
-int gitm_vert_batch_set_tile(gitm_vert_batch *b, int n_alts, int waves_per_column);
+/* Tile of a batch step.  Workgroups have waves_per_workgroup wavefronts
+ * (1..4, at least the tallest member's (nAlts+4)/64 rounded up).  A column of
+ * more than 64 levels takes a whole workgroup; columns of at most 64 levels
+ * take one wavefront each, columns_per_workgroup (1..waves_per_workgroup) of
+ * them per workgroup.  carries_in_lds: the RK4 carries in LDS (1), in
+ * device memory (0), or (2) in LDS for the members whose columns then still
+ * fit the CU as often as the kernel's registers allow (the short columns
+ * decide together, for a workgroup's slots).  persistent: 1 = as many
+ * workgroups as the GPU holds at
+ * once, each taking every gridDim-th task (valid when every column of more
+ * than 64 levels needs all waves_per_workgroup wavefronts); 0 = one
+ * workgroup per task.  The tile changes the speed, never a result bit. */
+typedef struct gitm_vert_batch_tile {
+  int32_t waves_per_workgroup;
+  int32_t columns_per_workgroup;
+  int32_t carries_in_lds;
+  int32_t persistent;
+} gitm_vert_batch_tile;
+
+/* Sets the tile (NULL: the automatic one, made at creation).  EINVAL for a
+ * tile the members do not fit (or whose LDS exceeds a CU's).  May grow the
+ * scratch memory (synchronises the handle's stream). */
+int gitm_vert_batch_set_tile(gitm_vert_batch *b, const gitm_vert_batch_tile *t);
+int gitm_vert_batch_get_tile(const gitm_vert_batch *b, gitm_vert_batch_tile *t);
 
 /* Advance every member by n_steps steps.  dt_device: NULL to use p->dt for
  * all members, or n_members device doubles, member m stepping with

@@ -4,9 +4,9 @@ column counts stepped together.
 Each member must come out exactly as the single-grid entry point leaves it
 (the same column code runs, so bit for bit), and, through that, within the
 parity bar of the oracle; per-member time steps from the batch prep must
-equal the oracle's; tiles must not change a bit; a hipGraph-captured loop of
-prep + step must reproduce the eager loop on every replay; an error must
-name its member and column.
+equal the oracle's; a step is one launch, and its tiles must not change a
+bit; a hipGraph-captured loop of prep + step must reproduce the eager loop
+on every replay; an error must name its member and column.
 """
 from __future__ import annotations
 
@@ -103,31 +103,92 @@ def test_batch_prepare_per_member_dt(solver):
     b.close()
 
 
-def test_batch_tiles_and_autotune(solver):
+def run_tiles(solver, cols, p, which, tiles):
+    """Steps a fresh batch of ``cols`` once with each tile the library
+    accepts; every member must equal its single-grid step bit for bit.
+    Returns the tiles that ran."""
+    from gitm_amd import vertical
+    ref = [single(solver, c, p, which) for c in cols]
+    ran = []
+    for tile in tiles:
+        dev = [vertical.to_device(c) for c in cols]
+        b = vertical.VerticalBatch(solver, dev, ions=p.use_ion_advection)
+        try:
+            b.set_tile(tile)
+        except vertical.VerticalStepError:
+            b.close()
+            continue
+        assert b.tile() == tuple(tile)
+        print("tile", tile, flush=True)
+        b.advance(p, which, 1)
+        for i in range(len(cols)):
+            assert_same(vertical.to_host(dev[i]), ref[i], f"tile {tile} member {i}")
+        b.close()
+        ran.append(tuple(tile))
+    return ran
+
+
+def test_batch_every_tile_is_bitwise_identical(solver):
+    """One launch per step under every tile of the search: workgroups of 2,
+    3 or 4 wavefronts, 1..4 short columns per workgroup, carries in LDS or in
+    device memory, one workgroup per task or a persistent grid."""
+    from gitm_amd import vertical
+    cols = members()
+    b = vertical.VerticalBatch(solver, [vertical.to_device(c) for c in cols])
+    tiles = b.candidate_tiles()
+    assert b.tile() == (2, 2, 2, 0)  # the automatic tile
+    with pytest.raises(vertical.VerticalStepError):
+        b.set_tile((1, 1, 1, 0))  # 124 levels need 2 wavefronts
+    with pytest.raises(vertical.VerticalStepError):
+        b.set_tile((3, 2, 0, 1))  # a persistent grid needs every tall column to fill it
+    b.close()
+    ran = run_tiles(solver, cols, synth.StepParams(), "ausm", tiles)
+    # 2 waves: 2 short column counts x 3 carry placements x 2 grids; 3 and
+    # 4 waves: one task per workgroup, 3 + 4 short column counts x 3 carry
+    # placements, less four 60-level columns with their carries in LDS
+    # (172 KiB, more than a CU's 160)
+    assert len(ran) == 32, ran
+    assert (4, 4, 1, 0) not in ran and (4, 4, 2, 0) in ran
+    assert any(t[3] == 1 for t in ran)
+
+
+@pytest.mark.parametrize("which,ions", [("rusanov", False), ("ausm", True)])
+def test_batch_tiles_other_paths(solver, which, ions):
+    cols = members(ions)
+    p = synth.StepParams(use_ion_advection=ions)
+    ran = run_tiles(solver, cols, p, which,
+                    [(2, 2, 1, 0), (2, 2, 0, 1), (2, 1, 0, 1), (4, 3, 0, 0), (2, 2, 2, 1)])
+    assert len(ran) == 5
+
+
+def test_batch_mixed_wavefront_counts(solver):
+    """Tall members needing 2 and 3 wavefronts in one launch: the 2-wave
+    columns leave a 3-wave workgroup's last wavefront empty (it ends at
+    once), so only a grid of one workgroup per task is accepted."""
+    from gitm_amd import vertical
+    ms = [synth.Member(synth.GridSpec(1, 1, na), tinf, 700 + i)
+          for i, (na, tinf) in enumerate(((150, 900.0), (70, 1200.0), (50, 800.0),
+                                          (188, 1000.0)))]
+    cols = [synth.member_columns(m) for m in ms]
+    ran = run_tiles(solver, cols, synth.StepParams(), "ausm",
+                    [(3, 3, 1, 0), (3, 1, 0, 0), (3, 3, 0, 1), (4, 4, 0, 0), (3, 3, 2, 0)])
+    assert ran == [(3, 3, 1, 0), (3, 1, 0, 0), (4, 4, 0, 0), (3, 3, 2, 0)]
+
+
+def test_batch_autotune(solver):
     from gitm_amd import vertical
     cols = members()
     p = synth.StepParams()
     ref = [single(solver, c, p, "ausm") for c in cols]
-    for w in (3, 4):
-        dev = [vertical.to_device(c) for c in cols]
-        b = vertical.VerticalBatch(solver, dev)
-        b.set_tile(0, w)
-        b.advance(p, "ausm", 1)
-        for i in range(len(cols)):
-            assert_same(vertical.to_host(dev[i]), ref[i], f"tile {w} member {i}")
-        b.close()
     dev = [vertical.to_device(c) for c in cols]
     b = vertical.VerticalBatch(solver, dev)
-    times = b.autotune(p, "ausm", probe_columns=81)
-    assert sorted(times) == [50, 60, 70, 120]
-    assert sorted(times[120]) == [2, 3, 4] and sorted(times[50]) == [1, 2, 3, 4]
-    assert all(t > 0 for ts in times.values() for t in ts.values())
-    b.advance(p, "ausm", 1)  # the tuned tiles leave the members untouched until now
+    times = b.autotune(p, "ausm", probe_columns=300)
+    assert len(times) == 32 and all(t > 0 for t in times.values())
+    assert b.tile() == min(times, key=times.get)
+    b.advance(p, "ausm", 1)  # the probe runs leave the members untouched until now
     for i in range(len(cols)):
         assert_same(vertical.to_host(dev[i]), ref[i], f"tuned member {i}")
     b.close()
-    with pytest.raises(vertical.VerticalStepError):
-        vertical.VerticalBatch(solver, dev).set_tile(120, 1)  # 124 levels need 2 waves
 
 
 def test_batch_graph_replays(solver):
