@@ -29,7 +29,8 @@ module ModVerticalGPU
      type(gitm_vert_field) :: ndensity_s, vertical_velocity, velocity, &
           temperature, rho, log_ns, ndensity, temp_unit, mean_major_mass, &
           gamma, cmax, eddy, altitude, gravity, inv_radial_distance, dalt, &
-          cos_latitude, idensity_s, ivelocity, sza, mlat, msis_bc
+          cos_latitude, idensity_s, ivelocity, sza, mlat, msis_bc, &
+          tides_east, tides_north, tides_temp
   end type gitm_vert_columns
 
   type, bind(C) :: gitm_vert_params

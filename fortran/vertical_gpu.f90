@@ -19,6 +19,7 @@ subroutine advance_vertical_all_gpu(iError)
   use ModPlanet, only: nSpecies, nSpeciesTotal, nSpeciesAll, nIons, Mass, VertTau
   use ModSources, only: KappaEddyDiffusion
   use ModEUV, only: SZA
+  use ModTides, only: TidesEast, TidesNorth, TidesTemp
   use ModVerticalGPU
 
   implicit none
@@ -69,7 +70,7 @@ subroutine advance_vertical_all_gpu(iError)
   call gpu_step(NDensityS, VerticalVelocity, Velocity, Temperature, Rho, LogNS, NDensity, &
        TempUnitB, MeanMajorMassB, Gamma, cMax_GDB, KappaEddyDiffusion, Altitude_GB, &
        Gravity_GB, InvRadialDistance_GB, dAlt_GB, CosLatitude, IDensityS, IVelocity, &
-       SZA, MLatitude, rc)
+       SZA, MLatitude, TidesEast, TidesNorth, TidesTemp, rc)
   if (rc /= 0) then
      iError = rc
      if (rc /= GITM_VERT_EUNSUPPORTED) then
@@ -95,11 +96,12 @@ contains
   ! the call; each descriptor points at the array's element (iLon=1, iLat=1,
   ! first level, first component, block 1).
   subroutine gpu_step(nds, vvel, vel, temp, rho_, logns_, nden, tu, mmm, gam, cmax, eddy, &
-       alt, grav, invr, dalt, coslat, ions, ivel, sza_, mlat, rc)
+       alt, grav, invr, dalt, coslat, ions, ivel, sza_, mlat, teast, tnorth, ttemp, rc)
     real(c_double), target, intent(inout) :: nds(*), vvel(*), vel(*), temp(*), rho_(*), &
          logns_(*), nden(*), ions(*)
     real(c_double), target, intent(in) :: tu(*), mmm(*), gam(*), cmax(*), eddy(*), alt(*), &
-         grav(*), invr(*), dalt(*), coslat(*), ivel(*), sza_(*), mlat(*)
+         grav(*), invr(*), dalt(*), coslat(*), ivel(*), sza_(*), mlat(*), teast(*), &
+         tnorth(*), ttemp(*)
     integer(c_int), intent(out) :: rc
     type(gitm_vert_columns) :: c
     integer :: nx, ny, nz, i0, ext(5)
@@ -155,6 +157,14 @@ contains
        c%msis_bc%s_lat = 9 * nLons
        c%msis_bc%s_blk = 9 * nLons * nLats
        c%msis_bc%first_level = 0
+    end if
+    ! Tides: TidesEast/North/Temp(-1:nLons+2,-1:nLats+2,2,nBlocksMax), levels
+    ! -1 and 0 (ModTides.f90:29-31): element (1,1,level -1,block 1)
+    if (UseGSWMTides .or. UseWACCMTides) then
+       ext = (/ nx, ny, 2, 0, 0 /)
+       c%tides_east = field_of(c_loc(teast(i0)), ext, 4, -1)
+       c%tides_north = field_of(c_loc(tnorth(i0)), ext, 4, -1)
+       c%tides_temp = field_of(c_loc(ttemp(i0)), ext, 4, -1)
     end if
     rc = gitm_vert_advance_host(VertHandle, p, c, &
          merge(GITM_VERT_AUSM, GITM_VERT_RUSANOV, UseAUSMSolver), 1_c_int)

@@ -21,7 +21,7 @@ ERRORS = {
     2: "singular neutral-friction matrix",
     3: "negative ion density too close to the upper boundary",
     4: "invalid arguments",
-    5: "unsupported option (tides, BC perturbation, UseDamping)",
+    5: "unsupported option (BC perturbation, UseDamping with the AUSM solver)",
     6: "HIP runtime error",
     7: "out of device memory",
     8: "adjoint tape too small for the column",
@@ -41,7 +41,8 @@ class Field(ctypes.Structure):
 FIELD_NAMES = ("ndensity_s", "vertical_velocity", "velocity", "temperature", "rho",
                "log_ns", "ndensity", "temp_unit", "mean_major_mass", "gamma",
                "cmax", "eddy", "altitude", "gravity", "inv_radial_distance", "dalt",
-               "cos_latitude", "idensity_s", "ivelocity", "sza", "mlat", "msis_bc")
+               "cos_latitude", "idensity_s", "ivelocity", "sza", "mlat", "msis_bc",
+               "tides_east", "tides_north", "tides_temp")
 
 
 class Columns(ctypes.Structure):
@@ -103,6 +104,10 @@ CANON = {
 PER_COLUMN = {"coslat": "cos_latitude", "sza": "sza", "mlat": "mlat"}
 # per-column fields with components ([column][component])
 PER_COLUMN_MULTI = {"msis": ("msis_bc", 9)}
+# the bottom tides, [column][east, north, temp][level -1, 0]
+TIDES = ("tides_east", "tides_north", "tides_temp")
+# every key columns_struct takes
+COLUMN_KEYS = set(CANON) | set(PER_COLUMN) | set(PER_COLUMN_MULTI) | {"tides"}
 
 
 def column_field(ptr: int, ncomp: int, nlev: int) -> Field:
@@ -131,6 +136,9 @@ def columns_struct(ptrs: dict, n_cols: int, n_alts: int) -> Columns:
         elif name in PER_COLUMN_MULTI:
             fname, ncomp = PER_COLUMN_MULTI[name]
             setattr(c, fname, Field(addr, ncomp, 0, 0, 0, 1, 0, 0))
+        elif name == "tides":
+            for i, fname in enumerate(TIDES):
+                setattr(c, fname, Field(addr + 16 * i, 6, 0, 0, 1, 0, -1, 0))
         else:
             raise KeyError(name)
     return c

@@ -61,7 +61,7 @@ const char *gitm_vert_strerror(int code) {
     case GITM_VERT_ENEGION: return "negative ion density too close to the upper boundary";
     case GITM_VERT_EINVAL: return "invalid arguments";
     case GITM_VERT_EUNSUPPORTED:
-      return "unsupported option (tides, BC perturbation, UseDamping)";
+      return "unsupported option (BC perturbation, UseDamping with the AUSM solver)";
     case GITM_VERT_EHIP: return "HIP runtime error";
     case GITM_VERT_ENOMEM: return "out of device memory";
     case GITM_VERT_ETAPE: return "adjoint tape too small for the column";
@@ -235,6 +235,7 @@ struct FieldSpec {
   bool written;
   bool per_column;
   bool ions_only;  // staged only with UseIonAdvection
+  bool bottom = false;  // levels first_level..0 only (the tides)
 };
 
 const FieldSpec kFields[] = {
@@ -260,13 +261,18 @@ const FieldSpec kFields[] = {
     {&gitm_vert_columns::sza, 1, 0, false, true, true},
     {&gitm_vert_columns::mlat, 1, 0, false, true, true},
     {&gitm_vert_columns::msis_bc, 9, 0, false, true, false},
+    {&gitm_vert_columns::tides_east, 1, 0, false, false, false, true},
+    {&gitm_vert_columns::tides_north, 1, 0, false, false, false, true},
+    {&gitm_vert_columns::tides_temp, 1, 0, false, false, false, true},
 };
 
 size_t field_span(const gitm_vert_columns &c, const gitm_vert_field &f, const FieldSpec &fs) {
   if (f.s_lon < 0 || f.s_lat < 0 || f.s_blk < 0 || f.s_lev < 0 || f.s_comp < 0) return 0;
   long long last = (long long)(c.n_lons - 1) * f.s_lon + (long long)(c.n_lats - 1) * f.s_lat +
                    (long long)(c.n_blocks - 1) * f.s_blk + (long long)(fs.ncomp - 1) * f.s_comp;
-  if (!fs.per_column)
+  if (fs.bottom)
+    last += (long long)(0 - f.first_level) * f.s_lev;
+  else if (!fs.per_column)
     last += (long long)(c.n_alts + fs.last_level_offset - f.first_level) * f.s_lev;
   return (size_t)(last + 1);
 }

@@ -250,7 +250,8 @@ int gitm_vert_adjoint_taped(gitm_vert_handle *h, const gitm_vert_params *p,
   if (!h || !gout || !gin) return GITM_VERT_EINVAL;
   int rc = gv::check_args(p, c, GITM_VERT_AUSM, 1);
   if (rc) return rc;
-  if (p->use_ion_advection || p->use_msis) return GITM_VERT_EUNSUPPORTED;
+  if (p->use_ion_advection || p->use_msis || p->use_gswm_tides || p->use_waccm_tides)
+    return GITM_VERT_EUNSUPPORTED;
   const gitm_vert_field *req[] = {&gout->ndensity_s, &gout->vertical_velocity, &gout->velocity,
                                   &gout->temperature, &gout->rho, &gin->ndensity_s,
                                   &gin->vertical_velocity, &gin->velocity, &gin->temperature,
@@ -389,7 +390,8 @@ int adjoint_hand(gitm_vert_handle *h, const gitm_vert_params *p, const gitm_vert
   if (!h || !gout || !gin) return GITM_VERT_EINVAL;
   int rc = gv::check_args(p, c, GITM_VERT_AUSM, 1);
   if (rc) return rc;
-  if (p->use_ion_advection || p->use_msis) return GITM_VERT_EUNSUPPORTED;
+  if (p->use_ion_advection || p->use_msis || p->use_gswm_tides || p->use_waccm_tides)
+    return GITM_VERT_EUNSUPPORTED;
   const gitm_vert_field *req[] = {&gout->ndensity_s, &gout->vertical_velocity, &gout->velocity,
                                   &gout->temperature, &gout->rho, &gin->ndensity_s,
                                   &gin->vertical_velocity, &gin->velocity, &gin->temperature,

@@ -112,6 +112,8 @@ struct KParams {
   int n_alts, n_steps, solver;
   int friction, coriolis, ions, improved_ions, nighttime_ion_bcs;
   int msis;                 // UseMSIS: level-0 ghost values from the caller's MSIS
+  int tides;                // bottom tides from the caller's values: 0 off, 1 GSWM, 2 WACCM
+  int damping;              // UseDamping: the Rusanov step's VertTau profile
   int scr_fields;           // fields per level carried in device memory (see carry_in_lds)
   unsigned long long *err;  // first-error word, see error_key()
   double *scratch;          // RK4 carries: n_cols x scr_fields x (nAlts+4) doubles
@@ -453,8 +455,12 @@ struct Column {
   bool plasma = false;  // nighttime plasmasphere column (bc :492-497)
   // UseMSIS: the caller's msis_bcs values of level 0, on the lanes that use
   // them: LogNS(0) of species j on lane j < 6; Temp(0) and the east and
-  // north winds on lane 1 (level 0)
+  // north winds on lane 1 (level 0).  With tides the winds are the tides'
+  // (set after MSIS's, :150-159), and with GSWM tides but no MSIS, msis_t
+  // holds the Temp(0) of the step's first boundary pass (see bcs()).
   R msis_ln = 0.0, msis_t = 0.0, msis_ve = 0.0, msis_vn = 0.0;
+  R tide_t = 0.0;  // tides: TidesTemp of level 0 on lane 1
+  R vtau = 1.0;    // VertTau of the owned level (Rusanov)
 
   // n_alts: the column's nAlts (a batch's members have their own)
   GV_MEM Column(Ctx &c, const KParams &p, R *l, int column, R step_dt, int n_alts)
@@ -605,6 +611,8 @@ struct Column {
     S.tu = tu;
     st(A_GRAV, j) = grav;
     S.gm1 = gam - 1.0;
+    // VertTau(iAlt), UseDamping: vertical_solver_rusanov.f90:581-584
+    vtau = kp.damping ? 15.0 - (1.0 - exp(-1.0 * alt / 1000.0 / 40.0)) * 5.0 : R(kp.vert_tau);
     const R rd = 1.0 / invr;  // RadialDistance_C (ausm :595-597)
     xb(0, j) = alt;
     xb(1, j) = S.eg;
@@ -701,20 +709,34 @@ struct Column {
   }
 
   // -------------------------------------------------------------------------
-  // set_vertical_bcs, set_vertical_bcs.Earth.f90:8-652 (neutrals, UseMSIS=F,
-  // no tides, no BC perturbation).  Ghost values depend only on the interior
-  // and on Temp(0) and LogNS(0) of the non-photochemical species, which the
-  // BCs never change.
+  // set_vertical_bcs, set_vertical_bcs.Earth.f90:8-652 (no BC perturbation).
+  // Without MSIS and tides, ghost values depend only on the interior and on
+  // Temp(0) and LogNS(0) of the non-photochemical species, which the BCs
+  // never change.  `first`: the step's first pass (before the RK stages).
   // -------------------------------------------------------------------------
-  GV_MEM void bcs() {
+  GV_MEM void bcs(bool first = false) {
     // UseMSIS (set_vertical_bcs.Earth.f90:113-137): Temp(0) and the horizontal
     // winds of level 0 take the MSIS values before anything reads them; the
     // MSIS LogNS(0) replace the non-photochemical densities below, after the
-    // hydrostatic fill of level -1 has used the entry values (NS, :93)
-    if (kp.msis && j == 1) {
-      U[F_T] = msis_t;
-      U[F_VE] = msis_ve;
-      U[F_VN] = msis_vn;
+    // hydrostatic fill of level -1 has used the entry values (NS, :93).
+    // Tides (:150-159) then set the winds of level 0 (held in msis_ve/vn)
+    // and GSWM adds TidesTemp to Temp(0), WACCM replaces it.  The drivers
+    // reset the ghost levels to their values after the first pass before
+    // every later pass (the stages leave them alone, vertical_solver_ausm.f90
+    // :163-377, vertical_solver_rusanov.f90:120-369), so GSWM adds to that
+    // Temp(0) every time; msis_t keeps it when MSIS does not set Temp(0).
+    if (j == 1) {
+      if (kp.msis || (kp.tides == 1 && !first)) U[F_T] = msis_t;
+      if (kp.msis || kp.tides) {
+        U[F_VE] = msis_ve;
+        U[F_VN] = msis_vn;
+      }
+      if (kp.tides == 1) {
+        U[F_T] = tide_t + U[F_T];
+        if (first && !kp.msis) msis_t = U[F_T];
+      } else if (kp.tides == 2) {
+        U[F_T] = tide_t;
+      }
     }
     // stage ln(6), w(6), T, vE, vN, vU
 #pragma unroll
@@ -836,9 +858,14 @@ struct Column {
       for (int sp = 0; sp < NSP; sp++)
         vu = vu + ns[sp] * kp.mass[sp] * sc(SC_BOT + 6 * sp + 2 + o) / er;
       U[F_VU] = vu;
-      if (!kp.msis) {  // :146-148
-        U[F_VE] = 0.0;
-        U[F_VN] = 0.0;
+      if (!kp.msis) {
+        if (!kp.tides) {  // :139-142
+          U[F_VE] = 0.0;
+          U[F_VN] = 0.0;
+        } else if (j == 0) {  // the tides' level 0 (:150-159), Vel(-1) = Vel(0) (:192-196)
+          U[F_VE] = xb(13, 1);
+          U[F_VN] = xb(14, 1);
+        }
       } else if (j == 0) {
         // Vel(-1) east and north extrapolated from levels 0..4 (:364-375)
         const R *CB = &sc(SC_CB);
@@ -1463,7 +1490,7 @@ struct Column {
                   dt * (vv * gw[s] -
                         (U[F_VN] * U[F_VN] + U[F_VE] * U[F_VE]) * st(A_INVR, j) +
                         T * gln[s] * KB / m[s] + ga[0] * KB / m[s] - st(A_GRAV, j)) +
-                  dt * dw[s] - vv / kp.vert_tau;
+                  dt * dw[s] - vv / vtau;
       if (kp.coriolis)
         ww = ww + dt * (centrifugal / st(A_INVR, j) + coriolis * U[F_VE]);
       ww = ww - dt * (kp.tdc[s] * KB * ga[0]) / m[s];
@@ -1546,7 +1573,7 @@ struct Column {
   // the reference's left-to-right order; the boundary conditions follow every
   // stage (ghost levels never enter the combination).
   GV_MEM void ausm_step() {
-    bcs();
+    bcs(true);
     if (cell()) {
 #pragma unroll
       for (int f = 0; f < NF; f++) Yv(f) = U[f];
@@ -1562,7 +1589,7 @@ struct Column {
   }
 
   GV_MEM void rusanov_step() {
-    bcs();
+    bcs(true);
     if (cell()) {
 #pragma unroll
       for (int f = 0; f < NF; f++) Yv(f) = U[f];
@@ -1674,6 +1701,13 @@ GV_FN void run_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io, i
       c.msis_ve = m[7 * f.s_comp];
       c.msis_vn = m[8 * f.s_comp];
     }
+  }
+  if (kp.tides && c.j == 1) {
+    // TidesEast/North/Temp of level 0 (:150-159); level -1's are overwritten
+    // by level 0's (:192-196)
+    c.msis_ve = *fld(io.tides_east, col_base(io.tides_east, ci, cj, cb), 0, 0);
+    c.msis_vn = *fld(io.tides_north, col_base(io.tides_north, ci, cj, cb), 0, 0);
+    c.tide_t = *fld(io.tides_temp, col_base(io.tides_temp, ci, cj, cb), 0, 0);
   }
   c.setup(alt, grav, invr, dalt, mmm, gam, eddy, cmax, tu, coslat);
 

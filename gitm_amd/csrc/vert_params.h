@@ -46,10 +46,12 @@ inline int check_args(const gitm_vert_params *p, const gitm_vert_columns *c,
   if (c->n_alts < 10 || c->n_alts > GITM_VERT_MAX_NALTS) return GITM_VERT_EINVAL;
   if (c->n_lons < 1 || c->n_lats < 1 || c->n_blocks < 1) return GITM_VERT_EINVAL;
   if ((long long)c->n_lons * c->n_lats * c->n_blocks > 0x7fffffffLL) return GITM_VERT_EINVAL;
-  if (p->use_gswm_tides || p->use_waccm_tides || p->use_bc_perturbation ||
-      p->during_perturb || p->use_damping)
-    return GITM_VERT_EUNSUPPORTED;
+  if (p->use_bc_perturbation || p->during_perturb) return GITM_VERT_EUNSUPPORTED;
+  if (p->use_damping && solver == GITM_VERT_AUSM) return GITM_VERT_EUNSUPPORTED;
   if (p->use_msis && !field_ok(c->msis_bc)) return GITM_VERT_EINVAL;
+  if ((p->use_gswm_tides || p->use_waccm_tides) &&
+      !(field_ok(c->tides_east) && field_ok(c->tides_north) && field_ok(c->tides_temp)))
+    return GITM_VERT_EINVAL;
   if (!(p->dt > 0.0) || !(p->vert_tau != 0.0)) return GITM_VERT_EINVAL;
   for (int s = 0; s < NSP; s++)
     if (!(p->mass[s] > 0.0)) return GITM_VERT_EINVAL;
@@ -99,6 +101,9 @@ inline KParams make_kparams(const gitm_vert_params *p, const gitm_vert_columns *
   k.improved_ions = p->use_improved_ion_advection != 0;
   k.nighttime_ion_bcs = p->use_nighttime_ion_bcs != 0;
   k.msis = p->use_msis != 0;
+  // WACCM's assignments follow GSWM's and replace all of them (:150-159)
+  k.tides = p->use_waccm_tides ? 2 : p->use_gswm_tides ? 1 : 0;
+  k.damping = p->use_damping != 0;
   k.scr_fields = 0;  // set with the tile (scratch_fields)
   k.err = err;
   return k;

@@ -286,6 +286,9 @@ class StepParams:
     use_msis: bool = False             # ModInputs.f90:190 (needs cols["msis"])
     min_tec: float = 2.0               # ModInputs.f90:261
     vert_tau: float = 1.0e9            # ModEarth.f90:325
+    use_gswm_tides: bool = False       # ModInputs.f90:195 (needs cols["tides"])
+    use_waccm_tides: bool = False      # ModInputs.f90:196 (needs cols["tides"])
+    use_damping: bool = False          # ModInputs.f90:221 (Rusanov only)
 
 
 # ---------------------------------------------------------------------------
@@ -349,6 +352,25 @@ def msis_bc_values(cols: dict, seed: int = 1, dlog: float = 0.05, dtemp: float =
     out[:, :6] = np.log(cols["nds"][:, :, 1]) + dlog * z[:, :6]
     out[:, 6] = cols["temp"][:, 1] * cols["tempunit"][:, 1] * (1.0 + dtemp * z[:, 6])
     out[:, 7:] = wind * z[:, 7:]
+    return out
+
+
+def tides_values(cols: dict, seed: int = 1, wind: float = 15.0, dtemp: float = 8.0,
+                 absolute: bool = False) -> np.ndarray:
+    """Stand-in values of GITM's TidesEast, TidesNorth and TidesTemp
+    (tides.f90) at each column's levels -1 and 0 ([column][east, north,
+    temp][level -1, 0]), seeded: winds N(0, wind) m/s; the temperature a
+    perturbation N(0, dtemp) K (GSWM's, added to Temp), or with ``absolute``
+    the column's level-0 temperature in K plus that perturbation (WACCM's,
+    which replaces it)."""
+    nc = cols["temp"].shape[0]
+    z = normal(seed, 303, (nc, 3, 2))
+    out = np.empty((nc, 3, 2))
+    out[:, :2, :] = wind * z[:, :2, :]
+    out[:, 2, :] = dtemp * z[:, 2, :]
+    if absolute:
+        t0 = cols["temp"][:, 0:2] * cols["tempunit"][:, 0:2]
+        out[:, 2, :] += t0
     return out
 
 

@@ -35,7 +35,9 @@ OUTPUTS = ("logns", "nden")
 INPUTS = ("tempunit", "mmm", "gamma", "cmax", "eddy", "alt", "grav", "invr", "dalt",
           "coslat")
 ION_FIELDS = ("ions", "ivel", "sza", "mlat")  # read/written with UseIonAdvection
-MSIS_FIELDS = ("msis",)  # UseMSIS: level-0 ghost values per column
+# bottom boundary values: UseMSIS's level 0 per column; the tides' levels -1
+# and 0 per column ([column][east, north, temp][level])
+BOTTOM_FIELDS = ("msis", "tides")
 
 _lib = None
 
@@ -136,6 +138,9 @@ def make_params(sp: StepParams | None = None) -> _abi.Params:
     p.use_improved_ion_advection = int(sp.use_improved_ion_advection)
     p.use_nighttime_ion_bcs = int(sp.use_nighttime_ion_bcs)
     p.use_msis = int(getattr(sp, "use_msis", False))
+    p.use_gswm_tides = int(getattr(sp, "use_gswm_tides", False))
+    p.use_waccm_tides = int(getattr(sp, "use_waccm_tides", False))
+    p.use_damping = int(getattr(sp, "use_damping", False))
     return p
 
 
@@ -173,8 +178,7 @@ def _device_columns(cols: dict, extra: tuple = ()):
 
     ptrs = {}
     for k, t in cols.items():
-        if (k not in _abi.CANON and k not in _abi.PER_COLUMN and k not in _abi.PER_COLUMN_MULTI
-                and k not in extra):
+        if k not in _abi.COLUMN_KEYS and k not in extra:
             continue
         if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
             raise ValueError(f"{k}: need a contiguous float64 device tensor")
@@ -487,8 +491,7 @@ class VerticalSolver:
         """Advance host columns in place (numpy FP64, C-contiguous)."""
         ptrs = {}
         for k, a in cols.items():
-            if (k not in _abi.CANON and k not in _abi.PER_COLUMN
-                    and k not in _abi.PER_COLUMN_MULTI):
+            if k not in _abi.COLUMN_KEYS:
                 continue
             if a.dtype != np.float64 or not a.flags.c_contiguous:
                 raise ValueError(f"{k}: need a C-contiguous float64 array")
@@ -677,7 +680,7 @@ def to_device(cols: dict, device="cuda") -> dict:
     import torch
 
     out = {}
-    for k in STATE + INPUTS + ION_FIELDS + MSIS_FIELDS:
+    for k in STATE + INPUTS + ION_FIELDS + BOTTOM_FIELDS:
         if k in cols:
             out[k] = torch.from_numpy(np.ascontiguousarray(cols[k],
                                                           dtype=np.float64)).to(device)

@@ -110,6 +110,12 @@ typedef struct gitm_vert_columns {
    * LogNS(0) of N(4S) and NO, by extrapolation
    * (set_vertical_bcs.Earth.f90:113-137,198-272,364-375). */
   gitm_vert_field msis_bc;
+  /* UseGSWMTides / UseWACCMTides only: TidesEast, TidesNorth, TidesTemp
+   * (ModTides.f90:29-31, (-1:nLons+2,-1:nLats+2,2,nBlocksMax)), levels -1
+   * and 0 (first_level = -1); the east and north winds of levels -1:0 and
+   * the temperature in K (GSWM: added to Temp(-1:0); WACCM: replaces it),
+   * set_vertical_bcs.Earth.f90:150-159. */
+  gitm_vert_field tides_east, tides_north, tides_temp;
 } gitm_vert_columns;
 
 /* Run parameters (ModInputs / ModPlanet values the path reads). */
@@ -128,11 +134,18 @@ typedef struct gitm_vert_params {
   int32_t use_improved_ion_advection; /* :259                               */
   int32_t use_nighttime_ion_bcs;      /* :260                               */
   int32_t use_msis;                   /* :190; needs c->msis_bc          */
-  /* Must be 0 (otherwise GITM_VERT_EUNSUPPORTED): */
-  int32_t use_gswm_tides, use_waccm_tides; /* :195-196                      */
+  int32_t use_gswm_tides, use_waccm_tides; /* :195-196; need c->tides_*   */
+  /* Must be 0 (otherwise GITM_VERT_EUNSUPPORTED): the user's BC
+   * perturbation (user_bc_perturbation, user.f90:176) */
   int32_t use_bc_perturbation;        /* :402                               */
   int32_t during_perturb;             /* :121                               */
-  int32_t use_damping;                /* :221                               */
+  /* UseDamping (:221): the Rusanov step's VertTau profile
+   * (vertical_solver_rusanov.f90:581-584).  With the AUSM solver
+   * GITM_VERT_EUNSUPPORTED: the reference's assignment there
+   * (vertical_solver_ausm.f90:857-860) indexes VertTau with the exhausted
+   * loop variable nAlts+3, outside the array, and the AUSM step never reads
+   * VertTau, so its effect is whatever that store overwrites. */
+  int32_t use_damping;
   int32_t reserved;
 } gitm_vert_params;
 

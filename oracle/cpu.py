@@ -25,7 +25,9 @@ class _Params(ctypes.Structure):
                 ("use_ion_advection", ctypes.c_int),
                 ("use_improved_ion_advection", ctypes.c_int),
                 ("use_nighttime_ion_bcs", ctypes.c_int), ("min_tec", ctypes.c_double),
-                ("vert_tau", ctypes.c_double), ("use_msis", ctypes.c_int)]
+                ("vert_tau", ctypes.c_double), ("use_msis", ctypes.c_int),
+                ("use_gswm_tides", ctypes.c_int), ("use_waccm_tides", ctypes.c_int),
+                ("use_damping", ctypes.c_int)]
 
 
 _P = ctypes.POINTER(ctypes.c_double)
@@ -36,7 +38,7 @@ class _Cols(ctypes.Structure):
         (n, _P) for n in ("nds", "vvel", "vel", "temp", "rho", "logns", "nden",
                           "tempunit", "mmm", "gamma", "cmax", "eddy", "alt", "grav",
                           "invr", "dalt", "coslat", "ions", "ivel", "sza", "mlat",
-                          "msis")]
+                          "msis", "tides")]
 
 
 class _Status(ctypes.Structure):
@@ -95,12 +97,17 @@ def advance(cols: dict, params, solver: int = AUSM, n_steps: int = 1,
     msis = bool(getattr(params, "use_msis", False))
     if msis:
         keep["msis"] = np.ascontiguousarray(cols["msis"], dtype=np.float64)
+    gswm = bool(getattr(params, "use_gswm_tides", False))
+    waccm = bool(getattr(params, "use_waccm_tides", False))
+    if gswm or waccm:
+        keep["tides"] = np.ascontiguousarray(cols["tides"], dtype=np.float64)
     p = _Params(na, params.dt, params.omega_body, params.max_vert_vel,
                 params.beta_limiter, int(params.use_neutral_friction),
                 int(params.use_coriolis), int(ions),
                 int(params.use_improved_ion_advection),
                 int(params.use_nighttime_ion_bcs), params.min_tec, params.vert_tau,
-                int(msis))
+                int(msis), int(gswm), int(waccm),
+                int(getattr(params, "use_damping", False)))
     c = _Cols()
     c.n_cols = nc
     c.n_alts = na

@@ -23,9 +23,10 @@
  * HydroNT/HydroPressure/HydroRho, the interim NewLogRho update, Grad/Diff of
  * LogPress/LogRho/LogNum, electron ghost values) is omitted.
  *
- * Only the UseMSIS=F, no-tides, no-BC-perturbation, UseDamping=F branch of
- * the boundary conditions is restated; the product library rejects the
- * other settings in the same way.
+ * The boundary conditions are restated with UseMSIS (from the caller's
+ * level-0 msis_bcs values), the GSWM and WACCM tides (from the caller's
+ * TidesEast/North/Temp) and UseDamping for the Rusanov step; the user's BC
+ * perturbation is not (the product library rejects it in the same way).
  */
 #include "gitm_vert_oracle.h"
 
@@ -90,6 +91,8 @@ typedef struct {
   double kp[LM], ku[LM]; /* LiouKpS, Ku, vertical_solver_ausm.f90:1320-1352 */
   double mass[NSP];
   double msis[9]; /* UseMSIS: msis_bcs' LogNS(0,1:6), Temp(0), east, north wind */
+  double tides[3][2]; /* TidesEast, TidesNorth, TidesTemp at levels -1, 0 */
+  double vtau[LM];    /* VertTau(iAlt) of the Rusanov step */
 } ov_col;
 
 typedef struct {
@@ -406,6 +409,20 @@ static int set_vertical_bcs(ov_col *c, const ov_params *p, ov_state *s) {
     s->vel[iU_][X(k)] = 0.0;
     for (sp = 0; sp < NSP; sp++) s->vv[sp][X(k)] = 0.0;
   }
+  /* :150-154 */
+  if (p->use_gswm_tides)
+    for (k = -1; k <= 0; k++) {
+      s->vel[iE_][X(k)] = c->tides[0][k + 1];
+      s->vel[iN_][X(k)] = c->tides[1][k + 1];
+      T[X(k)] = c->tides[2][k + 1] + T[X(k)];
+    }
+  /* :155-159 */
+  if (p->use_waccm_tides)
+    for (k = -1; k <= 0; k++) {
+      s->vel[iE_][X(k)] = c->tides[0][k + 1];
+      s->vel[iN_][X(k)] = c->tides[1][k + 1];
+      T[X(k)] = c->tides[2][k + 1];
+    }
   /* :192-196 */
   T[X(-1)] = T[X(0)];
   for (d = 0; d < 3; d++) s->vel[d][X(-1)] = s->vel[d][X(0)];
@@ -1035,7 +1052,7 @@ static int rusanov_stage(const ov_col *c, const ov_params *p, const ov_state *u,
                  dt * (vv * GVV[sp][i] - (vn * vn + ve * ve) * c->invr[X(i)] +
                        u->temp[X(i)] * GLNS[sp][i] * kB / m[sp] +
                        GT[i] * kB / m[sp] - c->gravG[X(i)]) +
-                 dt * DVV[sp][i] - vv / p->vert_tau;
+                 dt * DVV[sp][i] - vv / c->vtau[X(i)];
       if (p->use_coriolis)
         w = w + dt * (c->centrifugal / c->invr[X(i)] + c->coriolis * ve);
       w = w - dt * (TDC[sp] * kB * GT[i]) / m[sp];
@@ -1271,6 +1288,13 @@ static void static_column(ov_col *c, const ov_params *p, const ov_columns *cs,
   c->mlat = cs->mlat ? cs->mlat[col] : 0.0;
   if (p->use_msis)
     for (k = 0; k < 9; k++) c->msis[k] = cs->msis[(size_t)col * 9 + k];
+  if (p->use_gswm_tides || p->use_waccm_tides)
+    for (k = 0; k < 6; k++) c->tides[k / 2][k % 2] = cs->tides[(size_t)col * 6 + k];
+  /* VertTau: ModEarth.f90:325, or with UseDamping vertical_solver_rusanov.f90
+   * :581-584 (the AUSM step never reads it) */
+  for (k = 0; k < L; k++)
+    c->vtau[k] = p->use_damping ? 15 - (1 - exp(-1.0 * c->altG[k] / 1000.0 / 40.0)) * 5.0
+                                : p->vert_tau;
   /* :149-151 */
   for (k = 0; k <= nA + 2; k++) c->dAltF[X(k)] = c->altG[X(k)] - c->altG[X(k - 1)];
   c->dAltF[X(-1)] = c->dAltF[X(0)];

@@ -48,6 +48,9 @@ typedef struct {
   double min_tec;             /* ModInputs.f90:261                         */
   double vert_tau;            /* VertTau(:) (init_planet, ModEarth.f90:325) */
   int use_msis;               /* ModInputs.f90:190: level-0 ghosts from msis */
+  int use_gswm_tides;         /* ModInputs.f90:195                         */
+  int use_waccm_tides;        /* ModInputs.f90:196                         */
+  int use_damping;            /* ModInputs.f90:221 (Rusanov VertTau)       */
 } ov_params;
 
 typedef struct {
@@ -74,6 +77,9 @@ typedef struct {
   /* UseMSIS: msis_bcs' level-0 values [col][9]: LogNS(0,1:6), Temp(0) in
    * K, east and north wind */
   const double *msis;
+  /* tides: TidesEast, TidesNorth, TidesTemp (K) at levels -1, 0
+   * [col][3][2] */
+  const double *tides;
 } ov_columns;
 
 typedef struct {

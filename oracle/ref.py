@@ -42,6 +42,9 @@ def advance(cols: dict, params, solver: int = 1, n_steps: int = 1, exe: str = EX
                 int(params.use_coriolis), int(ions),
                 int(params.use_improved_ion_advection),
                 int(params.use_nighttime_ion_bcs)]
+    gswm = bool(getattr(params, "use_gswm_tides", False))
+    waccm = bool(getattr(params, "use_waccm_tides", False))
+    hdr[10:13] = [int(gswm), int(waccm), int(getattr(params, "use_damping", False))]
     par = np.zeros(8)
     par[:5] = [params.dt, params.omega_body, params.max_vert_vel,
                params.beta_limiter, params.min_tec]
@@ -49,6 +52,8 @@ def advance(cols: dict, params, solver: int = 1, n_steps: int = 1, exe: str = EX
              "cmax", "eddy", "alt", "grav", "invr", "dalt", "coslat"]
     if ions:
         order += ["ions", "ivel", "sza", "mlat"]
+    if gswm or waccm:
+        order += ["tides"]
     with tempfile.TemporaryDirectory() as td:
         fin = os.path.join(td, "in.bin")
         fout = os.path.join(td, "out.bin")

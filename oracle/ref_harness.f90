@@ -15,7 +15,9 @@
 !           (level fastest, [col][component][level]) in the order
 !           nds(6) vvel(6) vel(3) temp rho tempunit mmm gamma cmax eddy alt
 !           grav invr dalt, coslat(col), and if ions: ions(10) ivel(3),
-!           sza(col), mlat(col)
+!           sza(col), mlat(col), and if tides (hdr(11) GSWM or hdr(12) WACCM):
+!           TidesEast, TidesNorth, TidesTemp at levels -1, 0 ([col][3][2]);
+!           hdr(13): UseDamping
 !   output: int32 status(4), then nds(6) vvel(6) vel(3) temp rho logns(6)
 !           nden, and if ions: ions(10)
 program gitm_vert_ref
@@ -26,18 +28,19 @@ program gitm_vert_ref
   use ModInputs
   use ModSources, only: KappaEddyDiffusion, EuvHeating, init_mod_sources
   use ModEUV, only: SZA
+  use ModTides, only: TidesEast, TidesNorth, TidesTemp
 
   implicit none
 
   integer :: hdr(16), status(4)
   real :: par(8)
   integer :: nCols, nA, solver, nStepsH, iCol, iStep, L, k, s, d, nIonsIO
-  logical :: doIons
+  logical :: doIons, doTides
   real, allocatable :: nds(:,:,:), vvel(:,:,:), vel(:,:,:), temp(:,:), rhoH(:,:)
   real, allocatable :: tempunit_(:,:), mmm(:,:), gam(:,:), cmax(:,:), eddy(:,:)
   real, allocatable :: alt(:,:), grav(:,:), invr(:,:), dalt(:,:), coslat(:)
   real, allocatable :: ions(:,:,:), ivel(:,:,:), sza_(:), mlat(:)
-  real, allocatable :: logns_o(:,:,:), nden_o(:,:)
+  real, allocatable :: logns_o(:,:,:), nden_o(:,:), tides(:,:,:)
   character(len=512) :: fin, fout
 
   call get_command_argument(1, fin)
@@ -85,6 +88,11 @@ program gitm_vert_ref
      read(10) sza_
      read(10) mlat
   endif
+  doTides = hdr(11) /= 0 .or. hdr(12) /= 0
+  if (doTides) then
+     allocate(tides(2,3,nCols))
+     read(10) tides
+  endif
   close(10)
 
   ! Model set-up the reference performs before any step
@@ -101,10 +109,10 @@ program gitm_vert_ref
   UseImprovedIonAdvection    = (hdr(9) /= 0)
   UseNighttimeIonBCs         = (hdr(10) /= 0)
   UseMSIS                    = .false.
-  UseGSWMTides               = .false.
-  UseWACCMTides              = .false.
+  UseGSWMTides               = (hdr(11) /= 0)
+  UseWACCMTides              = (hdr(12) /= 0)
   UseBcPerturbation          = .false.
-  UseDamping                 = .false.
+  UseDamping                 = (hdr(13) /= 0)
   Dt                         = par(1)
   OmegaBodyInput             = par(2)
   MaximumVerticalVelocity    = par(3)
@@ -160,6 +168,11 @@ program gitm_vert_ref
         enddo
         SZA(1,1,1) = sza_(iCol)
         MLatitude(1,1,nA,1) = mlat(iCol)
+     endif
+     if (doTides) then
+        TidesEast(1,1,1:2,1) = tides(:,1,iCol)
+        TidesNorth(1,1,1:2,1) = tides(:,2,iCol)
+        TidesTemp(1,1,1:2,1) = tides(:,3,iCol)
      endif
 
      do iStep = 1, nStepsH

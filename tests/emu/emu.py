@@ -61,6 +61,9 @@ def params_from(sp) -> _abi.Params:
     p.use_improved_ion_advection = int(sp.use_improved_ion_advection)
     p.use_nighttime_ion_bcs = int(sp.use_nighttime_ion_bcs)
     p.use_msis = int(getattr(sp, "use_msis", False))
+    p.use_gswm_tides = int(getattr(sp, "use_gswm_tides", False))
+    p.use_waccm_tides = int(getattr(sp, "use_waccm_tides", False))
+    p.use_damping = int(getattr(sp, "use_damping", False))
     return p
 
 
@@ -86,6 +89,9 @@ def advance(cols: dict, sp, solver: int = 1, n_steps: int = 1, waves: int = 0,
             keep[k] = np.ascontiguousarray(cols[k], dtype=np.float64)
     if getattr(sp, "use_msis", False) and "msis" in cols:
         keep["msis"] = np.ascontiguousarray(cols["msis"], dtype=np.float64)
+    if "tides" in cols and (getattr(sp, "use_gswm_tides", False) or
+                            getattr(sp, "use_waccm_tides", False)):
+        keep["tides"] = np.ascontiguousarray(cols["tides"], dtype=np.float64)
     ptrs = {k: v.ctypes.data for k, v in out.items()}
     ptrs.update({k: v.ctypes.data for k, v in keep.items()})
     c = _abi.columns_struct(ptrs, nc, na)

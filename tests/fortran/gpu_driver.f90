@@ -15,6 +15,7 @@ program gitm_vert_gpu_driver
   use ModInputs
   use ModSources, only: KappaEddyDiffusion, EuvHeating, init_mod_sources
   use ModEUV, only: SZA
+  use ModTides, only: TidesEast, TidesNorth, TidesTemp
   use ModVerticalGPU
 
   implicit none
@@ -22,12 +23,12 @@ program gitm_vert_gpu_driver
   integer :: hdr(16), status(4)
   real :: par(8)
   integer :: nCols, nA, solver, nStepsH, iCol, iStep, L, s, d, iLon, iLat, iB, iError
-  logical :: doIons
+  logical :: doIons, doTides
   real, allocatable :: nds(:,:,:), vvel(:,:,:), vel(:,:,:), temp(:,:), rhoH(:,:)
   real, allocatable :: tempunit_(:,:), mmm(:,:), gam(:,:), cmax(:,:), eddy(:,:)
   real, allocatable :: alt(:,:), grav(:,:), invr(:,:), dalt(:,:), coslat(:)
   real, allocatable :: ions(:,:,:), ivel(:,:,:), sza_(:), mlat(:)
-  real, allocatable :: logns_o(:,:,:), nden_o(:,:)
+  real, allocatable :: logns_o(:,:,:), nden_o(:,:), tides(:,:,:)
   character(len=512) :: fin, fout
 
   call get_command_argument(1, fin)
@@ -75,6 +76,11 @@ program gitm_vert_gpu_driver
      read(10) sza_
      read(10) mlat
   endif
+  doTides = hdr(11) /= 0 .or. hdr(12) /= 0
+  if (doTides) then
+     allocate(tides(2,3,nCols))
+     read(10) tides
+  endif
   close(10)
 
   nBlocks = nCols / (nLons*nLats)
@@ -91,10 +97,10 @@ program gitm_vert_gpu_driver
   UseImprovedIonAdvection    = (hdr(9) /= 0)
   UseNighttimeIonBCs         = (hdr(10) /= 0)
   UseMSIS                    = .false.
-  UseGSWMTides               = .false.
-  UseWACCMTides              = .false.
+  UseGSWMTides               = (hdr(11) /= 0)
+  UseWACCMTides              = (hdr(12) /= 0)
   UseBcPerturbation          = .false.
-  UseDamping                 = .false.
+  UseDamping                 = (hdr(13) /= 0)
   Dt                         = par(1)
   OmegaBodyInput             = par(2)
   MaximumVerticalVelocity    = par(3)
@@ -159,6 +165,11 @@ program gitm_vert_gpu_driver
         enddo
         SZA(iLon,iLat,iB) = sza_(iCol)
         MLatitude(iLon,iLat,nA,iB) = mlat(iCol)
+     endif
+     if (doTides) then
+        TidesEast(iLon,iLat,1:2,iB) = tides(:,1,iCol)
+        TidesNorth(iLon,iLat,1:2,iB) = tides(:,2,iCol)
+        TidesTemp(iLon,iLat,1:2,iB) = tides(:,3,iCol)
      endif
   enddo
 

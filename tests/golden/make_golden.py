@@ -45,11 +45,25 @@ CASES = [
     # (check_for_negative_densities, :608-649)
     ("rusanov_ions_negative_top", 0, 2, 8, 31, {"use_ion_advection": True},
      {"ions": True}),
+    # the bottom tides (set_vertical_bcs.Earth.f90:150-159): GSWM adds its
+    # temperature to Temp(-1:0) at every boundary pass, WACCM replaces it
+    ("ausm_gswm_tides_3step", 1, 3, 9, 37, {"use_gswm_tides": True}, {}),
+    ("rusanov_gswm_tides_3step", 0, 3, 9, 37, {"use_gswm_tides": True}, {}),
+    ("ausm_ions_waccm_tides_2step", 1, 2, 12, 61,
+     {"use_ion_advection": True, "use_waccm_tides": True}, {"ions": True}),
+    # UseDamping: the Rusanov step's VertTau profile (vertical_solver_rusanov
+    # .f90:581-584), with WACCM tides
+    ("rusanov_damping_waccm_3step", 0, 3, 10, 43,
+     {"use_damping": True, "use_waccm_tides": True}, {}),
 ]
 
 
-def tweak(name: str, sub: dict) -> None:
-    """Edits of the generated columns that steer them into rare branches."""
+def tweak(name: str, sub: dict, seed: int, pover: dict) -> None:
+    """Edits of the generated columns that steer them into rare branches, and
+    the tides' values of the cases that use them."""
+    if pover.get("use_gswm_tides") or pover.get("use_waccm_tides"):
+        sub["tides"] = synth.tides_values(sub, seed=seed,
+                                          absolute=bool(pover.get("use_waccm_tides")))
     if name == "ausm_ions_low_tec":
         sub["ions"] *= 1e-5           # TEC ~1e-4 TECU, far below MinTEC = 2
         sub["ions"][:, :9] = np.maximum(sub["ions"][:, :9], 1.0)
@@ -75,7 +89,7 @@ def main(only=None) -> None:
         cols = synth.make_columns(spec, seed=seed, **gover)
         idx = np.arange(0, spec.ncols, stride)
         sub = {k: np.ascontiguousarray(v[idx]) for k, v in cols.items()}
-        tweak(name, sub)
+        tweak(name, sub, seed, pover)
         params = dataclasses.replace(synth.StepParams(), **pover)
         out = ref.advance(sub, params, solver=solver, n_steps=steps)
         meta = dict(solver=solver, steps=steps, seed=seed, columns=idx.tolist(),

@@ -54,6 +54,29 @@ def test_emulated_kernel_stretched_100_levels(solver, waves):
     assert_bitwise(out, ref, f"nAlts=100 W={waves}")
 
 
+@pytest.mark.parametrize("solver,flags", [
+    (1, {"use_gswm_tides": True}), (0, {"use_waccm_tides": True, "use_damping": True}),
+    (1, {"use_msis": True, "use_gswm_tides": True}), (0, {"use_msis": True, "use_waccm_tides": True}),
+    (0, {"use_damping": True})])
+def test_emulated_kernel_tides_msis_damping(solver, flags):
+    """The bottom tides (alone and after MSIS) and the Rusanov damping on 100
+    stretched levels, 3 steps: the host build of the kernel against the
+    oracle bit for bit (the oracle itself is pinned to the reference build
+    for tides and damping by the golden fixtures; the reference build cannot
+    run UseMSIS)."""
+    cols = synth.make_columns(synth.GridSpec(1, 1, 100, stretched=True), seed=14)
+    cols = subset(cols, [0, 30, 60, 80])
+    if flags.get("use_gswm_tides") or flags.get("use_waccm_tides"):
+        cols["tides"] = synth.tides_values(cols, seed=14,
+                                           absolute=bool(flags.get("use_waccm_tides")))
+    if flags.get("use_msis"):
+        cols["msis"] = synth.msis_bc_values(cols, seed=14)
+    p = synth.StepParams(**flags)
+    out = emu.advance(cols, p, solver=solver, n_steps=3)
+    ref = cpu.advance(cols, p, solver=solver, n_steps=3)
+    assert_bitwise(out, ref, f"{flags} solver {solver}")
+
+
 def test_emulated_kernel_short_column_one_wave():
     cols = synth.make_columns(synth.GridSpec(1, 1, 20), seed=13, tinf=1300.0)
     cols = subset(cols, [1, 2, 3])

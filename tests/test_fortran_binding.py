@@ -23,7 +23,8 @@ FC = "/opt/rocm/bin/amdflang"
 FIELDS = ("ndensity_s", "vertical_velocity", "velocity", "temperature", "rho", "log_ns",
           "ndensity", "temp_unit", "mean_major_mass", "gamma", "cmax", "eddy", "altitude",
           "gravity", "inv_radial_distance", "dalt", "cos_latitude", "idensity_s",
-          "ivelocity", "sza", "mlat", "msis_bc")
+          "ivelocity", "sza", "mlat", "msis_bc",
+          "tides_east", "tides_north", "tides_temp")
 PARAMS = ("dt", "omega_body", "max_vertical_velocity", "beta_limiter", "min_tec", "vert_tau",
           "mass", "thermal_diff_coef", "use_neutral_friction", "use_coriolis",
           "use_ion_advection", "use_improved_ion_advection", "use_nighttime_ion_bcs",
@@ -96,16 +97,23 @@ def test_bind_c_layout_matches_the_header():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("which,ions", [(1, False), (0, False), (1, True)])
-def test_fortran_drop_in_matches_reference(which, ions):
+@pytest.mark.parametrize("which,ions,flags", [(1, False, {}), (0, False, {}), (1, True, {}),
+                                              (1, False, {"use_gswm_tides": True}),
+                                              (0, False, {"use_waccm_tides": True,
+                                                          "use_damping": True})])
+def test_fortran_drop_in_matches_reference(which, ions, flags):
     """The 2x2 blocks of 9x9 columns, nAlts 50: GITM's arrays in place
-    (halo cells, lon-fastest strides, per-block TempUnit), one step, against
-    the reference build at 1e-11 (winds normwise, as in test_gpu_parity)."""
+    (halo cells, lon-fastest strides, per-block TempUnit; the tides'
+    TidesEast/North/Temp of ModTides), one step, against the reference build
+    at 1e-11 (winds normwise, as in test_gpu_parity)."""
     if not (os.path.exists(ref.EXE) and os.path.exists(ref.FORTRAN_GPU_EXE)):
         pytest.skip("the reference build and the Fortran driver are not built")
     cols = synth.make_columns(synth.CONFIGS["cpu_324x50"], seed=31, ions=ions)
-    p = synth.StepParams(use_ion_advection=ions)
-    print(f"fortran drop-in: solver {which}, ions {ions}, 324 columns", flush=True)
+    p = synth.StepParams(use_ion_advection=ions, **flags)
+    if flags.get("use_gswm_tides") or flags.get("use_waccm_tides"):
+        cols["tides"] = synth.tides_values(cols, seed=31,
+                                           absolute=bool(flags.get("use_waccm_tides")))
+    print(f"fortran drop-in: solver {which}, ions {ions}, {flags}, 324 columns", flush=True)
     got = ref.advance(cols, p, solver=which, n_steps=1, exe=ref.FORTRAN_GPU_EXE)
     want = ref.advance(cols, p, solver=which, n_steps=1)
     errs = assert_parity(got, want, 1e-11, what=f"fortran drop-in solver {which}")

@@ -260,3 +260,54 @@ def test_msis_lower_boundary(solver, which):
     solver.advance_host(host, p, which, 2)
     for k in ("nds", "vvel", "vel", "temp", "rho"):
         assert np.array_equal(host[k], dev[k]), k
+
+
+@pytest.mark.parametrize("which,flags", [
+    ("ausm", {"use_gswm_tides": True}),
+    ("rusanov", {"use_waccm_tides": True, "use_damping": True}),
+    ("ausm", {"use_msis": True, "use_gswm_tides": True}),
+    ("rusanov", {"use_damping": True})])
+def test_tides_and_damping(solver, which, flags):
+    """The bottom tides (set_vertical_bcs.Earth.f90:150-159; GSWM adds its
+    temperature at every boundary pass, WACCM replaces it) from the caller's
+    TidesEast/North/Temp, alone and after MSIS, and the Rusanov step's
+    UseDamping VertTau profile (vertical_solver_rusanov.f90:581-584): per
+    step against the oracle on 100 stretched levels, 5 steps, and the
+    host-array entry point equal to the device one."""
+    cols = subset(synth.make_columns(synth.GridSpec(1, 1, 100, stretched=True), seed=43),
+                  np.arange(0, 81, 3))
+    if flags.get("use_gswm_tides") or flags.get("use_waccm_tides"):
+        cols["tides"] = synth.tides_values(cols, seed=43,
+                                           absolute=bool(flags.get("use_waccm_tides")))
+    if flags.get("use_msis"):
+        cols["msis"] = synth.msis_bc_values(cols, seed=43)
+    p = dataclasses.replace(synth.StepParams(), **flags)
+    sid = 1 if which == "ausm" else 0
+    state = {k: np.array(v) for k, v in cols.items()}
+    for step in range(5):
+        out = gpu_run(solver, state, p, which)
+        ref = cpu.advance(state, p, solver=sid, threads=THREADS)
+        assert_parity(out, ref, RTOL, what=f"{flags} {which} step {step + 1}")
+        state.update({k: ref[k] for k in ("nds", "vvel", "vel", "temp", "rho")})
+    dev = gpu_run(solver, cols, p, which, n_steps=2)
+    host = {k: np.array(v) for k, v in cols.items()}
+    host["logns"] = np.zeros_like(host["nds"])
+    host["nden"] = np.zeros_like(host["temp"])
+    solver.advance_host(host, p, which, 2)
+    for k in ("nds", "vvel", "vel", "temp", "rho"):
+        assert np.array_equal(host[k], dev[k]), k
+
+
+def test_ausm_damping_and_bc_perturbation_are_unsupported(solver):
+    """UseDamping with the AUSM solver and the user's BC perturbation return
+    GITM_VERT_EUNSUPPORTED before any work, so the Fortran caller runs its
+    own path; a tides flag without the tides' arrays is an argument error."""
+    from gitm_amd import vertical
+    cols = subset(synth.make_columns(synth.CONFIGS["cpu_324x50"], seed=44), np.arange(4))
+    d = vertical.to_device(cols)
+    for which, flags, code in (("ausm", {"use_damping": True}, 5),
+                               ("rusanov", {"use_gswm_tides": True}, 4)):
+        with pytest.raises(vertical.VerticalStepError) as ei:
+            solver.advance(d, dataclasses.replace(synth.StepParams(), **flags), which, 1)
+        assert ei.value.code == code, (which, flags, ei.value.code)
+    assert np.array_equal(vertical.to_host(d)["temp"], cols["temp"])
