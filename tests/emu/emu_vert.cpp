@@ -7,8 +7,10 @@
 // against the CPU oracle bit for bit, without a GPU.  Nothing in the
 // product library loads this code.
 #define GV_HOST_EMULATION 1
+#include <algorithm>
 #include <atomic>
 #include <barrier>
+#include <memory>
 #include <thread>
 #include <vector>
 
@@ -411,6 +413,66 @@ extern "C" int gv_emu_advance(const gitm_vert_params *p, const gitm_vert_columns
     case 4: key = run_all<256>(kp, *c, c0, c1, device_math != 0); break;
     default: return GITM_VERT_EINVAL;
   }
+  gitm_vert_error tmp;
+  if (!e) e = &tmp;
+  gv::decode_error(key, e);
+  return e->code;
+}
+
+// The short-column path of the ensemble step (gitm_vert_ensemble.hip): the
+// columns [c0, c1), per_wg at a time, share one LDS block, each in a slot of
+// its own (slot_doubles: the column's LDS rounded up to even), and each
+// synchronises only its own lanes, as a wavefront's WaveCtx does on the GPU.
+// carry_lds: the RK4 carries in the slot (1) or in scratch (0).
+template <int SOLVER, bool IONS, bool CARRY>
+unsigned long long run_slots(const gv::KParams &kp, const gitm_vert_columns &io, int c0,
+                             int c1, int per_wg, bool device_math, double *scratch) {
+  const int L = kp.n_alts + 4;
+  int sd = (int)gv::lds_doubles(kp.n_alts, IONS, CARRY);
+  sd += sd & 1;
+  std::vector<double> lds((size_t)per_wg * sd, 0.0);
+  unsigned long long key = ~0ULL;
+  for (int g = c0; g < c1; g += per_wg) {
+    const int n = std::min(per_wg, c1 - g);
+    std::vector<std::unique_ptr<Shared>> sh;
+    for (int w = 0; w < n; w++) sh.push_back(std::make_unique<Shared>(L, device_math));
+    std::vector<std::thread> th;
+    for (int w = 0; w < n; w++)
+      for (int t = 0; t < L; t++)
+        th.emplace_back([&, w, t] {
+          EmuCtx cx{t, sh[w].get()};
+          gv::run_column<64, SOLVER, IONS, CARRY>(cx, kp, io, g + w, lds.data() + (size_t)w * sd,
+                                                  scratch, kp.dt, g + w, kp.n_alts,
+                                                  gv::scratch_fields(CARRY, IONS));
+        });
+    for (auto &x : th) x.join();
+    for (auto &s : sh) key = std::min(key, s->err.load());
+  }
+  return key;
+}
+
+extern "C" int gv_emu_advance_slots(const gitm_vert_params *p, const gitm_vert_columns *c,
+                                    int solver, int n_steps, int per_wg, int carry_lds,
+                                    int c0, int c1, int device_math, gitm_vert_error *e) {
+  int rc = gv::check_args(p, c, solver, n_steps);
+  if (rc) return rc;
+  if (c->n_alts + 4 > 64 || per_wg < 1 || per_wg > 4) return GITM_VERT_EINVAL;
+  gv::KParams kp = gv::make_kparams(p, c, solver, n_steps, nullptr);
+  const bool ions = kp.ions != 0, carry = carry_lds != 0;
+  std::vector<double> scratch((size_t)c->n_lons * c->n_lats * c->n_blocks *
+                              gv::scratch_fields(carry, ions) * (c->n_alts + 4));
+  const bool dm = device_math != 0;
+  double *sc = scratch.data();
+  unsigned long long key;
+#define GV_SLOTS(S, I, C) key = run_slots<S, I, C>(kp, *c, c0, c1, per_wg, dm, sc)
+  if (solver == GITM_VERT_AUSM) {
+    if (ions) { if (carry) GV_SLOTS(GITM_VERT_AUSM, true, true); else GV_SLOTS(GITM_VERT_AUSM, true, false); }
+    else { if (carry) GV_SLOTS(GITM_VERT_AUSM, false, true); else GV_SLOTS(GITM_VERT_AUSM, false, false); }
+  } else {
+    if (ions) { if (carry) GV_SLOTS(GITM_VERT_RUSANOV, true, true); else GV_SLOTS(GITM_VERT_RUSANOV, true, false); }
+    else { if (carry) GV_SLOTS(GITM_VERT_RUSANOV, false, true); else GV_SLOTS(GITM_VERT_RUSANOV, false, false); }
+  }
+#undef GV_SLOTS
   gitm_vert_error tmp;
   if (!e) e = &tmp;
   gv::decode_error(key, e);
