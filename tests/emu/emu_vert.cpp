@@ -14,9 +14,14 @@
 #include <thread>
 #include <vector>
 
+// GV_EMU_FORWARD_ONLY leaves out the adjoint builds and the operation count
+// (tests/emu/sanitize_main.cpp compiles only the forward step).
+#ifndef GV_EMU_FORWARD_ONLY
 #include "count.h"
 #include "vert_ad.h"
 #include "vert_adj.h"
+#endif
+#include "vert_column.h"
 #include "vert_params.h"
 
 namespace {
@@ -94,6 +99,7 @@ unsigned long long run_all(const gv::KParams &kp, const gitm_vert_columns &io, i
   return sh.err.load();
 }
 
+#ifndef GV_EMU_FORWARD_ONLY
 // The adjoint build: taped arithmetic (vert_ad.h), lanes as CPU threads.
 struct AdShared {
   std::barrier<> bar;
@@ -165,9 +171,11 @@ struct EmuAdCtx {
     }
   }
 };
+#endif  // GV_EMU_FORWARD_ONLY
 
 }  // namespace
 
+#ifndef GV_EMU_FORWARD_ONLY
 // Adjoint of one AUSM step (no ions) of columns c0..c1-1 of `c`: gout holds
 // the adjoints of the step's outputs, gin receives those of its inputs.
 // nmax/maxph size the tape of one column; the numbers of nodes and phases
@@ -389,6 +397,8 @@ extern "C" int gv_emu_count_ops(const gitm_vert_params *p, const gitm_vert_colum
   for (int i = 0; i < 6; i++) out[i] = sum[i];
   return GITM_VERT_OK;
 }
+
+#endif  // GV_EMU_FORWARD_ONLY
 
 // device_math != 0 evaluates as the GPU does (friction powers as exp(e ln T),
 // the log-space forms of Column); 0 uses the reference's forms, for bitwise

@@ -24,6 +24,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#define GV_EMU_FORWARD_ONLY 1
 #include "emu_vert.cpp"
 
 namespace {

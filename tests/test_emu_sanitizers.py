@@ -47,8 +47,11 @@ def build(kind: str, out_dir: str) -> str:
 
 @pytest.fixture(scope="module")
 def exes(tmp_path_factory):
+    from concurrent.futures import ThreadPoolExecutor
     d = str(tmp_path_factory.mktemp("sanitize"))
-    return {k: build(k, d) for k in ("thread", "address")}
+    kinds = ("thread", "address")
+    with ThreadPoolExecutor(2) as ex:  # the two builds side by side
+        return dict(zip(kinds, ex.map(lambda k: build(k, d), kinds)))
 
 
 def write_case(path, cols, solver, n_steps, waves, per_wg=0, ions=False, msis=False,
