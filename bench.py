@@ -292,6 +292,140 @@ def sources_bytes_per_column(n_alts: int) -> int:
     return 41 * 8 * (n_alts + 4)
 
 
+BLOCK_LON = BLOCK_LAT = 9   # GITM's blocks of 9x9 cells (ModSize.f90.orig)
+NSPECIES_TOTAL = 11         # nSpeciesTotal: NDensityS holds them all; the step uses six
+
+
+def block_layout(torch, dev: dict, n_alts: int):
+    """The column-contiguous device columns ``dev`` in GITM's own block
+    layout (ModGITM.f90:192-255): arrays (-1:nLons+2,-1:nLats+2,-1:nAlts+2
+    [,nComp],nBlocks) of 9x9 blocks with their halo cells (NaN), NDensityS with
+    all 11 species, cMax_GDB(0:nLons+1,0:nLats+1,0:nAlts+1,3,nBlocks),
+    KappaEddyDiffusion without halos, CosLatitude(-1:nLats+2,nBlocks).
+    Returns the device tensors (C order [block][comp][level][lat][lon]), the
+    C-ABI descriptor and the (i, j, b) index of every column."""
+    from gitm_amd import _abi
+    ncols = int(dev["temp"].shape[0])
+    nb, L = ncols // (BLOCK_LON * BLOCK_LAT), n_alts + 4
+    NLo, NLg = BLOCK_LON + 4, BLOCK_LAT + 4
+    c = torch.arange(ncols, device="cuda")
+    i, j, b = c % BLOCK_LON, (c // BLOCK_LON) % BLOCK_LAT, c // (BLOCK_LON * BLOCK_LAT)
+    nan = float("nan")
+
+    def halo(key, nc_alloc=None):
+        a = dev[key]
+        a3 = a[:, None, :] if a.dim() == 2 else a
+        out = torch.full((nb, nc_alloc or a3.shape[1], L, NLg, NLo), nan,
+                         dtype=torch.float64, device="cuda")
+        out[b, :a3.shape[1], :, j + 2, i + 2] = a3
+        return out
+
+    blk = {k: halo(k) for k in ("vvel", "vel", "temp", "rho", "gamma", "alt", "grav", "invr",
+                                "dalt", "tempunit", "mmm", "logns", "nden")}
+    blk["nds"] = halo("nds", NSPECIES_TOTAL)
+    cm = torch.full((nb, 3, L - 2, BLOCK_LAT + 2, BLOCK_LON + 2), nan, dtype=torch.float64,
+                    device="cuda")
+    cm[b, 2, :, j + 1, i + 1] = dev["cmax"][:, 1:L - 1]
+    blk["cmax"] = cm
+    ed = torch.full((nb, L, BLOCK_LAT, BLOCK_LON), nan, dtype=torch.float64, device="cuda")
+    ed[b, :, j, i] = dev["eddy"]
+    blk["eddy"] = ed
+    cl = torch.full((nb, NLg), nan, dtype=torch.float64, device="cuda")
+    cl[b, j + 2] = dev["coslat"]
+    blk["coslat"] = cl
+    return blk, (i, j, b)
+
+
+def block_columns(blk: dict, ptr_of, n_blocks: int, n_alts: int):
+    """C-ABI descriptor of block-layout arrays at the addresses ptr_of(key)."""
+    from gitm_amd import _abi
+    c = _abi.Columns()
+    c.n_lons, c.n_lats, c.n_blocks, c.n_alts = BLOCK_LON, BLOCK_LAT, n_blocks, n_alts
+
+    def f(key, nc, **kw):
+        return _abi.gitm_block_field(ptr_of(key), BLOCK_LON, BLOCK_LAT, n_alts, nc, **kw)
+
+    c.ndensity_s = f("nds", NSPECIES_TOTAL)
+    c.vertical_velocity, c.velocity, c.log_ns = f("vvel", 6), f("vel", 3), f("logns", 6)
+    for key, name in (("temp", "temperature"), ("rho", "rho"), ("nden", "ndensity"),
+                      ("tempunit", "temp_unit"), ("mmm", "mean_major_mass"),
+                      ("gamma", "gamma"), ("alt", "altitude"), ("grav", "gravity"),
+                      ("invr", "inv_radial_distance"), ("dalt", "dalt")):
+        setattr(c, name, f(key, 1))
+    c.cmax = f("cmax", 3, comp0=2, lon_lo=0, lat_lo=0, alt_lo=0, alt_hi=n_alts + 1)
+    c.eddy = f("eddy", 1, lon_lo=1, lat_lo=1)
+    c.cos_latitude = _abi.latitude_field(ptr_of("coslat"), BLOCK_LAT)
+    return c
+
+
+def bench_block_layout(torch, solver, dev, spec, params, which: str, reps: int = 10) -> dict:
+    """The step on GITM's own block layout at 1 deg (800 blocks of 9x9, the
+    layout the Fortran drop-in passes), device-resident: its time per launch
+    beside the column-contiguous layout's, after a check that one step gives
+    the same bits on both; and the host-array entry point on the same arrays
+    in host memory (what gitm_vert_advance_host stages over PCIe per call)."""
+    import ctypes
+
+    from gitm_amd import vertical
+    nb = spec.ncols // (BLOCK_LON * BLOCK_LAT)
+    ref = {k: v.clone() for k, v in dev.items()}
+    blk, (i, j, b) = block_layout(torch, ref, spec.nalts)
+    c = block_columns(blk, lambda k: blk[k].data_ptr(), nb, spec.nalts)
+    p = vertical.make_params(params)
+    sid = vertical.solver_id(which)
+    stream = torch.cuda.current_stream()
+    solver.set_stream(stream.cuda_stream)
+
+    def step():
+        rc = vertical.lib().gitm_vert_advance(solver._h, ctypes.byref(p), ctypes.byref(c), sid, 1)
+        if rc:
+            raise RuntimeError(f"gitm_vert_advance on the block layout: {rc}")
+
+    step()
+    solver.advance(ref, params, which, 1, check=False)
+    torch.cuda.synchronize()
+    solver.check()
+    same = all(torch.equal(blk[k][b, :ref[k].shape[1] if ref[k].dim() == 3 else 1, :,
+                                  j + 2, i + 2].reshape(ref[k].shape), ref[k])
+               for k in ("nds", "vvel", "vel", "temp", "rho"))
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(reps)]
+    for e0, e1 in ev:
+        e0.record(stream)
+        step()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    solver.check()
+    block_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev]))
+    for e0, e1 in ev:
+        e0.record(stream)
+        solver.advance(ref, params, which, 1, check=False)
+        e1.record(stream)
+    torch.cuda.synchronize()
+    column_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev]))
+    # the host-array entry point on the block arrays in host memory
+    host = {k: v.cpu().numpy() for k, v in blk.items()}
+    del blk
+    torch.cuda.empty_cache()
+    ch = block_columns(host, lambda k: host[k].ctypes.data, nb, spec.nalts)
+    staged = sum(v.nbytes for k, v in host.items() if k not in ("nds",)) + \
+        host["nds"][:, :6].nbytes
+    t = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        rc = vertical.lib().gitm_vert_advance_host(solver._h, ctypes.byref(p), ctypes.byref(ch),
+                                                   sid, 1)
+        t.append(time.perf_counter() - t0)
+        if rc:
+            raise RuntimeError(f"gitm_vert_advance_host on the block layout: {rc}")
+    return {"layout": "GITM blocks of 9x9 with halo cells, 800 blocks x 100 levels",
+            "bitwise_equal_to_column_layout": bool(same),
+            "launch_ms": block_ms, "column_layout_launch_ms": column_ms,
+            "ratio_to_column_layout": block_ms / column_ms,
+            "host_staged_ms_per_call": 1e3 * float(np.median(t[1:])),
+            "host_staged_bytes_in_approx": staged}
+
+
 def bench_sources(torch, solver, dev, spec, reps: int = 10) -> dict:
     """The sources step (gitm_vert_sources) on the same grid, increments
     only (the state is left as it is), timed with HIP events."""
@@ -541,6 +675,8 @@ def main() -> int:
     launch_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
     prep = time_prep(torch, solver, dev, max(args.steps, 10))
     src = bench_sources(torch, solver, dev, spec)
+    print("bench: block layout", file=sys.stderr, flush=True)
+    blk = bench_block_layout(torch, solver, dev, spec, params, args.solver)
 
     ncols = spec.ncols
     value = ncols * args.steps / wall
@@ -618,6 +754,7 @@ def main() -> int:
                  / (prep["launch_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
                  "cfl_dt": prep["dt"]},
         "sources": src,
+        "block_layout": blk,
         "horizontal": hor,
         "ensemble": ens,
         "adjoint_forward_ratio": adj["ratio"] if adj else None,

@@ -2,7 +2,7 @@
 // the single-grid step, the column prep, errors and the host-array entry.
 //
 // Entry points are declared in include/gitm_vert.h.  One workgroup per
-// column (gitm_vert_columns column index = blockIdx.x), NT = 64*W threads,
+// column (the column of workgroup blockIdx.x is xcd_column's), NT = 64*W threads,
 // one level per thread; see vert_column.h for the column algorithm.
 #include "vert_gpu.h"
 
@@ -12,7 +12,8 @@ template <int NT, int SOLVER, bool IONS>
 __global__ void __launch_bounds__(NT) vert_column_kernel(gv::KParams kp, gitm_vert_columns io) {
   extern __shared__ double lds[];
   GpuCtx cx{kp.err};
-  gv::run_column<NT, SOLVER, IONS>(cx, kp, io, (int)blockIdx.x, lds);
+  gv::run_column<NT, SOLVER, IONS>(cx, kp, io, xcd_column((int)blockIdx.x, (int)gridDim.x),
+                                   lds);
 }
 
 // One thread per level: blockDim = nAlts+4 (<= NT, the compile-time bound).
@@ -42,7 +43,7 @@ template <int NT>
 __global__ void __launch_bounds__(NT) prep_kernel(gv::PrepParams pp, gitm_vert_columns io,
                                                   gitm_vert_field pf, gitm_vert_field cpf,
                                                   unsigned long long *dt_bits) {
-  prep_column<NT>(pp, io, pf, cpf, (int)blockIdx.x, dt_bits);
+  prep_column<NT>(pp, io, pf, cpf, xcd_column((int)blockIdx.x, (int)gridDim.x), dt_bits);
 }
 
 }  // namespace

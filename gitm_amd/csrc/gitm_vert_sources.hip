@@ -95,7 +95,7 @@ template <int NT>
 __global__ void __launch_bounds__(NT)
     sources_kernel(SrcParams p, gitm_vert_columns io, SrcFields f) {
   extern __shared__ double lds[];
-  const int col = (int)blockIdx.x, j = (int)threadIdx.x, k = j - 1;
+  const int col = xcd_column((int)blockIdx.x, (int)gridDim.x), j = (int)threadIdx.x, k = j - 1;
   const int nA = p.n_alts;
   const int nlb = io.n_lons * io.n_lats;
   const bool lev = k >= 0 && k <= nA + 1;

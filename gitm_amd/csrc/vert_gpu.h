@@ -96,6 +96,18 @@ namespace {
 using gv::BatchMember;
 using gv::EnsLayout;
 
+// Column of workgroup b in a launch of G workgroups, one column each.  The
+// hardware hands consecutive workgroups to the eight XCDs in turn, and each
+// XCD has an L2 of its own; giving each XCD a contiguous range of columns
+// keeps neighbouring columns, which share cache lines in GITM's lon-fastest
+// block arrays, on one L2 while they run.  A bijection of [0, G) for any G.
+__device__ __forceinline__ int xcd_column(int b, int G) {
+  constexpr int NXCD = 8;
+  const int full = G / NXCD;
+  if (b >= NXCD * full) return b;
+  return (b % NXCD) * full + b / NXCD;
+}
+
 __device__ __forceinline__ int find_member(const BatchMember *__restrict__ m, int n,
                                            int block) {
   int lo = 0, hi = n - 1;
