@@ -148,3 +148,51 @@ def horizontal(grid: dict, dt: float, omega_body: float, beta_limiter: float,
         out[k] = vals[pos:pos + n].reshape(s).copy()
         pos += n
     return out
+
+
+SEXE = os.path.join(HERE, "_ref", "gitm_sources_ref")
+SMAGIC = 1196840533
+
+
+def sources_available() -> bool:
+    return os.path.exists(SEXE)
+
+
+def sources(cols: dict, dt: float, use_turbulent_cond: bool) -> dict:
+    """The reference's calc_thermal_conduction and calc_viscosity (with its
+    calc_viscosity_coef) on the columns of ``cols`` (whole 9x9 blocks,
+    nAlts = 50).  Returns conduction, viscosity, vert_viscosity, visc_coef
+    and visc_coef_s on levels 0..nAlts+1 (levels -1 and nAlts+2 zero)."""
+    nc, L = cols["temp"].shape
+    na = L - 4
+    if na != REF_NALTS or nc % (REF_BLOCK[0] * REF_BLOCK[1]) or \
+            nc // (REF_BLOCK[0] * REF_BLOCK[1]) > REF_BLOCK[2]:
+        raise ValueError("reference build is compiled for 9x9 blocks, nAlts=50, <= 4 blocks")
+    hdr = np.zeros(16, dtype=np.int32)
+    hdr[:4] = [SMAGIC, nc, na, int(use_turbulent_cond)]
+    par = np.zeros(8)
+    par[0] = dt
+    order = ["temp", "tempunit", "rho", "cp", "eddy", "nds", "kappa_temp", "vel", "vvel",
+             "alt", "raddist", "mmm"]
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        fout = os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(hdr.tobytes())
+            f.write(par.tobytes())
+            for k in order:
+                f.write(_fortran_order(cols[k]).tobytes())
+        subprocess.run([SEXE, fin, fout], check=True, capture_output=True)
+        raw = np.fromfile(fout, dtype=np.uint8)
+    status = raw[:16].view(np.int32)
+    if status[0]:
+        raise RuntimeError(f"gitm_sources_ref status {status}")
+    vals = raw[16:].view(np.float64)
+    out, pos = {}, 0
+    for k, shp in (("conduction", (nc, L)), ("viscosity", (nc, 3, L)),
+                   ("vert_viscosity", (nc, 6, L)), ("visc_coef", (nc, L)),
+                   ("visc_coef_s", (nc, 6, L))):
+        n = int(np.prod(shp))
+        out[k] = vals[pos:pos + n].reshape(shp).copy()
+        pos += n
+    return out

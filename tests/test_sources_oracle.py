@@ -1,5 +1,12 @@
-"""The oracle's sources step (ov_sources_step, ov_conduction_1d), pinned by
-what the mathematics fixes.
+"""The oracle's sources step (ov_sources_step, ov_conduction_1d), pinned to
+the reference build and by what the mathematics fixes.
+
+* Bit for bit against the reference's calc_thermal_conduction and
+  calc_viscosity (src/calc_conduction.f90, src/calc_viscosity.f90, compiled
+  from the reference sources: oracle/_ref/gitm_sources_ref): Conduction,
+  Viscosity and VerticalViscosityS, laminar and turbulent, on the golden
+  fixtures tests/golden/sources_*.npz and on fresh columns when the
+  reference build is present.
 
 * calc_conduction_1d (calc_conduction.f90:132-244) on a uniform grid with a
   constant radius reduces to the textbook implicit step of
@@ -19,12 +26,52 @@ what the mathematics fixes.
 """
 from __future__ import annotations
 
+import glob
+import json
+import os
+
 import numpy as np
 import pytest
 
 from gitm_amd import synth
-from oracle import cpu
+from oracle import cpu, ref
 from tests.helpers import subset
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "sources_*.npz")))
+INCREMENTS = ("conduction", "viscosity", "vert_viscosity")
+
+
+def test_sources_fixtures_present():
+    assert len(GOLDEN) >= 2
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def test_oracle_bitwise_equals_reference_sources(path):
+    z = np.load(path)
+    meta = json.loads(str(z["meta"]))
+    cols = {k[3:]: z[k] for k in z.files if k.startswith("in_")}
+    want = {k[4:]: z[k] for k in z.files if k.startswith("out_")}
+    got = cpu.sources(cols, meta["dt"], synth.species_masses(),
+                      use_turbulent_cond=meta["use_turbulent_cond"], apply=False)
+    L = cols["temp"].shape[1]
+    for k in INCREMENTS:  # levels 0..nAlts+1
+        assert np.array_equal(got[k][..., 1:L - 1], want[k][..., 1:L - 1]), k
+
+
+@pytest.mark.skipif(not ref.sources_available(), reason="reference build (oracle/_ref) absent")
+@pytest.mark.parametrize("turbulent", [False, True])
+def test_oracle_bitwise_equals_reference_sources_fresh(turbulent):
+    """Four blocks (324 columns), the reference's own ViscCoef and ViscCoefS
+    (calc_viscosity_coef) fed to the oracle."""
+    cols = synth.make_columns(synth.CONFIGS["cpu_324x50"], seed=33 + int(turbulent))
+    cols.update(synth.source_fields(cols, seed=34))
+    want = ref.sources(cols, 2.0, turbulent)
+    cols["visc_coef"], cols["visc_coef_s"] = want["visc_coef"], want["visc_coef_s"]
+    got = cpu.sources(cols, 2.0, synth.species_masses(), use_turbulent_cond=turbulent,
+                      apply=False)
+    L = cols["temp"].shape[1]
+    for k in INCREMENTS:
+        assert np.array_equal(got[k][..., 1:L - 1], want[k][..., 1:L - 1]), k
 
 
 def textbook(nA, dt, q, D, M, h):
@@ -169,12 +216,6 @@ def test_sources_step_updates(turbulent):
                           cols["vel"][0, :, 1:nA + 3] + out["viscosity"][0, :, 1:nA + 3])
     assert np.array_equal(out["vvel"][0, :, 1:nA + 3],
                           cols["vvel"][0, :, 1:nA + 3] + out["vert_viscosity"][0, :, 1:nA + 3])
-    # Viscosity(up): He's full-step solve
-    he = 5
-    diff = (4.0 / 3.0) * cols["visc_coef_s"][0, he] + mass[he] * cols["nds"][0, he] * cols["eddy"][0]
-    res = cpu.conduction_1d(nA, dt, True, cols["vvel"][0, he], diff,
-                            mass[he] * cols["nds"][0, he], cols["alt"][0], cols["raddist"][0])
-    assert np.array_equal(out["viscosity"][0, 2, 1:nA + 3], res[1:nA + 3])
     # the turbulent conduction changes the temperature increments only
     other = cpu.sources(cols, dt, mass, use_turbulent_cond=not turbulent, apply=False)
     assert not np.array_equal(other["conduction"], out["conduction"])
