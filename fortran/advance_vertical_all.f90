@@ -3,8 +3,8 @@
 ! same empty argument list.  The block loop's rate and viscosity updates
 ! stay in Fortran, as in the reference (:20-24); the column loop runs on the
 ! GPU in one call (vertical_gpu.f90).  When the library does not implement a
-! selected option (GITM_VERT_EUNSUPPORTED: tides, BC perturbation,
-! UseDamping) the original routine runs, renamed advance_vertical_all_fortran
+! selected option (GITM_VERT_EUNSUPPORTED: BC perturbation, UseDamping
+! with the AUSM solver) the original routine runs, renamed advance_vertical_all_fortran
 ! by the maintainer; any other failure stops the run, as the reference's
 ! stop_gitm calls on this path do.
 subroutine advance_vertical_all
@@ -34,8 +34,8 @@ subroutine advance_vertical_all
   enddo
 
   ! UseMSIS: the caller's level-0 MSIS values, one column at a time
-  ! (msis_at_level0 wraps the reference's msis_bcs as set_vertical_bcs.Earth.f90
-  ! :116-131 calls it for iAlt = 0)
+  ! (msis_at_level0.f90: the reference's msis_bcs, called as
+  ! set_vertical_bcs.Earth.f90:116-131 calls it for iAlt = 0)
   if (UseMSIS) call fill_msis_bc
 
   call advance_vertical_all_gpu(iError)
@@ -48,27 +48,3 @@ subroutine advance_vertical_all
   call end_timing("vertical_all")
 
 end subroutine advance_vertical_all
-
-! MsisBC(1:9, iLon, iLat, iBlock): LogNS(0,1:6), Temp(0), Vel(0, east),
-! Vel(0, north) of every column from msis_at_level0.
-subroutine fill_msis_bc
-
-  use ModSizeGitm, only: nLons, nLats
-  use ModGITM, only: nBlocks
-  use ModVerticalGPU
-
-  implicit none
-
-  integer :: iLon, iLat, iBlock
-
-  if (.not. allocated(MsisBC)) allocate(MsisBC(9, nLons, nLats, nBlocks))
-  do iBlock = 1, nBlocks
-     do iLat = 1, nLats
-        do iLon = 1, nLons
-           call msis_at_level0(iLon, iLat, iBlock, MsisBC(1:6,iLon,iLat,iBlock), &
-                MsisBC(7,iLon,iLat,iBlock), MsisBC(8:9,iLon,iLat,iBlock))
-        enddo
-     enddo
-  enddo
-
-end subroutine fill_msis_bc

@@ -31,7 +31,14 @@ def _fortran_order(a: np.ndarray) -> np.ndarray:
 FORTRAN_GPU_EXE = os.path.join(HERE, "_ref", "gitm_vert_fortran_gpu")
 
 
-def advance(cols: dict, params, solver: int = 1, n_steps: int = 1, exe: str = EXE) -> dict:
+def advance(cols: dict, params, solver: int = 1, n_steps: int = 1, exe: str = EXE,
+            msis_env: dict | None = None) -> dict:
+    """The reference vertical step on ``cols`` (nAlts 50).  With
+    params.use_msis, ``msis_env`` gives julian_day, utime, f107 and f107a
+    and cols holds each column's lat and lon in radians: the boundary
+    conditions then call the reference's msis_bcs (MSIS-00, no HWM winds),
+    and out["msis"] holds what it returns for each column's level 0
+    ([col][9], the layout of cols["msis"])."""
     nc, L = cols["temp"].shape
     na = L - 4
     if na != REF_NALTS:
@@ -45,15 +52,21 @@ def advance(cols: dict, params, solver: int = 1, n_steps: int = 1, exe: str = EX
     gswm = bool(getattr(params, "use_gswm_tides", False))
     waccm = bool(getattr(params, "use_waccm_tides", False))
     hdr[10:13] = [int(gswm), int(waccm), int(getattr(params, "use_damping", False))]
+    msis = bool(getattr(params, "use_msis", False))
     par = np.zeros(8)
     par[:5] = [params.dt, params.omega_body, params.max_vert_vel,
                params.beta_limiter, params.min_tec]
+    if msis:
+        hdr[13:15] = [1, msis_env["julian_day"]]
+        par[5:8] = [msis_env["utime"], msis_env["f107"], msis_env["f107a"]]
     order = ["nds", "vvel", "vel", "temp", "rho", "tempunit", "mmm", "gamma",
              "cmax", "eddy", "alt", "grav", "invr", "dalt", "coslat"]
     if ions:
         order += ["ions", "ivel", "sza", "mlat"]
     if gswm or waccm:
         order += ["tides"]
+    if msis:
+        order += ["lat", "lon"]
     with tempfile.TemporaryDirectory() as td:
         fin = os.path.join(td, "in.bin")
         fout = os.path.join(td, "out.bin")
@@ -75,6 +88,8 @@ def advance(cols: dict, params, solver: int = 1, n_steps: int = 1, exe: str = EX
               ("nden", (nc, L))]
     if ions:
         shapes.append(("ions", (nc, 10, L)))
+    if msis:
+        shapes.append(("msis", (nc, 9)))
     for k, shp in shapes:
         n = int(np.prod(shp))
         out[k] = vals[pos:pos + n].reshape(shp).copy()

@@ -17,9 +17,13 @@
 !           grav invr dalt, coslat(col), and if ions: ions(10) ivel(3),
 !           sza(col), mlat(col), and if tides (hdr(11) GSWM or hdr(12) WACCM):
 !           TidesEast, TidesNorth, TidesTemp at levels -1, 0 ([col][3][2]);
-!           hdr(13): UseDamping
+!           hdr(13): UseDamping; hdr(14): UseMSIS, with hdr(15) = iJulianDay,
+!           par(6:8) = UTime, F107, F107A and then Latitude(col),
+!           Longitude(col) in radians
 !   output: int32 status(4), then nds(6) vvel(6) vel(3) temp rho logns(6)
-!           nden, and if ions: ions(10)
+!           nden, and if ions: ions(10), and if UseMSIS: what msis_bcs
+!           returns for each column's level 0 ([col][9]: LogNS(1:6), Temp,
+!           east and north wind), computed as set_vertical_bcs computes it
 program gitm_vert_ref
 
   use ModSizeGitm
@@ -29,13 +33,19 @@ program gitm_vert_ref
   use ModSources, only: KappaEddyDiffusion, EuvHeating, init_mod_sources
   use ModEUV, only: SZA
   use ModTides, only: TidesEast, TidesNorth, TidesTemp
+  use ModTime, only: UTime, iJulianDay, CurrentTime
+  use ModIndicesInterfaces, only: get_HPI
+  use EUA_ModMsis00, only: meters, tselec
 
   implicit none
 
   integer :: hdr(16), status(4)
   real :: par(8)
   integer :: nCols, nA, solver, nStepsH, iCol, iStep, L, k, s, d, nIonsIO
-  logical :: doIons, doTides
+  logical :: doIons, doTides, doMsis
+  real, allocatable :: latr(:), lonr(:), msis_o(:,:)
+  real :: LatD, LonD, AltM, Lst, HP, Ap, temptemp, LogRhoM, v(2), logNS_Species(6)
+  integer :: iError
   real, allocatable :: nds(:,:,:), vvel(:,:,:), vel(:,:,:), temp(:,:), rhoH(:,:)
   real, allocatable :: tempunit_(:,:), mmm(:,:), gam(:,:), cmax(:,:), eddy(:,:)
   real, allocatable :: alt(:,:), grav(:,:), invr(:,:), dalt(:,:), coslat(:)
@@ -93,6 +103,12 @@ program gitm_vert_ref
      allocate(tides(2,3,nCols))
      read(10) tides
   endif
+  doMsis = hdr(14) /= 0
+  if (doMsis) then
+     allocate(latr(nCols), lonr(nCols), msis_o(9,nCols))
+     read(10) latr
+     read(10) lonr
+  endif
   close(10)
 
   ! Model set-up the reference performs before any step
@@ -108,7 +124,7 @@ program gitm_vert_ref
   UseIonAdvection            = doIons
   UseImprovedIonAdvection    = (hdr(9) /= 0)
   UseNighttimeIonBCs         = (hdr(10) /= 0)
-  UseMSIS                    = .false.
+  UseMSIS                    = doMsis
   UseGSWMTides               = (hdr(11) /= 0)
   UseWACCMTides              = (hdr(12) /= 0)
   UseBcPerturbation          = .false.
@@ -118,6 +134,34 @@ program gitm_vert_ref
   MaximumVerticalVelocity    = par(3)
   BetaLimiter                = par(4)
   MinTEC                     = par(5)
+  if (doMsis) then
+     ! the time and solar inputs msis_bcs reads (ModTime, ModInputs)
+     iJulianDay = hdr(15)
+     UTime = par(6)
+     CurrentTime = par(6)
+     f107 = par(7)
+     f107a = par(8)
+     ! no HWM-14 winds (their data files are not here): msis_bcs gives zero
+     ! winds (init_msis.Earth.f90:448-456)
+     UseMSISTides = .false.
+     ! MSIS-00 set up as init_msis sets it (init_msis.Earth.f90:165-186)
+     call meters(.true.)
+     if (UseMsisTides) then
+        sw_msis = 1
+     else if (UseMSISOnly) then
+        sw_msis = 1
+        sw_msis(7) = 0
+        sw_msis(8) = 0
+        sw_msis(14) = 0
+     else
+        sw_msis = 0
+        sw_msis(1) = 1
+        sw_msis(9) = 1
+     endif
+     sw_msis(9) = 0
+     sw_msis(2) = 0
+     call tselec(sw_msis)
+  endif
   ! calc_rates.Earth.f90:83-84
   ThermalDiffCoefS(1:nSpecies) = 0.0
   ThermalDiffCoefS(iHe_) = -0.38
@@ -169,6 +213,27 @@ program gitm_vert_ref
         SZA(1,1,1) = sza_(iCol)
         MLatitude(1,1,nA,1) = mlat(iCol)
      endif
+     if (doMsis) then
+        Latitude(1,1) = latr(iCol)
+        Longitude(1,1) = lonr(iCol)
+        ! level 0's msis_bcs values, as set_vertical_bcs.Earth.f90:112-137
+        ! computes them (advance_vertical.f90:137-138 for Lat and Lon)
+        LatD = Latitude(1,1) * 180.0/pi
+        LonD = Longitude(1,1) * 180.0/pi
+        call get_HPI(CurrentTime, HP, iError)
+        if (iError > 0) hp = 40.0
+        Ap = min(200.,max(-40.72 + 1.3 * HP, 10.))
+        AltM = Altitude_GB(1,1,0,1)/1000.0 * &
+             (1.0 - &
+             MsisOblateFactor/2.0 + &
+             MsisOblateFactor * cos(LatD*3.1415/180.0))
+        Lst = mod(UTime/3600.0+LonD/15.0,24.0)
+        call msis_bcs(iJulianDay,UTime,AltM,LatD,LonD,Lst, &
+             F107A,F107,AP,LogNS_Species, temptemp, LogRhoM, v)
+        msis_o(1:6,iCol) = LogNS_Species
+        msis_o(7,iCol) = temptemp
+        msis_o(8:9,iCol) = v
+     endif
      if (doTides) then
         TidesEast(1,1,1:2,1) = tides(:,1,iCol)
         TidesNorth(1,1,1:2,1) = tides(:,2,iCol)
@@ -207,6 +272,7 @@ program gitm_vert_ref
   write(11) logns_o
   write(11) nden_o
   if (doIons) write(11) ions
+  if (doMsis) write(11) msis_o
   close(11)
 
 end program gitm_vert_ref

@@ -16,6 +16,8 @@ program gitm_vert_gpu_driver
   use ModSources, only: KappaEddyDiffusion, EuvHeating, init_mod_sources
   use ModEUV, only: SZA
   use ModTides, only: TidesEast, TidesNorth, TidesTemp
+  use ModTime, only: UTime, iJulianDay, CurrentTime
+  use EUA_ModMsis00, only: meters, tselec
   use ModVerticalGPU
 
   implicit none
@@ -23,7 +25,8 @@ program gitm_vert_gpu_driver
   integer :: hdr(16), status(4)
   real :: par(8)
   integer :: nCols, nA, solver, nStepsH, iCol, iStep, L, s, d, iLon, iLat, iB, iError
-  logical :: doIons, doTides
+  logical :: doIons, doTides, doMsis
+  real, allocatable :: latr(:), lonr(:), msis_o(:,:)
   real, allocatable :: nds(:,:,:), vvel(:,:,:), vel(:,:,:), temp(:,:), rhoH(:,:)
   real, allocatable :: tempunit_(:,:), mmm(:,:), gam(:,:), cmax(:,:), eddy(:,:)
   real, allocatable :: alt(:,:), grav(:,:), invr(:,:), dalt(:,:), coslat(:)
@@ -81,6 +84,12 @@ program gitm_vert_gpu_driver
      allocate(tides(2,3,nCols))
      read(10) tides
   endif
+  doMsis = hdr(14) /= 0
+  if (doMsis) then
+     allocate(latr(nCols), lonr(nCols), msis_o(9,nCols))
+     read(10) latr
+     read(10) lonr
+  endif
   close(10)
 
   nBlocks = nCols / (nLons*nLats)
@@ -96,7 +105,7 @@ program gitm_vert_gpu_driver
   UseIonAdvection            = doIons
   UseImprovedIonAdvection    = (hdr(9) /= 0)
   UseNighttimeIonBCs         = (hdr(10) /= 0)
-  UseMSIS                    = .false.
+  UseMSIS                    = doMsis
   UseGSWMTides               = (hdr(11) /= 0)
   UseWACCMTides              = (hdr(12) /= 0)
   UseBcPerturbation          = .false.
@@ -106,6 +115,19 @@ program gitm_vert_gpu_driver
   MaximumVerticalVelocity    = par(3)
   BetaLimiter                = par(4)
   MinTEC                     = par(5)
+  if (doMsis) then
+     ! time, solar inputs and MSIS-00 set-up as oracle/ref_harness.f90 does
+     iJulianDay = hdr(15)
+     UTime = par(6)
+     CurrentTime = par(6)
+     f107 = par(7)
+     f107a = par(8)
+     UseMSISTides = .false.
+     call meters(.true.)
+     sw_msis = 0
+     sw_msis(1) = 1
+     call tselec(sw_msis)
+  endif
   ThermalDiffCoefS(1:nSpecies) = 0.0
   ThermalDiffCoefS(iHe_) = -0.38
 
@@ -166,6 +188,11 @@ program gitm_vert_gpu_driver
         SZA(iLon,iLat,iB) = sza_(iCol)
         MLatitude(iLon,iLat,nA,iB) = mlat(iCol)
      endif
+     if (doMsis) then
+        ! per latitude row and longitude of a block, as GITM holds them
+        Latitude(iLat,iB) = latr(iCol)
+        Longitude(iLon,iB) = lonr(iCol)
+     endif
      if (doTides) then
         TidesEast(iLon,iLat,1:2,iB) = tides(:,1,iCol)
         TidesNorth(iLon,iLat,1:2,iB) = tides(:,2,iCol)
@@ -175,6 +202,7 @@ program gitm_vert_gpu_driver
 
   status = 0
   do iStep = 1, nStepsH
+     if (doMsis) call fill_msis_bc
      call advance_vertical_all_gpu(iError)
      if (iError /= 0) then
         status(1) = iError
@@ -214,6 +242,16 @@ program gitm_vert_gpu_driver
   write(11) logns_o
   write(11) nden_o
   if (doIons) write(11) ions
+  if (doMsis) then
+     ! the level-0 MSIS values the drop-in passed, per column
+     do iCol = 1, nCols
+        iLon = mod(iCol-1, nLons) + 1
+        iLat = mod((iCol-1)/nLons, nLats) + 1
+        iB = (iCol-1)/(nLons*nLats) + 1
+        msis_o(:,iCol) = MsisBC(:,iLon,iLat,iB)
+     enddo
+     write(11) msis_o
+  endif
   close(11)
 
 end program gitm_vert_gpu_driver

@@ -55,7 +55,14 @@ CASES = [
     # .f90:581-584), with WACCM tides
     ("rusanov_damping_waccm_3step", 0, 3, 10, 43,
      {"use_damping": True, "use_waccm_tides": True}, {}),
+    # UseMSIS through the reference's own msis_bcs (MSIS-00 GTD7, no HWM
+    # winds): in_msis holds what it returned for each column's level 0, the
+    # values the library takes from its caller
+    ("ausm_msis_3step", 1, 3, 13, 23, {"use_msis": True}, {}),
+    ("rusanov_msis_gswm_2step", 0, 2, 14, 29, {"use_msis": True, "use_gswm_tides": True}, {}),
 ]
+# time and solar inputs of the UseMSIS cases (ModTime, ModInputs)
+MSIS_ENV = dict(julian_day=172, utime=30000.0, f107=180.0, f107a=160.0)
 
 
 def tweak(name: str, sub: dict, seed: int, pover: dict) -> None:
@@ -64,6 +71,12 @@ def tweak(name: str, sub: dict, seed: int, pover: dict) -> None:
     if pover.get("use_gswm_tides") or pover.get("use_waccm_tides"):
         sub["tides"] = synth.tides_values(sub, seed=seed,
                                           absolute=bool(pover.get("use_waccm_tides")))
+    if pover.get("use_msis"):
+        # each column's latitude (the sign seeded) and longitude, radians
+        nc = sub["temp"].shape[0]
+        z = synth.normal(seed, 305, (nc, 2))
+        sub["lat"] = np.arccos(np.clip(sub["coslat"], -1.0, 1.0)) * np.sign(z[:, 0])
+        sub["lon"] = np.mod(3.0 * z[:, 1], 2.0 * np.pi)
     if name == "ausm_ions_low_tec":
         sub["ions"] *= 1e-5           # TEC ~1e-4 TECU, far below MinTEC = 2
         sub["ions"][:, :9] = np.maximum(sub["ions"][:, :9], 1.0)
@@ -91,8 +104,12 @@ def main(only=None) -> None:
         sub = {k: np.ascontiguousarray(v[idx]) for k, v in cols.items()}
         tweak(name, sub, seed, pover)
         params = dataclasses.replace(synth.StepParams(), **pover)
-        out = ref.advance(sub, params, solver=solver, n_steps=steps)
+        env = MSIS_ENV if pover.get("use_msis") else None
+        out = ref.advance(sub, params, solver=solver, n_steps=steps, msis_env=env)
+        if env:
+            sub["msis"] = out.pop("msis")
         meta = dict(solver=solver, steps=steps, seed=seed, columns=idx.tolist(),
+                    msis_env=env,
                     params=dataclasses.asdict(params), generator=gover,
                     source="reference Fortran (oracle/_ref) via oracle/ref.py")
         arrays = {f"in_{k}": v for k, v in sub.items()}

@@ -100,12 +100,15 @@ def test_bind_c_layout_matches_the_header():
 @pytest.mark.parametrize("which,ions,flags", [(1, False, {}), (0, False, {}), (1, True, {}),
                                               (1, False, {"use_gswm_tides": True}),
                                               (0, False, {"use_waccm_tides": True,
-                                                          "use_damping": True})])
+                                                          "use_damping": True}),
+                                              (1, False, {"use_msis": True})])
 def test_fortran_drop_in_matches_reference(which, ions, flags):
     """The 2x2 blocks of 9x9 columns, nAlts 50: GITM's arrays in place
     (halo cells, lon-fastest strides, per-block TempUnit; the tides'
-    TidesEast/North/Temp of ModTides), one step, against the reference build
-    at 1e-11 (winds normwise, as in test_gpu_parity)."""
+    TidesEast/North/Temp of ModTides; with UseMSIS the level-0 values that
+    fill_msis_bc gets from the reference's msis_bcs), one step, against the
+    reference build at 1e-11 (winds normwise, as in test_gpu_parity); the
+    MSIS values the drop-in passed equal those set_vertical_bcs used."""
     if not (os.path.exists(ref.EXE) and os.path.exists(ref.FORTRAN_GPU_EXE)):
         pytest.skip("the reference build and the Fortran driver are not built")
     cols = synth.make_columns(synth.CONFIGS["cpu_324x50"], seed=31, ions=ions)
@@ -113,9 +116,20 @@ def test_fortran_drop_in_matches_reference(which, ions, flags):
     if flags.get("use_gswm_tides") or flags.get("use_waccm_tides"):
         cols["tides"] = synth.tides_values(cols, seed=31,
                                            absolute=bool(flags.get("use_waccm_tides")))
+    env = None
+    if flags.get("use_msis"):
+        # Latitude per (iLat, iBlock) and Longitude per (iLon, iBlock), radians
+        c = np.arange(cols["temp"].shape[0])
+        i, j, b = c % 9, (c // 9) % 9, c // 81
+        cols["lat"] = np.arccos(np.clip(cols["coslat"], -1.0, 1.0)) * \
+            np.where((j + b) % 2 == 0, 1.0, -1.0)
+        cols["lon"] = (i + 9 * b) * (2.0 * np.pi / 36.0)
+        env = dict(julian_day=45, utime=51000.0, f107=100.0, f107a=110.0)
     print(f"fortran drop-in: solver {which}, ions {ions}, {flags}, 324 columns", flush=True)
-    got = ref.advance(cols, p, solver=which, n_steps=1, exe=ref.FORTRAN_GPU_EXE)
-    want = ref.advance(cols, p, solver=which, n_steps=1)
+    got = ref.advance(cols, p, solver=which, n_steps=1, exe=ref.FORTRAN_GPU_EXE, msis_env=env)
+    want = ref.advance(cols, p, solver=which, n_steps=1, msis_env=env)
+    if env:
+        assert np.array_equal(got.pop("msis"), want.pop("msis"))
     errs = assert_parity(got, want, 1e-11, what=f"fortran drop-in solver {which}")
     print(errs)
     for k in ("logns", "nden"):

@@ -1,6 +1,14 @@
 """UseMSIS lower boundary (set_vertical_bcs.Earth.f90:113-137): the oracle's
-branch pinned by what the reference's code fixes, and the host build of the
-kernel against it bit for bit.
+branch pinned to the reference build, by what the reference's code fixes,
+and the host build of the kernel against it bit for bit.
+
+Against the reference: the reference build runs the branch itself, calling
+its own msis_bcs (init_msis.Earth.f90:367, MSIS-00 GTD7 with no HWM winds)
+at every boundary pass; the oracle, given what msis_bcs returns for each
+column's level 0, reproduces the reference's step bit for bit (fresh columns
+here, and the golden fixtures ausm_msis_3step and rusanov_msis_gswm_2step).
+That also pins what the invariants below cannot: the hydrostatic fill of
+level -1 uses the entry NS(0), not the MSIS value (:93, :198-223).
 
 The reference calls msis_bcs for levels -1 and 0 at every boundary pass; the
 values depend on time, place and solar indices only, so the caller computes
@@ -25,8 +33,27 @@ import numpy as np
 import pytest
 
 from gitm_amd import synth
-from oracle import cpu
+from oracle import cpu, ref
 from tests.helpers import subset
+
+
+@pytest.mark.skipif(not ref.available(), reason="reference build (oracle/_ref) absent")
+@pytest.mark.parametrize("solver", [0, 1])
+def test_oracle_bitwise_equals_reference_msis(solver):
+    cols = subset(synth.make_columns(synth.CONFIGS["cpu_324x50"], seed=61 + solver),
+                  np.arange(3, 324, 17))
+    nc = cols["temp"].shape[0]
+    rng = np.random.default_rng(solver)
+    cols["lat"] = np.arccos(np.clip(cols["coslat"], -1.0, 1.0)) * \
+        np.where(rng.random(nc) < 0.5, -1.0, 1.0)
+    cols["lon"] = rng.uniform(0.0, 2.0 * np.pi, nc)
+    env = dict(julian_day=300, utime=72000.0, f107=120.0, f107a=130.0)
+    p = synth.StepParams(use_msis=True)
+    want = ref.advance(cols, p, solver=solver, n_steps=3, msis_env=env)
+    cols["msis"] = want.pop("msis")
+    got = cpu.advance(cols, p, solver=solver, n_steps=3)
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
 
 STATE = ("nds", "vvel", "vel", "temp", "rho")
 
