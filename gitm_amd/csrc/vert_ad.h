@@ -149,7 +149,7 @@ GV_AD ad exp(ad a) {
   const double e = ::exp(a.v);
   return mk(e, a.i, e);
 }
-GV_AD ad log(ad a) { return mk(::log(a.v), a.i, 1.0 / a.v); }
+GV_AD ad log(ad a) { return mk(gv::ln(a.v), a.i, 1.0 / a.v); }
 GV_AD ad sqrt(ad a) {
   const double s = ::sqrt(a.v);
   return mk(s, a.i, 0.5 / s);
@@ -231,14 +231,14 @@ GV_FN void adjoint_forward(Ctx &cx, const KParams &kp, const gitm_vert_columns &
     return;
   }
   for (int s = 0; s < NSP; s++) {
-    c.U[F_LN + s] = log(in[s]);
+    c.U[F_LN + s] = ln(in[s]);
     c.U[F_W + s] = in[6 + s];
   }
   c.U[F_VE] = in[12];
   c.U[F_VN] = in[13];
   c.U[F_VU] = in[14];
   c.U[F_T] = in[15] * tu;
-  c.U[F_LR] = log(in[16]);
+  c.U[F_LR] = ln(in[16]);
   c.ausm_step();
   // the step's outputs as advance_vertical scatters them (:163-185)
   ad out[17];

@@ -319,7 +319,7 @@ struct AdjColumn {
         ns[sp] = exp(bc[6 * sp + o]);
         sr = sr + m[sp] * ns[sp];
       }
-      const double lr = log(sr);
+      const double lr = ln(sr);
       const double er = exp(lr);
       double vu = 0.0;
 #pragma unroll
@@ -1479,10 +1479,10 @@ GV_FN void adj_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io,
     if (bad) c.fail(GITM_VERT_ENONPOSITIVE, k);
     return;
   }
-  c.U[F_LR] = log(rho_in);
+  c.U[F_LR] = ln(rho_in);
   c.U[F_T] = temp_in * tu;
 #pragma unroll
-  for (int s = 0; s < NSP; s++) c.U[F_LN + s] = log(nds[s]);
+  for (int s = 0; s < NSP; s++) c.U[F_LN + s] = ln(nds[s]);
 
   // forward: the step (vertical_solver_ausm.f90:130-420) with its record
   cx.rec = rec;

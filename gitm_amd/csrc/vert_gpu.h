@@ -29,7 +29,7 @@ struct GpuCtx {
   // fifteen pow (a pow costs five times an exp here); within a few ulp of pow.
   __device__ static constexpr bool device_math() { return true; }
   __device__ void diff_powers(double temp, double sign, double (&out)[6]) const {
-    const double lt = log(temp);
+    const double lt = gv::ln(temp);
 #pragma unroll
     for (int u = 0; u < 6; u++) out[u] = exp((sign * gv::DEXU[u]) * lt);
   }
