@@ -248,6 +248,17 @@ GV_FN double ln_fast(double x) {
   if (x == __builtin_inf()) res = x;
   return res;
 }
+// is_double<R>: the plain build's number type (not the taped or counting
+// types of the adjoint and operation-count builds)
+template <class T>
+struct is_double {
+  static constexpr bool value = false;
+};
+template <>
+struct is_double<double> {
+  static constexpr bool value = true;
+};
+
 // ln: the column code's logarithm.  The device takes ln_fast; the host
 // build keeps the C library's log, so that it stays bit for bit with the
 // oracle; the adjoint build's taped numbers take their own log.
@@ -1088,6 +1099,53 @@ struct Column {
   // calc_all_fluxes_hydro (:1093-2070).  Writes New (valid on cell threads).
   // Ordered so that few per-level values stay live across the flux phase.
   // -------------------------------------------------------------------------
+  // The hydrostatic densities of the AUSM stage (ausm :725-751) in the
+  // device form: NS(level 1) times the running product of the per-level
+  // factors a(k) b_s(k) (xb(1), xb(2+s)), split into G segments of `seg`
+  // levels so that the serial chain is `seg` levels long instead of nAlts+2.
+  // Lane (g, s) forms the products within segment g (from NS(level 1) for
+  // g = 0, from 1 otherwise); lanes s then make the segment ends absolute
+  // in order, and every other level of segments 1.. takes the end of the
+  // segment below.  The products associate differently from the level-by-
+  // level order of the reference (one ulp per multiplication at most, far
+  // below the step's tolerance); the host build's reference forms keep that
+  // order.  Reads and writes of the three phases are separated by barriers;
+  // within a phase each slot has one writer and no other reader.
+  GV_MEM void hydro_segmented() {
+    const int G = (L / NSP) < 8 ? L / NSP : 8;  // >= 2: L >= 14 (nAlts >= 10)
+    const int seg = (L - 2 + G - 1) / G;
+    if (j < G * NSP) {
+      const int g = j / NSP, s = j - g * NSP;
+      const int lo = 2 + g * seg, hi = (lo + seg < L) ? lo + seg : L;
+      R x = (g == 0) ? R(xb(8 + s, 1)) : R(1.0);
+      for (int jj = lo; jj < hi; jj++) {
+        x = x * xb(1, jj) * xb(2 + s, jj);
+        xb(8 + s, jj) = x;
+      }
+      if (g == 0) xb(8 + s, 0) = xb(8 + s, 1) * xb(1, 0) * xb(2 + s, 0);
+    }
+    cx.sync();
+    if (j < NSP) {
+      const int e0 = (2 + seg < L ? 2 + seg : L) - 1;
+      R x = xb(8 + j, e0);
+      for (int g = 1; g < G && 2 + g * seg < L; g++) {
+        const int e = (2 + (g + 1) * seg < L ? 2 + (g + 1) * seg : L) - 1;
+        x = x * xb(8 + j, e);
+        xb(8 + j, e) = x;
+      }
+    }
+    cx.sync();
+    if (lv && j >= 2 + seg) {
+      const int g = (j - 2) / seg;
+      const int below = 2 + g * seg - 1;
+      const int end = (2 + (g + 1) * seg < L ? 2 + (g + 1) * seg : L) - 1;
+      if (j != end) {
+#pragma unroll
+        for (int s = 0; s < NSP; s++) xb(8 + s, j) = xb(8 + s, below) * xb(8 + s, j);
+      }
+    }
+  }
+
   GV_MEM void ausm_stage(R dtin, int stg) {
     const double *m = kp.mass;
     const R T = U[F_T];
@@ -1153,7 +1211,9 @@ struct Column {
       }
     }
     cx.sync();
-    if (j < NSP) {
+    if (is_double<R>::value && cx.device_math()) {
+      hydro_segmented();
+    } else if (j < NSP) {
       const int s = j;
       const R ns0 = xb(8 + s, 1);
       R x = ns0;

@@ -12,7 +12,7 @@
 // Case file (native little-endian): int32 hdr[8] = {magic 1196840532,
 // n_cols, n_alts, solver, n_steps, flags, waves, per_wg}, flags bit 0 ions,
 // 1 UseMSIS, 2 GSWM tides, 3 WACCM tides, 4 UseDamping, 5 carries in LDS for
-// the slot run; then float64 [col][component][level] arrays: nds(6) vvel(6)
+// the slot run, 6 the GPU's forms of the arithmetic (device_math); then float64 [col][component][level] arrays: nds(6) vvel(6)
 // vel(3) temp rho tempunit mmm gamma cmax eddy alt grav invr dalt, coslat
 // [col]; with ions ions(10) ivel(3) sza[col] mlat[col]; with UseMSIS
 // msis[col][9]; with tides tides[col][3][2].
@@ -133,9 +133,10 @@ int main(int argc, char **argv) {
       }
     }
     gitm_vert_error e{};
+    const int dm = (flags & 64) != 0;
     const int rc = slots ? gv_emu_advance_slots(&p, &c, solver, n_steps, per_wg,
-                                                (flags & 32) != 0, 0, nc, 0, &e)
-                         : gv_emu_advance(&p, &c, solver, n_steps, waves, 0, nc, 0, &e);
+                                                (flags & 32) != 0, 0, nc, dm, &e)
+                         : gv_emu_advance(&p, &c, solver, n_steps, waves, 0, nc, dm, &e);
     if (rc) {
       fprintf(stderr, "%s run: error %d at column %d level %d\n", slots ? "slot" : "workgroup",
               rc, e.column, e.level);

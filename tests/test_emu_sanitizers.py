@@ -9,6 +9,7 @@ outside it, into the scalar area or the int buffer of another column, or
 past the scratch, is a report.
 
 Cases: both solvers; ions (the TEC and top ion boundary on scalar slots);
+the GPU's forms of the arithmetic (the segmented hydrostatic product);
 UseMSIS with the GSWM tides and WACCM with UseDamping (the level-0 values on
 the boundary lanes); tiles of 1, 2 and 4 wavefronts (carries in LDS and in
 device memory); and the ensemble step's short-column slots, several columns
@@ -55,10 +56,10 @@ def exes(tmp_path_factory):
 
 
 def write_case(path, cols, solver, n_steps, waves, per_wg=0, ions=False, msis=False,
-               gswm=False, waccm=False, damping=False, slot_carry_lds=True):
+               gswm=False, waccm=False, damping=False, slot_carry_lds=True, device_math=False):
     nc, L = cols["temp"].shape
     flags = (int(ions) | int(msis) << 1 | int(gswm) << 2 | int(waccm) << 3 |
-             int(damping) << 4 | int(slot_carry_lds) << 5)
+             int(damping) << 4 | int(slot_carry_lds) << 5 | int(device_math) << 6)
     hdr = np.array([MAGIC, nc, L - 4, solver, n_steps, flags, waves, per_wg], dtype=np.int32)
     keys = list(ORDER)
     if ions:
@@ -97,6 +98,10 @@ CASES = [
     ("rusanov_slots_56_device_carries", (56, False), [1, 2, 3], 0, 1, 1, 3,
      {"slot_carry_lds": False}),
     ("ausm_slots_ions_40", (40, False), [5, 6, 7, 8], 1, 1, 1, 4, {"ions": True}),
+    # the GPU's forms (segmented hydrostatic product, log-space sums)
+    ("ausm_100_w2_device", (100, True), [6], 1, 1, 2, 0, {"device_math": True}),
+    ("ausm_10_device", (10, False), [1, 2], 1, 2, 1, 0, {"device_math": True}),
+    ("ausm_slots_50_device", (50, False), [3, 4, 5], 1, 1, 1, 3, {"device_math": True}),
 ]
 
 
@@ -116,5 +121,6 @@ def test_kernel_under_sanitizer(exes, tmp_path, kind, case):
     write_case(path, cols, solver, steps, waves, per_wg, ions=ions, msis=opt.get("msis", False),
                gswm=opt.get("gswm", False), waccm=opt.get("waccm", False),
                damping=opt.get("damping", False),
-               slot_carry_lds=opt.get("slot_carry_lds", True))
+               slot_carry_lds=opt.get("slot_carry_lds", True),
+               device_math=opt.get("device_math", False))
     run(exes[kind], path, kind)
