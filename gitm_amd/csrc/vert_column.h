@@ -869,9 +869,11 @@ struct Column {
       Rb[3] = wm1;
       Rb[4] = w0r;
       Rb[5] = wm1r;
-    } else if (j >= 8 && j < 8 + NSP) {
-      // top, one species per thread (:429-441, :471-486)
-      const int sp = j - 8;
+    } else if (j >= L - NSP && j < L) {
+      // top, one species per thread (:429-441, :471-486), on the column's
+      // last lanes: in a column of two or more wavefronts the top and the
+      // bottom run side by side in different wavefronts
+      const int sp = L - 1 - j;
       const int jn = nA + 1;  // offset of level nAlts
       const R tn = xb(12, jn);
       const R wn = xb(F_W - F_LN + sp, jn), wnm1 = xb(F_W - F_LN + sp, jn - 1);
