@@ -1499,6 +1499,9 @@ int ov_advance_threaded(const ov_params *p, const ov_columns *c, int solver,
 /* ------------------------------------------------------------------ */
 /* Column prep and the vertical time step                              */
 /* ------------------------------------------------------------------ */
+/* Bit for bit with the reference build's calc_pressure, calc_rates,
+   calc_eddy_diffusion_coefficient and calc_timestep_vertical
+   (oracle/ref_prep_harness.f90, tests/test_prep_reference.py). */
 double ov_prepare(const ov_prep_params *p, int n_cols, int n_alts, const double *nds,
                   const double *temp, const double *rho, const double *vvel,
                   const double *dalt, const double *ivel, double *nden, double *press,

@@ -211,3 +211,55 @@ def sources(cols: dict, dt: float, use_turbulent_cond: bool) -> dict:
         out[k] = vals[pos:pos + n].reshape(shp).copy()
         pos += n
     return out
+
+
+# --- the column prep and the vertical time step (gitm_prep_ref) ------------
+PEXE = os.path.join(HERE, "_ref", "gitm_prep_ref")
+PMAGIC = 1196840534
+PREP_OUT = ("nden", "press", "cp", "gamma", "mmm", "tempunit", "eddy", "cmax")
+
+
+def prep_available() -> bool:
+    return os.path.exists(PEXE)
+
+
+def prep(cols: dict, pp) -> tuple[dict, float, np.ndarray, np.ndarray]:
+    """The reference's calc_pressure, calc_rates (MeanMajorMass, TempUnit),
+    calc_eddy_diffusion_coefficient and calc_timestep_vertical on the
+    columns of ``cols`` (whole 9x9 blocks, nAlts = 50) with the prep
+    parameters ``pp`` (gitm_amd.synth.PrepParams; its mass and vibration
+    are not sent: the reference uses its own, which are returned).  Returns
+    the outputs of PREP_OUT on levels -1..nAlts+2 (cmax on 0..nAlts+1, the
+    others zero), Dt, Mass(1:6) and Vibration(1:6)."""
+    nc, L = cols["temp"].shape
+    na = L - 4
+    if na != REF_NALTS or nc % (REF_BLOCK[0] * REF_BLOCK[1]) or \
+            nc // (REF_BLOCK[0] * REF_BLOCK[1]) > REF_BLOCK[2]:
+        raise ValueError("reference build is compiled for 9x9 blocks, nAlts=50, <= 4 blocks")
+    hdr = np.zeros(16, dtype=np.int32)
+    hdr[:4] = [PMAGIC, nc, na, int(pp.use_ion_advection)]
+    par = np.zeros(8)
+    par[:6] = [pp.cfl, pp.fixed_dt, pp.dt_end, pp.eddy_coef, pp.eddy_p0, pp.eddy_p1]
+    ivel = cols["ivel"] if "ivel" in cols else np.zeros((nc, 3, L))
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        fout = os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(hdr.tobytes())
+            f.write(par.tobytes())
+            for a in (cols["nds"], cols["temp"], cols["rho"], cols["vvel"], cols["dalt"], ivel):
+                f.write(_fortran_order(a).tobytes())
+        subprocess.run([PEXE, fin, fout], check=True, capture_output=True)
+        raw = np.fromfile(fout, dtype=np.uint8)
+    status = raw[:16].view(np.int32)
+    if status[0]:
+        raise RuntimeError(f"gitm_prep_ref status {status}")
+    vals = raw[16:].view(np.float64)
+    out, pos = {}, 0
+    for k in PREP_OUT:
+        out[k] = vals[pos:pos + nc * L].reshape(nc, L).copy()
+        pos += nc * L
+    dt = float(vals[pos])
+    mass = vals[pos + 1:pos + 7].copy()
+    vib = vals[pos + 7:pos + 13].copy()
+    return out, dt, mass, vib
