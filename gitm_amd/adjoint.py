@@ -207,13 +207,20 @@ def _source(temp, heat, cp_tu, dt: float):
     return out
 
 
-def _prep(solver, state: dict, geometry: dict, pp: PrepParams) -> dict:
+def _prep_buffers(like) -> dict:
+    """Output tensors of the column prep, shaped like ``like``; zero, so that
+    cMax's levels -1 and nAlts+2, which the prep does not write, stay zero
+    when the buffers are reused."""
+    return {k: torch.zeros_like(like) for k in PREP + ("cp", "nden")}
+
+
+def _prep(solver, state: dict, geometry: dict, pp: PrepParams, bufs: dict | None = None) -> dict:
     """The column prep of ``state``: a dict with the prep outputs (PREP keys,
-    "cp") in fresh tensors."""
+    "cp"), in ``bufs`` (from _prep_buffers; reused from step to step, they
+    are overwritten) or in fresh tensors."""
     d = dict(state)
     d.update({k: geometry[k] for k in GEOMETRY})
-    for k in PREP + ("cp", "nden"):
-        d[k] = torch.zeros_like(state["temp"])
+    d.update(bufs if bufs is not None else _prep_buffers(state["temp"]))
     solver.prepare(d, pp)
     return {k: d[k] for k in PREP + ("cp",)}
 
@@ -235,9 +242,10 @@ class VerticalWindow(torch.autograd.Function):
         geometry = {k: fixed[k] for k in GEOMETRY}
         held = None if with_prep else {k: fixed[k] for k in PREP + ("cp",)}
         checkpoints = []
+        bufs = _prep_buffers(state["temp"]) if with_prep else None
         for _ in range(n_steps):
             checkpoints.append({k: state[k].clone() for k in STATE})
-            prep = _prep(solver, state, geometry, pp) if with_prep else held
+            prep = _prep(solver, state, geometry, pp, bufs) if with_prep else held
             cols = dict(state)
             cols.update(geometry)
             cols.update({k: prep[k] for k in PREP})
@@ -260,22 +268,27 @@ class VerticalWindow(torch.autograd.Function):
         heat = ctx.heat
         g_heat = torch.zeros_like(heat)
         gprep = {k: torch.empty_like(heat) for k in PREP}
+        bufs = _prep_buffers(heat) if ctx.held is None else None
+        # d temp_out / d heat on levels 1..nAlts times the temp adjoint; zero
+        # on the ghost levels, which the source does not change
+        a = torch.zeros_like(heat)
         for ck in reversed(ctx.checkpoints):
-            prep = _prep(ctx.solver, ck, ctx.geometry, ctx.pp) if ctx.held is None else ctx.held
+            prep = _prep(ctx.solver, ck, ctx.geometry, ctx.pp, bufs) if ctx.held is None \
+                else ctx.held
             cp, tu = prep["cp"], prep["tempunit"]
             # the source: temp += Dt Q / (cp TempUnit) on 1..nAlts
-            lt = torch.zeros_like(heat)
-            lt[:, 2:-2] = lam["temp"][:, 2:-2]
             q = dt / (cp * tu)
-            g_heat += lt * q
+            torch.mul(lam["temp"][:, 2:-2], q[:, 2:-2], out=a[:, 2:-2])
+            g_heat += a
             cols = dict(ck)
             cols.update(ctx.geometry)
             cols.update({k: prep[k] for k in PREP})
             if ctx.held is None:
                 lam = adjoint_step(ctx.solver, cols, lam, ctx.params, gin=lam, gprep=gprep)
                 # the source's cp and TempUnit come from the same prep
-                gprep["tempunit"] -= lt * heat * q / tu
-                gcp = -(lt * heat * q / cp).contiguous()
+                b = a * heat
+                gprep["tempunit"] -= b / tu
+                gcp = torch.div(b, cp).neg_()
                 lam = prepare_adjoint(ctx.solver, cols, gprep, lam, gcp=gcp, pp=ctx.pp)
             else:
                 lam = adjoint_step(ctx.solver, cols, lam, ctx.params, gin=lam)

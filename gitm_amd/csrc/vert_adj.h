@@ -170,9 +170,11 @@ struct TapCtx : Base {
 // order is stable: the transposed friction matrix (the friction matrix
 // I + dt (diag(sum_q C_iq) - C), C >= 0, is diagonally dominant by rows).
 GV_FN void solve6_dd(double (&a)[6][6], double (&b)[6]) {
+  double rd[6];  // reciprocals of the pivots, for the elimination and the substitution
 #pragma unroll
   for (int k = 0; k < 6; k++) {
     const double r = 1.0 / a[k][k];
+    rd[k] = r;
 #pragma unroll
     for (int i = k + 1; i < 6; i++) {
       const double f = a[i][k] * r;
@@ -186,7 +188,7 @@ GV_FN void solve6_dd(double (&a)[6][6], double (&b)[6]) {
     double t = b[i];
 #pragma unroll
     for (int q = i + 1; q < 6; q++) t = t - a[i][q] * b[q];
-    b[i] = t / a[i][i];
+    b[i] = t * rd[i];
   }
 }
 
@@ -297,8 +299,8 @@ struct AdjColumn {
         wb[q] = Us(F_W + j, 1 + q);    // VertVel of levels 0..5
       }
       t0 = Us(F_T, 1);
-    } else if (j >= 8 && j < 8 + NSP) {
-      const int sp = j - 8;
+    } else if (j >= L - NSP) {  // the top on the column's last lanes, as the forward
+      const int sp = L - 1 - j;
       tn = Us(F_T, jn);
       wn = Us(F_W + sp, jn);
       wnm1 = Us(F_W + sp, jn - 1);
@@ -320,20 +322,20 @@ struct AdjColumn {
         sr = sr + m[sp] * ns[sp];
       }
       const double lr = ln(sr);
-      const double er = exp(lr);
+      const double er = exp(lr), ier = 1.0 / er;
       double vu = 0.0;
 #pragma unroll
       for (int sp = 0; sp < NSP; sp++)
-        vu = vu + ns[sp] * m[sp] * bc[6 * sp + 2 + o] / er;
+        vu = vu + ns[sp] * m[sp] * bc[6 * sp + 2 + o] * ier;
       const double lvu = lam[F_VU];
-      const double ler = -lvu * vu / er;
+      const double ler = -lvu * vu * ier;
       const double lsr = (lam[F_LR] + ler * er) / sr;
 #pragma unroll
       for (int sp = 0; sp < NSP; sp++) {
         const double w = bc[6 * sp + 2 + o];
-        const double lns = lsr * m[sp] + lvu * m[sp] * w / er;
+        const double lns = lsr * m[sp] + lvu * m[sp] * w * ier;
         AB[sp * 6 + o] = lam[F_LN + sp] + lns * ns[sp];
-        AB[sp * 6 + 2 + o] = lvu * ns[sp] * m[sp] / er;
+        AB[sp * 6 + 2 + o] = lvu * ns[sp] * m[sp] * ier;
         AB[sp * 6 + 4 + o] = lam[F_W + sp];
       }
       if (j == 0) AS[0] = lam[F_T];
@@ -350,9 +352,10 @@ struct AdjColumn {
         e[sp] = exp(bc[36 + 4 * sp + o]);
         sr = sr + m[sp] * e[sp];
       }
+      const double isr = 1.0 / sr;
 #pragma unroll
       for (int sp = 0; sp < NSP; sp++) {
-        AT[sp * 4 + o] = lam[F_LN + sp] + lam[F_LR] * m[sp] * e[sp] / sr;
+        AT[sp * 4 + o] = lam[F_LN + sp] + lam[F_LR] * m[sp] * e[sp] * isr;
         AT[sp * 4 + 2 + o] = lam[F_W + sp];
       }
       AS[4 + o] = lam[F_T];
@@ -438,8 +441,8 @@ struct AdjColumn {
       }
       AS[18 + sp] = ct0;
       if (photochemical(sp) || sp == S_O) AS[12 + sp] = 0.0;
-    } else if (j >= 8 && j < 8 + NSP) {
-      const int sp = j - 8;
+    } else if (j >= L - NSP) {
+      const int sp = L - 1 - j;
       const double *lt = &AT[sp * 4];
       const double mm = kp.mass[sp];
       const double ish1 = -c.sc(SC_MGT1) * mm / ((0.5 * (tn + tn)) * KB);
@@ -514,7 +517,7 @@ struct AdjColumn {
     double U[NF];
 #pragma unroll
     for (int f = 0; f < NF; f++) U[f] = R(s, RC_U + f);
-    const double T = U[F_T];
+    const double T = U[F_T], iT = 1.0 / T;
     const double rho = exp(U[F_LR]);
     double NS[NSP], rhos[NSP], nt = 0.0;
 #pragma unroll
@@ -654,7 +657,7 @@ struct AdjColumn {
             // M_iq = -dtin C_iq, M_ii = 1 + dtin sum_q C_iq
             const double lC = dtin * mu[i] * (x[q] - x[i]);
             lnns[q] += lC * cc[i][q];
-            lT += lC * cc[i][q] * nns[q] * (1.0 - DEX[i][q]) / T;
+            lT += lC * cc[i][q] * nns[q] * (1.0 - DEX[i][q]) * iT;
           }
         }
         lam[F_T] += lT;
@@ -672,8 +675,9 @@ struct AdjColumn {
         const bool sh = (sub >> q) & 1;
         const double dev = rhos[q] - m[q] * hns[q];
         const double nm = moms - dtin * msf[q] + dtin * (sh ? dev : rhos[q]) * eg;
-        const double lnm = l / nrs[q];
-        lnrs[q] -= lnm * nm / nrs[q];
+        const double inrs = 1.0 / nrs[q];
+        const double lnm = l * inrs;
+        lnrs[q] -= lnm * nm * inrs;
         lVN += l * dtin * 2.0 * VN * invr;
         lVE += l * dtin * 2.0 * VE * invr;
         if (kp.coriolis) lVE += l * dtin * c.coriolis;
@@ -689,9 +693,10 @@ struct AdjColumn {
       lam[F_VN] += lVN;
       lam[F_VE] += lVE;
       // new LogNS = log(nns), new LogRho = log(sum m nns)
+      const double isr = 1.0 / sr;
 #pragma unroll
       for (int q = 0; q < NSP; q++) {
-        lnns[q] += lnv[F_LN + q] / nns[q] + lnv[F_LR] * m[q] / sr + lnnt;
+        lnns[q] += lnv[F_LN + q] / nns[q] + lnv[F_LR] * m[q] * isr + lnnt;
       }
       lnrho -= linrho * inrho * inrho;
 #pragma unroll
@@ -791,14 +796,14 @@ struct AdjColumn {
           if (j + 1 <= nA + 2) lf[i] -= xb(3 * i + 2, j + 1);
           if (f < 6) {
             lhns[f] += lf[i] / hns[f];  // ln(hns KB T)
-            lam[F_T] += lf[i] / T;
+            lam[F_T] += lf[i] * iT;
           } else if (f < 12) {
             lam[F_W + f - 6] += lf[i];
           } else if (f < 18) {
             lam[F_LN + f - 12] += lf[i];  // ln(m ns)
           } else if (f < 24) {
             lam[F_LN + f - 18] += lf[i];  // ln(ns KB T)
-            lam[F_T] += lf[i] / T;
+            lam[F_T] += lf[i] * iT;
           } else if (f == 24) {
             lam[F_VE] += lf[i];
           } else if (f == 25) {
@@ -999,8 +1004,8 @@ struct AdjColumn {
         const double kpx = side ? kp1 : kp0, kux = side ? ku1 : ku0;
         const double rdt = side ? rdt1 : rdt0;
         const double D = sh ? pdh : pd;
-        const double Q = mcs * (facs + rdt);
-        const double MP = kpx * f1 * D / Q;
+        const double Q = mcs * (facs + rdt), iQ = 1.0 / Q;
+        const double MP = kpx * f1 * D * iQ;
         const double NV = NVq[side];
         double lFR, lFM, lNP;
         if (side) {
@@ -1039,9 +1044,9 @@ struct AdjColumn {
         lMF4N += lNV * CS;
         const double lMP = -lNV * CS;
         // MP = Kp f1 D / (mcs (facs + rdt))
-        lf1 += lMP * kpx * D / Q;
-        const double lD = lMP * kpx * f1 / Q;
-        const double lQ = -lMP * MP / Q;
+        lf1 += lMP * kpx * D * iQ;
+        const double lD = lMP * kpx * f1 * iQ;
+        const double lQ = -lMP * MP * iQ;
         lmcs += lQ * (facs + rdt);
         lfacs += lQ * mcs;
         if (sh)
@@ -1230,12 +1235,13 @@ struct AdjColumn {
       const double MT = 0.5 * (T + tm1);
       const double dF = st(A_DF, j);
       double la = 0.0, lMT = 0.0;
+      const double ikmt = 1.0 / (KB * MT), imt = 1.0 / MT;
 #pragma unroll
       for (int q = 0; q < NSP; q++) {
         la += S[q];
-        const double ish = -1.0 * kp.mass[q] * c.S.mgh / (KB * MT);
+        const double ish = -1.0 * kp.mass[q] * c.S.mgh * ikmt;
         const double logb = -1.0 * dF * ish;
-        lMT += S[q] * (-logb / MT);
+        lMT += S[q] * (-logb * imt);
       }
       cown = -la / T + 0.5 * lMT;
       cnb = la / tm1 + 0.5 * lMT;
@@ -1318,7 +1324,8 @@ struct AdjColumn {
     double lL0[3] = {0, 0, 0}, lR0[3] = {0, 0, 0}, lL1[3] = {0, 0, 0}, lR1[3] = {0, 0, 0};
     double lcm0 = 0.0, lcm1 = 0.0;
     if (cl) {
-      const double g = 0.5 * lgt / dC;
+      const double idC = 1.0 / dC;
+      const double g = 0.5 * lgt * idC;
       lL1[0] = g;
       lR1[0] = g;
       lL0[0] = -g;
@@ -1326,13 +1333,13 @@ struct AdjColumn {
       const double ld[3] = {0.0, ldfe, ldfn};
 #pragma unroll
       for (int f = 1; f < 3; f++) {
-        lR1[f] = cm1 * ld[f] / dC;
-        lL1[f] = -cm1 * ld[f] / dC;
-        lR0[f] = -cm0 * ld[f] / dC;
-        lL0[f] = cm0 * ld[f] / dC;
+        lR1[f] = cm1 * ld[f] * idC;
+        lL1[f] = -cm1 * ld[f] * idC;
+        lR0[f] = -cm0 * ld[f] * idC;
+        lL0[f] = cm0 * ld[f] * idC;
         const double L0 = xb(3 + f, j - 1), R1 = xb(f, j + 1);
-        lcm1 += ld[f] * (R1 - ln[f]) / dC;
-        lcm0 -= ld[f] * (rf[f] - L0) / dC;
+        lcm1 += ld[f] * (R1 - ln[f]) * idC;
+        lcm0 -= ld[f] * (rf[f] - L0) * idC;
       }
     }
     cx.sync();
