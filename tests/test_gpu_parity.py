@@ -39,8 +39,8 @@ RTOL = 1e-11
 STATE = ("nds", "vvel", "vel", "temp", "rho", "ions")
 THREADS = min(16, os.cpu_count() or 1)
 GOLDEN = sorted(g for g in glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz"))
-                if not os.path.basename(g).startswith(("horizontal_", "sources_")))
-# (the vertical step's; horizontal_* and sources_* are the other steps')
+                if not os.path.basename(g).startswith(("horizontal_", "sources_", "prep_")))
+# (the vertical step's; horizontal_*, sources_* and prep_* are the other steps')
 
 
 def gpu_run(solver, cols, params, which="ausm", n_steps=1):

@@ -19,8 +19,8 @@ from gitm_amd import synth
 from oracle import cpu, ref
 
 GOLDEN = sorted(g for g in glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz"))
-                if not os.path.basename(g).startswith(("horizontal_", "sources_")))
-# (the vertical step's; horizontal_* and sources_* are the other steps')
+                if not os.path.basename(g).startswith(("horizontal_", "sources_", "prep_")))
+# (the vertical step's; horizontal_*, sources_* and prep_* are the other steps')
 OUT = ("nds", "vvel", "vel", "temp", "rho", "logns", "nden", "ions")
 
 
