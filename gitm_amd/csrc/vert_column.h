@@ -1197,9 +1197,12 @@ struct Column {
       if (lv && j >= 2) {
         xb(1, j) = tm1 / T;
         const R MeanTemp = 0.5 * (T + tm1);
+        // the device form divides once for the six species
+        const bool shared = is_double<R>::value && cx.device_math();
+        const R ikt = shared ? R(1.0 / (KB * MeanTemp)) : R(0.0);
 #pragma unroll
         for (int s = 0; s < NSP; s++) {
-          const R ish = -1.0 * m[s] * S.mgh / (KB * MeanTemp);
+          const R ish = shared ? -1.0 * m[s] * S.mgh * ikt : -1.0 * m[s] * S.mgh / (KB * MeanTemp);
           xb(2 + s, j) = exp(-1.0 * dF * ish);
         }
       } else if (j == 0) {
