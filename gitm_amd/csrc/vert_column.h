@@ -196,6 +196,16 @@ template <class A, class B>
 GV_FN auto dmin(A a, B b) -> decltype(a + b) {
   return a < b ? a : b;
 }
+#ifndef GV_HOST_EMULATION
+// On the device two doubles take v_max_f64 / v_min_f64 (one instruction in
+// place of a compare and two selects).  They agree with the selects above
+// except for NaN operands and the sign of a zero result (min(-0, +0) is -0
+// here, +0 above), which the column code does not meet: its operands are
+// limited slopes, sound speeds, Mach numbers and winds that are finite, and
+// a zero result is only ever added or multiplied.
+GV_FN double dmax(double a, double b) { return __builtin_fmax(a, b); }
+GV_FN double dmin(double a, double b) { return __builtin_fmin(a, b); }
+#endif
 
 // ln_fast: a natural logarithm in 33 FP64 instructions, against 76 for the
 // device library's log (which carries extra precision); the AUSM stage takes
