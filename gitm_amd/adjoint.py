@@ -287,7 +287,7 @@ class VerticalWindow(torch.autograd.Function):
                 lam = adjoint_step(ctx.solver, cols, lam, ctx.params, gin=lam, gprep=gprep)
                 # the source's cp and TempUnit come from the same prep
                 b = a * heat
-                gprep["tempunit"] -= b / tu
+                gprep["tempunit"].addcdiv_(b, tu, value=-1.0)
                 gcp = torch.div(b, cp).neg_()
                 lam = prepare_adjoint(ctx.solver, cols, gprep, lam, gcp=gcp, pp=ctx.pp)
             else:

@@ -201,7 +201,7 @@ GV_FN void limiter_mc_rev(double beta, double up, double dn, double ld, double &
   if (up > 0.0) {
     if (!(dn > 0.0)) return;
     const double a = beta * up, b = beta * dn, c = (up + dn) * 0.5;
-    const double m = a < b ? a : b;
+    const double m = dmin(a, b);
     if (m < c) {
       if (a < b)
         lup = beta * ld;
@@ -215,7 +215,7 @@ GV_FN void limiter_mc_rev(double beta, double up, double dn, double ld, double &
   }
   if (!(dn < 0.0)) return;
   const double a = beta * up, b = beta * dn, c = (up + dn) * 0.5;
-  const double m = a > b ? a : b;
+  const double m = dmax(a, b);
   if (m > c) {
     if (a > b)
       lup = beta * ld;
@@ -566,8 +566,8 @@ struct AdjColumn {
       double ww[NSP], nvu = 0.0, swn = 0.0;
 #pragma unroll
       for (int q = 0; q < NSP; q++) {
-        double x = w[q] > -kp.maxvv ? w[q] : -kp.maxvv;
-        x = kp.maxvv < x ? kp.maxvv : x;
+        double x = dmax(-kp.maxvv, w[q]);
+        x = dmin(kp.maxvv, x);
         ww[q] = x;
         nvu = nvu + x * nrs[q] * inrho;
         swn = swn + x * nrs[q];
@@ -577,13 +577,14 @@ struct AdjColumn {
       const double nte = te - dtin * ef + dtin * rho * VU * eg;
       const double K2n = nvu * nvu + nvN * nvN + nvE * nvE;
       const double np = (nte - 0.5 * nrho * K2n) * gm1;
-      const double nT = np / (KB * nnt);
+      const double iknt = 1.0 / (KB * nnt);
+      const double nT = np * iknt;
       double sr = 0.0;
 #pragma unroll
       for (int q = 0; q < NSP; q++) sr = sr + m[q] * nns[q];
       // T' = np / (KB nnt); np = (nte - nrho K2n/2) (gamma - 1)
-      const double lnp = lnv[F_T] / (KB * nnt);
-      double lnnt = -lnv[F_T] * nT / nnt;
+      const double lnp = lnv[F_T] * iknt;
+      double lnnt = -lnv[F_T] * nT * (KB * iknt);
       const double lnte = lnp * gm1;
       double lnrho = -0.5 * lnp * gm1 * K2n;
       const double lnvu = lnv[F_VU] - lnp * gm1 * nrho * nvu;
