@@ -547,8 +547,10 @@ def bench_adjoint(torch, solver, steps: int = 10, seed: int = 5) -> dict:
     400 km.  The forward pass (the plain HIP prep and step, one checkpoint per
     step) and the backward pass (per step: the prep recomputed from the
     checkpoint, the hand-written adjoint of the step with the prep inputs'
-    adjoints, the prep's adjoint) are timed after one untimed pass.  The
-    ratio is backward / forward."""
+    adjoints, the prep's adjoint) are timed over five passes after one
+    untimed pass; each is the median of its five (a pass is tens of ms of
+    small launches, whose time moves by several per cent from pass to pass).
+    The ratio is the backward median over the forward median."""
     from gitm_amd import adjoint, synth, vertical
 
     spec = synth.CONFIGS["5deg_2592x50"]
@@ -562,9 +564,9 @@ def bench_adjoint(torch, solver, steps: int = 10, seed: int = 5) -> dict:
     heat = torch.from_numpy(adjoint.heating_profile(cols["alt"])).cuda()
     w = torch.from_numpy(adjoint.level_weights(cols["alt"], 400e3)).cuda()
     p = synth.StepParams()
-    fwd_ms = bwd_ms = 0.0
+    fwd, bwd = [], []
     loss = 0.0
-    for rep in range(2):
+    for rep in range(6):
         state = {k: d[k].clone().requires_grad_(True) for k in adjoint.STATE}
         q = heat.clone().requires_grad_(True)
         torch.cuda.synchronize()
@@ -576,12 +578,17 @@ def bench_adjoint(torch, solver, steps: int = 10, seed: int = 5) -> dict:
         J.backward()
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        fwd_ms, bwd_ms, loss = (t1 - t0) * 1e3, (t2 - t1) * 1e3, float(J.detach())
+        if rep:
+            fwd.append((t1 - t0) * 1e3)
+            bwd.append((t2 - t1) * 1e3)
+        loss = float(J.detach())
+    fwd_ms, bwd_ms = float(np.median(fwd)), float(np.median(bwd))
     ok = bool(torch.isfinite(state["temp"].grad).all()) and bool(torch.isfinite(q.grad).all())
     return {"workload": f"5deg_2592x50: 2592 columns x 50 levels, {steps} x (column prep + "
                         "AUSM step + heating source), loss = mean mass density at 400 km",
             "forward_ms": fwd_ms, "backward_ms": bwd_ms, "ratio": bwd_ms / fwd_ms,
             "forward_ms_per_step": fwd_ms / steps, "backward_ms_per_step": bwd_ms / steps,
+            "forward_ms_samples": fwd, "backward_ms_samples": bwd,
             "loss": loss, "gradients_finite": ok}
 
 

@@ -81,10 +81,15 @@ enum AdjFix { FX_TU = 0, FX_GAMMA, FX_MMM, FX_CMAX, FX_EDDY, NFIX };
 // conditions' adjoint hand-offs.
 // Layout: [0, 36) bottom Rb adjoints, [36, 60) top ones, [60, 84) scalars,
 // [84, 156) bottom species contributions, [156, 180) top ones, then the
-// column's cMax and Gamma by lane (nAlts+4 each).  At nAlts = 50 the
-// column's LDS is 39.8 KiB, four columns per CU.
+// column's cMax and Gamma by lane (nAlts+4 each), then the AX_PRE values of
+// the boundary levels' state that the boundary conditions' reverse reads
+// (share_boundary_state).  At nAlts = 50 the column's LDS is 40.3 KiB, four
+// columns per CU.
 constexpr int AX_CMAX = 180;
-GV_HD constexpr int adj_lds_extra(int n_alts) { return AX_CMAX + 2 * (n_alts + 4); }
+constexpr int AX_PRE_LEN = 87;
+GV_HD constexpr int adj_lds_extra(int n_alts) {
+  return AX_CMAX + 2 * (n_alts + 4) + AX_PRE_LEN;
+}
 GV_HD constexpr unsigned long adj_lds_bytes(int n_alts, bool carry_lds) {
   return (lds_doubles(n_alts, false, carry_lds) + adj_lds_extra(n_alts)) * sizeof(double);
 }
@@ -280,30 +285,64 @@ struct AdjColumn {
   // stage s (its interior levels are the state before the boundary
   // conditions; the ghost values are recomputed here).
   // -------------------------------------------------------------------------
+  // The LDS values of the boundary levels' state (AX_PRE_LEN doubles after
+  // cMax and Gamma): [sp*12 + q] LogNS of species sp and [sp*12 + 6 + q] its
+  // VertVel at level q = 0..5; [72] Temp(0); [73] Temp(nAlts);
+  // [74 + 2 sp], [75 + 2 sp] VertVel of species sp at nAlts and nAlts-1;
+  // [86] Vel(up) at nAlts.
+  GV_MEM double *pre() const { return ax + AX_CMAX + 2 * L; }
+  // Each lane that holds one of those levels' state (U, the state the
+  // boundary conditions' reverse of the same pass reads) puts it there, so
+  // that bcs_rev reads LDS instead of other lanes' records in device memory.
+  // Called before a barrier that precedes the bcs_rev of the pass, and after
+  // one that follows the previous bcs_rev's reads.
+  GV_MEM void share_boundary_state(const double (&U)[NF]) {
+    double *P = pre();
+    if (j >= 1 && j <= 6) {
+      const int q = j - 1;
+#pragma unroll
+      for (int sp = 0; sp < NSP; sp++) {
+        P[sp * 12 + q] = U[F_LN + sp];
+        P[sp * 12 + 6 + q] = U[F_W + sp];
+      }
+      if (j == 1) P[72] = U[F_T];
+    } else if (j == nA + 1) {
+      P[73] = U[F_T];
+#pragma unroll
+      for (int sp = 0; sp < NSP; sp++) P[74 + 2 * sp] = U[F_W + sp];
+      P[86] = U[F_VU];
+    } else if (j == nA) {
+#pragma unroll
+      for (int sp = 0; sp < NSP; sp++) P[75 + 2 * sp] = U[F_W + sp];
+    }
+  }
+
   GV_MEM void bcs_rev(int s) {
     const double *m = kp.mass;
     const int jn = nA + 1;  // lane of level nAlts
     // the pass's results: Rb of the bottom, 6 per species, then the top's,
     // 4 per species (the forward recorded them)
-    const double *bc = bcrec + s * ADJ_BC;
+    // (stage_rev copied them from the record to the LDS scalars they came from)
+    const double *bc = &c.sc(SC_BOT);
     // the values the species lanes read, from the record of state s: its
     // interior levels are the state before the boundary conditions, and
     // Temp(0) and the non-photochemical LogNS(0) are never changed by them
-    auto Us = [&](int f, int lane) { return rec[((long)s * NREC + RC_U + f) * NT + lane]; };
+    // (share_boundary_state left them in LDS, P)
+    const double *P = pre();
     double lnb[6], wb[6], t0 = 0.0;
     double tn = 0.0, wn = 0.0, wnm1 = 0.0;
     if (j < NSP) {
 #pragma unroll
       for (int q = 0; q < 6; q++) {
-        lnb[q] = Us(F_LN + j, 1 + q);  // LogNS of levels 0..5
-        wb[q] = Us(F_W + j, 1 + q);    // VertVel of levels 0..5
+        lnb[q] = P[j * 12 + q];     // LogNS of levels 0..5
+        wb[q] = P[j * 12 + 6 + q];  // VertVel of levels 0..5
       }
-      t0 = Us(F_T, 1);
+      t0 = P[72];
     } else if (j >= L - NSP) {  // the top on the column's last lanes, as the forward
       const int sp = L - 1 - j;
-      tn = Us(F_T, jn);
-      wn = Us(F_W + sp, jn);
-      wnm1 = Us(F_W + sp, jn - 1);
+      tn = P[73];
+      wn = P[74 + 2 * sp];
+      wnm1 = P[75 + 2 * sp];
     }
     // hand-off area: bottom [sp*6 + 0..5] adjoints of Rb (n0, nm1, w0, wm1,
     // w0r, wm1r); top [36 + sp*4 + 0..3] of (l1, l2, w1, w2); [60..63]
@@ -494,7 +533,7 @@ struct AdjColumn {
       lam[F_VE] += AS[6] + AS[7];
       lam[F_VN] += AS[8] + AS[9];
       // Vel(up) of the top ghosts = |Vel(up, nAlts)|
-      const double sgn = (Us(F_VU, jn) < 0.0) ? -1.0 : 1.0;
+      const double sgn = (P[86] < 0.0) ? -1.0 : 1.0;
       lam[F_VU] += sgn * (AS[10] + AS[11]);
     }
     if (j == jn - 1) {
@@ -517,6 +556,12 @@ struct AdjColumn {
     double U[NF];
 #pragma unroll
     for (int f = 0; f < NF; f++) U[f] = R(s, RC_U + f);
+    share_boundary_state(U);  // for this pass's bcs_rev, after this stage's barriers
+    // this pass's boundary results (bcs_rev reads them in the LDS scalars):
+    // loaded here, stored before the hydrostatic reverse's barriers, so that
+    // the load is waited for late
+    const double bcv0 = (j < ADJ_BC) ? bcrec[s * ADJ_BC + j] : 0.0;
+    const double bcv1 = (j + L < ADJ_BC) ? bcrec[s * ADJ_BC + j + L] : 0.0;
     const double T = U[F_T], iT = 1.0 / T;
     const double rho = exp(U[F_LR]);
     double NS[NSP], rhos[NSP], nt = 0.0;
@@ -818,6 +863,8 @@ struct AdjColumn {
     }
 
     // --- R5: SubtractHydrostatic's deviation: done in R1 (lhns) ----------
+    if (j < ADJ_BC) c.sc(SC_BOT + j) = bcv0;
+    if (j + L < ADJ_BC) c.sc(SC_BOT + j + L) = bcv1;
     // --- R6: the hydrostatic background (:694-751) ----------------------
     // hns(j) = ns0 * prod_{i=2..j} a(i) b_s(i) for lanes j >= 2, hns(1) =
     // NS(level 0), hns(0) = ns0 a(0) b_s(0).  With G(j) = lhns(j) hns(j),
@@ -1544,6 +1591,8 @@ GV_FN void adj_column(Ctx &cx, const KParams &kp, const gitm_vert_columns &io,
   // reverse: the boundary conditions after the last stage, then the RK4
   // combination stage by stage (Stage_{k+1} = Y + New_k - U_k; Final =
   // (acc + New_4 - U_4)/3 with acc = -Y + S2 + 2 S3 + S4 on cell levels)
+  a.share_boundary_state(c.U);
+  cx.sync();
   a.bcs_rev(4);
   // the RK4 combination's adjoints acc and Y live where the forward kept
   // the carries (LDS)
