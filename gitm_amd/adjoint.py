@@ -251,8 +251,11 @@ class VerticalWindow(torch.autograd.Function):
             cols.update({k: prep[k] for k in PREP})
             solver.advance(cols, params, "ausm", 1, check=False)
             state = {k: cols[k] for k in STATE}
-            state["temp"] = _source(state["temp"], heat.detach(), prep["cp"] * prep["tempunit"],
-                                    params.dt)
+            # cp TempUnit of this step's prep: kept with the checkpoint for
+            # the source's adjoint (a fresh tensor; the prep buffers are reused)
+            cp_tu = prep["cp"] * prep["tempunit"]
+            checkpoints[-1]["cp_tu"] = cp_tu
+            state["temp"] = _source(state["temp"], heat.detach(), cp_tu, params.dt)
         solver.check()
         ctx.solver, ctx.geometry, ctx.held, ctx.params, ctx.pp = solver, geometry, held, params, pp
         ctx.checkpoints, ctx.heat = checkpoints, heat.detach()
@@ -277,7 +280,7 @@ class VerticalWindow(torch.autograd.Function):
                 else ctx.held
             cp, tu = prep["cp"], prep["tempunit"]
             # the source: temp += Dt Q / (cp TempUnit) on 1..nAlts
-            q = dt / (cp * tu)
+            q = dt / ck["cp_tu"]
             torch.mul(lam["temp"][:, 2:-2], q[:, 2:-2], out=a[:, 2:-2])
             g_heat += a
             cols = dict(ck)
