@@ -800,6 +800,9 @@ struct AdjColumn {
       const unsigned long long kt = __builtin_bit_cast(unsigned long long, R(s, RC_KT));
       const bool act = j >= 1 && j <= nA + 2;
       const double iF0 = st(A_IF, j), iF1 = st(A_IF, j + 1), hdF0 = 0.5 * st(A_DF, j);
+      // unrolled: every field index, and so every index into lam, the work
+      // area and the limiter word, is a constant
+#pragma unroll
       for (int b0 = 0; b0 < 27; b0 += 7) {
         double lo[7], lf[7];
 #pragma unroll
