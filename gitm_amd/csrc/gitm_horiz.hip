@@ -136,13 +136,23 @@ __global__ void __launch_bounds__(256, 2)
   // live in device memory (accg, [workgroup][altitude][v][cell]): they are
   // touched once per stage, and the LDS they would take costs a workgroup
   // per CU.
-  for (int e = tid; e < napw * slab; e += (int)blockDim.x) {
-    const int aa = e / slab, r = e % slab;
-    const int v = r / (NLg * NLo), r2 = r % (NLg * NLo);
-    const int lat = r2 / NLo - 1, lon = r2 % NLo - 1;
-    int comp;
-    const gitm_vert_field &g = field(v, comp);
-    lds[e] = *hf(g, lon, lat, k0 + aa, comp, b);
+  // Each thread takes the cells r2 = tid, tid + blockDim.x, .. of a slice
+  // (two at most: a slice has (nla+4)(nlo+4) cells) and stages them for
+  // every variable and altitude, so that the only integer divisions are the
+  // per-cell ones, not one per element.
+  {
+    const int plane = NLg * NLo;
+    for (int r2 = tid; r2 < plane; r2 += (int)blockDim.x) {
+      const int lat = r2 / NLo - 1, lon = r2 % NLo - 1;
+      for (int aa = 0; aa < napw; aa++) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          int comp;
+          const gitm_vert_field &g = field(v, comp);
+          lds[(aa * NV + v) * plane + r2] = *hf(g, lon, lat, k0 + aa, comp, b);
+        }
+      }
+    }
   }
   const int aw = active ? a : 0;
   double *x = lds + (long long)aw * slab;
