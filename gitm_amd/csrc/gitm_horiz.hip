@@ -107,12 +107,16 @@ __device__ __forceinline__ void rusanov_cell(int i, int n, VF v, const LineGeo &
 // per CU holds about that many anyway).  TOPO: UseTopography, launched for
 // one altitude at a time (the host walks k = 1..nAlts), so that the level
 // below already holds the step's result when dVarDAlt reads it.
-template <int NV, bool TOPO>
+// BLON, BLAT: the block's cells per direction when fixed at compile time
+// (GITM's are, 9 x 9 by default), so that the LDS and cell indexing is
+// constant; 0 takes them from the parameters.
+template <int NV, bool TOPO, int BLON = 0, int BLAT = 0>
 __global__ void __launch_bounds__(256, 2)
     horiz_kernel(HzParams p, gitm_vert_columns io, HzFields f, unsigned long long *err,
                  double *accg) {
   extern __shared__ double lds[];
-  const int nlo = p.n_lons, nla = p.n_lats, NLo = nlo + 4, NLg = nla + 4;
+  const int nlo = BLON ? BLON : p.n_lons, nla = BLAT ? BLAT : p.n_lats, NLo = nlo + 4,
+            NLg = nla + 4;
   const int cells = nlo * nla, slab = NV * NLg * NLo;
   const int b = (int)blockIdx.x / p.alt_groups;
   const int k0 = p.k_first + ((int)blockIdx.x % p.alt_groups) * p.apw;
@@ -644,20 +648,39 @@ extern "C" int gitm_horiz_advance(gitm_vert_handle *h, const gitm_horiz_params *
     if (hipMalloc(&h->scratch, need) != hipSuccess) return GITM_VERT_ENOMEM;
     h->scratch_bytes = need;
   }
+  // GITM's default 9 x 9 blocks take the kernels with the block shape fixed
+  const bool b99 = nlo == 9 && nla == 9;
   auto launch = [&](const HzParams &q) -> int {
     const dim3 grid((unsigned)nwg), block(threads);
-    if (ions && topo)
-      hipLaunchKernelGGL((horiz_kernel<NV_IONS, true>), grid, block, lds, h->stream, q, *c, f,
-                         h->d_err, h->scratch);
-    else if (ions)
-      hipLaunchKernelGGL((horiz_kernel<NV_IONS, false>), grid, block, lds, h->stream, q, *c,
-                         f, h->d_err, h->scratch);
-    else if (topo)
-      hipLaunchKernelGGL((horiz_kernel<NV_NEUTRAL, true>), grid, block, lds, h->stream, q, *c,
-                         f, h->d_err, h->scratch);
-    else
-      hipLaunchKernelGGL((horiz_kernel<NV_NEUTRAL, false>), grid, block, lds, h->stream, q,
-                         *c, f, h->d_err, h->scratch);
+    if (ions && topo) {
+      if (b99)
+        hipLaunchKernelGGL((horiz_kernel<NV_IONS, true, 9, 9>), grid, block, lds, h->stream, q,
+                           *c, f, h->d_err, h->scratch);
+      else
+        hipLaunchKernelGGL((horiz_kernel<NV_IONS, true>), grid, block, lds, h->stream, q, *c,
+                           f, h->d_err, h->scratch);
+    } else if (ions) {
+      if (b99)
+        hipLaunchKernelGGL((horiz_kernel<NV_IONS, false, 9, 9>), grid, block, lds, h->stream,
+                           q, *c, f, h->d_err, h->scratch);
+      else
+        hipLaunchKernelGGL((horiz_kernel<NV_IONS, false>), grid, block, lds, h->stream, q, *c,
+                           f, h->d_err, h->scratch);
+    } else if (topo) {
+      if (b99)
+        hipLaunchKernelGGL((horiz_kernel<NV_NEUTRAL, true, 9, 9>), grid, block, lds,
+                           h->stream, q, *c, f, h->d_err, h->scratch);
+      else
+        hipLaunchKernelGGL((horiz_kernel<NV_NEUTRAL, true>), grid, block, lds, h->stream, q,
+                           *c, f, h->d_err, h->scratch);
+    } else {
+      if (b99)
+        hipLaunchKernelGGL((horiz_kernel<NV_NEUTRAL, false, 9, 9>), grid, block, lds,
+                           h->stream, q, *c, f, h->d_err, h->scratch);
+      else
+        hipLaunchKernelGGL((horiz_kernel<NV_NEUTRAL, false>), grid, block, lds, h->stream, q,
+                           *c, f, h->d_err, h->scratch);
+    }
     GV_HIP(hipGetLastError());
     return GITM_VERT_OK;
   };
