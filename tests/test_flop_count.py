@@ -21,7 +21,8 @@ def test_frozen_counts(workload, n_alts):
 
 
 def test_count_is_near_the_estimate():
-    # SURVEY's estimate: 2.0e4 (+-35%) weighted operations per level per step
+    # the estimate made by counting the reference's operations by hand:
+    # 2.0e4 (+-35%) weighted operations per level per step
     per_level = bench.ops_per_column(100) / 104
     assert 1.3e4 < per_level < 2.7e4
     # the reference forms (more logs and divisions) count more
