@@ -141,6 +141,18 @@ def test_stretched_100_levels(solver, which):
     per_step(solver, cols, p, which, 3)
 
 
+@pytest.mark.parametrize("n_alts", [10, 13, 17, 29, 60, 61, 124, 125, 188, 252])
+def test_column_heights(solver, n_alts):
+    """AUSM at column heights that change how the device forms split the
+    column: the hydrostatic product's segments (two of 6 levels at nAlts 10,
+    eight of 32 at 252, shorter last segments in between), the top
+    boundary's lanes (in the first wavefront up to nAlts 60, across two at
+    61, in the last one above), and one to four wavefronts per column."""
+    spec = synth.GridSpec(1, 1, n_alts, stretched=n_alts >= 100)
+    cols = subset(synth.make_columns(spec, seed=n_alts), [0, 7])
+    per_step(solver, cols, synth.StepParams(), "ausm", 2)
+
+
 @pytest.mark.parametrize("which", ["ausm", "rusanov"])
 @pytest.mark.parametrize("improved", [True, False])
 def test_ions_stretched_100_levels(solver, which, improved):
