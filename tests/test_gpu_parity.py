@@ -147,10 +147,13 @@ def test_column_heights(solver, n_alts):
     column: the hydrostatic product's segments (two of 6 levels at nAlts 10,
     eight of 32 at 252, shorter last segments in between), the top
     boundary's lanes (in the first wavefront up to nAlts 60, across two at
-    61, in the last one above), and one to four wavefronts per column."""
+    61, in the last one above), and one to four wavefronts per column; the
+    Rusanov step too at three of them, for the boundary lanes."""
     spec = synth.GridSpec(1, 1, n_alts, stretched=n_alts >= 100)
     cols = subset(synth.make_columns(spec, seed=n_alts), [0, 7])
     per_step(solver, cols, synth.StepParams(), "ausm", 2)
+    if n_alts in (10, 61, 252):  # the boundary lanes are the Rusanov step's too
+        per_step(solver, cols, synth.StepParams(), "rusanov", 2)
 
 
 @pytest.mark.parametrize("which", ["ausm", "rusanov"])
