@@ -8,7 +8,8 @@ intermediates in device memory, and reverses the stages' phases on the device.
 ``taped=True`` takes the same adjoint through ``gitm_vert_adjoint_taped``
 (every operation on a tape, swept back with atomics), the slow independent
 check.  ``prepare_adjoint`` wraps ``gitm_vert_prepare_adjoint``, the adjoint
-of the column prep.
+of the column prep, and ``source_adjoint`` ``gitm_vert_source_adjoint``, that
+of the window's heating source.
 
 ``VerticalWindow`` is a ``torch.autograd.Function`` over a window of steps
 that follows GITM's order within a time step: the column prep from the
@@ -78,6 +79,10 @@ def _lib():
                                                   ctypes.POINTER(AdjointPrepFields),
                                                   ctypes.POINTER(_abi.Field),
                                                   ctypes.POINTER(AdjointFields)]
+        dp = ctypes.POINTER(ctypes.c_double)
+        lib.gitm_vert_source_adjoint.restype = ctypes.c_int
+        lib.gitm_vert_source_adjoint.argtypes = [vp, ctypes.c_longlong, ctypes.c_int,
+                                                 ctypes.c_double, dp, dp, dp, dp, dp, dp, dp]
         lib.gitm_vert_set_adjoint_memory.restype = ctypes.c_int
         lib.gitm_vert_set_adjoint_memory.argtypes = [vp, ctypes.c_size_t]
         lib.gitm_vert_release_adjoint_memory.restype = ctypes.c_int
@@ -167,6 +172,35 @@ def prepare_adjoint(solver: "vertical.VerticalSolver", cols: dict, gprep: dict, 
     return gin
 
 
+def source_adjoint(solver: "vertical.VerticalSolver", g_temp, cp, tempunit, heat, dt: float,
+                   g_heat, g_tempunit=None, g_cp=None) -> None:
+    """The adjoint of the heating source Temperature(1:nAlts) += Dt * Q /
+    (cp * TempUnit) (``gitm_vert_source_adjoint``): from ``g_temp``, the
+    temperature's adjoint after the source, ADDS a = g_temp Dt / (cp TempUnit)
+    (levels 1..nAlts) to ``g_heat`` and, when given, -a Q / TempUnit to
+    ``g_tempunit`` and writes -a Q / cp to ``g_cp``.  All [column][level]
+    float64 device tensors.  Queued on the current stream."""
+    ts = [g_temp, cp, tempunit, heat, g_heat] + ([g_tempunit, g_cp] if g_cp is not None else [])
+    for i, t in enumerate(ts):
+        _check(t, f"source_adjoint argument {i}")
+        if t.shape != g_temp.shape:
+            raise ValueError("source_adjoint: every tensor shaped like g_temp")
+    if (g_tempunit is None) != (g_cp is None):
+        raise ValueError("source_adjoint: g_tempunit and g_cp go together")
+    ncol, L = int(g_temp.shape[0]), int(g_temp.shape[1])
+    dp = ctypes.POINTER(ctypes.c_double)
+
+    def ptr(t):
+        return ctypes.cast(t.data_ptr(), dp) if t is not None else None
+
+    solver.set_stream(torch.cuda.current_stream().cuda_stream)
+    rc = _lib().gitm_vert_source_adjoint(solver._h, ncol, L - 4, float(dt), ptr(g_temp),
+                                         ptr(cp), ptr(tempunit), ptr(heat), ptr(g_heat),
+                                         ptr(g_tempunit), ptr(g_cp))
+    if rc:
+        raise vertical.VerticalStepError(rc)
+
+
 def set_adjoint_memory(solver: "vertical.VerticalSolver", nbytes: int) -> None:
     rc = _lib().gitm_vert_set_adjoint_memory(solver._h, nbytes)
     if rc:
@@ -251,10 +285,7 @@ class VerticalWindow(torch.autograd.Function):
             cols.update({k: prep[k] for k in PREP})
             solver.advance(cols, params, "ausm", 1, check=False)
             state = {k: cols[k] for k in STATE}
-            # cp TempUnit of this step's prep: kept with the checkpoint for
-            # the source's adjoint (a fresh tensor; the prep buffers are reused)
             cp_tu = prep["cp"] * prep["tempunit"]
-            checkpoints[-1]["cp_tu"] = cp_tu
             state["temp"] = _source(state["temp"], heat.detach(), cp_tu, params.dt)
         solver.check()
         ctx.solver, ctx.geometry, ctx.held, ctx.params, ctx.pp = solver, geometry, held, params, pp
@@ -267,34 +298,34 @@ class VerticalWindow(torch.autograd.Function):
         for k, g in zip(STATE, grads):
             ref = ctx.checkpoints[0][k]
             lam[k] = torch.zeros_like(ref) if g is None else g.contiguous().clone()
+        # the adjoints before the step (the step's adjoint reads lam, the
+        # adjoints after the step and the source, and writes nxt)
+        nxt = {k: torch.empty_like(lam[k]) for k in STATE}
         dt = ctx.params.dt
         heat = ctx.heat
         g_heat = torch.zeros_like(heat)
         gprep = {k: torch.empty_like(heat) for k in PREP}
+        gcp = torch.empty_like(heat) if ctx.held is None else None
         bufs = _prep_buffers(heat) if ctx.held is None else None
-        # d temp_out / d heat on levels 1..nAlts times the temp adjoint; zero
-        # on the ghost levels, which the source does not change
-        a = torch.zeros_like(heat)
         for ck in reversed(ctx.checkpoints):
             prep = _prep(ctx.solver, ck, ctx.geometry, ctx.pp, bufs) if ctx.held is None \
                 else ctx.held
-            cp, tu = prep["cp"], prep["tempunit"]
-            # the source: temp += Dt Q / (cp TempUnit) on 1..nAlts
-            q = dt / ck["cp_tu"]
-            torch.mul(lam["temp"][:, 2:-2], q[:, 2:-2], out=a[:, 2:-2])
-            g_heat += a
             cols = dict(ck)
             cols.update(ctx.geometry)
             cols.update({k: prep[k] for k in PREP})
+            # the source (temp += Dt Q / (cp TempUnit) on 1..nAlts) passes
+            # the temperature's adjoint through unchanged; its cp and
+            # TempUnit come from the same prep as the step's
             if ctx.held is None:
-                lam = adjoint_step(ctx.solver, cols, lam, ctx.params, gin=lam, gprep=gprep)
-                # the source's cp and TempUnit come from the same prep
-                b = a * heat
-                gprep["tempunit"].addcdiv_(b, tu, value=-1.0)
-                gcp = torch.div(b, cp).neg_()
-                lam = prepare_adjoint(ctx.solver, cols, gprep, lam, gcp=gcp, pp=ctx.pp)
+                adjoint_step(ctx.solver, cols, lam, ctx.params, gin=nxt, gprep=gprep)
+                source_adjoint(ctx.solver, lam["temp"], prep["cp"], prep["tempunit"], heat, dt,
+                               g_heat, gprep["tempunit"], gcp)
+                prepare_adjoint(ctx.solver, cols, gprep, nxt, gcp=gcp, pp=ctx.pp)
             else:
-                lam = adjoint_step(ctx.solver, cols, lam, ctx.params, gin=lam)
+                adjoint_step(ctx.solver, cols, lam, ctx.params, gin=nxt)
+                source_adjoint(ctx.solver, lam["temp"], prep["cp"], prep["tempunit"], heat, dt,
+                               g_heat)
+            lam, nxt = nxt, lam
         ctx.solver.check()
         return (lam["nds"], lam["vvel"], lam["vel"], lam["temp"], lam["rho"], g_heat,
                 None, None, None, None, None, None)

@@ -460,6 +460,25 @@ __global__ void __launch_bounds__(256)
   gv::prep_level_adjoint(pp, io, gf, gcp, gin, ci, cj, cb, k);
 }
 
+// The heating source's adjoint, one thread per (column, level) of L levels:
+// a = lam Dt / (cp TempUnit) on the interior levels, g_heat += a, and with
+// g_cp, g_tu -= a Q / TempUnit, g_cp = -(a Q / cp).
+__global__ void __launch_bounds__(256)
+    source_adjoint_kernel(long long n, int L, double dt, const double *lam, const double *cp,
+                          const double *tu, const double *heat, double *g_heat, double *g_tu,
+                          double *g_cp) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int k = (int)(i % L);
+  const double a = (k >= 2 && k < L - 2) ? lam[i] * (dt / (cp[i] * tu[i])) : 0.0;
+  g_heat[i] += a;
+  if (g_cp) {
+    const double b = a * heat[i];
+    g_tu[i] -= b / tu[i];
+    g_cp[i] = -(b / cp[i]);
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -497,6 +516,24 @@ int gitm_vert_prepare_adjoint(gitm_vert_handle *h, const gitm_vert_prep_params *
   const long long n = (long long)c->n_lons * c->n_lats * c->n_blocks * (c->n_alts + 4);
   hipLaunchKernelGGL(prep_adjoint_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      h->stream, pp, *c, *gprep, gcp ? *gcp : none, *gin, n);
+  GV_HIP(hipGetLastError());
+  return GITM_VERT_OK;
+}
+
+int gitm_vert_source_adjoint(gitm_vert_handle *h, long long n_cols, int n_alts, double dt,
+                             const double *g_temp, const double *cp, const double *temp_unit,
+                             const double *heat, double *g_heat, double *g_temp_unit,
+                             double *g_cp) {
+  if (!h || n_cols < 0 || n_alts < 1 || !g_temp || !cp || !temp_unit || !g_heat)
+    return GITM_VERT_EINVAL;
+  if ((g_temp_unit == nullptr) != (g_cp == nullptr) || (g_cp && !heat)) return GITM_VERT_EINVAL;
+  const long long n = n_cols * (long long)(n_alts + 4);
+  if (n_cols > ((long long)1 << 38) / (n_alts + 4)) return GITM_VERT_EINVAL;
+  if (n == 0) return GITM_VERT_OK;
+  GV_HIP(hipSetDevice(h->device));
+  hipLaunchKernelGGL(source_adjoint_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     h->stream, n, n_alts + 4, dt, g_temp, cp, temp_unit, heat, g_heat,
+                     g_temp_unit, g_cp);
   GV_HIP(hipGetLastError());
   return GITM_VERT_OK;
 }

@@ -372,6 +372,21 @@ int gitm_vert_prepare_adjoint(gitm_vert_handle *h, const gitm_vert_prep_params *
                               const gitm_vert_adjoint_prep_fields *gprep,
                               const gitm_vert_field *gcp, const gitm_vert_adjoint_fields *gin);
 
+/* The adjoint of a heating source applied after the step, Temperature
+ * (1:nAlts) += Dt * Q / (cp * TempUnit) (the form of add_sources.f90:53-66
+ * for a heating rate Q), for n_cols contiguous columns of n_alts + 4 levels
+ * ([column][level], level -1 first).  From the adjoint `g_temp` of the
+ * temperature after the source, with a = g_temp * (Dt / (cp * TempUnit)) on
+ * levels 1..nAlts and 0 on the others: g_heat += a, and when g_temp_unit and
+ * g_cp are given (both or neither), g_temp_unit -= a * Q / TempUnit and
+ * g_cp = -(a * Q / cp) (overwritten, every level).  The temperature's own
+ * adjoint passes through unchanged.  The reference has no adjoint; this is
+ * the differentiable window's.  Asynchronous. */
+int gitm_vert_source_adjoint(gitm_vert_handle *h, long long n_cols, int n_alts, double dt,
+                             const double *g_temp, const double *cp, const double *temp_unit,
+                             const double *heat, double *g_heat, double *g_temp_unit,
+                             double *g_cp);
+
 /* The same adjoint as gitm_vert_adjoint, taken by recomputing the step with
  * a taped number type (every FP64 operation a node of a tape in device
  * memory) and sweeping the tape back with atomic adds: slow (about 170x the

@@ -33,6 +33,8 @@ differentiable window (gitm_amd.adjoint.VerticalWindow).
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -170,10 +172,51 @@ def test_prep_adjoint_against_differences(solver):
         assert rel < 1e-6, (field, adj, fd)
 
 
+def test_source_adjoint_bits(solver):
+    """gitm_vert_source_adjoint against the heating source's adjoint written
+    out in numpy in the same order of operations (a = lam * (Dt / (cp *
+    TempUnit)) on levels 1..nAlts, g_heat += a, g_tu -= a Q / TempUnit, g_cp
+    = -(a Q / cp)): the same bits, also with the prep held (g_heat only).
+    Mismatched or partial arguments are refused before any launch."""
+    import torch
+    from gitm_amd import adjoint
+    rng = np.random.default_rng(11)
+    n, L, dt = 333, 54, 2.0
+    lam, heat, gh0, gtu0 = (rng.standard_normal((n, L)) for _ in range(4))
+    cp = 1000.0 + 500.0 * rng.random((n, L))
+    tu = 1.0 + 30.0 * rng.random((n, L))
+    a = np.zeros((n, L))
+    a[:, 2:-2] = lam[:, 2:-2] * (dt / (cp[:, 2:-2] * tu[:, 2:-2]))
+    b = a * heat
+    want_gh, want_gtu, want_gcp = gh0 + a, gtu0 - b / tu, -(b / cp)
+    dv = {k: torch.from_numpy(v.copy()).cuda()
+          for k, v in dict(lam=lam, heat=heat, gh=gh0, gtu=gtu0, cp=cp, tu=tu).items()}
+    gcp = torch.full_like(dv["cp"], float("nan"))
+    adjoint.source_adjoint(solver, dv["lam"], dv["cp"], dv["tu"], dv["heat"], dt, dv["gh"],
+                           dv["gtu"], gcp)
+    solver.check()
+    assert np.array_equal(dv["gh"].cpu().numpy(), want_gh)
+    assert np.array_equal(dv["gtu"].cpu().numpy(), want_gtu)
+    assert np.array_equal(gcp.cpu().numpy(), want_gcp)
+    gh = torch.from_numpy(gh0.copy()).cuda()
+    adjoint.source_adjoint(solver, dv["lam"], dv["cp"], dv["tu"], dv["heat"], dt, gh)
+    assert np.array_equal(gh.cpu().numpy(), want_gh)
+    with pytest.raises(ValueError):
+        adjoint.source_adjoint(solver, dv["lam"], dv["cp"], dv["tu"], dv["heat"], dt, gh,
+                               dv["gtu"], None)
+    lib = adjoint._lib()
+    p = ctypes.cast(gh.data_ptr(), ctypes.POINTER(ctypes.c_double))
+    assert lib.gitm_vert_source_adjoint(solver._h, n, L - 4, dt, p, p, p, p, p, p, None) != 0
+    assert lib.gitm_vert_source_adjoint(solver._h, -1, L - 4, dt, p, p, p, p, p, None, None) != 0
+    assert lib.gitm_vert_source_adjoint(solver._h, n, L - 4, dt, None, p, p, p, p, None,
+                                        None) != 0
+
+
 def test_adjoint_record_chunks(solver):
     """A small record budget runs the columns in chunks: the same bits.  A
-    column's record is 5 x 106 x 64 doubles (271,360 B): 64 MiB holds 247
-    columns, so the 324 columns run as a chunk of 247 and one of 77."""
+    column's record, work area and boundary record are (5 x 125 + 54) x 64
+    + 300 doubles (350,048 B): 64 MiB holds 191 columns, so the 324 columns
+    run as a chunk of 191 and one of 133."""
     import torch
     from gitm_amd import adjoint, vertical
     cols = synth.make_columns(synth.CONFIGS["cpu_324x50"], seed=4)
