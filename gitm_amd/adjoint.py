@@ -180,13 +180,13 @@ def source_adjoint(solver: "vertical.VerticalSolver", g_temp, cp, tempunit, heat
     (levels 1..nAlts) to ``g_heat`` and, when given, -a Q / TempUnit to
     ``g_tempunit`` and writes -a Q / cp to ``g_cp``.  All [column][level]
     float64 device tensors.  Queued on the current stream."""
+    if (g_tempunit is None) != (g_cp is None):
+        raise ValueError("source_adjoint: g_tempunit and g_cp go together")
     ts = [g_temp, cp, tempunit, heat, g_heat] + ([g_tempunit, g_cp] if g_cp is not None else [])
     for i, t in enumerate(ts):
         _check(t, f"source_adjoint argument {i}")
         if t.shape != g_temp.shape:
             raise ValueError("source_adjoint: every tensor shaped like g_temp")
-    if (g_tempunit is None) != (g_cp is None):
-        raise ValueError("source_adjoint: g_tempunit and g_cp go together")
     ncol, L = int(g_temp.shape[0]), int(g_temp.shape[1])
     dp = ctypes.POINTER(ctypes.c_double)
 
@@ -289,7 +289,7 @@ class VerticalWindow(torch.autograd.Function):
             state["temp"] = _source(state["temp"], heat.detach(), cp_tu, params.dt)
         solver.check()
         ctx.solver, ctx.geometry, ctx.held, ctx.params, ctx.pp = solver, geometry, held, params, pp
-        ctx.checkpoints, ctx.heat = checkpoints, heat.detach()
+        ctx.checkpoints, ctx.heat = checkpoints, heat.detach().contiguous()
         return tuple(state[k] for k in STATE)
 
     @staticmethod

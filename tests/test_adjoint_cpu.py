@@ -105,6 +105,19 @@ def test_level_weights_and_heating_profile():
     assert q[0] == pytest.approx(1e-5) and q[1] == pytest.approx(1e-5 * np.exp(-1.0))
 
 
+def test_source_adjoint_argument_checks():
+    """The heating source's adjoint takes contiguous float64 device tensors
+    of one shape, with the prep's cp and TempUnit adjoints both or neither:
+    anything else is refused on the host, before the library is reached."""
+    import torch
+    from gitm_amd import adjoint
+    t = torch.zeros(3, 8, dtype=torch.float64)
+    with pytest.raises(ValueError):
+        adjoint.source_adjoint(None, t, t, t, t, 1.0, t)  # host tensors
+    with pytest.raises(ValueError, match="go together"):
+        adjoint.source_adjoint(None, t, t, t, t, 1.0, t, t, None)
+
+
 # ---------------------------------------------------------------------------
 # The hand-written adjoint (gitm_amd/csrc/vert_adj.h), host build.
 # ---------------------------------------------------------------------------
